@@ -1,0 +1,852 @@
+/* ks_oracle.c -- FP64 CPU restatement of MILC's staggered hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker, never the product): see ks_oracle.h.
+ *
+ * Every function cites the MILC 7.8.1 source it restates.  Pinning:
+ *   kso_dslash        pinned by: MILC golden (M^dag M)^-1 src on 8^4 asqtad
+ *                     (binary_samples/ksvector.ans.MdaggerMinv.sample.l8),
+ *                     free-field closed form, anti-Hermiticity, gauge
+ *                     covariance, compiled MILC dslash_fn_field (oracle/_ref).
+ *   kso_congrad       pinned by: the same golden, dense solve on 4^4,
+ *                     iteration-for-iteration against compiled MILC.
+ *   kso_multicg_offset pinned by: per-shift true residuals, agreement with
+ *                     independent single-shift CG, compiled MILC.
+ *   kso_fat_links / kso_long_links / kso_asqtad_links
+ *                     pinned by: MILC golden asqtad fat links
+ *                     (binary_samples/fat.l8 from lat.sample.l8, u0=0.502),
+ *                     free-field sums of coefficients, gauge covariance.
+ *   kso_u3_unitarize  pinned by: unitarity, polar-decomposition identity
+ *                     W = V (V^dag V)^-1/2 checked against numpy SVD.
+ *   kso_hisq_links    pinned by: compiled MILC load_hisq_aux_links/load_X
+ *                     (oracle/_ref) and the free-field closed form.
+ *   kso_force_oprod   pinned by: central finite differences of the action,
+ *                     compiled MILC smearing0 outer products.
+ *   kso_gauge_random  pinned by: unitarity, det = 1, numpy expm of i*0.4*H.
+ */
+#include "ks_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* complex 3x3 helpers on double[18] (row-major e[i][j], re/im)               */
+/* ------------------------------------------------------------------------ */
+#define RE(m, i, j) (m)[2 * (3 * (i) + (j))]
+#define IM(m, i, j) (m)[2 * (3 * (i) + (j)) + 1]
+
+static inline void m_nn(const double *a, const double *b, double *c) {
+  double t[18];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      double re = 0, im = 0;
+      for (int k = 0; k < 3; k++) {
+        re += RE(a, i, k) * RE(b, k, j) - IM(a, i, k) * IM(b, k, j);
+        im += RE(a, i, k) * IM(b, k, j) + IM(a, i, k) * RE(b, k, j);
+      }
+      RE(t, i, j) = re; IM(t, i, j) = im;
+    }
+  memcpy(c, t, sizeof t);
+}
+/* c = a * b^dag  (libraries/m_mat_na.c) */
+static inline void m_na(const double *a, const double *b, double *c) {
+  double t[18];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      double re = 0, im = 0;
+      for (int k = 0; k < 3; k++) {
+        re += RE(a, i, k) * RE(b, j, k) + IM(a, i, k) * IM(b, j, k);
+        im += IM(a, i, k) * RE(b, j, k) - RE(a, i, k) * IM(b, j, k);
+      }
+      RE(t, i, j) = re; IM(t, i, j) = im;
+    }
+  memcpy(c, t, sizeof t);
+}
+/* c = a^dag * b  (libraries/m_mat_an.c) */
+static inline void m_an(const double *a, const double *b, double *c) {
+  double t[18];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      double re = 0, im = 0;
+      for (int k = 0; k < 3; k++) {
+        re += RE(a, k, i) * RE(b, k, j) + IM(a, k, i) * IM(b, k, j);
+        im += RE(a, k, i) * IM(b, k, j) - IM(a, k, i) * RE(b, k, j);
+      }
+      RE(t, i, j) = re; IM(t, i, j) = im;
+    }
+  memcpy(c, t, sizeof t);
+}
+static inline void m_adj(const double *a, double *c) {
+  double t[18];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) { RE(t, i, j) = RE(a, j, i); IM(t, i, j) = -IM(a, j, i); }
+  memcpy(c, t, sizeof t);
+}
+static inline void m_axpy(double *y, double s, const double *x) {
+  for (int k = 0; k < 18; k++) y[k] += s * x[k];
+}
+/* out (+)= a * v */
+static inline void mv(const double *a, const double *v, double *out, int accumulate) {
+  double t[6];
+  for (int i = 0; i < 3; i++) {
+    double re = 0, im = 0;
+    for (int k = 0; k < 3; k++) {
+      re += RE(a, i, k) * v[2 * k] - IM(a, i, k) * v[2 * k + 1];
+      im += RE(a, i, k) * v[2 * k + 1] + IM(a, i, k) * v[2 * k];
+    }
+    t[2 * i] = re; t[2 * i + 1] = im;
+  }
+  if (accumulate) for (int k = 0; k < 6; k++) out[k] += t[k];
+  else memcpy(out, t, sizeof t);
+}
+/* out (+)= a^dag * v */
+static inline void amv(const double *a, const double *v, double *out, int accumulate) {
+  double t[6];
+  for (int i = 0; i < 3; i++) {
+    double re = 0, im = 0;
+    for (int k = 0; k < 3; k++) {
+      re += RE(a, k, i) * v[2 * k] + IM(a, k, i) * v[2 * k + 1];
+      im += RE(a, k, i) * v[2 * k + 1] - IM(a, k, i) * v[2 * k];
+    }
+    t[2 * i] = re; t[2 * i + 1] = im;
+  }
+  if (accumulate) for (int k = 0; k < 6; k++) out[k] += t[k];
+  else memcpy(out, t, sizeof t);
+}
+
+/* ------------------------------------------------------------------------ */
+/* geometry (generic/layout_hyper_prime.c:509-520, include/dirs.h)            */
+/* ------------------------------------------------------------------------ */
+size_t kso_node_index(const kso_geom *g, int x, int y, int z, int t) {
+  const int *d = g->dims;
+  x = ((x % d[0]) + d[0]) % d[0];
+  y = ((y % d[1]) + d[1]) % d[1];
+  z = ((z % d[2]) + d[2]) % d[2];
+  t = ((t % d[3]) + d[3]) % d[3];
+  size_t i = (size_t)x + (size_t)d[0] * ((size_t)y + (size_t)d[1] * ((size_t)z + (size_t)d[2] * (size_t)t));
+  if ((x + y + z + t) % 2 == 0) return i / 2;
+  return (i + g->V) / 2;
+}
+
+int kso_geom_init(kso_geom *g, const int dims[4]) {
+  memcpy(g->dims, dims, sizeof g->dims);
+  g->V = (size_t)dims[0] * dims[1] * dims[2] * dims[3];
+  g->Vh = g->V / 2;
+  if (dims[0] % 2 || g->V % 2) return -1;
+  g->coords = (int *)malloc(sizeof(int) * 4 * g->V);
+  g->nbr = (size_t *)malloc(sizeof(size_t) * 16 * g->V);
+  if (!g->coords || !g->nbr) return -2;
+  for (int t = 0; t < dims[3]; t++)
+    for (int z = 0; z < dims[2]; z++)
+      for (int y = 0; y < dims[1]; y++)
+        for (int x = 0; x < dims[0]; x++) {
+          size_t i = kso_node_index(g, x, y, z, t);
+          int *c = g->coords + 4 * i;
+          c[0] = x; c[1] = y; c[2] = z; c[3] = t;
+        }
+#pragma omp parallel for
+  for (size_t i = 0; i < g->V; i++) {
+    const int *c = g->coords + 4 * i;
+    for (int dir = 0; dir < 4; dir++) {
+      for (int hop = 1; hop <= 3; hop += 2) {
+        int cp[4], cm[4];
+        memcpy(cp, c, sizeof cp); memcpy(cm, c, sizeof cm);
+        cp[dir] += hop; cm[dir] -= hop;
+        size_t ip = kso_node_index(g, cp[0], cp[1], cp[2], cp[3]);
+        size_t im = kso_node_index(g, cm[0], cm[1], cm[2], cm[3]);
+        int base = hop == 1 ? 0 : 8;
+        g->nbr[(size_t)(base + dir) * g->V + i] = ip;         /* XUP..TUP / X3UP..T3UP */
+        g->nbr[(size_t)(base + 7 - dir) * g->V + i] = im;     /* OPP_DIR = 7-dir */
+      }
+    }
+  }
+  return 0;
+}
+
+void kso_geom_free(kso_geom *g) {
+  free(g->coords); free(g->nbr);
+  g->coords = NULL; g->nbr = NULL;
+}
+
+#define NB(g, k, i) ((g)->nbr[(size_t)(k) * (g)->V + (i)])
+
+static void parity_range(const kso_geom *g, int parity, size_t *lo, size_t *hi) {
+  switch (parity) {
+    case KSO_EVEN: *lo = 0; *hi = g->Vh; break;
+    case KSO_ODD: *lo = g->Vh; *hi = g->V; break;
+    default: *lo = 0; *hi = g->V; break;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* counter-based RNG shared (by specification) with the HIP generator       */
+/*   u64 draw(seed, ctr) = mix(mix(seed ^ K0) + ctr), mix = splitmix64 final */
+/*   uniform = ((draw >> 11) + 0.5) * 2^-53                                   */
+/*   gaussian pair p: Box-Muller on uniforms 2p, 2p+1                         */
+/* ------------------------------------------------------------------------ */
+static inline uint64_t sm_mix(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+static inline double uni(uint64_t key, uint64_t ctr) {
+  uint64_t r = sm_mix(key + ctr);
+  return ((double)(r >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+}
+static inline void gauss_pair(uint64_t key, uint64_t p, double *g0, double *g1) {
+  double u1 = uni(key, 2 * p), u2 = uni(key, 2 * p + 1);
+  double r = sqrt(-2.0 * log(u1));
+  double th = 6.283185307179586476925286766559 * u2;
+  *g0 = r * cos(th);
+  *g1 = r * sin(th);
+}
+
+/* exp(X) for anti-Hermitian 3x3 X: 4 halvings, degree-14 Taylor (Horner),
+ * 4 squarings.  Only + and * after the Gaussians, so CPU and GPU agree to
+ * rounding. */
+static void expm3(const double *X, double *E) {
+  double Y[18], T[18];
+  for (int k = 0; k < 18; k++) Y[k] = X[k] * 0.0625;
+  /* Horner: E = I + Y(I + Y/2(I + Y/3(...))) */
+  memset(E, 0, sizeof(double) * 18);
+  RE(E, 0, 0) = RE(E, 1, 1) = RE(E, 2, 2) = 1.0;
+  for (int n = 14; n >= 1; n--) {
+    m_nn(Y, E, T);
+    for (int k = 0; k < 18; k++) E[k] = T[k] / (double)n;
+    RE(E, 0, 0) += 1.0; RE(E, 1, 1) += 1.0; RE(E, 2, 2) += 1.0;
+  }
+  for (int s = 0; s < 4; s++) { m_nn(E, E, T); memcpy(E, T, sizeof T); }
+}
+
+/* U = exp(i*scale*H), H = (A+A^dag)/2 - Tr/3, A_jk ~ N(0,1)+iN(0,1); per link
+ * (lexicographic site s, dir mu) the 9 Gaussian pairs use p = (4s+mu)*9+n.
+ * Parity unpinned against MILC (MILC has no such generator): pinned by
+ * unitarity, det U = 1 and numpy's expm of the same H in tests. */
+void kso_gauge_random(const kso_geom *g, uint64_t seed, double scale, double *links) {
+  uint64_t key = sm_mix(seed ^ 0x5851F42D4C957F2DULL);
+#pragma omp parallel for
+  for (size_t i = 0; i < g->V; i++) {
+    const int *c = g->coords + 4 * i;
+    uint64_t s = (uint64_t)c[0] + (uint64_t)g->dims[0] * ((uint64_t)c[1] + (uint64_t)g->dims[1] *
+                 ((uint64_t)c[2] + (uint64_t)g->dims[2] * (uint64_t)c[3]));
+    for (int mu = 0; mu < 4; mu++) {
+      double A[18], H[18], X[18];
+      for (int n = 0; n < 9; n++) gauss_pair(key, (4 * s + mu) * 9 + n, &A[2 * n], &A[2 * n + 1]);
+      for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) {
+          RE(H, a, b) = 0.5 * (RE(A, a, b) + RE(A, b, a));
+          IM(H, a, b) = 0.5 * (IM(A, a, b) - IM(A, b, a));
+        }
+      double tr = (RE(H, 0, 0) + RE(H, 1, 1) + RE(H, 2, 2)) / 3.0;
+      RE(H, 0, 0) -= tr; RE(H, 1, 1) -= tr; RE(H, 2, 2) -= tr;
+      /* X = i*scale*H */
+      for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) {
+          RE(X, a, b) = -scale * IM(H, a, b);
+          IM(X, a, b) = scale * RE(H, a, b);
+        }
+      expm3(X, links + (4 * i + mu) * 18);
+    }
+  }
+}
+
+/* Gaussian complex vector on one parity, <re^2> = <im^2> = 1/2 as in
+ * grsource_plain_field (generic_ks/grsource_imp.c:169-195); site s (lex) uses
+ * pairs p = 3s + c.  Other-parity sites are zeroed. */
+void kso_gaussian_vector(const kso_geom *g, uint64_t seed, int parity, double *v) {
+  uint64_t key = sm_mix(seed ^ 0x2545F4914F6CDD1DULL);
+  size_t lo, hi;
+  parity_range(g, parity, &lo, &hi);
+  const double s2 = 0.70710678118654752440;
+#pragma omp parallel for
+  for (size_t i = 0; i < g->V; i++) {
+    double *vi = v + 6 * i;
+    if (i < lo || i >= hi) { memset(vi, 0, 6 * sizeof(double)); continue; }
+    const int *c = g->coords + 4 * i;
+    uint64_t s = (uint64_t)c[0] + (uint64_t)g->dims[0] * ((uint64_t)c[1] + (uint64_t)g->dims[1] *
+                 ((uint64_t)c[2] + (uint64_t)g->dims[2] * (uint64_t)c[3]));
+    for (int k = 0; k < 3; k++) {
+      double a, b;
+      gauss_pair(key, 3 * s + k, &a, &b);
+      vi[2 * k] = s2 * a; vi[2 * k + 1] = s2 * b;
+    }
+  }
+}
+
+/* KS phases and antiperiodic time boundary (generic_ks/rephase.c:64-118);
+ * multiplying twice restores the input, as rephase(OFF) does. */
+void kso_rephase(const kso_geom *g, double *links) {
+#pragma omp parallel for
+  for (size_t i = 0; i < g->V; i++) {
+    const int *c = g->coords + 4 * i;
+    double ph[4];
+    ph[3] = 1.0;
+    ph[0] = (c[3] % 2 == 1) ? -1.0 : 1.0;
+    ph[1] = (c[0] % 2 == 1) ? -ph[0] : ph[0];
+    ph[2] = (c[1] % 2 == 1) ? -ph[1] : ph[1];
+    if (c[3] == g->dims[3] - 1) ph[3] = -ph[3];
+    for (int mu = 0; mu < 4; mu++) {
+      double *m = links + (4 * i + mu) * 18;
+      for (int k = 0; k < 18; k++) m[k] *= ph[mu];
+    }
+  }
+}
+
+/* Average plaquette Re Tr/3 over all 6 planes (for test diagnostics). */
+double kso_plaquette(const kso_geom *g, const double *L) {
+  double sum = 0;
+#pragma omp parallel for reduction(+ : sum)
+  for (size_t i = 0; i < g->V; i++) {
+    for (int mu = 0; mu < 4; mu++)
+      for (int nu = mu + 1; nu < 4; nu++) {
+        double a[18], b[18];
+        m_nn(L + (4 * i + mu) * 18, L + (4 * NB(g, mu, i) + nu) * 18, a);
+        m_nn(L + (4 * i + nu) * 18, L + (4 * NB(g, nu, i) + mu) * 18, b);
+        m_na(a, b, a);
+        sum += (RE(a, 0, 0) + RE(a, 1, 1) + RE(a, 2, 2)) / 3.0;
+      }
+  }
+  return sum / (6.0 * g->V);
+}
+
+/* ------------------------------------------------------------------------ */
+/* link smearing                                                             */
+/* ------------------------------------------------------------------------ */
+
+/* Generalised staple (generic/general_staple.c:42-124): for the "mu link"
+ * field L (stride floats of 18 per site) and thin links U,
+ *   upper(x) = U_nu(x) [L(x+nu) U_nu(x+mu)^dag]
+ *   lower(x) = [U_nu(x-nu)^dag L(x-nu)] U_nu(x-nu+mu)
+ * If staple != NULL: staple = upper + lower and fat_mu += coef*staple;
+ * else fat_mu += coef*upper then fat_mu += coef*lower. */
+static void gen_staple(const kso_geom *g, double *staple, int mu, int nu, const double *L,
+                       int stride, double *fat, double coef, const double *U, double *tmp) {
+  const size_t V = g->V;
+#pragma omp parallel for
+  for (size_t i = 0; i < V; i++) {
+    double t1[18], t2[18];
+    size_t xn = NB(g, nu, i), xm = NB(g, mu, i);
+    m_na(L + (size_t)stride * 18 * xn, U + (4 * xm + nu) * 18, t1);
+    if (staple) m_nn(U + (4 * i + nu) * 18, t1, staple + 18 * i);
+    else {
+      m_nn(U + (4 * i + nu) * 18, t1, t2);
+      m_axpy(fat + (4 * i + mu) * 18, coef, t2);
+    }
+  }
+#pragma omp parallel for
+  for (size_t i = 0; i < V; i++) {
+    double t1[18];
+    m_an(U + (4 * i + nu) * 18, L + (size_t)stride * 18 * i, t1);
+    m_nn(t1, U + (4 * NB(g, mu, i) + nu) * 18, tmp + 18 * i);
+  }
+#pragma omp parallel for
+  for (size_t i = 0; i < V; i++) {
+    const double *low = tmp + 18 * NB(g, 7 - nu, i);
+    double *f = fat + (4 * i + mu) * 18;
+    if (staple) {
+      for (int k = 0; k < 18; k++) staple[18 * i + k] += low[k];
+      m_axpy(f, coef, staple + 18 * i);
+    } else {
+      m_axpy(f, coef, low);
+    }
+  }
+}
+
+/* Optimised asqtad fattening (generic_ks/fermion_links_fn_load_milc.c:222-252,
+ * ASQ_OPTIMIZED_FATTENING): one-link term corrected by -6*lepage because the
+ * Lepage staple-of-staple contains six retracing paths equal to U_mu. */
+void kso_fat_links(const kso_geom *g, const double coeff[6], const double *U, double *fat) {
+  const size_t V = g->V;
+  double one_link = coeff[0] - 6.0 * coeff[5];
+  double *staple = (double *)malloc(sizeof(double) * 18 * V);
+  double *tempmat1 = (double *)malloc(sizeof(double) * 18 * V);
+  double *tmp = (double *)malloc(sizeof(double) * 18 * V);
+  for (int dir = 0; dir < 4; dir++) {
+#pragma omp parallel for
+    for (size_t i = 0; i < V; i++) {
+      const double *u = U + (4 * i + dir) * 18;
+      double *f = fat + (4 * i + dir) * 18;
+      for (int k = 0; k < 18; k++) f[k] = one_link * u[k];
+    }
+    if (coeff[2] == 0.0 && coeff[5] == 0.0 && coeff[3] == 0.0) continue;
+    for (int nu = 0; nu < 4; nu++) {
+      if (nu == dir) continue;
+      gen_staple(g, staple, dir, nu, U + 18 * dir, 4, fat, coeff[2], U, tmp);
+      gen_staple(g, NULL, dir, nu, staple, 1, fat, coeff[5], U, tmp);
+      for (int rho = 0; rho < 4; rho++) {
+        if (rho == dir || rho == nu) continue;
+        gen_staple(g, tempmat1, dir, rho, staple, 1, fat, coeff[3], U, tmp);
+        for (int sig = 0; sig < 4; sig++) {
+          if (sig == dir || sig == nu || sig == rho) continue;
+          gen_staple(g, NULL, dir, sig, tempmat1, 1, fat, coeff[4], U, tmp);
+        }
+      }
+    }
+  }
+  free(staple); free(tempmat1); free(tmp);
+}
+
+/* Naik links (generic_ks/fermion_links_fn_load_milc.c:51-131): the only path
+ * with net displacement 3*mu in the HISQ/asqtad tables is the straight one,
+ * lng_mu(x) = naik * U_mu(x) U_mu(x+mu) U_mu(x+2mu). */
+void kso_long_links(const kso_geom *g, double naik, const double *U, double *lng) {
+#pragma omp parallel for
+  for (size_t i = 0; i < g->V; i++) {
+    for (int mu = 0; mu < 4; mu++) {
+      size_t x1 = NB(g, mu, i), x2 = NB(g, mu, x1);
+      double t[18];
+      m_nn(U + (4 * x1 + mu) * 18, U + (4 * x2 + mu) * 18, t);
+      m_nn(U + (4 * i + mu) * 18, t, t);
+      double *l = lng + (4 * i + mu) * 18;
+      for (int k = 0; k < 18; k++) l[k] = naik * t[k];
+    }
+  }
+}
+
+/* Analytic U(3) projection W = V (V^dag V)^-1/2
+ * (generic_ks/su3_mat_op.c:828-1211, Hasenfratz-Hoffmann-Schaefer):
+ * eigenvalues g_k of Q = V^dag V from the trigonometric cubic solution, then
+ * Q^-1/2 = f0 + f1 Q + f2 Q^2.  Returns 1 when MILC's SVD fallback condition
+ * (relative det mismatch > 1e-8 or |det|^2 < 1e-8) holds; the analytic result
+ * is still returned then (callers count and report these links). */
+int kso_u3_unitarize(const double *V, double *W) {
+  double Q[18], Q2[18];
+  m_an(V, V, Q);
+  m_nn(Q, Q, Q2);
+  double c0 = RE(Q, 0, 0) + RE(Q, 1, 1) + RE(Q, 2, 2);
+  double c1 = (RE(Q2, 0, 0) + RE(Q2, 1, 1) + RE(Q2, 2, 2)) / 2;
+  double q3 = 0;
+  for (int i = 0; i < 3; i++)
+    for (int k = 0; k < 3; k++) q3 += RE(Q2, i, k) * RE(Q, k, i) - IM(Q2, i, k) * IM(Q, k, i);
+  double c2 = q3 / 3;
+  double g0, g1, g2;
+  double S = c1 / 3 - c0 * (c0 / 18);
+  if (fabs(S) < 1.0e-14) {
+    g0 = g1 = g2 = c0 / 3;
+  } else {
+    double R = c2 / 2 - c0 * (c1 / 3) + c0 * c0 * (c0 / 27);
+    S = sqrt(S);
+    double RoS = R / (S * S * S);
+    double theta;
+    if (!(fabs(RoS) < 1.0)) theta = R > 0 ? 0.0 : 3.14159265358979323846264338328;
+    else theta = acos(RoS);
+    double th3 = theta / 3, pi23 = 6.28318530717958647692528676656 / 3;
+    g0 = c0 / 3 + 2 * S * cos(th3);
+    g1 = c0 / 3 + 2 * S * cos(th3 + pi23);
+    g2 = c0 / 3 + 2 * S * cos(th3 + 2 * pi23);
+  }
+  /* determinant check used by MILC to decide on the SVD fallback */
+  double dre = 0, dim = 0;
+  {
+    double a1re = RE(V,1,1)*RE(V,2,2) - IM(V,1,1)*IM(V,2,2) - RE(V,1,2)*RE(V,2,1) + IM(V,1,2)*IM(V,2,1);
+    double a1im = RE(V,1,1)*IM(V,2,2) + IM(V,1,1)*RE(V,2,2) - RE(V,1,2)*IM(V,2,1) - IM(V,1,2)*RE(V,2,1);
+    double a2re = RE(V,1,0)*RE(V,2,2) - IM(V,1,0)*IM(V,2,2) - RE(V,1,2)*RE(V,2,0) + IM(V,1,2)*IM(V,2,0);
+    double a2im = RE(V,1,0)*IM(V,2,2) + IM(V,1,0)*RE(V,2,2) - RE(V,1,2)*IM(V,2,0) - IM(V,1,2)*RE(V,2,0);
+    double a3re = RE(V,1,0)*RE(V,2,1) - IM(V,1,0)*IM(V,2,1) - RE(V,1,1)*RE(V,2,0) + IM(V,1,1)*IM(V,2,0);
+    double a3im = RE(V,1,0)*IM(V,2,1) + IM(V,1,0)*RE(V,2,1) - RE(V,1,1)*IM(V,2,0) - IM(V,1,1)*RE(V,2,0);
+    dre = RE(V,0,0)*a1re - IM(V,0,0)*a1im - RE(V,0,1)*a2re + IM(V,0,1)*a2im + RE(V,0,2)*a3re - IM(V,0,2)*a3im;
+    dim = IM(V,0,0)*a1re + RE(V,0,0)*a1im - IM(V,0,1)*a2re - RE(V,0,1)*a2im + IM(V,0,2)*a3re + RE(V,0,2)*a3im;
+  }
+  double det_check = dre * dre + dim * dim;
+  int svd = 0;
+  if (det_check != 0 && fabs(det_check - g0 * g1 * g2) / fabs(det_check) > 1e-8) svd = 1;
+  if (det_check < 1e-8) svd = 1;
+
+  double g0s = sqrt(g0), g1s = sqrt(g1), g2s = sqrt(g2);
+  double us = g1s + g2s, ws = g1s * g2s;
+  double vs = g0s * us + ws;
+  us += g0s;
+  ws *= g0s;
+  double denom = ws * (us * vs - ws);
+  double f0 = (us * vs * vs - ws * (us * us + vs)) / denom;
+  double f1 = (2 * us * vs - ws - us * us * us) / denom;
+  double f2 = us / denom;
+  double S2[18];
+  for (int k = 0; k < 18; k++) S2[k] = f1 * Q[k] + f2 * Q2[k];
+  RE(S2, 0, 0) += f0; RE(S2, 1, 1) += f0; RE(S2, 2, 2) += f0;
+  m_nn(V, S2, W);
+  return svd;
+}
+
+/* HISQ U(3) link chain (generic_ks/fermion_links_hisq_load_milc.c:546-702)
+ * with the path coefficients of generic_ks/imp_actions/hisq/hisq_u3_action.h:
+ *   level 1 (fat7):  one-link 1/8, 3-staple -1/16, 5-staple 1/64, 7-staple -1/384
+ *   level 2:         one-link 1, Naik -1/24, 3-staple -1/16, 5-staple 1/64,
+ *                    7-staple -1/384, Lepage -1/8;
+ * unitarisation with phases out (rephase_field_offset OFF/ON around
+ * u3_unitarize_analytic, :251-270), U(3) group so W = Y (WeqY).
+ * eps_naik = 0, a single Naik set. */
+int kso_hisq_links(const kso_geom *g, const double *U, double *fat, double *lng, double *W_out) {
+  const size_t n = 4 * g->V * 18;
+  double *Vl = (double *)malloc(sizeof(double) * n);
+  double *W = W_out ? W_out : (double *)malloc(sizeof(double) * n);
+  const double c1[6] = {1.0 / 8.0, 0.0, -1.0 / 8.0 * 0.5, 1.0 / 8.0 * 0.25 * 0.5,
+                        -1.0 / 8.0 * 0.125 * (1.0 / 6.0), 0.0};
+  const double c2[6] = {(1.0 / 8.0) + (2.0 * 6.0 / 16.0) + (1.0 / 8.0), -1.0 / 24.0,
+                        -1.0 / 8.0 * 0.5, 1.0 / 8.0 * 0.25 * 0.5,
+                        -1.0 / 8.0 * 0.125 * (1.0 / 6.0), -2.0 / 16.0};
+  kso_fat_links(g, c1, U, Vl);
+  kso_rephase(g, Vl);
+  int nsvd = 0;
+#pragma omp parallel for reduction(+ : nsvd)
+  for (size_t l = 0; l < 4 * g->V; l++) nsvd += kso_u3_unitarize(Vl + 18 * l, W + 18 * l);
+  kso_rephase(g, W);
+  kso_fat_links(g, c2, W, fat);
+  kso_long_links(g, c2[1], W, lng);
+  free(Vl);
+  if (!W_out) free(W);
+  return nsvd;
+}
+
+/* Asqtad links as ks_imp_utilities/check_ks_invert builds them
+ * (generic_ks/imp_actions/asqtad_action.h path coefficients, tadpole
+ * factors u0^-(length-1), ks_action_paths.c build_act_path_coeff). */
+void kso_asqtad_links(const kso_geom *g, double u0, const double *U, double *fat, double *lng) {
+  double u2 = u0 * u0, u4 = u2 * u2, u6 = u4 * u2;
+  const double c[6] = {5.0 / 8.0, -1.0 / 24.0 / u2, -1.0 / 16.0 / u2, 1.0 / 64.0 / u4,
+                       -1.0 / 384.0 / u6, -1.0 / 16.0 / u4};
+  kso_fat_links(g, c, U, fat);
+  kso_long_links(g, c[1], U, lng);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Dslash (generic_ks/dslash_fn_dblstore.c:311-563)                           */
+/*   dst(x) = sum_mu F_mu(x) s(x+mu) + L_mu(x) s(x+3mu)                       */
+/*          - F_mu(x-mu)^dag s(x-mu) - L_mu(x-3mu)^dag s(x-3mu)               */
+/* on sites of `parity`; the back links are the adjoints of the neighbours'    */
+/* forward links (fn_links_milc.c:115-195).  src and dst may alias (the two   */
+/* parities are disjoint halves).                                             */
+/* ------------------------------------------------------------------------ */
+void kso_dslash(const kso_geom *g, const double *fat, const double *lng, const double *src,
+                double *dst, int parity) {
+  size_t lo, hi;
+  parity_range(g, parity, &lo, &hi);
+#pragma omp parallel for
+  for (size_t i = lo; i < hi; i++) {
+    double fwd[6], t[6], b[6], bl[6];
+    /* forward fat 4dir sum, then forward long 4dir sum (tvec), then add */
+    mv(fat + (4 * i + 0) * 18, src + 6 * NB(g, 0, i), fwd, 0);
+    for (int mu = 1; mu < 4; mu++) mv(fat + (4 * i + mu) * 18, src + 6 * NB(g, mu, i), fwd, 1);
+    mv(lng + (4 * i + 0) * 18, src + 6 * NB(g, 8, i), t, 0);
+    for (int mu = 1; mu < 4; mu++) mv(lng + (4 * i + mu) * 18, src + 6 * NB(g, 8 + mu, i), t, 1);
+    for (int k = 0; k < 6; k++) fwd[k] += t[k];
+    /* backward: fatback(x,mu) = F_mu(x-mu)^dag, lngback(x,mu) = L_mu(x-3mu)^dag */
+    for (int mu = 0; mu < 4; mu++) {
+      size_t xm = NB(g, 7 - mu, i), x3m = NB(g, 15 - mu, i);
+      amv(fat + (4 * xm + mu) * 18, src + 6 * xm, b, mu > 0);
+      amv(lng + (4 * x3m + mu) * 18, src + 6 * x3m, bl, mu > 0);
+    }
+    for (int k = 0; k < 6; k++) { fwd[k] -= b[k]; fwd[k] -= bl[k]; }
+    memcpy(dst + 6 * i, fwd, sizeof fwd);
+  }
+}
+
+double kso_norm2(const kso_geom *g, const double *v, int parity) {
+  size_t lo, hi;
+  parity_range(g, parity, &lo, &hi);
+  double s = 0;
+#pragma omp parallel for reduction(+ : s)
+  for (size_t i = lo; i < hi; i++)
+    for (int k = 0; k < 6; k++) s += v[6 * i + k] * v[6 * i + k];
+  return s;
+}
+
+/* dst = (offset - D_{p,p'} D_{p',p}) src on `parity`, i.e. (M^dag M) for
+ * offset = 4m^2 (d_congrad5_fn_milc.c:196-211 computes the negative). */
+void kso_mdagm(const kso_geom *g, const double *fat, const double *lng, double offset,
+               const double *src, double *dst, double *tmp, int parity) {
+  int other = parity == KSO_EVEN ? KSO_ODD : KSO_EVEN;
+  kso_dslash(g, fat, lng, src, tmp, other);
+  kso_dslash(g, fat, lng, tmp, tmp, parity);
+  size_t lo, hi;
+  parity_range(g, parity, &lo, &hi);
+#pragma omp parallel for
+  for (size_t i = lo; i < hi; i++)
+    for (int k = 0; k < 6; k++) dst[6 * i + k] = offset * src[6 * i + k] - tmp[6 * i + k];
+}
+
+/* ------------------------------------------------------------------------ */
+/* CG (generic_ks/d_congrad5_fn_milc.c:60-407, FEWSUMS variant)               */
+/* ------------------------------------------------------------------------ */
+int kso_congrad(const kso_geom *g, const double *fat, const double *lng, const double *src,
+                double *dest, kso_qic *qic, double mass) {
+  const size_t V = g->V;
+  int parity = qic->parity;
+  int otherparity = parity == KSO_EVEN ? KSO_ODD : KSO_EVEN;
+  int niter = qic->max, max_restarts = qic->nrestart;
+  double rsqmin = qic->resid * qic->resid;
+  double relrsqmin = qic->relresid * qic->relresid;
+  int max_cg = max_restarts * niter;
+  double msq_x4 = 4.0 * mass * mass;
+  size_t lo, hi;
+  parity_range(g, parity, &lo, &hi);
+
+  double source_norm = kso_norm2(g, src, parity);
+  int nrestart = 0, iteration = 0;
+  qic->size_r = 0; qic->size_relr = 1.0;
+  qic->final_iters = 0; qic->final_restart = 0; qic->converged = 1;
+  qic->final_rsq = 0; qic->final_relrsq = 0;
+  if (source_norm == 0.0) {
+    for (size_t i = lo; i < hi; i++) memset(dest + 6 * i, 0, 6 * sizeof(double));
+    return 0;
+  }
+  double *ttt = (double *)calloc(6 * V, sizeof(double));
+  double *cg_p = (double *)calloc(6 * V, sizeof(double));
+  double *resid = (double *)calloc(6 * V, sizeof(double));
+  double rsq = 0, actual_rsq = 999., oldrsq, pkp, c_tr, c_tt, relrsq = 1.0;
+  double a, b;
+  for (;;) {
+    if ((iteration % niter == 0) ||
+        ((rsqmin <= 0 || rsqmin > qic->size_r) && (relrsqmin <= 0 || relrsqmin > qic->size_relr))) {
+      rsq = 0.0;
+      kso_dslash(g, fat, lng, dest, ttt, otherparity);
+      kso_dslash(g, fat, lng, ttt, ttt, parity);
+#pragma omp parallel for reduction(+ : rsq)
+      for (size_t i = lo; i < hi; i++)
+        for (int k = 0; k < 6; k++) {
+          size_t j = 6 * i + k;
+          ttt[j] = ttt[j] + (-msq_x4) * dest[j];
+          resid[j] = src[j] + ttt[j];
+          cg_p[j] = resid[j];
+          rsq += resid[j] * resid[j];
+        }
+      actual_rsq = rsq;
+      qic->final_rsq = rsq / source_norm;
+      qic->final_relrsq = relrsq;
+      iteration++;
+      if (iteration >= max_cg || nrestart >= max_restarts ||
+          ((rsqmin <= 0 || rsqmin > qic->final_rsq) && (relrsqmin <= 0 || relrsqmin > qic->final_relrsq)))
+        break;
+      nrestart++;
+    }
+    oldrsq = actual_rsq;
+    kso_dslash(g, fat, lng, cg_p, ttt, otherparity);
+    kso_dslash(g, fat, lng, ttt, ttt, parity);
+    pkp = 0; c_tr = 0; c_tt = 0;
+#pragma omp parallel for reduction(+ : pkp, c_tr, c_tt)
+    for (size_t i = lo; i < hi; i++)
+      for (int k = 0; k < 6; k++) {
+        size_t j = 6 * i + k;
+        ttt[j] = ttt[j] + (-msq_x4) * cg_p[j];
+        pkp += cg_p[j] * ttt[j];
+        c_tr += ttt[j] * resid[j];
+        c_tt += ttt[j] * ttt[j];
+      }
+    iteration++;
+    a = -rsq / pkp;
+    actual_rsq = 0;
+#pragma omp parallel for reduction(+ : actual_rsq)
+    for (size_t i = lo; i < hi; i++)
+      for (int k = 0; k < 6; k++) {
+        size_t j = 6 * i + k;
+        dest[j] = dest[j] + a * cg_p[j];
+        resid[j] = resid[j] + a * ttt[j];
+        actual_rsq += resid[j] * resid[j];
+      }
+    rsq = oldrsq + 2.0 * a * c_tr + a * a * c_tt;
+    qic->size_r = rsq / source_norm;
+    qic->size_relr = relrsq;
+    qic->final_iters = iteration;
+    qic->final_restart = nrestart;
+    qic->converged = 1;
+    b = rsq / oldrsq;
+#pragma omp parallel for
+    for (size_t i = lo; i < hi; i++)
+      for (int k = 0; k < 6; k++) {
+        size_t j = 6 * i + k;
+        cg_p[j] = resid[j] + b * cg_p[j];
+      }
+  }
+  qic->final_iters = iteration;
+  qic->final_restart = nrestart;
+  qic->converged = 1;
+  if (nrestart == max_restarts || iteration == max_cg) qic->converged = 0;
+  free(ttt); free(cg_p); free(resid);
+  return iteration;
+}
+
+/* ------------------------------------------------------------------------ */
+/* multi-shift CG (generic_ks/ks_multicg_offset.c:63-505), one parity          */
+/* ------------------------------------------------------------------------ */
+int kso_multicg_offset(const kso_geom *g, const double *fat, const double *lng, const double *src,
+                       double **psim, const double *offsets, int num_offsets, kso_qic *qic) {
+  const size_t V = g->V;
+  int niter = qic[0].max * qic[0].nrestart;
+  double rsqmin = qic[0].resid * qic[0].resid;
+  int parity = qic[0].parity;
+  int l_parity = parity == KSO_ODD ? KSO_ODD : KSO_EVEN;
+  int l_other = l_parity == KSO_EVEN ? KSO_ODD : KSO_EVEN;
+  for (int j = 0; j < num_offsets; j++) {
+    qic[j].final_rsq = 0; qic[j].final_relrsq = 0; qic[j].size_r = 0; qic[j].size_relr = 0;
+    qic[j].final_iters = 0; qic[j].final_restart = 0; qic[j].converged = 1;
+  }
+  if (num_offsets == 0) return 0;
+  for (int j = 0; j < num_offsets; j++)
+    if (offsets[j] <= 0) return -1;
+  size_t lo, hi;
+  parity_range(g, l_parity, &lo, &hi);
+
+  double *shifts = (double *)malloc(sizeof(double) * num_offsets);
+  double *zeta_i = (double *)malloc(sizeof(double) * num_offsets);
+  double *zeta_im1 = (double *)malloc(sizeof(double) * num_offsets);
+  double *zeta_ip1 = (double *)malloc(sizeof(double) * num_offsets);
+  double *beta_i = (double *)malloc(sizeof(double) * num_offsets);
+  double *beta_im1 = (double *)malloc(sizeof(double) * num_offsets);
+  double *alpha = (double *)malloc(sizeof(double) * num_offsets);
+  int *finished = (int *)calloc(num_offsets, sizeof(int));
+  double **pm = (double **)malloc(sizeof(double *) * num_offsets);
+  double offset_low = 1.0e20;
+  int j_low = -1;
+  for (int j = 0; j < num_offsets; j++) {
+    shifts[j] = offsets[j];
+    if (offsets[j] < offset_low) { offset_low = offsets[j]; j_low = j; }
+  }
+  for (int j = 0; j < num_offsets; j++) {
+    pm[j] = (double *)calloc(6 * V, sizeof(double));
+    if (j != j_low) shifts[j] -= shifts[j_low];
+  }
+  double shift0 = -shifts[j_low];
+  double *ttt = (double *)calloc(6 * V, sizeof(double));
+  double *cg_p = (double *)calloc(6 * V, sizeof(double));
+  double *resid = (double *)calloc(6 * V, sizeof(double));
+
+  int iteration = 0;
+  int num_offsets_now = num_offsets;
+  double source_norm = 0;
+#pragma omp parallel for reduction(+ : source_norm)
+  for (size_t i = lo; i < hi; i++)
+    for (int k = 0; k < 6; k++) {
+      size_t q = 6 * i + k;
+      source_norm += src[q] * src[q];
+      resid[q] = src[q];
+      cg_p[q] = src[q];
+      for (int j = 0; j < num_offsets; j++) { psim[j][q] = 0.0; pm[j][q] = src[q]; }
+    }
+  double rsq = source_norm, oldrsq, pkp, c1, c2;
+  int ret = 0;
+  if (source_norm == 0.0) goto done;
+  iteration++;
+  double rsqstop = rsqmin * source_norm;
+  for (int j = 0; j < num_offsets; j++) {
+    zeta_im1[j] = zeta_i[j] = 1.0;
+    beta_im1[j] = -1.0;
+    alpha[j] = 0.0;
+  }
+  do {
+    oldrsq = rsq;
+    kso_dslash(g, fat, lng, cg_p, ttt, l_other);
+    kso_dslash(g, fat, lng, ttt, ttt, l_parity);
+    pkp = 0.0;
+#pragma omp parallel for reduction(+ : pkp)
+    for (size_t i = lo; i < hi; i++)
+      for (int k = 0; k < 6; k++) {
+        size_t q = 6 * i + k;
+        ttt[q] = ttt[q] + shift0 * cg_p[q];
+        pkp += cg_p[q] * ttt[q];
+      }
+    iteration++;
+    beta_i[j_low] = -rsq / pkp;
+    zeta_ip1[j_low] = 1.0;
+    for (int j = 0; j < num_offsets_now; j++) {
+      if (j == j_low) continue;
+      zeta_ip1[j] = zeta_i[j] * zeta_im1[j] * beta_im1[j_low];
+      c1 = beta_i[j_low] * alpha[j_low] * (zeta_im1[j] - zeta_i[j]);
+      c2 = zeta_im1[j] * beta_im1[j_low] * (1.0 + shifts[j] * beta_i[j_low]);
+      if (c1 + c2 != 0.0) zeta_ip1[j] /= c1 + c2;
+      else { zeta_ip1[j] = 0.0; finished[j] = 1; }
+      if (zeta_i[j] != 0.0) beta_i[j] = beta_i[j_low] * zeta_ip1[j] / zeta_i[j];
+      else {
+        zeta_ip1[j] = 0.0; beta_i[j] = 0.0; finished[j] = 1;
+        if (j == num_offsets_now - 1 && j > j_low) num_offsets_now--;
+      }
+    }
+    rsq = 0.0;
+#pragma omp parallel for reduction(+ : rsq)
+    for (size_t i = lo; i < hi; i++)
+      for (int k = 0; k < 6; k++) {
+        size_t q = 6 * i + k;
+        for (int j = 0; j < num_offsets_now; j++) psim[j][q] = psim[j][q] + beta_i[j] * pm[j][q];
+        resid[q] = resid[q] + beta_i[j_low] * ttt[q];
+        rsq += resid[q] * resid[q];
+      }
+    if (rsqstop > 0 && rsq <= rsqstop) {
+      for (int j = 0; j < num_offsets; j++) {
+        qic[j].final_rsq = rsq / source_norm;
+        qic[j].size_r = qic[j].final_rsq;
+        qic[j].size_relr = qic[j].final_relrsq;
+        qic[j].final_iters = iteration;
+      }
+      goto done;
+    }
+    alpha[j_low] = rsq / oldrsq;
+    for (int j = 0; j < num_offsets_now; j++) {
+      if (j == j_low) continue;
+      if (zeta_i[j] * beta_i[j_low] != 0.0)
+        alpha[j] = alpha[j_low] * zeta_ip1[j] * beta_i[j] / (zeta_i[j] * beta_i[j_low]);
+      else { alpha[j] = 0.0; finished[j] = 1; }
+    }
+#pragma omp parallel for
+    for (size_t i = lo; i < hi; i++)
+      for (int k = 0; k < 6; k++) {
+        size_t q = 6 * i + k;
+        for (int j = 0; j < num_offsets_now; j++) pm[j][q] = zeta_ip1[j] * resid[q] + alpha[j] * pm[j][q];
+        cg_p[q] = pm[j_low][q];
+      }
+    for (int j = 0; j < num_offsets_now; j++) {
+      beta_im1[j] = beta_i[j];
+      zeta_im1[j] = zeta_i[j];
+      zeta_i[j] = zeta_ip1[j];
+    }
+  } while (iteration < niter);
+  for (int j = 0; j < num_offsets; j++) {
+    qic[j].final_rsq = rsq / source_norm;
+    qic[j].size_r = qic[j].final_rsq;
+    qic[j].final_iters = iteration;
+    qic[j].converged = 0;
+  }
+done:
+  ret = iteration;
+  for (int j = 0; j < num_offsets; j++) free(pm[j]);
+  free(pm); free(shifts); free(zeta_i); free(zeta_im1); free(zeta_ip1);
+  free(beta_i); free(beta_im1); free(alpha); free(finished);
+  free(ttt); free(cg_p); free(resid);
+  return ret;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Force level 0 (generic_ks/fermion_force_hisq_multi.c:1478-1636):          */
+/*   oprod1[4x+mu] = sum_t r_t Z_t(x+mu)  Z_t(x)^dag                          */
+/*   oprod3[4x+mu] = sum_t r_t Z_t(x+3mu) Z_t(x)^dag                          */
+/* over all sites (EVENANDODD), Z_t = multi_x[t] on both parities.            */
+/* ------------------------------------------------------------------------ */
+void kso_force_oprod(const kso_geom *g, const double *const *multi_x, const double *residues,
+                     int nterms, double *oprod1, double *oprod3) {
+#pragma omp parallel for
+  for (size_t i = 0; i < g->V; i++) {
+    for (int mu = 0; mu < 4; mu++) {
+      for (int hop = 0; hop < 2; hop++) {
+        size_t y = NB(g, hop ? 8 + mu : mu, i);
+        double acc[18];
+        memset(acc, 0, sizeof acc);
+        for (int t = 0; t < nterms; t++) {
+          const double *a = multi_x[t] + 6 * y, *b = multi_x[t] + 6 * i;
+          double r = residues[t];
+          for (int p = 0; p < 3; p++)
+            for (int q = 0; q < 3; q++) {
+              /* a_p * conj(b_q) (libraries/su3_proj.c) */
+              double re = a[2 * p] * b[2 * q] + a[2 * p + 1] * b[2 * q + 1];
+              double im = a[2 * p + 1] * b[2 * q] - a[2 * p] * b[2 * q + 1];
+              acc[2 * (3 * p + q)] += r * re;
+              acc[2 * (3 * p + q) + 1] += r * im;
+            }
+        }
+        memcpy((hop ? oprod3 : oprod1) + (4 * i + mu) * 18, acc, sizeof acc);
+      }
+    }
+  }
+}

@@ -1,0 +1,225 @@
+"""ctypes front end of the FP64 oracle (libks_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: the checker for the HIP product path.  Only tests/,
+``__graft_entry__.smoke()`` and ``bench.py``'s cpu_baseline leg import this
+module.  All fields use MILC's layout and site order (see ks_oracle.h):
+vectors are float64 arrays of shape (V, 3, 2) [re/im], link fields (V, 4, 3, 3, 2),
+site index = MILC ``node_index`` (even sites first).
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+EVEN, ODD, EVENANDODD = 2, 1, 3
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libks_oracle.so")
+_lib = None
+
+
+class _Geom(ctypes.Structure):
+    _fields_ = [("dims", ctypes.c_int * 4), ("V", ctypes.c_size_t), ("Vh", ctypes.c_size_t),
+                ("coords", ctypes.POINTER(ctypes.c_int)), ("nbr", ctypes.POINTER(ctypes.c_size_t))]
+
+
+class Qic(ctypes.Structure):
+    """Mirror of quark_invert_control's solver fields (include/generic_quark_types.h:167-190)."""
+    _fields_ = [("max", ctypes.c_int), ("nrestart", ctypes.c_int), ("parity", ctypes.c_int),
+                ("resid", ctypes.c_double), ("relresid", ctypes.c_double),
+                ("final_rsq", ctypes.c_double), ("final_relrsq", ctypes.c_double),
+                ("size_r", ctypes.c_double), ("size_relr", ctypes.c_double),
+                ("final_iters", ctypes.c_int), ("final_restart", ctypes.c_int),
+                ("converged", ctypes.c_int)]
+
+
+def build():
+    """Compile libks_oracle.so with the committed Makefile."""
+    subprocess.run(["make", "-s", "-C", _HERE, "libks_oracle.so"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        P = ctypes.c_void_p
+        G = ctypes.POINTER(_Geom)
+        L.kso_geom_init.argtypes = [G, ctypes.POINTER(ctypes.c_int)]
+        L.kso_geom_free.argtypes = [G]
+        L.kso_node_index.argtypes = [G, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.kso_node_index.restype = ctypes.c_size_t
+        L.kso_gauge_random.argtypes = [G, ctypes.c_uint64, ctypes.c_double, P]
+        L.kso_gaussian_vector.argtypes = [G, ctypes.c_uint64, ctypes.c_int, P]
+        L.kso_rephase.argtypes = [G, P]
+        L.kso_plaquette.argtypes = [G, P]
+        L.kso_plaquette.restype = ctypes.c_double
+        L.kso_fat_links.argtypes = [G, P, P, P]
+        L.kso_long_links.argtypes = [G, ctypes.c_double, P, P]
+        L.kso_u3_unitarize.argtypes = [P, P]
+        L.kso_hisq_links.argtypes = [G, P, P, P, P]
+        L.kso_asqtad_links.argtypes = [G, ctypes.c_double, P, P, P]
+        L.kso_dslash.argtypes = [G, P, P, P, P, ctypes.c_int]
+        L.kso_congrad.argtypes = [G, P, P, P, P, ctypes.POINTER(Qic), ctypes.c_double]
+        L.kso_multicg_offset.argtypes = [G, P, P, P, P, P, ctypes.c_int, ctypes.POINTER(Qic)]
+        L.kso_force_oprod.argtypes = [G, P, P, ctypes.c_int, P, P]
+        L.kso_norm2.argtypes = [G, P, ctypes.c_int]
+        L.kso_norm2.restype = ctypes.c_double
+        L.kso_mdagm.argtypes = [G, P, P, ctypes.c_double, P, P, P, ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Lattice:
+    """Geometry of an nx*ny*nz*nt lattice in MILC site order."""
+
+    def __init__(self, dims):
+        self.dims = tuple(int(d) for d in dims)
+        self._g = _Geom()
+        d = (ctypes.c_int * 4)(*self.dims)
+        if lib().kso_geom_init(ctypes.byref(self._g), d) != 0:
+            raise ValueError("lattice dims must have even nx and even volume")
+        self.V = int(self._g.V)
+        self.Vh = int(self._g.Vh)
+        self.coords = np.ctypeslib.as_array(self._g.coords, shape=(self.V, 4)).copy()
+
+    def __del__(self):
+        try:
+            lib().kso_geom_free(ctypes.byref(self._g))
+        except Exception:
+            pass
+
+    @property
+    def g(self):
+        return ctypes.byref(self._g)
+
+    def node_index(self, x, y, z, t):
+        return int(lib().kso_node_index(self.g, x, y, z, t))
+
+    def lex_of_site(self):
+        """lexicographic index x + nx(y + ny(z + nz t)) of each MILC site."""
+        c = self.coords.astype(np.int64)
+        nx, ny, nz, _ = self.dims
+        return c[:, 0] + nx * (c[:, 1] + ny * (c[:, 2] + nz * c[:, 3]))
+
+    def parity_slice(self, parity):
+        if parity == EVEN:
+            return slice(0, self.Vh)
+        if parity == ODD:
+            return slice(self.Vh, self.V)
+        return slice(0, self.V)
+
+    # ---- fields ---------------------------------------------------------
+    def new_links(self):
+        return np.zeros((self.V, 4, 3, 3, 2))
+
+    def new_vector(self):
+        return np.zeros((self.V, 3, 2))
+
+    def gauge_random(self, seed, scale=0.4):
+        U = self.new_links()
+        lib().kso_gauge_random(self.g, seed, scale, _p(U))
+        return U
+
+    def gaussian_vector(self, seed, parity=EVEN):
+        v = self.new_vector()
+        lib().kso_gaussian_vector(self.g, seed, parity, _p(v))
+        return v
+
+    def rephase(self, U):
+        U = np.ascontiguousarray(U, dtype=np.float64).copy()
+        lib().kso_rephase(self.g, _p(U))
+        return U
+
+    def plaquette(self, U):
+        return lib().kso_plaquette(self.g, _p(np.ascontiguousarray(U)))
+
+    def fat_links(self, coeff, U):
+        fat = self.new_links()
+        c = np.ascontiguousarray(coeff, dtype=np.float64)
+        lib().kso_fat_links(self.g, _p(c), _p(np.ascontiguousarray(U)), _p(fat))
+        return fat
+
+    def long_links(self, naik, U):
+        lng = self.new_links()
+        lib().kso_long_links(self.g, naik, _p(np.ascontiguousarray(U)), _p(lng))
+        return lng
+
+    def hisq_links(self, U_phased, want_W=False):
+        fat, lng, W = self.new_links(), self.new_links(), self.new_links()
+        nsvd = lib().kso_hisq_links(self.g, _p(np.ascontiguousarray(U_phased)), _p(fat), _p(lng), _p(W))
+        return (fat, lng, nsvd, W) if want_W else (fat, lng, nsvd)
+
+    def asqtad_links(self, U_phased, u0):
+        fat, lng = self.new_links(), self.new_links()
+        lib().kso_asqtad_links(self.g, u0, _p(np.ascontiguousarray(U_phased)), _p(fat), _p(lng))
+        return fat, lng
+
+    def dslash(self, fat, lng, src, parity, dst=None):
+        if dst is None:
+            dst = self.new_vector()
+        lib().kso_dslash(self.g, _p(fat), _p(lng), _p(src), _p(dst), parity)
+        return dst
+
+    def mdagm(self, fat, lng, offset, src, parity):
+        dst, tmp = self.new_vector(), self.new_vector()
+        lib().kso_mdagm(self.g, _p(fat), _p(lng), offset, _p(src), _p(dst), _p(tmp), parity)
+        return dst
+
+    def norm2(self, v, parity=EVENANDODD):
+        return lib().kso_norm2(self.g, _p(v), parity)
+
+    def congrad(self, fat, lng, src, mass, resid, parity=EVEN, max_iters=1000, nrestart=5,
+                dest=None, relresid=0.0):
+        if dest is None:
+            dest = self.new_vector()
+        q = Qic(max=max_iters, nrestart=nrestart, parity=parity, resid=resid, relresid=relresid)
+        it = lib().kso_congrad(self.g, _p(fat), _p(lng), _p(src), _p(dest), ctypes.byref(q), mass)
+        return dest, it, q
+
+    def multicg_offset(self, fat, lng, src, offsets, resid, parity=EVEN, max_iters=1000, nrestart=5):
+        n = len(offsets)
+        psim = [self.new_vector() for _ in range(n)]
+        ptrs = (ctypes.c_void_p * n)(*[p.ctypes.data for p in psim])
+        offs = np.ascontiguousarray(offsets, dtype=np.float64)
+        qics = (Qic * n)()
+        for q in qics:
+            q.max, q.nrestart, q.parity, q.resid = max_iters, nrestart, parity, resid
+        it = lib().kso_multicg_offset(self.g, _p(fat), _p(lng), _p(src), ptrs, _p(offs), n, qics)
+        return psim, it, list(qics)
+
+    def force_oprod(self, multi_x, residues):
+        n = len(multi_x)
+        ptrs = (ctypes.c_void_p * n)(*[np.ascontiguousarray(x).ctypes.data for x in multi_x])
+        res = np.ascontiguousarray(residues, dtype=np.float64)
+        o1, o3 = self.new_links(), self.new_links()
+        lib().kso_force_oprod(self.g, ptrs, _p(res), n, _p(o1), _p(o3))
+        return o1, o3
+
+
+def u3_unitarize(V):
+    V = np.ascontiguousarray(V, dtype=np.float64).reshape(3, 3, 2)
+    W = np.zeros_like(V)
+    svd = lib().kso_u3_unitarize(_p(V), _p(W))
+    return W, svd
+
+
+def to_complex(a):
+    """(..., 2) float array -> complex array."""
+    return a[..., 0] + 1j * a[..., 1]
+
+
+def from_complex(c):
+    return np.stack([c.real, c.imag], axis=-1).astype(np.float64)
+
+
+# HISQ level-2 free-field coefficients: fat = 9/8, Naik = -1/24 (see kso_hisq_links)
+HISQ_FREE_C1 = 9.0 / 8.0
+HISQ_FREE_C3 = -1.0 / 24.0
