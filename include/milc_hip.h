@@ -1,0 +1,144 @@
+/* milc_hip.h -- C-ABI of the MI355X-native HISQ/asqtad staggered library.
+ *
+ * The entry points mirror the GPU-offload calls that MILC 7.8.1's generic_ks
+ * "_gpu" glue makes (the QUDA interface used at the call sites cited below),
+ * with plain pointers and sizes only.  MILC host fields are passed exactly as
+ * MILC holds them:
+ *   - colour vectors: su3_vector[sites_on_node], even sites first
+ *     (node_index order, generic/layout_hyper_prime.c:509-520), complex
+ *     re/im interleaved, 6 Reals per site;
+ *   - links: su3_matrix[4*sites_on_node], element (4*i + dir), 18 Reals, KS
+ *     phases and the antiperiodic time boundary already multiplied in
+ *     (generic_ks/rephase.c:122-143);
+ *   - "Real" is float when external_precision == 1, double when 2
+ *     (MILC_PRECISION).
+ * Operator conventions are MILC's: D has no factor 1/2; M = 2m + D;
+ * the single-parity solver inverts M^dag M = 4m^2 - D_eo D_oe; multi-shift
+ * offsets already include 4m^2 (ks_multicg_offset.c:184-194).
+ *
+ * Return value: >= 0 on success (iteration count where the mirrored call
+ * returns one), negative on error.  The library keeps one device context per
+ * process (the reference's GPU path is single-threaded per rank and not
+ * reentrant, d_congrad5_fn_gpu.c:120-126); calls must not overlap.
+ */
+#ifndef MILC_HIP_H
+#define MILC_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* parity codes of inv_args.evenodd */
+#define MILC_HIP_EVEN_PARITY 0
+#define MILC_HIP_ODD_PARITY 1
+
+/* Mirrors the QudaInvertArgs_t members MILC sets before qudaInvert /
+ * qudaMultishiftInvert / qudaDslash (d_congrad5_fn_gpu.c:95-131,
+ * ks_multicg_offset_gpu.c:161-199, dslash_fn.c:313-325).
+ * mixed_precision: 0 = solve entirely in internal precision; 1 = FP32 sloppy
+ * iterations with FP64 reliable updates; 2 = 16-bit (per-site normed) sloppy
+ * iterations with FP64 reliable updates (MILC's HALF_MIXED / MAX_MIXED,
+ * Makefile:463-466). */
+typedef struct {
+  int max_iter;
+  int evenodd;
+  int mixed_precision;
+  double naik_epsilon;
+  double tadpole;
+  /* Addition without a QUDA counterpart: MILC's per-restart iteration count
+   * (qic->max, d_congrad5_fn_milc.c:87); the FP64 solver recomputes the true
+   * residual every restart_iters iterations exactly as the CPU path does.
+   * 0 = max_iter. */
+  int restart_iters;
+} milc_hip_inv_args;
+
+/* ---- lifecycle ----------------------------------------------------------- */
+/* Replaces initialize_quda()/qudaInit + layout set-up
+ * (generic/milc_to_quda_utilities.c:11-43): one lattice nx*ny*nz*nt on one
+ * device.  nx and the volume must be even. */
+int milc_hip_init(const int dims[4], int device);
+/* Replaces qudaFinalize (ks_imp_rhmc/control.c:189). */
+int milc_hip_finalize(void);
+/* Library version string. */
+const char* milc_hip_version(void);
+
+/* ---- solvers --------------------------------------------------------------- */
+/* Replaces qudaInvert (generic_ks/d_congrad5_fn_gpu.c:136-148): CG on
+ * M^dag M for parity args.evenodd.  `solution` is the initial guess and is
+ * overwritten on that parity only.  On entry *num_iters == -1 signals that the
+ * links changed (d_congrad5_fn_gpu.c:120-126); otherwise the device copy of
+ * the links from the previous call is reused when the pointers match.
+ * internal_precision 2 with mixed_precision 0 reproduces MILC's FP64
+ * restarted CG (d_congrad5_fn_milc.c:60-407).  final_resid = |r|/|b| of the
+ * FP64 true residual.  target_relresid must be 0. */
+int milc_hip_invert(int external_precision, int internal_precision, double mass, milc_hip_inv_args args,
+                    double target_resid, double target_relresid, const void* fatlink, const void* longlink,
+                    const void* source, void* solution, double* final_resid, double* final_rel_resid,
+                    int* num_iters);
+
+/* Replaces qudaMultishiftInvert (generic_ks/ks_multicg_offset_gpu.c:203-217):
+ * Jegerlehner multi-mass CG for (M^dag M + offset_j - 4m^2) x_j = b, i.e.
+ * (-D_eo D_oe + offset_j) x_j = b, all offsets > 0.  Solutions are zeroed
+ * first.  The base system (smallest offset) stops at target_resid[0];
+ * internal_precision 1 runs the vector updates in FP32 with FP64
+ * recurrences, 2 runs everything in FP64. */
+int milc_hip_multishift_invert(int external_precision, int internal_precision, int num_offsets,
+                               const double* offsets, milc_hip_inv_args args, const double* target_resid,
+                               const double* target_relresid, const void* fatlink, const void* longlink,
+                               const void* source, void** solutions, double* final_resid,
+                               double* final_rel_resid, int* num_iters);
+
+/* Replaces qudaDslash (generic_ks/dslash_fn.c:336-341): dst = D src on
+ * parity args.evenodd (dslash_fn_dblstore.c:311-563 semantics). */
+int milc_hip_dslash(int external_precision, int internal_precision, milc_hip_inv_args args,
+                    const void* fatlink, const void* longlink, const void* src, void* dst, int* num_iters);
+
+/* ---- fermion force --------------------------------------------------------- */
+/* Replaces qudaComputeOprod (generic_ks/fermion_force_asqtad_gpu.c:60) and
+ * restates the level-0 outer products of the HISQ force
+ * (fermion_force_hisq_multi.c:1478-1636):
+ *   oprod[0][dir][i] = sum_t coeff[t][0] * Z_t(x_i + dir)  Z_t(x_i)^dag
+ *   oprod[1][dir][i] = sum_t coeff[t][1] * Z_t(x_i + 3dir) Z_t(x_i)^dag
+ * quark_field[t] = Z_t on all sites (X_t on even, D_oe X_t on odd,
+ * update_h_rhmc.c:82-84); oprod[k] points to an array of 4 su3_matrix[V]
+ * fields, overwritten. */
+int milc_hip_compute_oprod(int precision, int num_terms, double** coeff, void** quark_field, void* oprod[2]);
+
+/* ---- link construction ----------------------------------------------------- */
+/* Replaces qudaLoadUnitarizedLink + qudaLoadKSLink for the HISQ U(3) action
+ * (generic_ks/fermion_links_hisq_load_milc.c:546-702,
+ * fermion_links_fn_load_gpu.c:18-123): thin links U (phases in) ->
+ * fat7 V -> analytic U(3) projection W -> fat and Naik links.  n_svd, if not
+ * NULL, receives the number of links at which MILC would have used its SVD
+ * fallback (the analytic result is kept for them). */
+int milc_hip_load_hisq_links(int precision, const void* thin, void* fatlink, void* longlink, void* W_link,
+                             int* n_svd);
+
+/* ---- device-resident workloads (inputs generated in HBM) -------------------- */
+/* Thin links U = exp(i*scale*H) from `seed` (H traceless Hermitian, Gaussian
+ * entries; counter-based generator shared with the oracle), KS phases in,
+ * then the HISQ fat/Naik links, kept on the device as the current links. */
+int milc_hip_dev_gauge_hisq(uint64_t seed, double scale, int* n_svd);
+/* Gaussian even/odd source (<re^2> = 1/2, grsource_imp.c:169-195) into device
+ * slot `slot` (0..15). */
+int milc_hip_dev_source(uint64_t seed, int parity, int slot);
+/* Solve on device slots: x[slot_x] = (M^dag M)^-1 b[slot_b]; returns
+ * iterations; stats[0..5] = final |r|/|b|, seconds, restarts/reliable
+ * updates, converged, iterations, final_rsq. */
+int milc_hip_dev_invert(int slot_b, int slot_x, double mass, int parity, double resid, int mixed_precision,
+                        int max_iter, double* stats);
+/* Copy a device slot to / from a MILC-layout host field (double). */
+int milc_hip_dev_read(int slot, int parity, double* host);
+int milc_hip_dev_write(int slot, int parity, const double* host);
+/* Current device links to host MILC arrays (double). */
+int milc_hip_dev_read_links(double* fat, double* lng);
+/* Time `reps` Dslash applications on device slots (HIP events on the
+ * library stream); returns the average in *usec. */
+int milc_hip_dev_dslash_bench(int prec, int parity, int reps, double* usec);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,552 @@
+// mh_api.cpp -- extern "C" entry points (include/milc_hip.h).
+//
+// Each entry point restates the behaviour of the MILC GPU glue it replaces;
+// the citations are in the header.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../include/milc_hip.h"
+#include "mh_internal.h"
+
+using namespace mh;
+
+namespace {
+
+const int kSlots = 16;
+VecField g_slots[kSlots];
+
+int parity_from_evenodd(int evenodd) {
+  if (evenodd == MILC_HIP_EVEN_PARITY) return EVEN;
+  if (evenodd == MILC_HIP_ODD_PARITY) return ODD;
+  return -1;
+}
+
+// Upload a MILC host link pair (float or double) into the FP64 device links.
+int upload_links(int ext_prec, const void* fat, const void* lng) {
+  Context& c = ctx();
+  size_t n = (size_t)2 * c.Vh * 4 * 18;
+  std::vector<double> conv;
+  const double* hf = (const double*)fat;
+  const double* hl = (const double*)lng;
+  if (ext_prec == 1) {
+    conv.resize(2 * n);
+    for (size_t i = 0; i < n; i++) conv[i] = ((const float*)fat)[i];
+    for (size_t i = 0; i < n; i++) conv[n + i] = ((const float*)lng)[i];
+    hf = conv.data();
+    hl = conv.data() + n;
+  }
+  for (int p = 0; p < 3; p++) {
+    if (c.links[p].valid) {
+      free_links(c.links[p].fat);
+      free_links(c.links[p].lng);
+      c.links[p].valid = false;
+    }
+  }
+  DevLinks& L = c.links[P_F64];
+  int rc;
+  if ((rc = alloc_links(L.fat, P_F64, c.stride)) || (rc = alloc_links(L.lng, P_F64, c.stride))) return rc;
+  double* stage = nullptr;
+  if ((rc = dev_alloc((void**)&stage, n * sizeof(double)))) return rc;
+  MH_CHECK(hipMemcpyAsync(stage, hf, n * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  if ((rc = links_from_milc_dev(stage, L.fat, P_F64))) return rc;
+  MH_CHECK(hipMemcpyAsync(stage, hl, n * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  if ((rc = links_from_milc_dev(stage, L.lng, P_F64))) return rc;
+  MH_CHECK(hipStreamSynchronize(c.stream));
+  dev_free(stage);
+  L.prec = P_F64;
+  L.valid = true;
+  return 0;
+}
+
+// Sloppy copies of the FP64 links, built on demand.
+int ensure_links(int prec) {
+  Context& c = ctx();
+  if (c.links[prec].valid) return 0;
+  if (!c.links[P_F64].valid) return -3;
+  DevLinks& L = c.links[prec];
+  int rc;
+  if ((rc = alloc_links(L.fat, prec, c.stride)) || (rc = alloc_links(L.lng, prec, c.stride))) return rc;
+  if ((rc = convert_links(c.links[P_F64].fat, P_F64, L.fat, prec))) return rc;
+  if ((rc = convert_links(c.links[P_F64].lng, P_F64, L.lng, prec))) return rc;
+  L.prec = prec;
+  L.valid = true;
+  return 0;
+}
+
+// Links refresh rule of the reference glue: re-upload when the caller signals
+// new links (*num_iters == -1) or passes a different link array.
+int refresh_links(int ext_prec, const void* fat, const void* lng, int* num_iters) {
+  Context& c = ctx();
+  bool fresh = (num_iters && *num_iters == -1) || !c.links[P_F64].valid || c.link_key != fat ||
+               c.link_key2 != lng;
+  if (!fresh) return 0;
+  int rc = upload_links(ext_prec, fat, lng);
+  if (rc) return rc;
+  c.link_key = fat;
+  c.link_key2 = lng;
+  return 0;
+}
+
+// host MILC vector (float/double) -> device FP64 parity field
+int host_vec_in(int ext_prec, const void* h, VecField& v, int pb) {
+  Context& c = ctx();
+  if (ext_prec == 2) return upload_vec_milc((const double*)h, v, P_F64, pb);
+  std::vector<double> tmp((size_t)2 * c.Vh * 6, 0.0);
+  const float* f = (const float*)h;
+  size_t off = (size_t)pb * c.Vh * 6;
+  for (size_t i = 0; i < (size_t)c.Vh * 6; i++) tmp[off + i] = f[off + i];
+  return upload_vec_milc(tmp.data(), v, P_F64, pb);
+}
+int host_vec_out(int ext_prec, const VecField& v, int pb, void* h) {
+  Context& c = ctx();
+  if (ext_prec == 2) return download_vec_milc(v, P_F64, pb, (double*)h);
+  std::vector<double> tmp((size_t)2 * c.Vh * 6, 0.0);
+  int rc = download_vec_milc(v, P_F64, pb, tmp.data());
+  if (rc) return rc;
+  float* f = (float*)h;
+  size_t off = (size_t)pb * c.Vh * 6;
+  for (size_t i = 0; i < (size_t)c.Vh * 6; i++) f[off + i] = (float)tmp[off + i];
+  return 0;
+}
+
+int sloppy_of(int internal_precision, int mixed) {
+  if (mixed == 2) return P_H16;
+  if (mixed == 1) return P_F32;
+  return internal_precision == 1 ? P_F32 : P_F64;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* milc_hip_version(void) { return "milc_hip 0.1 (gfx950)"; }
+
+int milc_hip_init(const int dims[4], int device) {
+  Context& c = ctx();
+  if (c.initialised) milc_hip_finalize();
+  if (dims[0] % 2 != 0 || ((long)dims[0] * dims[1] * dims[2] * dims[3]) % 2 != 0) return -2;
+  for (int i = 0; i < 4; i++)
+    if (dims[i] < 4) return -2;  // the Naik hop must not wrap onto itself
+  MH_CHECK(hipSetDevice(device));
+  c.device = device;
+  for (int i = 0; i < 4; i++) c.dims[i] = dims[i];
+  c.geom = make_geom(dims);
+  c.Vh = c.geom.Vh;
+  c.stride = (c.Vh + 63) / 64 * 64;
+  MH_CHECK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+  c.ws.max_blocks = (c.Vh + 255) / 256 + 1024;
+  int rc = dev_alloc((void**)&c.ws.partials, (size_t)c.ws.max_blocks * 4 * 2 * sizeof(double));
+  if (rc) return rc;
+  if ((rc = dev_alloc((void**)&c.ws.scal, sizeof(CGScalars)))) return rc;
+  MH_CHECK(hipHostMalloc((void**)&c.ws.scal_host, sizeof(CGScalars), hipHostMallocDefault));
+  c.initialised = true;
+  return 0;
+}
+
+int milc_hip_finalize(void) {
+  Context& c = ctx();
+  if (!c.initialised) return 0;
+  (void)hipStreamSynchronize(c.stream);
+  for (int p = 0; p < 3; p++) {
+    if (c.links[p].valid) {
+      free_links(c.links[p].fat);
+      free_links(c.links[p].lng);
+      c.links[p].valid = false;
+    }
+  }
+  for (int s = 0; s < kSlots; s++) free_vec(g_slots[s]);
+  dev_free(c.ws.partials);
+  dev_free(c.ws.scal);
+  if (c.ws.scal_host) (void)hipHostFree(c.ws.scal_host);
+  dev_free(c.staging);
+  c.staging = nullptr;
+  c.staging_elems = 0;
+  c.ws = Workspace();
+  (void)hipStreamDestroy(c.stream);
+  c.stream = nullptr;
+  c.link_key = c.link_key2 = nullptr;
+  c.initialised = false;
+  return 0;
+}
+
+int milc_hip_invert(int external_precision, int internal_precision, double mass, milc_hip_inv_args args,
+                    double target_resid, double target_relresid, const void* fatlink, const void* longlink,
+                    const void* source, void* solution, double* final_resid, double* final_rel_resid,
+                    int* num_iters) {
+  Context& c = ctx();
+  if (!c.initialised) return -1;
+  int parity = parity_from_evenodd(args.evenodd);
+  if (parity < 0) return -2;
+  if (target_relresid != 0.0) return -4;
+  int rc = refresh_links(external_precision, fatlink, longlink, num_iters);
+  if (rc) return rc;
+  int ps = sloppy_of(internal_precision, args.mixed_precision);
+  if (ps != P_F64 && (rc = ensure_links(ps))) return rc;
+  int pb = pbit(parity);
+  VecField b, x;
+  if ((rc = alloc_vec(b, P_F64, c.stride)) || (rc = alloc_vec(x, P_F64, c.stride))) return rc;
+  if ((rc = host_vec_in(external_precision, source, b, pb)) || (rc = host_vec_in(external_precision, solution, x, pb)))
+    return rc;
+  SolveParams sp;
+  sp.parity = parity;
+  int per = args.restart_iters > 0 ? args.restart_iters : args.max_iter;
+  sp.max_iters = per > 0 ? per : 1;
+  sp.max_restarts = std::max(1, (args.max_iter + sp.max_iters - 1) / sp.max_iters);
+  sp.resid = target_resid;
+  sp.relresid = 0;
+  sp.sloppy_prec = ps;
+  sp.delta = 0.1;
+  sp.batch = 16;
+  SolveStats st;
+  rc = cg_solve(b, x, mass, sp, st);
+  if (!rc) rc = host_vec_out(external_precision, x, pb, solution);
+  free_vec(b);
+  free_vec(x);
+  if (rc) return rc;
+  if (final_resid) *final_resid = std::sqrt(st.final_rsq);
+  if (final_rel_resid) *final_rel_resid = 0.0;
+  if (num_iters) *num_iters = st.iters;
+  return st.iters;
+}
+
+int milc_hip_dslash(int external_precision, int internal_precision, milc_hip_inv_args args, const void* fatlink,
+                    const void* longlink, const void* src, void* dst, int* num_iters) {
+  Context& c = ctx();
+  if (!c.initialised) return -1;
+  int parity = parity_from_evenodd(args.evenodd);
+  if (parity < 0) return -2;
+  int rc = refresh_links(external_precision, fatlink, longlink, num_iters);
+  if (rc) return rc;
+  int prec = internal_precision == 1 ? P_F32 : P_F64;
+  if ((rc = ensure_links(prec))) return rc;
+  int pb = pbit(parity);
+  VecField s, d, s64, d64;
+  if ((rc = alloc_vec(s64, P_F64, c.stride)) || (rc = alloc_vec(d64, P_F64, c.stride))) return rc;
+  if ((rc = host_vec_in(external_precision, src, s64, pb ^ 1))) return rc;
+  if (prec == P_F64) {
+    rc = dslash(c.links[P_F64], s64, d64, pb, P_F64);
+  } else {
+    if ((rc = alloc_vec(s, prec, c.stride)) || (rc = alloc_vec(d, prec, c.stride))) return rc;
+    if (!(rc = convert_vec(s64, P_F64, s, prec)) && !(rc = dslash(c.links[prec], s, d, pb, prec)))
+      rc = convert_vec(d, prec, d64, P_F64);
+    free_vec(s);
+    free_vec(d);
+  }
+  if (!rc) rc = host_vec_out(external_precision, d64, pb, dst);
+  free_vec(s64);
+  free_vec(d64);
+  return rc;
+}
+
+int milc_hip_dev_source(uint64_t seed, int parity, int slot) {
+  Context& c = ctx();
+  if (!c.initialised || slot < 0 || slot >= kSlots) return -1;
+  int rc;
+  if (!g_slots[slot].data && (rc = alloc_vec(g_slots[slot], P_F64, c.stride))) return rc;
+  return gaussian_vec_dev(seed, pbit(parity), g_slots[slot]);
+}
+
+int milc_hip_dev_read(int slot, int parity, double* host) {
+  Context& c = ctx();
+  if (!c.initialised || slot < 0 || slot >= kSlots || !g_slots[slot].data) return -1;
+  return download_vec_milc(g_slots[slot], P_F64, pbit(parity), host);
+}
+
+int milc_hip_dev_write(int slot, int parity, const double* host) {
+  Context& c = ctx();
+  if (!c.initialised || slot < 0 || slot >= kSlots) return -1;
+  int rc;
+  if (!g_slots[slot].data && (rc = alloc_vec(g_slots[slot], P_F64, c.stride))) return rc;
+  rc = upload_vec_milc(host, g_slots[slot], P_F64, pbit(parity));
+  if (!rc) MH_CHECK(hipStreamSynchronize(c.stream));
+  return rc;
+}
+
+int milc_hip_dev_invert(int slot_b, int slot_x, double mass, int parity, double resid, int mixed_precision,
+                        int max_iter, double* stats) {
+  Context& c = ctx();
+  if (!c.initialised || slot_b < 0 || slot_b >= kSlots || slot_x < 0 || slot_x >= kSlots || !g_slots[slot_b].data)
+    return -1;
+  int rc;
+  if (!g_slots[slot_x].data && (rc = alloc_vec(g_slots[slot_x], P_F64, c.stride))) return rc;
+  int ps = sloppy_of(2, mixed_precision);
+  if (ps != P_F64 && (rc = ensure_links(ps))) return rc;
+  SolveParams sp;
+  sp.parity = parity;
+  sp.max_iters = max_iter;
+  sp.max_restarts = 1;
+  sp.resid = resid;
+  sp.relresid = 0;
+  sp.sloppy_prec = ps;
+  sp.delta = 0.1;
+  sp.batch = 16;
+  SolveStats st;
+  if ((rc = vec_zero(g_slots[slot_x], P_F64))) return rc;
+  rc = cg_solve(g_slots[slot_b], g_slots[slot_x], mass, sp, st);
+  if (rc) return rc;
+  if (stats) {
+    stats[0] = std::sqrt(st.final_rsq);
+    stats[1] = st.seconds;
+    stats[2] = st.restarts;
+    stats[3] = st.converged;
+    stats[4] = st.iters;
+    stats[5] = st.final_rsq;
+  }
+  return st.iters;
+}
+
+int milc_hip_dev_read_links(double* fat, double* lng) {
+  Context& c = ctx();
+  if (!c.initialised || !c.links[P_F64].valid) return -1;
+  size_t n = (size_t)2 * c.Vh * 4 * 18;
+  double* stage = nullptr;
+  int rc;
+  if ((rc = dev_alloc((void**)&stage, n * sizeof(double)))) return rc;
+  if (!(rc = links_to_milc_dev(c.links[P_F64].fat, P_F64, stage)))
+    MH_CHECK(hipMemcpyAsync(fat, stage, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  if (!rc && !(rc = links_to_milc_dev(c.links[P_F64].lng, P_F64, stage)))
+    MH_CHECK(hipMemcpyAsync(lng, stage, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  MH_CHECK(hipStreamSynchronize(c.stream));
+  dev_free(stage);
+  return rc;
+}
+
+int milc_hip_dev_dslash_bench(int prec, int parity, int reps, double* usec) {
+  Context& c = ctx();
+  if (!c.initialised || reps <= 0) return -1;
+  int rc;
+  if (prec < P_F64 || prec > P_H16) return -2;
+  if ((rc = ensure_links(prec))) return rc;
+  int pb = pbit(parity);
+  VecField s, d;
+  if ((rc = alloc_vec(s, prec, c.stride)) || (rc = alloc_vec(d, prec, c.stride))) return rc;
+  VecField s64;
+  if ((rc = alloc_vec(s64, P_F64, c.stride))) return rc;
+  if ((rc = gaussian_vec_dev(12345, pb ^ 1, s64)) || (rc = convert_vec(s64, P_F64, s, prec))) return rc;
+  for (int i = 0; i < 3; i++)
+    if ((rc = dslash(c.links[prec], s, d, pb, prec))) return rc;
+  hipEvent_t e0, e1;
+  MH_CHECK(hipEventCreate(&e0));
+  MH_CHECK(hipEventCreate(&e1));
+  MH_CHECK(hipEventRecord(e0, c.stream));
+  for (int i = 0; i < reps; i++)
+    if ((rc = dslash(c.links[prec], s, d, pb, prec))) return rc;
+  MH_CHECK(hipEventRecord(e1, c.stream));
+  MH_CHECK(hipEventSynchronize(e1));
+  float ms = 0;
+  MH_CHECK(hipEventElapsedTime(&ms, e0, e1));
+  *usec = 1000.0 * ms / reps;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  free_vec(s);
+  free_vec(d);
+  free_vec(s64);
+  return 0;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// link construction
+// ---------------------------------------------------------------------------
+extern "C" int milc_hip_dev_gauge_hisq(uint64_t seed, double scale, int* n_svd) {
+  Context& c = ctx();
+  if (!c.initialised) return -1;
+  for (int p = 0; p < 3; p++) {
+    if (c.links[p].valid) {
+      free_links(c.links[p].fat);
+      free_links(c.links[p].lng);
+      c.links[p].valid = false;
+    }
+  }
+  c.link_key = c.link_key2 = nullptr;
+  LinkField U;
+  int rc;
+  if ((rc = alloc_links(U, P_F64, c.stride))) return rc;
+  DevLinks& L = c.links[P_F64];
+  if ((rc = alloc_links(L.fat, P_F64, c.stride)) || (rc = alloc_links(L.lng, P_F64, c.stride))) return rc;
+  if ((rc = gauge_random_dev(seed, scale, U)) || (rc = rephase_dev(U))) return rc;
+  if ((rc = hisq_links_dev(U, L.fat, L.lng, nullptr, n_svd))) return rc;
+  free_links(U);
+  L.prec = P_F64;
+  L.valid = true;
+  return 0;
+}
+
+extern "C" int milc_hip_load_hisq_links(int precision, const void* thin, void* fatlink, void* longlink, void* W_link,
+                                        int* n_svd) {
+  Context& c = ctx();
+  if (!c.initialised) return -1;
+  if (precision != 1 && precision != 2) return -2;
+  size_t n = (size_t)2 * c.Vh * 4 * 18;
+  std::vector<double> conv;
+  const double* h = (const double*)thin;
+  if (precision == 1) {
+    conv.resize(n);
+    for (size_t i = 0; i < n; i++) conv[i] = ((const float*)thin)[i];
+    h = conv.data();
+  }
+  LinkField U, fat, lng, W;
+  int rc;
+  if ((rc = alloc_links(U, P_F64, c.stride)) || (rc = alloc_links(fat, P_F64, c.stride)) ||
+      (rc = alloc_links(lng, P_F64, c.stride)) || (rc = alloc_links(W, P_F64, c.stride)))
+    return rc;
+  double* stage = nullptr;
+  if ((rc = dev_alloc((void**)&stage, n * sizeof(double)))) return rc;
+  MH_CHECK(hipMemcpyAsync(stage, h, n * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  if ((rc = links_from_milc_dev(stage, U, P_F64))) return rc;
+  if ((rc = hisq_links_dev(U, fat, lng, &W, n_svd))) return rc;
+  std::vector<double> out(n);
+  void* dsts[3] = {fatlink, longlink, W_link};
+  LinkField* srcs[3] = {&fat, &lng, &W};
+  for (int k = 0; k < 3; k++) {
+    if (!dsts[k]) continue;
+    if ((rc = links_to_milc_dev(*srcs[k], P_F64, stage))) return rc;
+    MH_CHECK(hipMemcpyAsync(out.data(), stage, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    MH_CHECK(hipStreamSynchronize(c.stream));
+    if (precision == 2) memcpy(dsts[k], out.data(), n * sizeof(double));
+    else
+      for (size_t i = 0; i < n; i++) ((float*)dsts[k])[i] = (float)out[i];
+  }
+  dev_free(stage);
+  free_links(U);
+  free_links(fat);
+  free_links(lng);
+  free_links(W);
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// multi-shift solver and force outer products
+// ---------------------------------------------------------------------------
+namespace mh {
+int force_oprod_coeff(const std::vector<VecField>& xe, const std::vector<VecField>& xo, const std::vector<double>& c1,
+                      const std::vector<double>& c3, LinkField& o1, LinkField& o3);
+}
+
+extern "C" int milc_hip_multishift_invert(int external_precision, int internal_precision, int num_offsets,
+                                          const double* offsets, milc_hip_inv_args args, const double* target_resid,
+                                          const double* target_relresid, const void* fatlink, const void* longlink,
+                                          const void* source, void** solutions, double* final_resid,
+                                          double* final_rel_resid, int* num_iters) {
+  Context& c = ctx();
+  if (!c.initialised) return -1;
+  int parity = parity_from_evenodd(args.evenodd);
+  if (parity < 0 || num_offsets <= 0 || num_offsets > 32) return -2;
+  for (int j = 0; j < num_offsets; j++)
+    if (!(offsets[j] > 0)) return -2;  // ks_multicg_offset.c:139-145
+  (void)target_relresid;
+  int rc = refresh_links(external_precision, fatlink, longlink, num_iters);
+  if (rc) return rc;
+  int pb = pbit(parity);
+  bool mixed = args.mixed_precision != 0;
+  int bulk = (internal_precision == 1 || mixed) ? P_F32 : P_F64;
+  if ((rc = ensure_links(bulk))) return rc;
+  VecField b;
+  std::vector<VecField> xs(num_offsets);
+  if ((rc = alloc_vec(b, P_F64, c.stride))) return rc;
+  for (auto& v : xs)
+    if ((rc = alloc_vec(v, P_F64, c.stride))) return rc;
+  if ((rc = host_vec_in(external_precision, source, b, pb))) return rc;
+  std::vector<double> offs(offsets, offsets + num_offsets);
+  MultiParams mp;
+  mp.parity = parity;
+  mp.max_iters = args.max_iter > 0 ? args.max_iter : 1000;
+  // the bulk FP32 solve cannot reach FP64 targets; it stops at its own floor
+  // and the per-shift mixed-precision CG below finishes each system
+  mp.resid = mixed ? std::max(target_resid[0], 1e-6) : target_resid[0];
+  mp.prec = bulk;
+  mp.batch = 16;
+  SolveStats st;
+  if ((rc = multicg_solve(b, xs, offs, mp, st))) return rc;
+  int total_iters = st.iters;
+  std::vector<double> fres(num_offsets, 0.0);
+  if (mixed) {
+    // per-shift refinement: (offset_j - D D) x_j = b with mass = sqrt(offset_j)/2
+    // (the mass <-> offset relation of ks_multicg.c:142,222)
+    if ((rc = ensure_links(P_F32))) return rc;
+    for (int j = 0; j < num_offsets; j++) {
+      SolveParams sp;
+      sp.parity = parity;
+      sp.max_iters = mp.max_iters;
+      sp.max_restarts = 1;
+      sp.resid = target_resid[j] > 0 ? target_resid[j] : target_resid[0];
+      sp.relresid = 0;
+      sp.sloppy_prec = P_F32;
+      sp.delta = 0.1;
+      sp.batch = 16;
+      SolveStats s2;
+      if ((rc = cg_solve(b, xs[j], 0.5 * std::sqrt(offs[j]), sp, s2))) return rc;
+      total_iters += s2.iters;
+      fres[j] = std::sqrt(s2.final_rsq);
+    }
+  } else {
+    for (int j = 0; j < num_offsets; j++) fres[j] = std::sqrt(st.final_rsq);
+  }
+  for (int j = 0; j < num_offsets; j++)
+    if ((rc = host_vec_out(external_precision, xs[j], pb, solutions[j]))) return rc;
+  for (int j = 0; j < num_offsets; j++) {
+    if (final_resid) final_resid[j] = fres[j];
+    if (final_rel_resid) final_rel_resid[j] = 0.0;
+  }
+  free_vec(b);
+  for (auto& v : xs) free_vec(v);
+  if (num_iters) *num_iters = total_iters;
+  return total_iters;
+}
+
+extern "C" int milc_hip_compute_oprod(int precision, int num_terms, double** coeff, void** quark_field,
+                                      void* oprod[2]) {
+  Context& c = ctx();
+  if (!c.initialised) return -1;
+  if (num_terms <= 0 || num_terms > 32 || (precision != 1 && precision != 2)) return -2;
+  int rc;
+  std::vector<VecField> xe(num_terms), xo(num_terms);
+  std::vector<double> c1(num_terms), c3(num_terms);
+  for (int t = 0; t < num_terms; t++) {
+    c1[t] = coeff[t][0];
+    c3[t] = coeff[t][1];
+    if ((rc = alloc_vec(xe[t], P_F64, c.stride)) || (rc = alloc_vec(xo[t], P_F64, c.stride))) return rc;
+    if ((rc = host_vec_in(precision, quark_field[t], xe[t], 0)) || (rc = host_vec_in(precision, quark_field[t], xo[t], 1)))
+      return rc;
+  }
+  LinkField o1, o3;
+  if ((rc = alloc_links(o1, P_F64, c.stride)) || (rc = alloc_links(o3, P_F64, c.stride))) return rc;
+  if ((rc = force_oprod_coeff(xe, xo, c1, c3, o1, o3))) return rc;
+  // device layout -> MILC AoS (4*i+dir) -> the caller's per-direction fields
+  size_t n = (size_t)2 * c.Vh * 4 * 18;
+  double* stage = nullptr;
+  if ((rc = dev_alloc((void**)&stage, n * sizeof(double)))) return rc;
+  std::vector<double> h(n);
+  LinkField* src[2] = {&o1, &o3};
+  size_t V = (size_t)2 * c.Vh;
+  for (int k = 0; k < 2; k++) {
+    if ((rc = links_to_milc_dev(*src[k], P_F64, stage))) return rc;
+    MH_CHECK(hipMemcpyAsync(h.data(), stage, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    MH_CHECK(hipStreamSynchronize(c.stream));
+    void** dirs = (void**)oprod[k];
+    for (int mu = 0; mu < 4; mu++) {
+      if (precision == 2) {
+        double* d = (double*)dirs[mu];
+        for (size_t i = 0; i < V; i++) memcpy(d + 18 * i, h.data() + 18 * (4 * i + mu), 18 * sizeof(double));
+      } else {
+        float* d = (float*)dirs[mu];
+        for (size_t i = 0; i < V; i++)
+          for (int e = 0; e < 18; e++) d[18 * i + e] = (float)h[18 * (4 * i + mu) + e];
+      }
+    }
+  }
+  dev_free(stage);
+  free_links(o1);
+  free_links(o3);
+  for (int t = 0; t < num_terms; t++) {
+    free_vec(xe[t]);
+    free_vec(xo[t]);
+  }
+  return 0;
+}

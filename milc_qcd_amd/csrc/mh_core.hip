@@ -1,0 +1,883 @@
+// mh_core.hip -- context, field memory, MILC<->device layout, Dslash and CG kernels.
+//
+// Reference behaviour restated here (MILC 7.8.1):
+//   Dslash  generic_ks/dslash_fn_dblstore.c:311-563 (dslash_fn_field_special)
+//   CG      generic_ks/d_congrad5_fn_milc.c:60-407 (ks_congrad_parity_cpu, FEWSUMS)
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "mh_internal.h"
+
+namespace mh {
+
+static Context g_ctx;
+Context& ctx() { return g_ctx; }
+
+size_t prec_complex_bytes(int prec) { return prec == P_F64 ? 16 : (prec == P_F32 ? 8 : 4); }
+
+int dev_alloc(void** p, size_t bytes) {
+  *p = nullptr;
+  if (bytes == 0) return 0;
+  MH_CHECK(hipMalloc(p, bytes));
+  return 0;
+}
+void dev_free(void* p) {
+  if (p) (void)hipFree(p);
+}
+
+int alloc_vec(VecField& v, int prec, int stride) {
+  v.stride = stride;
+  v.norm = nullptr;
+  int rc = dev_alloc(&v.data, 3 * (size_t)stride * prec_complex_bytes(prec));
+  if (rc) return rc;
+  if (prec == P_H16) {
+    rc = dev_alloc((void**)&v.norm, (size_t)stride * sizeof(float));
+    if (rc) return rc;
+  }
+  MH_CHECK(hipMemsetAsync(v.data, 0, 3 * (size_t)stride * prec_complex_bytes(prec), ctx().stream));
+  if (v.norm) MH_CHECK(hipMemsetAsync(v.norm, 0, (size_t)stride * sizeof(float), ctx().stream));
+  return 0;
+}
+void free_vec(VecField& v) {
+  dev_free(v.data);
+  dev_free(v.norm);
+  v.data = nullptr;
+  v.norm = nullptr;
+}
+int alloc_links(LinkField& l, int prec, int stride) {
+  l.stride = stride;
+  l.scale = 1.0f;
+  return dev_alloc(&l.data, 2ull * 4 * 9 * (size_t)stride * prec_complex_bytes(prec));
+}
+void free_links(LinkField& l) {
+  dev_free(l.data);
+  l.data = nullptr;
+}
+
+static inline int nblocks_for(int n, int block) { return (n + block - 1) / block; }
+
+// ===========================================================================
+// Layout kernels: MILC AoS (node_index order) <-> per-parity SoA planes
+// ===========================================================================
+template <int P>
+__global__ void k_vec_from_milc(const double* __restrict__ aos, VecField out, int Vh, int pb) {
+  int cb = blockIdx.x * blockDim.x + threadIdx.x;
+  if (cb >= Vh) return;
+  const double* s = aos + 6 * ((size_t)pb * Vh + cb);
+  Vec3<typename PrecTraits<P>::Real> v;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    v.c[k].re = (typename PrecTraits<P>::Real)s[2 * k];
+    v.c[k].im = (typename PrecTraits<P>::Real)s[2 * k + 1];
+  }
+  store_vec<P>(out, cb, v);
+}
+template <int P>
+__global__ void k_vec_to_milc(VecField in, double* __restrict__ aos, int Vh, int pb) {
+  int cb = blockIdx.x * blockDim.x + threadIdx.x;
+  if (cb >= Vh) return;
+  Vec3<typename PrecTraits<P>::Real> v;
+  load_vec<P>(in, cb, v);
+  double* d = aos + 6 * ((size_t)pb * Vh + cb);
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    d[2 * k] = (double)v.c[k].re;
+    d[2 * k + 1] = (double)v.c[k].im;
+  }
+}
+
+// links: aos[(4*i + mu)*18], i = MILC site index = pb*Vh + cb
+template <int P>
+__global__ void k_links_from_milc(const double* __restrict__ aos, LinkField out, int Vh, float inv_scale) {
+  int cb = blockIdx.x * blockDim.x + threadIdx.x;
+  int pm = blockIdx.y;  // pb*4 + mu
+  if (cb >= Vh) return;
+  int pb = pm >> 2, mu = pm & 3;
+  const double* s = aos + 18 * (4 * ((size_t)pb * Vh + cb) + mu);
+  size_t base = (size_t)pm * 9 * out.stride + cb;
+  for (int e = 0; e < 9; e++) {
+    double re = s[2 * e], im = s[2 * e + 1];
+    if (P == P_F64) {
+      ((double2*)out.data)[base + (size_t)e * out.stride] = make_double2(re, im);
+    } else if (P == P_F32) {
+      ((float2*)out.data)[base + (size_t)e * out.stride] = make_float2((float)re, (float)im);
+    } else {
+      short2 q;
+      q.x = f_to_h16((float)(re * inv_scale));
+      q.y = f_to_h16((float)(im * inv_scale));
+      ((short2*)out.data)[base + (size_t)e * out.stride] = q;
+    }
+  }
+}
+template <int P>
+__global__ void k_links_to_milc(LinkField in, double* __restrict__ aos, int Vh) {
+  int cb = blockIdx.x * blockDim.x + threadIdx.x;
+  int pm = blockIdx.y;
+  if (cb >= Vh) return;
+  int pb = pm >> 2, mu = pm & 3;
+  Mat3<typename PrecTraits<P>::Real> m;
+  load_link<P>(in, pb, mu, cb, m);
+  double* d = aos + 18 * (4 * ((size_t)pb * Vh + cb) + mu);
+  for (int e = 0; e < 9; e++) {
+    d[2 * e] = (double)m.e[e / 3][e % 3].re;
+    d[2 * e + 1] = (double)m.e[e / 3][e % 3].im;
+  }
+}
+
+// max |element| of an FP64 device link field, for the 16-bit scale
+__global__ void k_link_absmax(const double2* __restrict__ p, size_t n, double* out_partial) {
+  double m = 0;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    double2 a = p[i];
+    m = fmax(m, fmax(fabs(a.x), fabs(a.y)));
+  }
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+  __shared__ double sm[16];
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = fmax(m, sm[w]);
+    out_partial[blockIdx.x] = m;
+  }
+}
+
+template <int PI, int PO>
+__global__ void k_convert_links(LinkField in, LinkField out, int Vh, float inv_scale) {
+  int cb = blockIdx.x * blockDim.x + threadIdx.x;
+  int pm = blockIdx.y;
+  if (cb >= Vh) return;
+  int pb = pm >> 2, mu = pm & 3;
+  Mat3<typename PrecTraits<PI>::Real> m;
+  load_link<PI>(in, pb, mu, cb, m);
+  size_t base = (size_t)pm * 9 * out.stride + cb;
+  for (int e = 0; e < 9; e++) {
+    double re = (double)m.e[e / 3][e % 3].re, im = (double)m.e[e / 3][e % 3].im;
+    if (PO == P_F64) ((double2*)out.data)[base + (size_t)e * out.stride] = make_double2(re, im);
+    else if (PO == P_F32) ((float2*)out.data)[base + (size_t)e * out.stride] = make_float2((float)re, (float)im);
+    else {
+      short2 q;
+      q.x = f_to_h16((float)(re * inv_scale));
+      q.y = f_to_h16((float)(im * inv_scale));
+      ((short2*)out.data)[base + (size_t)e * out.stride] = q;
+    }
+  }
+}
+
+template <int PI, int PO>
+__global__ void k_convert_vec(VecField in, VecField out, int Vh) {
+  int cb = blockIdx.x * blockDim.x + threadIdx.x;
+  if (cb >= Vh) return;
+  Vec3<typename PrecTraits<PI>::Real> a;
+  load_vec<PI>(in, cb, a);
+  Vec3<typename PrecTraits<PO>::Real> b;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    b.c[k].re = (typename PrecTraits<PO>::Real)a.c[k].re;
+    b.c[k].im = (typename PrecTraits<PO>::Real)a.c[k].im;
+  }
+  store_vec<PO>(out, cb, b);
+}
+
+#define DISPATCH_PREC(prec, FN, ...)                 \
+  do {                                               \
+    if ((prec) == P_F64) FN<P_F64>(__VA_ARGS__);     \
+    else if ((prec) == P_F32) FN<P_F32>(__VA_ARGS__); \
+    else FN<P_H16>(__VA_ARGS__);                     \
+  } while (0)
+
+template <int P>
+static void launch_vec_from_milc(const double* aos, VecField& out, int pb) {
+  Context& c = ctx();
+  hipLaunchKernelGGL(k_vec_from_milc<P>, dim3(nblocks_for(c.Vh, 256)), dim3(256), 0, c.stream, aos, out, c.Vh, pb);
+}
+template <int P>
+static void launch_vec_to_milc(const VecField& in, double* aos, int pb) {
+  Context& c = ctx();
+  hipLaunchKernelGGL(k_vec_to_milc<P>, dim3(nblocks_for(c.Vh, 256)), dim3(256), 0, c.stream, in, aos, c.Vh, pb);
+}
+
+int vec_from_milc_dev(const double* dev_aos, VecField& out, int prec, int pb) {
+  DISPATCH_PREC(prec, launch_vec_from_milc, dev_aos, out, pb);
+  MH_CHECK(hipGetLastError());
+  return 0;
+}
+int vec_to_milc_dev(const VecField& in, int prec, int pb, double* dev_aos) {
+  DISPATCH_PREC(prec, launch_vec_to_milc, in, dev_aos, pb);
+  MH_CHECK(hipGetLastError());
+  return 0;
+}
+
+static int ensure_staging(size_t elems) {
+  Context& c = ctx();
+  if (c.staging_elems >= elems) return 0;
+  dev_free(c.staging);
+  c.staging = nullptr;
+  c.staging_elems = 0;
+  int rc = dev_alloc((void**)&c.staging, elems * sizeof(double));
+  if (rc) return rc;
+  c.staging_elems = elems;
+  return 0;
+}
+
+int upload_vec_milc(const double* host_aos, VecField& out, int prec, int pb) {
+  Context& c = ctx();
+  int rc = ensure_staging((size_t)2 * c.Vh * 6);
+  if (rc) return rc;
+  // only the parity half is needed
+  MH_CHECK(hipMemcpyAsync(c.staging + (size_t)pb * c.Vh * 6, host_aos + (size_t)pb * c.Vh * 6,
+                          (size_t)c.Vh * 6 * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  return vec_from_milc_dev(c.staging, out, prec, pb);
+}
+int download_vec_milc(const VecField& in, int prec, int pb, double* host_aos) {
+  Context& c = ctx();
+  int rc = ensure_staging((size_t)2 * c.Vh * 6);
+  if (rc) return rc;
+  rc = vec_to_milc_dev(in, prec, pb, c.staging);
+  if (rc) return rc;
+  MH_CHECK(hipMemcpyAsync(host_aos + (size_t)pb * c.Vh * 6, c.staging + (size_t)pb * c.Vh * 6,
+                          (size_t)c.Vh * 6 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  MH_CHECK(hipStreamSynchronize(c.stream));
+  return 0;
+}
+
+static int link_absmax_f64(const LinkField& f, double* out) {
+  Context& c = ctx();
+  const int nb = 512;
+  size_t n = 2ull * 4 * 9 * f.stride;
+  hipLaunchKernelGGL(k_link_absmax, dim3(nb), dim3(256), 0, c.stream, (const double2*)f.data, n, c.ws.partials);
+  std::vector<double> h(nb);
+  MH_CHECK(hipMemcpyAsync(h.data(), c.ws.partials, nb * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  MH_CHECK(hipStreamSynchronize(c.stream));
+  double m = 0;
+  for (double v : h) m = std::max(m, v);
+  *out = m;
+  return 0;
+}
+
+int links_from_milc_dev(const double* dev_aos, LinkField& out, int prec) {
+  Context& c = ctx();
+  dim3 grid(nblocks_for(c.Vh, 256), 8);
+  if (prec == P_F64) {
+    hipLaunchKernelGGL(k_links_from_milc<P_F64>, grid, dim3(256), 0, c.stream, dev_aos, out, c.Vh, 1.0f);
+  } else if (prec == P_F32) {
+    hipLaunchKernelGGL(k_links_from_milc<P_F32>, grid, dim3(256), 0, c.stream, dev_aos, out, c.Vh, 1.0f);
+  } else {
+    // scale from an FP64 copy
+    LinkField tmp;
+    int rc = alloc_links(tmp, P_F64, out.stride);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_links_from_milc<P_F64>, grid, dim3(256), 0, c.stream, dev_aos, tmp, c.Vh, 1.0f);
+    double m;
+    rc = link_absmax_f64(tmp, &m);
+    free_links(tmp);
+    if (rc) return rc;
+    out.scale = (float)(m > 0 ? m : 1.0);
+    hipLaunchKernelGGL(k_links_from_milc<P_H16>, grid, dim3(256), 0, c.stream, dev_aos, out, c.Vh,
+                       (float)(1.0 / out.scale));
+  }
+  MH_CHECK(hipGetLastError());
+  return 0;
+}
+
+int links_to_milc_dev(const LinkField& in, int prec, double* dev_aos) {
+  Context& c = ctx();
+  dim3 grid(nblocks_for(c.Vh, 256), 8);
+  if (prec == P_F64) hipLaunchKernelGGL(k_links_to_milc<P_F64>, grid, dim3(256), 0, c.stream, in, dev_aos, c.Vh);
+  else if (prec == P_F32) hipLaunchKernelGGL(k_links_to_milc<P_F32>, grid, dim3(256), 0, c.stream, in, dev_aos, c.Vh);
+  else hipLaunchKernelGGL(k_links_to_milc<P_H16>, grid, dim3(256), 0, c.stream, in, dev_aos, c.Vh);
+  MH_CHECK(hipGetLastError());
+  return 0;
+}
+
+// FP64 -> lower precision link conversion (the input must be FP64)
+int convert_links(const LinkField& in, int pin, LinkField& out, int pout) {
+  Context& c = ctx();
+  if (pin != P_F64) return -2;
+  dim3 grid(nblocks_for(c.Vh, 256), 8);
+  if (pout == P_F64) {
+    hipLaunchKernelGGL((k_convert_links<P_F64, P_F64>), grid, dim3(256), 0, c.stream, in, out, c.Vh, 1.0f);
+  } else if (pout == P_F32) {
+    hipLaunchKernelGGL((k_convert_links<P_F64, P_F32>), grid, dim3(256), 0, c.stream, in, out, c.Vh, 1.0f);
+  } else {
+    double m;
+    int rc = link_absmax_f64(in, &m);
+    if (rc) return rc;
+    out.scale = (float)(m > 0 ? m : 1.0);
+    hipLaunchKernelGGL((k_convert_links<P_F64, P_H16>), grid, dim3(256), 0, c.stream, in, out, c.Vh,
+                       (float)(1.0 / out.scale));
+  }
+  MH_CHECK(hipGetLastError());
+  return 0;
+}
+
+int convert_vec(const VecField& in, int pin, VecField& out, int pout) {
+  Context& c = ctx();
+  dim3 grid(nblocks_for(c.Vh, 256));
+#define CV(a, b)                                                                                   \
+  if (pin == a && pout == b) {                                                                     \
+    hipLaunchKernelGGL((k_convert_vec<a, b>), grid, dim3(256), 0, c.stream, in, out, c.Vh);       \
+    MH_CHECK(hipGetLastError());                                                                   \
+    return 0;                                                                                      \
+  }
+  CV(P_F64, P_F64) CV(P_F64, P_F32) CV(P_F64, P_H16) CV(P_F32, P_F64) CV(P_F32, P_F32) CV(P_F32, P_H16)
+  CV(P_H16, P_F64) CV(P_H16, P_F32) CV(P_H16, P_H16)
+#undef CV
+  return -2;
+}
+
+// ===========================================================================
+// Dslash
+//   dst(x) = sum_mu F_mu(x) s(x+mu) + L_mu(x) s(x+3mu)
+//                 - F_mu(x-mu)^dag s(x-mu) - L_mu(x-3mu)^dag s(x-3mu)
+// One thread per output site; the epilogue variants fuse the CG algebra.
+// ===========================================================================
+template <int P>
+__device__ __forceinline__ void dslash_site(const Geom& g, const LinkField& fat, const LinkField& lng,
+                                            const VecField& src, int cb, int pb,
+                                            Vec3<typename PrecTraits<P>::Real>& acc) {
+  typedef typename PrecTraits<P>::Real R;
+  Coord c = site_coords(g, cb, pb);
+  int po = pb ^ 1;
+  zero(acc);
+#pragma unroll
+  for (int mu = 0; mu < 4; mu++) {
+    Mat3<R> m;
+    Vec3<R> v;
+    int n;
+    // forward one hop
+    n = neighbour(g, c, cb, mu, 1);
+    load_link<P>(fat, pb, mu, cb, m);
+    load_vec<P>(src, n, v);
+    mv_acc(m, v, acc);
+    // forward three hops
+    n = neighbour(g, c, cb, mu, 3);
+    load_link<P>(lng, pb, mu, cb, m);
+    load_vec<P>(src, n, v);
+    mv_acc(m, v, acc);
+    // backward one hop: F_mu(x-mu) lives on the other parity
+    n = neighbour(g, c, cb, mu, -1);
+    load_link<P>(fat, po, mu, n, m);
+    load_vec<P>(src, n, v);
+    amv_sub(m, v, acc);
+    // backward three hops
+    n = neighbour(g, c, cb, mu, -3);
+    load_link<P>(lng, po, mu, n, m);
+    load_vec<P>(src, n, v);
+    amv_sub(m, v, acc);
+  }
+}
+
+template <int P>
+__global__ void __launch_bounds__(256) k_dslash(Geom g, LinkField fat, LinkField lng, VecField src, VecField dst,
+                                                int pb, const int* __restrict__ stop) {
+  if (stop && *stop) return;
+  int bid = xcd_block(blockIdx.x, gridDim.x);
+  int cb = bid * blockDim.x + threadIdx.x;
+  if (cb >= g.Vh) return;
+  Vec3<typename PrecTraits<P>::Real> acc;
+  dslash_site<P>(g, fat, lng, src, cb, pb, acc);
+  store_vec<P>(dst, cb, acc);
+}
+
+// Block reduction of N doubles into partials[blockIdx.x * 4 + k].
+template <int N>
+__device__ __forceinline__ void block_sum_store(double (&v)[N], double* __restrict__ partials) {
+#pragma unroll
+  for (int k = 0; k < N; k++)
+    for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off, 64);
+  __shared__ double sm[N][16];
+  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < N; k++) sm[k][wid] = v[k];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int nw = blockDim.x >> 6;
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+      double s = 0;
+      for (int w = 0; w < nw; w++) s += sm[k][w];
+      partials[blockIdx.x * 4 + k] = s;
+    }
+  }
+}
+
+// Second half of the normal operator plus the CG dot products
+// (d_congrad5_fn_milc.c:281-308):
+//   ttt = D tmp - msq_x4 * p   (= -M^dag M p)
+//   pkp = <p,ttt>, c_tr = <ttt,r>, c_tt = <ttt,ttt>   (real parts)
+template <int PL, int PP, int PA>
+__global__ void __launch_bounds__(256) k_dslash_mdagm(Geom g, LinkField fat, LinkField lng, VecField tmp, VecField p,
+                                                      VecField r, VecField ttt, int pb, double msq_x4,
+                                                      double* __restrict__ partials, const int* __restrict__ stop) {
+  if (*stop) return;
+  typedef typename PrecTraits<PA>::Real RA;
+  int bid = xcd_block(blockIdx.x, gridDim.x);
+  int cb = bid * blockDim.x + threadIdx.x;
+  double s[3] = {0, 0, 0};
+  if (cb < g.Vh) {
+    Vec3<typename PrecTraits<PL>::Real> acc;
+    dslash_site<PL>(g, fat, lng, tmp, cb, pb, acc);
+    Vec3<typename PrecTraits<PP>::Real> pv;
+    load_vec<PP>(p, cb, pv);
+    Vec3<RA> rv, t;
+    load_vec<PA>(r, cb, rv);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      t.c[k].re = (RA)((RA)acc.c[k].re - (RA)msq_x4 * (RA)pv.c[k].re);
+      t.c[k].im = (RA)((RA)acc.c[k].im - (RA)msq_x4 * (RA)pv.c[k].im);
+      s[0] += (double)pv.c[k].re * (double)t.c[k].re + (double)pv.c[k].im * (double)t.c[k].im;
+      s[1] += (double)t.c[k].re * (double)rv.c[k].re + (double)t.c[k].im * (double)rv.c[k].im;
+      s[2] += (double)t.c[k].re * (double)t.c[k].re + (double)t.c[k].im * (double)t.c[k].im;
+    }
+    store_vec<PA>(ttt, cb, t);
+  }
+  block_sum_store<3>(s, partials);
+}
+
+// Restart block (d_congrad5_fn_milc.c:177-240): with acc = D tmp,
+//   ttt = acc - msq_x4*x; resid = src + ttt (= src - M^dag M x); p = resid;
+// |resid|^2 partials go to partials_rr.  Also used for the FP64 true residual
+// of the reliable updates (then p/ttt are not written).
+template <int P>
+__global__ void __launch_bounds__(256) k_dslash_resid(Geom g, LinkField fat, LinkField lng, VecField tmp, VecField x,
+                                                      VecField src, VecField resid, VecField p, int write_p,
+                                                      int pb, double msq_x4, double* __restrict__ partials_rr) {
+  typedef typename PrecTraits<P>::Real R;
+  int bid = xcd_block(blockIdx.x, gridDim.x);
+  int cb = bid * blockDim.x + threadIdx.x;
+  double s[1] = {0};
+  if (cb < g.Vh) {
+    Vec3<R> acc, xv, bv, r;
+    dslash_site<P>(g, fat, lng, tmp, cb, pb, acc);
+    load_vec<P>(x, cb, xv);
+    load_vec<P>(src, cb, bv);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      R tre = acc.c[k].re + (R)(-msq_x4) * xv.c[k].re;
+      R tim = acc.c[k].im + (R)(-msq_x4) * xv.c[k].im;
+      r.c[k].re = bv.c[k].re + tre;
+      r.c[k].im = bv.c[k].im + tim;
+      s[0] += (double)r.c[k].re * (double)r.c[k].re + (double)r.c[k].im * (double)r.c[k].im;
+    }
+    store_vec<P>(resid, cb, r);
+    if (write_p) store_vec<P>(p, cb, r);
+  }
+  block_sum_store<1>(s, partials_rr);
+}
+
+// Deterministic sum of nparts block partials (fixed order per thread, fixed tree).
+__device__ __forceinline__ void sum_partials(const double* __restrict__ parts, int nblocks, int k, double& out,
+                                             double* sm) {
+  double v = 0;
+  for (int i = threadIdx.x; i < nblocks; i += blockDim.x) v += parts[i * 4 + k];
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sm[wid] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) s += sm[w];
+    out = s;
+  }
+}
+
+// after a restart: rsq = actual_rsq = |resid|^2, reset the device loop state
+__global__ void k_restart_scalars(const double* __restrict__ partials_rr, int nblocks, CGScalars* sc) {
+  __shared__ double sm[16];
+  double rr = 0;
+  sum_partials(partials_rr, nblocks, 0, rr, sm);
+  if (threadIdx.x == 0) {
+    sc->rsq = rr;
+    sc->actual_rsq = rr;
+    sc->rsq_first = rr;
+    sc->rsq_max = rr;
+    sc->halt = 0;
+    sc->go = 0;
+    sc->iters = 0;
+  }
+}
+
+// FEWSUMS scalar step (d_congrad5_fn_milc.c:309-362):
+//   oldrsq = actual_rsq; a = -rsq/pkp; rsq = oldrsq + 2a c_tr + a^2 c_tt; b = rsq/oldrsq
+// Halts the loop once rsq < rsq_stop (the target, or for the mixed solver the
+// reliable-update trigger).
+__global__ void k_cg_scalars(const double* __restrict__ partials, const double* __restrict__ partials_rr,
+                             int nblocks, CGScalars* sc) {
+  if (sc->halt) {
+    if (threadIdx.x == 0) sc->go = 0;
+    return;
+  }
+  __shared__ double sm[16];
+  double pkp = 0, c_tr = 0, c_tt = 0, oldrsq = 0;
+  sum_partials(partials, nblocks, 0, pkp, sm);
+  sum_partials(partials, nblocks, 1, c_tr, sm);
+  sum_partials(partials, nblocks, 2, c_tt, sm);
+  sum_partials(partials_rr, nblocks, 0, oldrsq, sm);
+  if (threadIdx.x == 0) {
+    double a = -sc->rsq / pkp;
+    double rsq = oldrsq + 2.0 * a * c_tr + a * a * c_tt;
+    double b = rsq / oldrsq;
+    sc->oldrsq = oldrsq;
+    sc->actual_rsq = oldrsq;
+    sc->pkp = pkp;
+    sc->c_tr = c_tr;
+    sc->c_tt = c_tt;
+    sc->a = a;
+    sc->b = b;
+    sc->rsq = rsq;
+    sc->go = 1;
+    sc->iters += 1;
+    if (rsq > sc->rsq_max) sc->rsq_max = rsq;
+    if (rsq < sc->rsq_stop) sc->halt = 1;
+  }
+}
+
+// dest += a p; resid += a ttt; |resid|^2 partials; p = resid + b p
+// (d_congrad5_fn_milc.c:318-367)
+template <int PP, int PA>
+__global__ void __launch_bounds__(256) k_cg_update(VecField x, VecField p, VecField r, VecField ttt, int Vh,
+                                                   const CGScalars* __restrict__ sc, double* __restrict__ partials_rr) {
+  if (!sc->go) return;
+  typedef typename PrecTraits<PA>::Real RA;
+  typedef typename PrecTraits<PP>::Real RP;
+  int cb = blockIdx.x * blockDim.x + threadIdx.x;
+  double s[1] = {0};
+  if (cb < Vh) {
+    RA a = (RA)sc->a, b = (RA)sc->b;
+    Vec3<RA> xv, rv, tv;
+    Vec3<RP> pv;
+    load_vec<PA>(x, cb, xv);
+    load_vec<PA>(r, cb, rv);
+    load_vec<PA>(ttt, cb, tv);
+    load_vec<PP>(p, cb, pv);
+    Vec3<RP> pn;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      xv.c[k].re = xv.c[k].re + a * (RA)pv.c[k].re;
+      xv.c[k].im = xv.c[k].im + a * (RA)pv.c[k].im;
+      rv.c[k].re = rv.c[k].re + a * tv.c[k].re;
+      rv.c[k].im = rv.c[k].im + a * tv.c[k].im;
+      s[0] += (double)rv.c[k].re * (double)rv.c[k].re + (double)rv.c[k].im * (double)rv.c[k].im;
+      pn.c[k].re = (RP)(rv.c[k].re + b * (RA)pv.c[k].re);
+      pn.c[k].im = (RP)(rv.c[k].im + b * (RA)pv.c[k].im);
+    }
+    store_vec<PA>(x, cb, xv);
+    store_vec<PA>(r, cb, rv);
+    store_vec<PP>(p, cb, pn);
+  }
+  block_sum_store<1>(s, partials_rr);
+}
+
+// Reliable update, vector part: x_hi += x_s; x_s = 0
+__global__ void k_accum_x(VecField xhi, VecField xs, int Vh) {
+  int cb = blockIdx.x * blockDim.x + threadIdx.x;
+  if (cb >= Vh) return;
+  Vec3<double> a;
+  Vec3<float> b, z;
+  load_vec<P_F64>(xhi, cb, a);
+  load_vec<P_F32>(xs, cb, b);
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    a.c[k].re += (double)b.c[k].re;
+    a.c[k].im += (double)b.c[k].im;
+    z.c[k].re = z.c[k].im = 0.f;
+  }
+  store_vec<P_F64>(xhi, cb, a);
+  store_vec<P_F32>(xs, cb, z);
+}
+// Reliable update, sloppy vectors: p_s += r_hi - r_s; r_s = r_hi, so that p_s
+// equals r_hi + b p_old as if the true residual had been used in the last step.
+template <int PS>
+__global__ void k_ru_vectors(VecField rhi, VecField rs, VecField ps, int Vh) {
+  int cb = blockIdx.x * blockDim.x + threadIdx.x;
+  if (cb >= Vh) return;
+  Vec3<double> rh;
+  Vec3<float> r;
+  Vec3<typename PrecTraits<PS>::Real> p;
+  load_vec<P_F64>(rhi, cb, rh);
+  load_vec<P_F32>(rs, cb, r);
+  load_vec<PS>(ps, cb, p);
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    p.c[k].re = (float)((double)p.c[k].re + (rh.c[k].re - (double)r.c[k].re));
+    p.c[k].im = (float)((double)p.c[k].im + (rh.c[k].im - (double)r.c[k].im));
+    r.c[k].re = (float)rh.c[k].re;
+    r.c[k].im = (float)rh.c[k].im;
+  }
+  store_vec<P_F32>(rs, cb, r);
+  store_vec<PS>(ps, cb, p);
+}
+
+template <int P>
+__global__ void k_norm2(VecField v, int Vh, double* __restrict__ partials) {
+  int cb = blockIdx.x * blockDim.x + threadIdx.x;
+  double s[1] = {0};
+  if (cb < Vh) {
+    Vec3<typename PrecTraits<P>::Real> a;
+    load_vec<P>(v, cb, a);
+#pragma unroll
+    for (int k = 0; k < 3; k++) s[0] += (double)a.c[k].re * a.c[k].re + (double)a.c[k].im * a.c[k].im;
+  }
+  block_sum_store<1>(s, partials);
+}
+
+// ===========================================================================
+// host side
+// ===========================================================================
+static int dslash_grid() { return nblocks_for(ctx().Vh, 256); }
+
+template <int P>
+static void launch_dslash(const DevLinks& L, const VecField& src, VecField& dst, int pb, const int* stop) {
+  Context& c = ctx();
+  hipLaunchKernelGGL(k_dslash<P>, dim3(dslash_grid()), dim3(256), 0, c.stream, c.geom, L.fat, L.lng, src, dst,
+                     pb, stop);
+}
+
+int dslash(const DevLinks& L, const VecField& src, VecField& dst, int pb_out, int prec) {
+  if (!L.valid || L.prec != prec) return -3;
+  DISPATCH_PREC(prec, launch_dslash, L, src, dst, pb_out, (const int*)nullptr);
+  MH_CHECK(hipGetLastError());
+  return 0;
+}
+
+int vec_norm2(const VecField& v, int prec, double* out) {
+  Context& c = ctx();
+  int nb = nblocks_for(c.Vh, 256);
+  if (prec == P_F64) hipLaunchKernelGGL(k_norm2<P_F64>, dim3(nb), dim3(256), 0, c.stream, v, c.Vh, c.ws.partials);
+  else if (prec == P_F32) hipLaunchKernelGGL(k_norm2<P_F32>, dim3(nb), dim3(256), 0, c.stream, v, c.Vh, c.ws.partials);
+  else hipLaunchKernelGGL(k_norm2<P_H16>, dim3(nb), dim3(256), 0, c.stream, v, c.Vh, c.ws.partials);
+  MH_CHECK(hipGetLastError());
+  std::vector<double> h((size_t)nb * 4);
+  MH_CHECK(hipMemcpyAsync(h.data(), c.ws.partials, h.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  MH_CHECK(hipStreamSynchronize(c.stream));
+  double s = 0;
+  for (int i = 0; i < nb; i++) s += h[(size_t)i * 4];
+  *out = s;
+  return 0;
+}
+
+int vec_zero(VecField& v, int prec) {
+  Context& c = ctx();
+  MH_CHECK(hipMemsetAsync(v.data, 0, 3 * (size_t)v.stride * prec_complex_bytes(prec), c.stream));
+  if (prec == P_H16 && v.norm) MH_CHECK(hipMemsetAsync(v.norm, 0, (size_t)v.stride * sizeof(float), c.stream));
+  return 0;
+}
+int vec_copy(const VecField& in, VecField& out, int prec) {
+  Context& c = ctx();
+  MH_CHECK(hipMemcpyAsync(out.data, in.data, 3 * (size_t)in.stride * prec_complex_bytes(prec),
+                          hipMemcpyDeviceToDevice, c.stream));
+  if (prec == P_H16)
+    MH_CHECK(hipMemcpyAsync(out.norm, in.norm, (size_t)in.stride * sizeof(float), hipMemcpyDeviceToDevice, c.stream));
+  return 0;
+}
+
+static int read_scalars(CGScalars* h) {
+  Context& c = ctx();
+  MH_CHECK(hipMemcpyAsync(h, c.ws.scal, sizeof(CGScalars), hipMemcpyDeviceToHost, c.stream));
+  MH_CHECK(hipStreamSynchronize(c.stream));
+  return 0;
+}
+static int write_scalars(const CGScalars* h) {
+  Context& c = ctx();
+  MH_CHECK(hipMemcpyAsync(c.ws.scal, h, sizeof(CGScalars), hipMemcpyHostToDevice, c.stream));
+  return 0;
+}
+
+// partials_rr lives in the second half of the partials buffer
+static double* partials_rr() { return ctx().ws.partials + (size_t)ctx().ws.max_blocks * 4; }
+
+// True residual resid = src - M^dag M x on parity pb with FP64 links; p = resid
+// if write_p.  Leaves rsq in the device scalars (restart semantics).
+static int true_residual(const VecField& src, const VecField& x, VecField& tmp_other, VecField& resid, VecField& p,
+                         int write_p, int pb, double msq_x4) {
+  Context& c = ctx();
+  const DevLinks& L = c.links[P_F64];
+  if (!L.valid) return -3;
+  launch_dslash<P_F64>(L, x, tmp_other, pb ^ 1, nullptr);
+  hipLaunchKernelGGL(k_dslash_resid<P_F64>, dim3(dslash_grid()), dim3(256), 0, c.stream, c.geom, L.fat, L.lng,
+                     tmp_other, x, src, resid, p, write_p, pb, msq_x4, partials_rr());
+  hipLaunchKernelGGL(k_restart_scalars, dim3(1), dim3(256), 0, c.stream, partials_rr(), dslash_grid(), c.ws.scal);
+  MH_CHECK(hipGetLastError());
+  return 0;
+}
+
+template <int PL, int PP, int PA>
+static void enqueue_cg_iteration(const DevLinks& L, VecField& p, VecField& tmp, VecField& r, VecField& ttt,
+                                 VecField& x, int pb, double msq_x4) {
+  Context& c = ctx();
+  int nb = dslash_grid();
+  const int* halt = &c.ws.scal->halt;
+  hipLaunchKernelGGL(k_dslash<PL>, dim3(nb), dim3(256), 0, c.stream, c.geom, L.fat, L.lng, p, tmp, pb ^ 1, halt);
+  hipLaunchKernelGGL((k_dslash_mdagm<PL, PP, PA>), dim3(nb), dim3(256), 0, c.stream, c.geom, L.fat, L.lng, tmp, p,
+                     r, ttt, pb, msq_x4, c.ws.partials, halt);
+  hipLaunchKernelGGL(k_cg_scalars, dim3(1), dim3(256), 0, c.stream, c.ws.partials, partials_rr(), nb, c.ws.scal);
+  hipLaunchKernelGGL((k_cg_update<PP, PA>), dim3(nblocks_for(c.Vh, 256)), dim3(256), 0, c.stream, x, p, r, ttt,
+                     c.Vh, c.ws.scal, partials_rr());
+}
+
+// Plain FP64 CG, MILC's restart/iteration semantics (d_congrad5_fn_milc.c:60-407)
+static int cg_solve_f64(const VecField& src, VecField& x, double mass, const SolveParams& sp, SolveStats& st) {
+  Context& c = ctx();
+  const DevLinks& L = c.links[P_F64];
+  if (!L.valid) return -3;
+  int pb = pbit(sp.parity);
+  double msq_x4 = 4.0 * mass * mass;
+  double rsqmin = sp.resid * sp.resid;
+  int niter = sp.max_iters, max_restarts = sp.max_restarts;
+  long max_cg = (long)max_restarts * niter;
+  st = SolveStats();
+  double source_norm;
+  int rc = vec_norm2(src, P_F64, &source_norm);
+  if (rc) return rc;
+  if (source_norm == 0.0) {
+    st.converged = 1;
+    return vec_zero(x, P_F64);
+  }
+  VecField ttt, p, r, tmp;
+  if ((rc = alloc_vec(ttt, P_F64, c.stride)) || (rc = alloc_vec(p, P_F64, c.stride)) ||
+      (rc = alloc_vec(r, P_F64, c.stride)) || (rc = alloc_vec(tmp, P_F64, c.stride)))
+    return rc;
+  CGScalars h;
+  memset(&h, 0, sizeof h);
+  h.source_norm = source_norm;
+  h.rsq_stop = rsqmin * source_norm;
+  write_scalars(&h);
+  long iteration = 0;
+  int nrestart = 0;
+  double final_rsq = 0, size_r = 0;
+  int batch = sp.batch > 0 ? sp.batch : 16;
+  for (;;) {
+    // restart block: true residual
+    if ((rc = true_residual(src, x, tmp, r, p, 1, pb, msq_x4))) break;
+    if ((rc = read_scalars(&h))) break;
+    final_rsq = h.rsq / source_norm;
+    iteration++;
+    if (iteration >= max_cg || nrestart >= max_restarts || rsqmin > final_rsq) break;
+    nrestart++;
+    // normal iterations until the next restart point or the recurrence target
+    long left = niter - (iteration % niter);
+    if (left == niter) left = 0;
+    left = std::min(left, max_cg - iteration);
+    while (left > 0) {
+      int nb = (int)std::min<long>(batch, left);
+      for (int k = 0; k < nb; k++) enqueue_cg_iteration<P_F64, P_F64, P_F64>(L, p, tmp, r, ttt, x, pb, msq_x4);
+      MH_CHECK(hipGetLastError());
+      int before = h.iters;
+      if ((rc = read_scalars(&h))) break;
+      int done_now = h.iters - before;
+      iteration += done_now;
+      left -= done_now;
+      size_r = h.rsq / source_norm;
+      if (h.halt) break;
+    }
+    if (rc) break;
+  }
+  st.final_rsq = final_rsq;
+  st.size_r = size_r;
+  st.iters = (int)iteration;
+  st.restarts = nrestart;
+  st.converged = (nrestart == max_restarts || iteration == max_cg) ? 0 : 1;
+  free_vec(ttt); free_vec(p); free_vec(r); free_vec(tmp);
+  return rc;
+}
+
+// Mixed precision CG: sloppy (PS = F32 or H16 links and p, F32 r/x/Ap)
+// iterations with the FEWSUMS recurrence, FP64 reliable updates when the
+// sloppy residual has fallen by delta (or reached the target), FP64 true
+// residual for the stopping test.
+template <int PS>
+static int cg_solve_mixed(const VecField& src, VecField& x, double mass, const SolveParams& sp, SolveStats& st) {
+  Context& c = ctx();
+  const DevLinks& LS = c.links[PS];
+  if (!LS.valid || !c.links[P_F64].valid) return -3;
+  int pb = pbit(sp.parity);
+  double msq_x4 = 4.0 * mass * mass;
+  double rsqmin = sp.resid * sp.resid;
+  long max_cg = (long)sp.max_restarts * sp.max_iters;
+  double delta = sp.delta > 0 ? sp.delta : 0.1;
+  st = SolveStats();
+  double source_norm;
+  int rc = vec_norm2(src, P_F64, &source_norm);
+  if (rc) return rc;
+  if (source_norm == 0.0) {
+    st.converged = 1;
+    return vec_zero(x, P_F64);
+  }
+  VecField rhi, tmp64, ps, tmps, ttts, rs, xs, dummy{};
+  if ((rc = alloc_vec(rhi, P_F64, c.stride)) || (rc = alloc_vec(tmp64, P_F64, c.stride)) ||
+      (rc = alloc_vec(ps, PS, c.stride)) || (rc = alloc_vec(tmps, PS, c.stride)) ||
+      (rc = alloc_vec(ttts, P_F32, c.stride)) || (rc = alloc_vec(rs, P_F32, c.stride)) ||
+      (rc = alloc_vec(xs, P_F32, c.stride)))
+    return rc;
+  CGScalars h;
+  memset(&h, 0, sizeof h);
+  h.source_norm = source_norm;
+  write_scalars(&h);
+  int nb = nblocks_for(c.Vh, 256);
+  long iteration = 0;
+  int nru = 0;
+  double final_rsq = 0;
+  int batch = sp.batch > 0 ? sp.batch : 16;
+  bool first = true;
+  for (;;) {
+    // reliable update (or the initial residual): FP64 true residual
+    if (!first) hipLaunchKernelGGL(k_accum_x, dim3(nb), dim3(256), 0, c.stream, x, xs, c.Vh);
+    if ((rc = true_residual(src, x, tmp64, rhi, dummy, 0, pb, msq_x4))) break;
+    if (first) {
+      if ((rc = convert_vec(rhi, P_F64, rs, P_F32)) || (rc = convert_vec(rhi, P_F64, ps, PS))) break;
+    } else {
+      hipLaunchKernelGGL(k_ru_vectors<PS>, dim3(nb), dim3(256), 0, c.stream, rhi, rs, ps, c.Vh);
+      nru++;
+    }
+    first = false;
+    if ((rc = read_scalars(&h))) break;
+    final_rsq = h.rsq / source_norm;
+    iteration++;
+    if (rsqmin > final_rsq || iteration >= max_cg) break;
+    // sloppy iterations until |r|^2 fell by delta^2 or reached the target
+    h.rsq_stop = std::max(rsqmin * source_norm, delta * delta * h.rsq);
+    h.halt = 0;
+    h.iters = 0;
+    if ((rc = write_scalars(&h))) break;
+    long left = max_cg - iteration;
+    while (left > 0) {
+      int n = (int)std::min<long>(batch, left);
+      for (int k = 0; k < n; k++) enqueue_cg_iteration<PS, PS, P_F32>(LS, ps, tmps, rs, ttts, xs, pb, msq_x4);
+      MH_CHECK(hipGetLastError());
+      int before = h.iters;
+      if ((rc = read_scalars(&h))) break;
+      int done_now = h.iters - before;
+      iteration += done_now;
+      left -= done_now;
+      if (h.halt) break;
+    }
+    if (rc) break;
+  }
+  st.final_rsq = final_rsq;
+  st.size_r = final_rsq;
+  st.iters = (int)iteration;
+  st.restarts = nru;
+  st.reliable_updates = nru;
+  st.converged = rsqmin > final_rsq ? 1 : 0;
+  free_vec(rhi); free_vec(tmp64); free_vec(ps); free_vec(tmps); free_vec(ttts); free_vec(rs); free_vec(xs);
+  return rc;
+}
+
+int cg_solve(const VecField& src, VecField& x, double mass, const SolveParams& sp, SolveStats& st) {
+  auto t0 = std::chrono::steady_clock::now();
+  int rc;
+  if (sp.relresid > 0) return -4;  // Fermilab relative residual is not supported
+  if (sp.sloppy_prec == P_F64) rc = cg_solve_f64(src, x, mass, sp, st);
+  else if (sp.sloppy_prec == P_F32) rc = cg_solve_mixed<P_F32>(src, x, mass, sp, st);
+  else rc = cg_solve_mixed<P_H16>(src, x, mass, sp, st);
+  (void)hipStreamSynchronize(ctx().stream);
+  st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return rc;
+}
+
+}  // namespace mh

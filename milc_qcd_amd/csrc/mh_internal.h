@@ -1,0 +1,131 @@
+// mh_internal.h -- host-side declarations shared by the library's translation units.
+#pragma once
+#include <vector>
+#include "mh_common.h"
+
+namespace mh {
+
+// Device-resident scalars of one Krylov solve (all FP64).  The reduction
+// kernels write them and the vector-update kernels read them, so the host
+// never sits between two iterations.
+struct CGScalars {
+  double rsq;          // recurrence residual |r|^2 (FEWSUMS, d_congrad5_fn_milc.c:339)
+  double actual_rsq;   // explicitly summed |r|^2 of the last update
+  double oldrsq;
+  double pkp, c_tr, c_tt;
+  double a, b;
+  double source_norm;
+  double rsq_stop;     // stop when rsq < rsq_stop (absolute)
+  double rsq_first;    // |r|^2 right after the last reliable update
+  double rsq_max;      // max |r|^2 since the last reliable update
+  int halt;            // set by the scalar step once the target is met: later
+                       // iterations' kernels return immediately
+  int go;              // written by every scalar step: 1 if this iteration's
+                       // vector update must run
+  int iters;           // scalar steps executed since the last reset
+  int pad;
+};
+
+// Reduction scratch: partial sums per block (FP64) and the scalar block.
+struct Workspace {
+  double* partials = nullptr;  // [max_blocks * 4]
+  int max_blocks = 0;
+  CGScalars* scal = nullptr;   // device
+  CGScalars* scal_host = nullptr;  // pinned host mirror
+};
+
+// A set of links on the device in one precision.
+struct DevLinks {
+  LinkField fat{}, lng{};
+  int prec = -1;
+  bool valid = false;
+};
+
+struct Context {
+  int device = 0;
+  int dims[4] = {0, 0, 0, 0};
+  Geom geom{};
+  int Vh = 0;
+  int stride = 0;     // padded plane stride (elements) for per-parity fields
+  hipStream_t stream = nullptr;
+  Workspace ws;
+  // link cache (keyed like d_congrad5_fn_gpu.c:120-126: fn pointer + refresh flag)
+  const void* link_key = nullptr;
+  const void* link_key2 = nullptr;
+  DevLinks links[3];   // by Prec
+  double* staging = nullptr;   // FP64 AoS staging buffer, 4*V*18 doubles
+  size_t staging_elems = 0;
+  int block = 256;
+  bool initialised = false;
+};
+
+Context& ctx();
+
+// device memory helpers
+int dev_alloc(void** p, size_t bytes);
+void dev_free(void* p);
+int alloc_vec(VecField& v, int prec, int stride);
+void free_vec(VecField& v);
+int alloc_links(LinkField& l, int prec, int stride);
+void free_links(LinkField& l);
+size_t prec_complex_bytes(int prec);
+
+// ---- layout ----------------------------------------------------------------
+int upload_vec_milc(const double* host_aos, VecField& out, int prec, int pb);
+int download_vec_milc(const VecField& in, int prec, int pb, double* host_aos);
+int vec_from_milc_dev(const double* dev_aos, VecField& out, int prec, int pb);
+int vec_to_milc_dev(const VecField& in, int prec, int pb, double* dev_aos);
+int links_from_milc_dev(const double* dev_aos, LinkField& out, int prec);
+int links_to_milc_dev(const LinkField& in, int prec, double* dev_aos);
+int convert_links(const LinkField& in, int pin, LinkField& out, int pout);
+int convert_vec(const VecField& in, int pin, VecField& out, int pout);
+
+// ---- stencil ---------------------------------------------------------------
+// dst(pb_out) = D src(1-pb_out)
+int dslash(const DevLinks& L, const VecField& src, VecField& dst, int pb_out, int prec_vec);
+
+// ---- solver entry points (device fields, FP64 x/src) -------------------------
+struct SolveParams {
+  int parity;          // MILC code
+  int max_iters;       // per restart (qic->max)
+  int max_restarts;    // qic->nrestart
+  double resid;        // target |r|/|b|
+  double relresid;     // unsupported (must be 0)
+  int sloppy_prec;     // P_F64 (plain), P_F32 or P_H16
+  double delta;        // reliable-update threshold (mixed)
+  int batch;           // iterations enqueued per host check
+};
+struct SolveStats {
+  double final_rsq = 0, final_relrsq = 0, size_r = 0;
+  int iters = 0, restarts = 0, converged = 0, reliable_updates = 0;
+  double seconds = 0;
+};
+int cg_solve(const VecField& src, VecField& x, double mass, const SolveParams& sp, SolveStats& st);
+
+struct MultiParams {
+  int parity;
+  int max_iters;       // total iteration cap (qic->max * qic->nrestart)
+  double resid;        // base-system target |r|/|b|
+  int prec;            // P_F64 or P_F32 bulk
+  int batch;
+};
+int multicg_solve(const VecField& src, std::vector<VecField>& x, const std::vector<double>& offsets,
+                  const MultiParams& mp, SolveStats& st);
+
+// ---- force outer products ---------------------------------------------------
+int force_oprod(const std::vector<VecField>& xe, const std::vector<VecField>& xo,
+                const std::vector<double>& residues, LinkField& o1, LinkField& o3);
+
+// ---- links ------------------------------------------------------------------
+int gauge_random_dev(uint64_t seed, double scale, LinkField& U);
+int hisq_links_dev(const LinkField& U, LinkField& fat, LinkField& lng, LinkField* W_out, int* nsvd);
+int asqtad_links_dev(const LinkField& U, double u0, LinkField& fat, LinkField& lng);
+int rephase_dev(LinkField& U);
+int gaussian_vec_dev(uint64_t seed, int pb, VecField& v);
+
+// ---- BLAS --------------------------------------------------------------------
+int vec_norm2(const VecField& v, int prec, double* out);  // synchronous
+int vec_zero(VecField& v, int prec);
+int vec_copy(const VecField& in, VecField& out, int prec);
+
+}  // namespace mh

@@ -1,0 +1,384 @@
+// mh_multicg.hip -- multi-shift CG on the device.
+//
+// Restates generic_ks/ks_multicg_offset.c:63-505 (ks_multicg_offset_field_cpu):
+// Jegerlehner multi-mass CG for (-D_eo D_oe + offset_j) x_j = b with the
+// smallest offset as the base system, zeta/beta/alpha recurrences in FP64 and
+// retirement of shifts whose zeta underflows (:324-355, :429-444).
+// The recurrences run in single-block kernels on device-resident scalars, so
+// the host only checks convergence every `batch` iterations.  The update of
+// the non-base search directions pm_j (:447-453) is deferred into the next
+// iteration's solution update, so every shift's pm_j/psim_j pair is streamed
+// once per iteration.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "mh_internal.h"
+
+namespace mh {
+
+#define MS_MAX 32
+
+struct MSScalars {
+  double zeta_i[MS_MAX], zeta_im1[MS_MAX], zeta_ip1[MS_MAX];
+  double beta_i[MS_MAX], beta_im1[MS_MAX], alpha[MS_MAX];
+  double shifts[MS_MAX];
+  double rsq, oldrsq, pkp, source_norm, rsqstop;
+  int num_now, j_low, n, halt, iters, pending, converged, pad;
+};
+
+struct VecSet {
+  VecField v[MS_MAX];
+};
+
+template <int N>
+__device__ __forceinline__ void block_sum4(double (&v)[N], double* __restrict__ partials) {
+#pragma unroll
+  for (int k = 0; k < N; k++)
+    for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off, 64);
+  __shared__ double sm[N][16];
+  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < N; k++) sm[k][wid] = v[k];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int nw = blockDim.x >> 6;
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+      double s = 0;
+      for (int w = 0; w < nw; w++) s += sm[k][w];
+      partials[blockIdx.x * 4 + k] = s;
+    }
+  }
+}
+
+__device__ double reduce_one(const double* __restrict__ parts, int nblocks) {
+  __shared__ double sm[16];
+  double v = 0;
+  for (int i = threadIdx.x; i < nblocks; i += blockDim.x) v += parts[i * 4];
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) s += sm[w];
+  return s;
+}
+
+// Dslash forward-declared from mh_core.hip is a template there; the multi-shift
+// solver carries its own copies of the two stencil launches.
+template <int P>
+__device__ __forceinline__ void ms_dslash_site(const Geom& g, const LinkField& fat, const LinkField& lng,
+                                               const VecField& src, int cb, int pb,
+                                               Vec3<typename PrecTraits<P>::Real>& acc) {
+  typedef typename PrecTraits<P>::Real R;
+  Coord c = site_coords(g, cb, pb);
+  int po = pb ^ 1;
+  zero(acc);
+#pragma unroll
+  for (int mu = 0; mu < 4; mu++) {
+    Mat3<R> m;
+    Vec3<R> v;
+    int n;
+    n = neighbour(g, c, cb, mu, 1);
+    load_link<P>(fat, pb, mu, cb, m);
+    load_vec<P>(src, n, v);
+    mv_acc(m, v, acc);
+    n = neighbour(g, c, cb, mu, 3);
+    load_link<P>(lng, pb, mu, cb, m);
+    load_vec<P>(src, n, v);
+    mv_acc(m, v, acc);
+    n = neighbour(g, c, cb, mu, -1);
+    load_link<P>(fat, po, mu, n, m);
+    load_vec<P>(src, n, v);
+    amv_sub(m, v, acc);
+    n = neighbour(g, c, cb, mu, -3);
+    load_link<P>(lng, po, mu, n, m);
+    load_vec<P>(src, n, v);
+    amv_sub(m, v, acc);
+  }
+}
+
+template <int P>
+__global__ void __launch_bounds__(256) k_ms_dslash1(Geom g, LinkField fat, LinkField lng, VecField src, VecField dst,
+                                                    int pb, const MSScalars* __restrict__ sc) {
+  if (sc->halt) return;
+  int bid = xcd_block(blockIdx.x, gridDim.x);
+  int cb = bid * blockDim.x + threadIdx.x;
+  if (cb >= g.Vh) return;
+  Vec3<typename PrecTraits<P>::Real> acc;
+  ms_dslash_site<P>(g, fat, lng, src, cb, pb, acc);
+  store_vec<P>(dst, cb, acc);
+}
+
+// ttt = D tmp + shift0 * cg_p ; pkp = <cg_p, ttt>   (ks_multicg_offset.c:296-318)
+template <int P>
+__global__ void __launch_bounds__(256) k_ms_dslash2(Geom g, LinkField fat, LinkField lng, VecField tmp, VecField p,
+                                                    VecField ttt, int pb, const MSScalars* __restrict__ sc,
+                                                    double* __restrict__ partials) {
+  if (sc->halt) return;
+  typedef typename PrecTraits<P>::Real R;
+  int bid = xcd_block(blockIdx.x, gridDim.x);
+  int cb = bid * blockDim.x + threadIdx.x;
+  double s[1] = {0};
+  if (cb < g.Vh) {
+    R shift0 = (R)(-sc->shifts[sc->j_low]);
+    Vec3<R> acc, pv;
+    ms_dslash_site<P>(g, fat, lng, tmp, cb, pb, acc);
+    load_vec<P>(p, cb, pv);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      acc.c[k].re = acc.c[k].re + shift0 * pv.c[k].re;
+      acc.c[k].im = acc.c[k].im + shift0 * pv.c[k].im;
+      s[0] += (double)pv.c[k].re * (double)acc.c[k].re + (double)pv.c[k].im * (double)acc.c[k].im;
+    }
+    store_vec<P>(ttt, cb, acc);
+  }
+  block_sum4<1>(s, partials);
+}
+
+// beta/zeta recurrences (ks_multicg_offset.c:322-356)
+__global__ void k_ms_beta(const double* __restrict__ partials, int nblocks, MSScalars* sc) {
+  if (sc->halt) return;
+  double pkp = reduce_one(partials, nblocks);
+  if (threadIdx.x != 0) return;
+  int jl = sc->j_low;
+  sc->pkp = pkp;
+  sc->iters += 1;
+  sc->beta_i[jl] = -sc->rsq / pkp;
+  sc->zeta_ip1[jl] = 1.0;
+  int now = sc->num_now;
+  for (int j = 0; j < now; j++) {
+    if (j == jl) continue;
+    sc->zeta_ip1[j] = sc->zeta_i[j] * sc->zeta_im1[j] * sc->beta_im1[jl];
+    double c1 = sc->beta_i[jl] * sc->alpha[jl] * (sc->zeta_im1[j] - sc->zeta_i[j]);
+    double c2 = sc->zeta_im1[j] * sc->beta_im1[jl] * (1.0 + sc->shifts[j] * sc->beta_i[jl]);
+    if (c1 + c2 != 0.0) sc->zeta_ip1[j] /= c1 + c2;
+    else sc->zeta_ip1[j] = 0.0;
+    if (sc->zeta_i[j] != 0.0) {
+      sc->beta_i[j] = sc->beta_i[jl] * sc->zeta_ip1[j] / sc->zeta_i[j];
+    } else {
+      sc->zeta_ip1[j] = 0.0;
+      sc->beta_i[j] = 0.0;
+      if (j == now - 1 && j > jl) now--;
+    }
+  }
+  sc->num_now = now;
+}
+
+// psim_j += beta_j pm_j (pm_j first brought up to date with the deferred
+// pm_j = zeta_j r + alpha_j pm_j of the previous iteration); r += beta_low ttt
+template <int P>
+__global__ void __launch_bounds__(256) k_ms_update(VecSet psim, VecSet pm, VecField p, VecField r, VecField ttt,
+                                                   int Vh, const MSScalars* __restrict__ sc,
+                                                   double* __restrict__ partials) {
+  if (sc->halt) return;
+  typedef typename PrecTraits<P>::Real R;
+  int cb = blockIdx.x * blockDim.x + threadIdx.x;
+  double s[1] = {0};
+  if (cb < Vh) {
+    int now = sc->num_now, jl = sc->j_low, pending = sc->pending;
+    Vec3<R> rv, tv, pv;
+    load_vec<P>(r, cb, rv);
+    load_vec<P>(ttt, cb, tv);
+    load_vec<P>(p, cb, pv);
+    for (int j = 0; j < now; j++) {
+      Vec3<R> d, x;
+      R b = (R)sc->beta_i[j];
+      if (j == jl) {
+        d = pv;
+      } else {
+        load_vec<P>(pm.v[j], cb, d);
+        if (pending) {
+          R z = (R)sc->zeta_i[j], a = (R)sc->alpha[j];
+#pragma unroll
+          for (int k = 0; k < 3; k++) {
+            d.c[k].re = z * rv.c[k].re + a * d.c[k].re;
+            d.c[k].im = z * rv.c[k].im + a * d.c[k].im;
+          }
+          store_vec<P>(pm.v[j], cb, d);
+        }
+      }
+      load_vec<P>(psim.v[j], cb, x);
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        x.c[k].re = x.c[k].re + b * d.c[k].re;
+        x.c[k].im = x.c[k].im + b * d.c[k].im;
+      }
+      store_vec<P>(psim.v[j], cb, x);
+    }
+    R bl = (R)sc->beta_i[jl];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      rv.c[k].re = rv.c[k].re + bl * tv.c[k].re;
+      rv.c[k].im = rv.c[k].im + bl * tv.c[k].im;
+      s[0] += (double)rv.c[k].re * (double)rv.c[k].re + (double)rv.c[k].im * (double)rv.c[k].im;
+    }
+    store_vec<P>(r, cb, rv);
+  }
+  block_sum4<1>(s, partials);
+}
+
+// convergence test, alpha recurrences and scalar scroll (ks_multicg_offset.c:358-463)
+__global__ void k_ms_alpha(const double* __restrict__ partials, int nblocks, MSScalars* sc) {
+  if (sc->halt) return;
+  double rsq = reduce_one(partials, nblocks);
+  if (threadIdx.x != 0) return;
+  double oldrsq = sc->rsq;
+  sc->oldrsq = oldrsq;
+  sc->rsq = rsq;
+  if (sc->rsqstop > 0 && rsq <= sc->rsqstop) {
+    sc->halt = 1;
+    sc->converged = 1;
+    return;
+  }
+  int jl = sc->j_low, now = sc->num_now;
+  sc->alpha[jl] = rsq / oldrsq;
+  for (int j = 0; j < now; j++) {
+    if (j == jl) continue;
+    if (sc->zeta_i[j] * sc->beta_i[jl] != 0.0)
+      sc->alpha[j] = sc->alpha[jl] * sc->zeta_ip1[j] * sc->beta_i[j] / (sc->zeta_i[j] * sc->beta_i[jl]);
+    else
+      sc->alpha[j] = 0.0;
+  }
+  for (int j = 0; j < now; j++) {
+    sc->beta_im1[j] = sc->beta_i[j];
+    sc->zeta_im1[j] = sc->zeta_i[j];
+    sc->zeta_i[j] = sc->zeta_ip1[j];
+  }
+  sc->pending = 1;
+}
+
+// cg_p = r + alpha_low cg_p  (pm[j_low] with zeta_low = 1)
+template <int P>
+__global__ void k_ms_p(VecField p, VecField r, int Vh, const MSScalars* __restrict__ sc) {
+  if (sc->halt) return;
+  typedef typename PrecTraits<P>::Real R;
+  int cb = blockIdx.x * blockDim.x + threadIdx.x;
+  if (cb >= Vh) return;
+  R a = (R)sc->alpha[sc->j_low];
+  Vec3<R> pv, rv;
+  load_vec<P>(p, cb, pv);
+  load_vec<P>(r, cb, rv);
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    Cplx<R> t;
+    t.re = (R)1.0 * rv.c[k].re;
+    t.im = (R)1.0 * rv.c[k].im;
+    pv.c[k].re = t.re + a * pv.c[k].re;
+    pv.c[k].im = t.im + a * pv.c[k].im;
+  }
+  store_vec<P>(p, cb, pv);
+}
+
+template <int P>
+static int multicg_run(const VecField& src64, std::vector<VecField>& x64, const std::vector<double>& offsets,
+                       const MultiParams& mp, SolveStats& st) {
+  Context& c = ctx();
+  const DevLinks& L = c.links[P];
+  if (!L.valid) return -3;
+  int n = (int)offsets.size();
+  if (n <= 0 || n > MS_MAX) return -2;
+  int pb = pbit(mp.parity);
+  int rc;
+  double source_norm;
+  if ((rc = vec_norm2(src64, P_F64, &source_norm))) return rc;
+  for (int j = 0; j < n; j++)
+    if ((rc = vec_zero(x64[j], P_F64))) return rc;
+  st = SolveStats();
+  if (source_norm == 0.0) {
+    st.converged = 1;
+    return 0;
+  }
+  VecSet psim{}, pm{};
+  VecField p, r, ttt, tmp;
+  for (int j = 0; j < n; j++) {
+    if ((rc = alloc_vec(psim.v[j], P, c.stride))) return rc;
+    if ((rc = alloc_vec(pm.v[j], P, c.stride))) return rc;
+    if ((rc = convert_vec(src64, P_F64, pm.v[j], P))) return rc;
+  }
+  if ((rc = alloc_vec(p, P, c.stride)) || (rc = alloc_vec(r, P, c.stride)) || (rc = alloc_vec(ttt, P, c.stride)) ||
+      (rc = alloc_vec(tmp, P, c.stride)))
+    return rc;
+  if ((rc = convert_vec(src64, P_F64, p, P)) || (rc = convert_vec(src64, P_F64, r, P))) return rc;
+
+  MSScalars h;
+  memset(&h, 0, sizeof h);
+  double offset_low = 1.0e20;
+  int j_low = -1;
+  for (int j = 0; j < n; j++) {
+    h.shifts[j] = offsets[j];
+    if (offsets[j] < offset_low) {
+      offset_low = offsets[j];
+      j_low = j;
+    }
+  }
+  for (int j = 0; j < n; j++)
+    if (j != j_low) h.shifts[j] -= h.shifts[j_low];
+  h.n = n;
+  h.num_now = n;
+  h.j_low = j_low;
+  h.source_norm = source_norm;
+  h.rsq = source_norm;
+  h.rsqstop = mp.resid * mp.resid * source_norm;
+  for (int j = 0; j < n; j++) {
+    h.zeta_im1[j] = h.zeta_i[j] = 1.0;
+    h.beta_im1[j] = -1.0;
+    h.alpha[j] = 0.0;
+  }
+  // MILC stores shift0 = -shifts[j_low] with shifts[j_low] = offset_low
+  MSScalars* d_sc = nullptr;
+  if ((rc = dev_alloc((void**)&d_sc, sizeof(MSScalars)))) return rc;
+  MH_CHECK(hipMemcpyAsync(d_sc, &h, sizeof h, hipMemcpyHostToDevice, c.stream));
+  int nb = (c.Vh + 255) / 256;
+  long iteration = 1;  // the initialisation counts as one (ks_multicg_offset.c:246)
+  int batch = mp.batch > 0 ? mp.batch : 16;
+  long max_iter = mp.max_iters;
+  while (iteration < max_iter) {
+    int k = (int)std::min<long>(batch, max_iter - iteration);
+    for (int it = 0; it < k; it++) {
+      hipLaunchKernelGGL(k_ms_dslash1<P>, dim3(nb), dim3(256), 0, c.stream, c.geom, L.fat, L.lng, p, tmp, pb ^ 1,
+                         d_sc);
+      hipLaunchKernelGGL(k_ms_dslash2<P>, dim3(nb), dim3(256), 0, c.stream, c.geom, L.fat, L.lng, tmp, p, ttt, pb,
+                         d_sc, c.ws.partials);
+      hipLaunchKernelGGL(k_ms_beta, dim3(1), dim3(256), 0, c.stream, c.ws.partials, nb, d_sc);
+      hipLaunchKernelGGL(k_ms_update<P>, dim3(nb), dim3(256), 0, c.stream, psim, pm, p, r, ttt, c.Vh, d_sc,
+                         c.ws.partials);
+      hipLaunchKernelGGL(k_ms_alpha, dim3(1), dim3(256), 0, c.stream, c.ws.partials, nb, d_sc);
+      hipLaunchKernelGGL(k_ms_p<P>, dim3(nb), dim3(256), 0, c.stream, p, r, c.Vh, d_sc);
+    }
+    MH_CHECK(hipGetLastError());
+    MH_CHECK(hipMemcpyAsync(&h, d_sc, sizeof h, hipMemcpyDeviceToHost, c.stream));
+    MH_CHECK(hipStreamSynchronize(c.stream));
+    iteration = 1 + h.iters;
+    if (h.halt) break;
+  }
+  for (int j = 0; j < n; j++)
+    if ((rc = convert_vec(psim.v[j], P, x64[j], P_F64))) return rc;
+  MH_CHECK(hipStreamSynchronize(c.stream));
+  st.iters = (int)iteration;
+  st.converged = h.converged;
+  st.final_rsq = h.rsq / source_norm;
+  st.size_r = st.final_rsq;
+  for (int j = 0; j < n; j++) {
+    free_vec(psim.v[j]);
+    free_vec(pm.v[j]);
+  }
+  free_vec(p); free_vec(r); free_vec(ttt); free_vec(tmp);
+  dev_free(d_sc);
+  return 0;
+}
+
+int multicg_solve(const VecField& src, std::vector<VecField>& x, const std::vector<double>& offsets,
+                  const MultiParams& mp, SolveStats& st) {
+  auto t0 = std::chrono::steady_clock::now();
+  int rc = mp.prec == P_F64 ? multicg_run<P_F64>(src, x, offsets, mp, st) : multicg_run<P_F32>(src, x, offsets, mp, st);
+  (void)hipStreamSynchronize(ctx().stream);
+  st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return rc;
+}
+
+}  // namespace mh

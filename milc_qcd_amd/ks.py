@@ -1,0 +1,306 @@
+"""Host-side mirror of MILC's generic_ks GPU interface over the C-ABI.
+
+Every call goes through ``libmilc_hip.so`` (include/milc_hip.h).  The Python
+names follow the MILC functions they stand in for, with the same argument
+meaning and the same error behaviour (non-convergence is reported in the
+returned control record, invalid arguments raise):
+
+    ks_congrad_parity      generic_ks/d_congrad5_fn_gpu.c:35  (-> milc_hip_invert)
+    ks_multicg_offset_field generic_ks/ks_multicg_offset_gpu.c:38 (-> milc_hip_multishift_invert)
+    dslash_fn_field        generic_ks/dslash_fn.c:306  (-> milc_hip_dslash)
+    compute_oprod          generic_ks/fermion_force_asqtad_gpu.c:60 (-> milc_hip_compute_oprod)
+    load_hisq_links        generic_ks/fermion_links_hisq_load_milc.c:546-702 (-> milc_hip_load_hisq_links)
+
+Host fields are numpy float64 arrays in MILC's own layout and site order:
+vectors (V, 3, 2), links (V, 4, 3, 3, 2), even sites first.
+"""
+import ctypes
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+EVEN, ODD, EVENANDODD = 2, 1, 3  # include/macros.h:17-19
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmilc_hip.so")
+_lib = None
+_active = None  # the Lattice whose geometry the library context currently holds
+
+
+class MilcHipError(RuntimeError):
+    pass
+
+
+class InvArgs(ctypes.Structure):
+    _fields_ = [("max_iter", ctypes.c_int), ("evenodd", ctypes.c_int), ("mixed_precision", ctypes.c_int),
+                ("naik_epsilon", ctypes.c_double), ("tadpole", ctypes.c_double),
+                ("restart_iters", ctypes.c_int)]
+
+
+EXPORTS = [
+    "milc_hip_init", "milc_hip_finalize", "milc_hip_version", "milc_hip_invert",
+    "milc_hip_multishift_invert", "milc_hip_dslash", "milc_hip_compute_oprod",
+    "milc_hip_load_hisq_links", "milc_hip_dev_gauge_hisq", "milc_hip_dev_source",
+    "milc_hip_dev_invert", "milc_hip_dev_read", "milc_hip_dev_write", "milc_hip_dev_read_links",
+    "milc_hip_dev_dslash_bench",
+]
+
+
+def load_library(path=LIB_PATH):
+    """Load the HIP library; raises if it has not been built (no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise MilcHipError(f"{path} is missing: build it with __graft_entry__.build() or "
+                           "make -C milc_qcd_amd/csrc")
+    L = ctypes.CDLL(path)
+    P, I, D = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+    L.milc_hip_init.argtypes = [ctypes.POINTER(ctypes.c_int), I]
+    L.milc_hip_version.restype = ctypes.c_char_p
+    L.milc_hip_invert.argtypes = [I, I, D, InvArgs, D, D, P, P, P, P, ctypes.POINTER(D), ctypes.POINTER(D),
+                                  ctypes.POINTER(I)]
+    L.milc_hip_multishift_invert.argtypes = [I, I, I, ctypes.POINTER(D), InvArgs, ctypes.POINTER(D),
+                                             ctypes.POINTER(D), P, P, P, ctypes.POINTER(P), ctypes.POINTER(D),
+                                             ctypes.POINTER(D), ctypes.POINTER(I)]
+    L.milc_hip_dslash.argtypes = [I, I, InvArgs, P, P, P, P, ctypes.POINTER(I)]
+    L.milc_hip_compute_oprod.argtypes = [I, I, ctypes.POINTER(ctypes.POINTER(D)), ctypes.POINTER(P),
+                                         ctypes.POINTER(P)]
+    L.milc_hip_load_hisq_links.argtypes = [I, P, P, P, P, ctypes.POINTER(I)]
+    L.milc_hip_dev_gauge_hisq.argtypes = [ctypes.c_uint64, D, ctypes.POINTER(I)]
+    L.milc_hip_dev_source.argtypes = [ctypes.c_uint64, I, I]
+    L.milc_hip_dev_invert.argtypes = [I, I, D, I, D, I, I, ctypes.POINTER(D)]
+    L.milc_hip_dev_read.argtypes = [I, I, P]
+    L.milc_hip_dev_write.argtypes = [I, I, P]
+    L.milc_hip_dev_read_links.argtypes = [P, P]
+    L.milc_hip_dev_dslash_bench.argtypes = [I, I, I, ctypes.POINTER(D)]
+    _lib = L
+    return L
+
+
+def _p(a):
+    if a.dtype != np.float64 or not a.flags.c_contiguous:
+        raise MilcHipError("fields must be C-contiguous float64 arrays")
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _check(rc, what):
+    if rc < 0:
+        raise MilcHipError(f"{what} failed with status {rc}")
+    return rc
+
+
+def _evenodd(parity):
+    if parity == EVEN:
+        return 0
+    if parity == ODD:
+        return 1
+    # the GPU glue rejects EVENANDODD (d_congrad5_fn_gpu.c:95-102)
+    raise MilcHipError("parity must be EVEN or ODD")
+
+
+@dataclass
+class QuarkInvertControl:
+    """Solver fields of quark_invert_control (include/generic_quark_types.h:167-190)."""
+    max: int = 1000
+    nrestart: int = 5
+    parity: int = EVEN
+    resid: float = 1e-8
+    relresid: float = 0.0
+    prec: int = 2
+    mixed: int = 0  # 0 plain, 1 FP32 sloppy, 2 16-bit sloppy (with FP64 reliable updates)
+    final_rsq: float = 0.0
+    final_relrsq: float = 0.0
+    size_r: float = 0.0
+    size_relr: float = 0.0
+    converged: int = 0
+    final_iters: int = 0
+    final_restart: int = 0
+
+
+@dataclass
+class KsParam:
+    """ks_param (include/generic_quark_types.h:131-139); offset includes 4 m^2."""
+    offset: float
+    residue: float = 1.0
+    mass: float = 0.0
+    naik_term_epsilon: float = 0.0
+
+
+class Lattice:
+    """One nx*ny*nz*nt lattice on the device (milc_hip_init)."""
+
+    def __init__(self, dims, device=0):
+        self.dims = tuple(int(d) for d in dims)
+        self.V = int(np.prod(self.dims))
+        self.Vh = self.V // 2
+        self.device = device
+        self.lib = load_library()
+        self._fresh = True
+        self._activate()
+
+    def _activate(self):
+        """The library holds one device context (one lattice) per process, as
+        MILC's GPU glue does; re-initialise it when another Lattice was used
+        since (device-resident fields and links of the other one are dropped)."""
+        global _active
+        if _active is self:
+            return
+        d = (ctypes.c_int * 4)(*self.dims)
+        _check(self.lib.milc_hip_init(d, self.device), "milc_hip_init")
+        _active = self
+        self._fresh = True
+
+    def close(self):
+        global _active
+        if _active is self:
+            self.lib.milc_hip_finalize()
+            _active = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def parity_slice(self, parity):
+        return slice(0, self.Vh) if parity == EVEN else slice(self.Vh, self.V)
+
+    # ---------------------------------------------------------------- links
+    def set_links(self, fat, lng):
+        """Hold the (fat, long) pair as the current fn links; the next call
+        uploads them (MILC's refresh signal, d_congrad5_fn_gpu.c:120-126)."""
+        self.fat = np.ascontiguousarray(fat, dtype=np.float64)
+        self.lng = np.ascontiguousarray(lng, dtype=np.float64)
+        self._fresh = True
+
+    def _refresh_flag(self):
+        n = ctypes.c_int(-1 if getattr(self, "_fresh", True) else 0)
+        self._fresh = False
+        return n
+
+    def load_hisq_links(self, U_phased):
+        """HISQ U(3) fat and Naik links from thin links with phases in."""
+        self._activate()
+        U = np.ascontiguousarray(U_phased, dtype=np.float64)
+        fat, lng, W = np.zeros_like(U), np.zeros_like(U), np.zeros_like(U)
+        nsvd = ctypes.c_int(0)
+        _check(self.lib.milc_hip_load_hisq_links(2, _p(U), _p(fat), _p(lng), _p(W), ctypes.byref(nsvd)),
+               "milc_hip_load_hisq_links")
+        return fat, lng, W, nsvd.value
+
+    # ---------------------------------------------------------------- operators
+    def dslash_fn_field(self, src, parity, prec=2):
+        """dest = D src on `parity` (src read on the other parity)."""
+        self._activate()
+        src = np.ascontiguousarray(src, dtype=np.float64)
+        dst = np.zeros_like(src)
+        args = InvArgs(0, _evenodd(parity), 0, 0.0, 1.0, 0)
+        _check(self.lib.milc_hip_dslash(2, prec, args, _p(self.fat), _p(self.lng), _p(src), _p(dst),
+                                        ctypes.byref(self._refresh_flag())), "milc_hip_dslash")
+        return dst
+
+    def ks_congrad_parity(self, src, dest, qic: QuarkInvertControl, mass):
+        """Solve (M^dag M) dest = src on qic.parity; dest is the initial guess."""
+        self._activate()
+        if qic.relresid != 0.0:
+            raise MilcHipError("Fermilab relative residual is not supported")
+        src = np.ascontiguousarray(src, dtype=np.float64)
+        if dest.dtype != np.float64 or not dest.flags.c_contiguous:
+            raise MilcHipError("dest must be a C-contiguous float64 array")
+        args = InvArgs(qic.max * qic.nrestart, _evenodd(qic.parity), qic.mixed, 0.0, 1.0, qic.max)
+        res, rel = ctypes.c_double(0), ctypes.c_double(0)
+        n = self._refresh_flag()
+        it = _check(self.lib.milc_hip_invert(2, qic.prec, mass, args, qic.resid, 0.0, _p(self.fat),
+                                             _p(self.lng), _p(src), _p(dest), ctypes.byref(res),
+                                             ctypes.byref(rel), ctypes.byref(n)), "milc_hip_invert")
+        qic.final_rsq = res.value ** 2
+        qic.final_relrsq = rel.value ** 2
+        qic.final_iters = it
+        qic.converged = int(res.value < qic.resid)
+        return it
+
+    def ks_multicg_offset_field(self, src, ksp, qic, mixed=0, prec=2):
+        """Multi-shift CG; returns the list of solutions psim[j]."""
+        self._activate()
+        src = np.ascontiguousarray(src, dtype=np.float64)
+        n = len(ksp)
+        psim = [np.zeros_like(src) for _ in range(n)]
+        offs = (ctypes.c_double * n)(*[k.offset for k in ksp])
+        resid = (ctypes.c_double * n)(*[q.resid for q in qic])
+        relr = (ctypes.c_double * n)(*[0.0] * n)
+        fres = (ctypes.c_double * n)()
+        frel = (ctypes.c_double * n)()
+        ptrs = (ctypes.c_void_p * n)(*[p.ctypes.data for p in psim])
+        args = InvArgs(qic[0].max * qic[0].nrestart, _evenodd(qic[0].parity), mixed, 0.0, 1.0, 0)
+        it = ctypes.c_int(-1 if getattr(self, "_fresh", True) else 0)
+        self._fresh = False
+        _check(self.lib.milc_hip_multishift_invert(2, prec, n, offs, args, resid, relr, _p(self.fat),
+                                                   _p(self.lng), _p(src), ptrs, fres, frel, ctypes.byref(it)),
+               "milc_hip_multishift_invert")
+        for j, q in enumerate(qic):
+            q.final_rsq = fres[j] ** 2
+            q.final_iters = it.value
+            q.converged = int(fres[j] <= max(q.resid, 1e-14) * 1.0000001) if mixed else 1
+        return psim
+
+    def compute_oprod(self, multi_x, coeff):
+        """O1, O3 (each (V, 4, 3, 3, 2)) of the level-0 force; coeff[t] = (c1, c3)."""
+        self._activate()
+        n = len(multi_x)
+        xs = [np.ascontiguousarray(x, dtype=np.float64) for x in multi_x]
+        rows = [(ctypes.c_double * 2)(*c) for c in coeff]
+        cptr = (ctypes.POINTER(ctypes.c_double) * n)(*[ctypes.cast(r, ctypes.POINTER(ctypes.c_double))
+                                                       for r in rows])
+        qptr = (ctypes.c_void_p * n)(*[x.ctypes.data for x in xs])
+        o1 = [np.zeros((self.V, 3, 3, 2)) for _ in range(4)]
+        o3 = [np.zeros((self.V, 3, 3, 2)) for _ in range(4)]
+        d1 = (ctypes.c_void_p * 4)(*[a.ctypes.data for a in o1])
+        d3 = (ctypes.c_void_p * 4)(*[a.ctypes.data for a in o3])
+        op = (ctypes.c_void_p * 2)(ctypes.cast(d1, ctypes.c_void_p), ctypes.cast(d3, ctypes.c_void_p))
+        _check(self.lib.milc_hip_compute_oprod(2, n, cptr, qptr, op), "milc_hip_compute_oprod")
+        return np.stack(o1, axis=1), np.stack(o3, axis=1)
+
+    # ---------------------------------------------------------------- device-resident API
+    def dev_gauge_hisq(self, seed, scale=0.4):
+        self._activate()
+        nsvd = ctypes.c_int(0)
+        _check(self.lib.milc_hip_dev_gauge_hisq(seed, scale, ctypes.byref(nsvd)), "milc_hip_dev_gauge_hisq")
+        return nsvd.value
+
+    def dev_source(self, seed, parity, slot):
+        self._activate()
+        _check(self.lib.milc_hip_dev_source(seed, parity, slot), "milc_hip_dev_source")
+
+    def dev_invert(self, slot_b, slot_x, mass, parity, resid, mixed, max_iter=20000):
+        self._activate()
+        st = (ctypes.c_double * 6)()
+        it = _check(self.lib.milc_hip_dev_invert(slot_b, slot_x, mass, parity, resid, mixed, max_iter, st),
+                    "milc_hip_dev_invert")
+        return it, {"resid": st[0], "seconds": st[1], "reliable_updates": int(st[2]), "converged": int(st[3]),
+                    "iters": int(st[4]), "final_rsq": st[5]}
+
+    def dev_read(self, slot, parity):
+        self._activate()
+        out = np.zeros((self.V, 3, 2))
+        _check(self.lib.milc_hip_dev_read(slot, parity, _p(out)), "milc_hip_dev_read")
+        return out
+
+    def dev_write(self, slot, parity, v):
+        self._activate()
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        _check(self.lib.milc_hip_dev_write(slot, parity, _p(v)), "milc_hip_dev_write")
+
+    def dev_read_links(self):
+        self._activate()
+        fat = np.zeros((self.V, 4, 3, 3, 2))
+        lng = np.zeros((self.V, 4, 3, 3, 2))
+        _check(self.lib.milc_hip_dev_read_links(_p(fat), _p(lng)), "milc_hip_dev_read_links")
+        return fat, lng
+
+    def dev_dslash_bench(self, prec, parity, reps):
+        self._activate()
+        us = ctypes.c_double(0)
+        _check(self.lib.milc_hip_dev_dslash_bench(prec, parity, reps, ctypes.byref(us)), "milc_hip_dev_dslash_bench")
+        return us.value
