@@ -24,6 +24,7 @@
 #ifndef MILC_HIP_H
 #define MILC_HIP_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -137,6 +138,11 @@ int milc_hip_dev_read_links(double* fat, double* lng);
 /* Time `reps` Dslash applications on device slots (HIP events on the
  * library stream); returns the average in *usec. */
 int milc_hip_dev_dslash_bench(int prec, int parity, int reps, double* usec);
+/* STREAM-style device copy of `bytes` bytes, `reps` times (HIP events):
+ * the measured HBM reference next to the vendor peak. */
+int milc_hip_dev_copy_bench(size_t bytes, int reps, double* usec);
+/* |a - b| / |b| over the parity half held in two device slots. */
+int milc_hip_dev_diff(int slot_a, int slot_b, double* rel);
 
 #ifdef __cplusplus
 }

@@ -279,7 +279,7 @@ int milc_hip_dev_invert(int slot_b, int slot_x, double mass, int parity, double 
   SolveParams sp;
   sp.parity = parity;
   sp.max_iters = max_iter;
-  sp.max_restarts = 1;
+  sp.max_restarts = 5;
   sp.resid = resid;
   sp.relresid = 0;
   sp.sloppy_prec = ps;
@@ -548,5 +548,96 @@ extern "C" int milc_hip_compute_oprod(int precision, int num_terms, double** coe
     free_vec(xe[t]);
     free_vec(xo[t]);
   }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// measurement helpers for the benchmark
+// ---------------------------------------------------------------------------
+namespace {
+__global__ void k_copy(const float4* __restrict__ a, float4* __restrict__ b, size_t n) {
+  size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) b[i] = a[i];
+}
+__global__ void k_diff_norm(mh::VecField a, mh::VecField b, int Vh, double* out) {
+  int cb = blockIdx.x * blockDim.x + threadIdx.x;
+  double s[2] = {0, 0};
+  if (cb < Vh) {
+    mh::Vec3<double> x, y;
+    mh::load_vec<mh::P_F64>(a, cb, x);
+    mh::load_vec<mh::P_F64>(b, cb, y);
+    for (int k = 0; k < 3; k++) {
+      double dr = x.c[k].re - y.c[k].re, di = x.c[k].im - y.c[k].im;
+      s[0] += dr * dr + di * di;
+      s[1] += y.c[k].re * y.c[k].re + y.c[k].im * y.c[k].im;
+    }
+  }
+  for (int k = 0; k < 2; k++)
+    for (int off = 32; off > 0; off >>= 1) s[k] += __shfl_xor(s[k], off, 64);
+  __shared__ double sm[2][16];
+  if ((threadIdx.x & 63) == 0) {
+    sm[0][threadIdx.x >> 6] = s[0];
+    sm[1][threadIdx.x >> 6] = s[1];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t0 = 0, t1 = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) {
+      t0 += sm[0][w];
+      t1 += sm[1][w];
+    }
+    out[2 * blockIdx.x] = t0;
+    out[2 * blockIdx.x + 1] = t1;
+  }
+}
+}  // namespace
+
+extern "C" int milc_hip_dev_copy_bench(size_t bytes, int reps, double* usec) {
+  Context& c = ctx();
+  if (!c.initialised || reps <= 0 || bytes < 16) return -1;
+  size_t n = bytes / 16;
+  float4 *a = nullptr, *b = nullptr;
+  int rc;
+  if ((rc = dev_alloc((void**)&a, n * 16)) || (rc = dev_alloc((void**)&b, n * 16))) return rc;
+  MH_CHECK(hipMemsetAsync(a, 0, n * 16, c.stream));
+  int nb = 256 * 64;
+  for (int i = 0; i < 2; i++) hipLaunchKernelGGL(k_copy, dim3(nb), dim3(256), 0, c.stream, a, b, n);
+  hipEvent_t e0, e1;
+  MH_CHECK(hipEventCreate(&e0));
+  MH_CHECK(hipEventCreate(&e1));
+  MH_CHECK(hipEventRecord(e0, c.stream));
+  for (int i = 0; i < reps; i++) hipLaunchKernelGGL(k_copy, dim3(nb), dim3(256), 0, c.stream, a, b, n);
+  MH_CHECK(hipEventRecord(e1, c.stream));
+  MH_CHECK(hipEventSynchronize(e1));
+  float ms = 0;
+  MH_CHECK(hipEventElapsedTime(&ms, e0, e1));
+  *usec = 1000.0 * ms / reps;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  dev_free(a);
+  dev_free(b);
+  return 0;
+}
+
+extern "C" int milc_hip_dev_diff(int slot_a, int slot_b, double* rel) {
+  Context& c = ctx();
+  if (!c.initialised || slot_a < 0 || slot_a >= kSlots || slot_b < 0 || slot_b >= kSlots || !g_slots[slot_a].data ||
+      !g_slots[slot_b].data)
+    return -1;
+  int nb = (c.Vh + 255) / 256;
+  if (nb * 2 > c.ws.max_blocks * 8) return -2;
+  hipLaunchKernelGGL(k_diff_norm, dim3(nb), dim3(256), 0, c.stream, g_slots[slot_a], g_slots[slot_b], c.Vh,
+                     c.ws.partials);
+  MH_CHECK(hipGetLastError());
+  std::vector<double> h((size_t)nb * 2);
+  MH_CHECK(hipMemcpyAsync(h.data(), c.ws.partials, h.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  MH_CHECK(hipStreamSynchronize(c.stream));
+  double d = 0, n = 0;
+  for (int i = 0; i < nb; i++) {
+    d += h[2 * i];
+    n += h[2 * i + 1];
+  }
+  *rel = n > 0 ? std::sqrt(d / n) : 0.0;
   return 0;
 }

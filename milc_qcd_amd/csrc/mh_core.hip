@@ -781,7 +781,9 @@ static int cg_solve_f64(const VecField& src, VecField& x, double mass, const Sol
   st.size_r = size_r;
   st.iters = (int)iteration;
   st.restarts = nrestart;
-  st.converged = (nrestart == max_restarts || iteration == max_cg) ? 0 : 1;
+  // converged = the FP64 true residual met the target (the GPU glue's test,
+  // d_congrad5_fn_gpu.c:153)
+  st.converged = rsqmin > final_rsq ? 1 : 0;
   free_vec(ttt); free_vec(p); free_vec(r); free_vec(tmp);
   return rc;
 }

@@ -43,7 +43,7 @@ EXPORTS = [
     "milc_hip_multishift_invert", "milc_hip_dslash", "milc_hip_compute_oprod",
     "milc_hip_load_hisq_links", "milc_hip_dev_gauge_hisq", "milc_hip_dev_source",
     "milc_hip_dev_invert", "milc_hip_dev_read", "milc_hip_dev_write", "milc_hip_dev_read_links",
-    "milc_hip_dev_dslash_bench",
+    "milc_hip_dev_dslash_bench", "milc_hip_dev_copy_bench", "milc_hip_dev_diff",
 ]
 
 
@@ -75,6 +75,8 @@ def load_library(path=LIB_PATH):
     L.milc_hip_dev_write.argtypes = [I, I, P]
     L.milc_hip_dev_read_links.argtypes = [P, P]
     L.milc_hip_dev_dslash_bench.argtypes = [I, I, I, ctypes.POINTER(D)]
+    L.milc_hip_dev_copy_bench.argtypes = [ctypes.c_size_t, I, ctypes.POINTER(D)]
+    L.milc_hip_dev_diff.argtypes = [I, I, ctypes.POINTER(D)]
     _lib = L
     return L
 
@@ -304,3 +306,15 @@ class Lattice:
         us = ctypes.c_double(0)
         _check(self.lib.milc_hip_dev_dslash_bench(prec, parity, reps, ctypes.byref(us)), "milc_hip_dev_dslash_bench")
         return us.value
+
+    def dev_copy_bench(self, nbytes, reps):
+        self._activate()
+        us = ctypes.c_double(0)
+        _check(self.lib.milc_hip_dev_copy_bench(nbytes, reps, ctypes.byref(us)), "milc_hip_dev_copy_bench")
+        return us.value
+
+    def dev_diff(self, slot_a, slot_b):
+        self._activate()
+        r = ctypes.c_double(0)
+        _check(self.lib.milc_hip_dev_diff(slot_a, slot_b, ctypes.byref(r)), "milc_hip_dev_diff")
+        return r.value
