@@ -161,150 +161,254 @@ __device__ __forceinline__ void amv_sub(const Mat3<R>& m, const Vec3<R>& v, Vec3
 }
 
 // ---------------------------------------------------------------------------
-// Precision formats of device fields.
-//   F64: double2 per complex.
-//   F32: float2 per complex.
-//   H16: 16-bit fixed point (int16, value = q / 32767 * scale).  Links share
-//        one scale per field (max |element|); vectors carry one float norm per
-//        site (max |component| of that site), stored in a separate norm plane.
+// Precision formats of device fields.  Every per-site record is read with
+// 16-byte loads where its size allows, and link loads are non-temporal: each
+// link is used once per Dslash, so it should not displace the colour vectors
+// (read 16 times each) from L2 and the Infinity Cache.
+//
+//   F64  vector: 3 planes of double2 (one per colour).
+//        link  : 9 planes of double2 per (parity, dir).
+//   F32  vector: one float4 plane (colours 0,1) + one float2 plane (colour 2).
+//        link  : 4 float4 planes (elements 0..7) + 1 float2 plane (element 8)
+//                per (parity, dir); the float2 planes follow all float4 planes.
+//   H16  16-bit fixed point, value = q/32767 * scale.
+//        vector: one 16-byte record per site: 3 complex as (short,short) words
+//                and the site's float norm (max |component|) in the last word.
+//        link  : 2 int4 planes (elements 0..7) + 1 int plane (element 8) per
+//                (parity, dir); one scale per field (max |element|).
 // ---------------------------------------------------------------------------
 enum Prec { P_F64 = 0, P_F32 = 1, P_H16 = 2 };
 
 template <int P> struct PrecTraits;
 template <> struct PrecTraits<P_F64> {
-  typedef double Real;      // arithmetic type
-  typedef double2 Store;    // storage of one complex
-  static constexpr int bytes_per_complex = 16;
+  typedef double Real;
+  static constexpr int vec_bytes = 48;
+  static constexpr int link_bytes = 144;
 };
 template <> struct PrecTraits<P_F32> {
   typedef float Real;
-  typedef float2 Store;
-  static constexpr int bytes_per_complex = 8;
+  static constexpr int vec_bytes = 24;
+  static constexpr int link_bytes = 72;
 };
 template <> struct PrecTraits<P_H16> {
   typedef float Real;
-  typedef short2 Store;
-  static constexpr int bytes_per_complex = 4;
+  static constexpr int vec_bytes = 16;
+  static constexpr int link_bytes = 36;
 };
+inline int vec_site_bytes(int prec) { return prec == P_F64 ? 48 : (prec == P_F32 ? 24 : 16); }
+inline int link_site_bytes(int prec) { return prec == P_F64 ? 144 : (prec == P_F32 ? 72 : 36); }
 
-// A colour-vector field of one parity in SoA layout: plane k (k = colour) at
-// data + k*stride; H16 fields also have a norm plane.
+// A colour-vector field of one parity (stride = sites per plane, >= Vh).
 struct VecField {
   void* data;
-  float* norm;   // H16 only
-  int stride;    // elements per plane (>= Vh)
+  float* norm;   // unused (the 16-bit norm lives in the site record)
+  int stride;
 };
 
-// A link field (fat or long) for both parities: element plane e (0..8) of
-// direction mu and parity bit pb at data + ((pb*4 + mu)*9 + e)*stride.
+// A link field (fat or long) for both parities and all four directions.
 struct LinkField {
   void* data;
   float scale;  // H16: value = q * scale / 32767
   int stride;
 };
 
-__device__ __forceinline__ float h16_to_f(short q) { return (float)q; }
-__device__ __forceinline__ short f_to_h16(float v) {
-  // v in [-1, 1] after normalisation; round to nearest
-  float s = v * 32767.0f;
-  s = fminf(fmaxf(s, -32767.0f), 32767.0f);
-  return (short)__float2int_rn(s);
+typedef double nat_d2 __attribute__((ext_vector_type(2)));
+typedef float nat_f2 __attribute__((ext_vector_type(2)));
+typedef float nat_f4 __attribute__((ext_vector_type(4)));
+typedef int nat_i4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ nat_d2 ldnt(const double2* p) { return __builtin_nontemporal_load((const nat_d2*)p); }
+__device__ __forceinline__ nat_f4 ldnt(const float4* p) { return __builtin_nontemporal_load((const nat_f4*)p); }
+__device__ __forceinline__ nat_f2 ldnt(const float2* p) { return __builtin_nontemporal_load((const nat_f2*)p); }
+__device__ __forceinline__ nat_i4 ldnt(const int4* p) { return __builtin_nontemporal_load((const nat_i4*)p); }
+__device__ __forceinline__ int ldnt(const int* p) { return __builtin_nontemporal_load(p); }
+
+__device__ __forceinline__ float q16lo(int w) { return (float)(short)(w & 0xffff); }
+__device__ __forceinline__ float q16hi(int w) { return (float)(w >> 16); }
+__device__ __forceinline__ int q16pack(float a, float b) {
+  int qa = __float2int_rn(fminf(fmaxf(a * 32767.0f, -32767.0f), 32767.0f));
+  int qb = __float2int_rn(fminf(fmaxf(b * 32767.0f, -32767.0f), 32767.0f));
+  return (qa & 0xffff) | (qb << 16);
 }
 
+// ---- vectors ----------------------------------------------------------------
 template <int P>
 __device__ __forceinline__ void load_vec(const VecField& f, int cb, Vec3<typename PrecTraits<P>::Real>& v);
-
 template <>
 __device__ __forceinline__ void load_vec<P_F64>(const VecField& f, int cb, Vec3<double>& v) {
   const double2* p = (const double2*)f.data;
 #pragma unroll
   for (int k = 0; k < 3; k++) {
-    double2 a = p[k * f.stride + cb];
+    double2 a = p[(size_t)k * f.stride + cb];
     v.c[k].re = a.x; v.c[k].im = a.y;
   }
 }
 template <>
 __device__ __forceinline__ void load_vec<P_F32>(const VecField& f, int cb, Vec3<float>& v) {
-  const float2* p = (const float2*)f.data;
-#pragma unroll
-  for (int k = 0; k < 3; k++) {
-    float2 a = p[k * f.stride + cb];
-    v.c[k].re = a.x; v.c[k].im = a.y;
-  }
+  float4 a = ((const float4*)f.data)[cb];
+  float2 b = ((const float2*)((const float4*)f.data + f.stride))[cb];
+  v.c[0].re = a.x; v.c[0].im = a.y; v.c[1].re = a.z; v.c[1].im = a.w; v.c[2].re = b.x; v.c[2].im = b.y;
 }
 template <>
 __device__ __forceinline__ void load_vec<P_H16>(const VecField& f, int cb, Vec3<float>& v) {
-  const short2* p = (const short2*)f.data;
-  float n = f.norm[cb] * (1.0f / 32767.0f);
-#pragma unroll
-  for (int k = 0; k < 3; k++) {
-    short2 a = p[k * f.stride + cb];
-    v.c[k].re = h16_to_f(a.x) * n; v.c[k].im = h16_to_f(a.y) * n;
-  }
+  int4 a = ((const int4*)f.data)[cb];
+  float n = __int_as_float(a.w) * (1.0f / 32767.0f);
+  v.c[0].re = q16lo(a.x) * n; v.c[0].im = q16hi(a.x) * n;
+  v.c[1].re = q16lo(a.y) * n; v.c[1].im = q16hi(a.y) * n;
+  v.c[2].re = q16lo(a.z) * n; v.c[2].im = q16hi(a.z) * n;
 }
 
 template <int P>
 __device__ __forceinline__ void store_vec(const VecField& f, int cb, const Vec3<typename PrecTraits<P>::Real>& v);
-
 template <>
 __device__ __forceinline__ void store_vec<P_F64>(const VecField& f, int cb, const Vec3<double>& v) {
   double2* p = (double2*)f.data;
 #pragma unroll
-  for (int k = 0; k < 3; k++) p[k * f.stride + cb] = make_double2(v.c[k].re, v.c[k].im);
+  for (int k = 0; k < 3; k++) p[(size_t)k * f.stride + cb] = make_double2(v.c[k].re, v.c[k].im);
 }
 template <>
 __device__ __forceinline__ void store_vec<P_F32>(const VecField& f, int cb, const Vec3<float>& v) {
-  float2* p = (float2*)f.data;
-#pragma unroll
-  for (int k = 0; k < 3; k++) p[k * f.stride + cb] = make_float2(v.c[k].re, v.c[k].im);
+  ((float4*)f.data)[cb] = make_float4(v.c[0].re, v.c[0].im, v.c[1].re, v.c[1].im);
+  ((float2*)((float4*)f.data + f.stride))[cb] = make_float2(v.c[2].re, v.c[2].im);
 }
 template <>
 __device__ __forceinline__ void store_vec<P_H16>(const VecField& f, int cb, const Vec3<float>& v) {
-  short2* p = (short2*)f.data;
   float m = 0.f;
 #pragma unroll
   for (int k = 0; k < 3; k++) m = fmaxf(m, fmaxf(fabsf(v.c[k].re), fabsf(v.c[k].im)));
   float inv = m > 0.f ? 1.0f / m : 0.f;
-  f.norm[cb] = m;
-#pragma unroll
-  for (int k = 0; k < 3; k++) {
-    short2 q;
-    q.x = f_to_h16(v.c[k].re * inv);
-    q.y = f_to_h16(v.c[k].im * inv);
-    p[k * f.stride + cb] = q;
-  }
+  int4 o;
+  o.x = q16pack(v.c[0].re * inv, v.c[0].im * inv);
+  o.y = q16pack(v.c[1].re * inv, v.c[1].im * inv);
+  o.z = q16pack(v.c[2].re * inv, v.c[2].im * inv);
+  o.w = __float_as_int(m);
+  ((int4*)f.data)[cb] = o;
 }
 
+// ---- links --------------------------------------------------------------------
 template <int P>
 __device__ __forceinline__ void load_link(const LinkField& f, int pb, int mu, int cb,
                                           Mat3<typename PrecTraits<P>::Real>& m);
-
 template <>
 __device__ __forceinline__ void load_link<P_F64>(const LinkField& f, int pb, int mu, int cb, Mat3<double>& m) {
   const double2* p = (const double2*)f.data + (size_t)((pb * 4 + mu) * 9) * f.stride + cb;
 #pragma unroll
   for (int e = 0; e < 9; e++) {
-    double2 a = p[(size_t)e * f.stride];
+    nat_d2 a = ldnt(p + (size_t)e * f.stride);
     m.e[e / 3][e % 3].re = a.x; m.e[e / 3][e % 3].im = a.y;
   }
 }
 template <>
 __device__ __forceinline__ void load_link<P_F32>(const LinkField& f, int pb, int mu, int cb, Mat3<float>& m) {
-  const float2* p = (const float2*)f.data + (size_t)((pb * 4 + mu) * 9) * f.stride + cb;
+  const float4* p4 = (const float4*)f.data + (size_t)((pb * 4 + mu) * 4) * f.stride + cb;
+  const float2* p2 = (const float2*)((const float4*)f.data + (size_t)32 * f.stride) + (size_t)(pb * 4 + mu) * f.stride + cb;
+  float r[18];
 #pragma unroll
-  for (int e = 0; e < 9; e++) {
-    float2 a = p[(size_t)e * f.stride];
-    m.e[e / 3][e % 3].re = a.x; m.e[e / 3][e % 3].im = a.y;
+  for (int ch = 0; ch < 4; ch++) {
+    nat_f4 a = ldnt(p4 + (size_t)ch * f.stride);
+    r[4 * ch] = a.x; r[4 * ch + 1] = a.y; r[4 * ch + 2] = a.z; r[4 * ch + 3] = a.w;
   }
+  nat_f2 b = ldnt(p2);
+  r[16] = b.x; r[17] = b.y;
+#pragma unroll
+  for (int e = 0; e < 9; e++) { m.e[e / 3][e % 3].re = r[2 * e]; m.e[e / 3][e % 3].im = r[2 * e + 1]; }
 }
 template <>
 __device__ __forceinline__ void load_link<P_H16>(const LinkField& f, int pb, int mu, int cb, Mat3<float>& m) {
-  const short2* p = (const short2*)f.data + (size_t)((pb * 4 + mu) * 9) * f.stride + cb;
+  const int4* p4 = (const int4*)f.data + (size_t)((pb * 4 + mu) * 2) * f.stride + cb;
+  const int* p1 = (const int*)((const int4*)f.data + (size_t)16 * f.stride) + (size_t)(pb * 4 + mu) * f.stride + cb;
   float s = f.scale * (1.0f / 32767.0f);
+  int w[9];
 #pragma unroll
-  for (int e = 0; e < 9; e++) {
-    short2 a = p[(size_t)e * f.stride];
-    m.e[e / 3][e % 3].re = h16_to_f(a.x) * s; m.e[e / 3][e % 3].im = h16_to_f(a.y) * s;
+  for (int ch = 0; ch < 2; ch++) {
+    nat_i4 a = ldnt(p4 + (size_t)ch * f.stride);
+    w[4 * ch] = a.x; w[4 * ch + 1] = a.y; w[4 * ch + 2] = a.z; w[4 * ch + 3] = a.w;
+  }
+  w[8] = ldnt(p1);
+#pragma unroll
+  for (int e = 0; e < 9; e++) { m.e[e / 3][e % 3].re = q16lo(w[e]) * s; m.e[e / 3][e % 3].im = q16hi(w[e]) * s; }
+}
+
+// Store one link; for H16 `inv_scale` = 1/scale of the destination field.
+template <int P, typename R>
+__device__ __forceinline__ void store_link(const LinkField& f, int pb, int mu, int cb, const Mat3<R>& m,
+                                           float inv_scale);
+template <>
+__device__ __forceinline__ void store_link<P_F64, double>(const LinkField& f, int pb, int mu, int cb,
+                                                          const Mat3<double>& m, float) {
+  double2* p = (double2*)f.data + (size_t)((pb * 4 + mu) * 9) * f.stride + cb;
+#pragma unroll
+  for (int e = 0; e < 9; e++) p[(size_t)e * f.stride] = make_double2(m.e[e / 3][e % 3].re, m.e[e / 3][e % 3].im);
+}
+template <>
+__device__ __forceinline__ void store_link<P_F32, double>(const LinkField& f, int pb, int mu, int cb,
+                                                          const Mat3<double>& m, float) {
+  float4* p4 = (float4*)f.data + (size_t)((pb * 4 + mu) * 4) * f.stride + cb;
+  float2* p2 = (float2*)((float4*)f.data + (size_t)32 * f.stride) + (size_t)(pb * 4 + mu) * f.stride + cb;
+  float r[18];
+#pragma unroll
+  for (int e = 0; e < 9; e++) { r[2 * e] = (float)m.e[e / 3][e % 3].re; r[2 * e + 1] = (float)m.e[e / 3][e % 3].im; }
+#pragma unroll
+  for (int ch = 0; ch < 4; ch++) p4[(size_t)ch * f.stride] = make_float4(r[4 * ch], r[4 * ch + 1], r[4 * ch + 2], r[4 * ch + 3]);
+  *p2 = make_float2(r[16], r[17]);
+}
+template <>
+__device__ __forceinline__ void store_link<P_H16, double>(const LinkField& f, int pb, int mu, int cb,
+                                                          const Mat3<double>& m, float inv_scale) {
+  int4* p4 = (int4*)f.data + (size_t)((pb * 4 + mu) * 2) * f.stride + cb;
+  int* p1 = (int*)((int4*)f.data + (size_t)16 * f.stride) + (size_t)(pb * 4 + mu) * f.stride + cb;
+  int w[9];
+#pragma unroll
+  for (int e = 0; e < 9; e++)
+    w[e] = q16pack((float)(m.e[e / 3][e % 3].re * inv_scale), (float)(m.e[e / 3][e % 3].im * inv_scale));
+  p4[0] = make_int4(w[0], w[1], w[2], w[3]);
+  p4[f.stride] = make_int4(w[4], w[5], w[6], w[7]);
+  *p1 = w[8];
+}
+
+// Sum of NV values over `nblocks` block records parts[b*4 + k] (k < NV) by
+// one block: thread t takes records t, t+blockDim, ... with four independent
+// 32-byte loads in flight per pass.  Fixed association order, so the result
+// is bit-reproducible.  Only thread 0 holds the result.
+template <int NV>
+__device__ __forceinline__ void reduce_records(const double* __restrict__ parts, int nblocks, double (&out)[NV]) {
+  double acc[NV];
+#pragma unroll
+  for (int k = 0; k < NV; k++) acc[k] = 0.0;
+  const double4* p4 = (const double4*)parts;
+  const int bd = blockDim.x;
+  for (int base = threadIdx.x; base < nblocks; base += 4 * bd) {
+    double4 r[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      int i = base + u * bd;
+      r[u] = i < nblocks ? p4[i] : make_double4(0.0, 0.0, 0.0, 0.0);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      acc[0] += r[u].x;
+      if (NV > 1) acc[NV > 1 ? 1 : 0] += r[u].y;
+      if (NV > 2) acc[NV > 2 ? 2 : 0] += r[u].z;
+      if (NV > 3) acc[NV > 3 ? 3 : 0] += r[u].w;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NV; k++)
+    for (int off = 32; off > 0; off >>= 1) acc[k] += __shfl_xor(acc[k], off, 64);
+  __shared__ double sm[NV][16];
+  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < NV; k++) sm[k][wid] = acc[k];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+      double t = 0.0;
+      for (int w = 0; w < (bd >> 6); w++) t += sm[k][w];
+      out[k] = t;
+    }
   }
 }
 

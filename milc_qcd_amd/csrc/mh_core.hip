@@ -9,6 +9,7 @@
 #include <cstring>
 #include <vector>
 
+#include "mh_dslash.h"
 #include "mh_internal.h"
 
 namespace mh {
@@ -16,7 +17,8 @@ namespace mh {
 static Context g_ctx;
 Context& ctx() { return g_ctx; }
 
-size_t prec_complex_bytes(int prec) { return prec == P_F64 ? 16 : (prec == P_F32 ? 8 : 4); }
+size_t vec_bytes(int prec, int stride) { return (size_t)vec_site_bytes(prec) * stride; }
+size_t link_bytes(int prec, int stride) { return (size_t)8 * link_site_bytes(prec) * stride; }
 
 int dev_alloc(void** p, size_t bytes) {
   *p = nullptr;
@@ -31,26 +33,19 @@ void dev_free(void* p) {
 int alloc_vec(VecField& v, int prec, int stride) {
   v.stride = stride;
   v.norm = nullptr;
-  int rc = dev_alloc(&v.data, 3 * (size_t)stride * prec_complex_bytes(prec));
+  int rc = dev_alloc(&v.data, vec_bytes(prec, stride));
   if (rc) return rc;
-  if (prec == P_H16) {
-    rc = dev_alloc((void**)&v.norm, (size_t)stride * sizeof(float));
-    if (rc) return rc;
-  }
-  MH_CHECK(hipMemsetAsync(v.data, 0, 3 * (size_t)stride * prec_complex_bytes(prec), ctx().stream));
-  if (v.norm) MH_CHECK(hipMemsetAsync(v.norm, 0, (size_t)stride * sizeof(float), ctx().stream));
+  MH_CHECK(hipMemsetAsync(v.data, 0, vec_bytes(prec, stride), ctx().stream));
   return 0;
 }
 void free_vec(VecField& v) {
   dev_free(v.data);
-  dev_free(v.norm);
   v.data = nullptr;
-  v.norm = nullptr;
 }
 int alloc_links(LinkField& l, int prec, int stride) {
   l.stride = stride;
   l.scale = 1.0f;
-  return dev_alloc(&l.data, 2ull * 4 * 9 * (size_t)stride * prec_complex_bytes(prec));
+  return dev_alloc(&l.data, link_bytes(prec, stride));
 }
 void free_links(LinkField& l) {
   dev_free(l.data);
@@ -97,20 +92,13 @@ __global__ void k_links_from_milc(const double* __restrict__ aos, LinkField out,
   if (cb >= Vh) return;
   int pb = pm >> 2, mu = pm & 3;
   const double* s = aos + 18 * (4 * ((size_t)pb * Vh + cb) + mu);
-  size_t base = (size_t)pm * 9 * out.stride + cb;
+  Mat3<double> m;
+#pragma unroll
   for (int e = 0; e < 9; e++) {
-    double re = s[2 * e], im = s[2 * e + 1];
-    if (P == P_F64) {
-      ((double2*)out.data)[base + (size_t)e * out.stride] = make_double2(re, im);
-    } else if (P == P_F32) {
-      ((float2*)out.data)[base + (size_t)e * out.stride] = make_float2((float)re, (float)im);
-    } else {
-      short2 q;
-      q.x = f_to_h16((float)(re * inv_scale));
-      q.y = f_to_h16((float)(im * inv_scale));
-      ((short2*)out.data)[base + (size_t)e * out.stride] = q;
-    }
+    m.e[e / 3][e % 3].re = s[2 * e];
+    m.e[e / 3][e % 3].im = s[2 * e + 1];
   }
+  store_link<P, double>(out, pb, mu, cb, m, inv_scale);
 }
 template <int P>
 __global__ void k_links_to_milc(LinkField in, double* __restrict__ aos, int Vh) {
@@ -152,18 +140,13 @@ __global__ void k_convert_links(LinkField in, LinkField out, int Vh, float inv_s
   int pb = pm >> 2, mu = pm & 3;
   Mat3<typename PrecTraits<PI>::Real> m;
   load_link<PI>(in, pb, mu, cb, m);
-  size_t base = (size_t)pm * 9 * out.stride + cb;
+  Mat3<double> d;
+#pragma unroll
   for (int e = 0; e < 9; e++) {
-    double re = (double)m.e[e / 3][e % 3].re, im = (double)m.e[e / 3][e % 3].im;
-    if (PO == P_F64) ((double2*)out.data)[base + (size_t)e * out.stride] = make_double2(re, im);
-    else if (PO == P_F32) ((float2*)out.data)[base + (size_t)e * out.stride] = make_float2((float)re, (float)im);
-    else {
-      short2 q;
-      q.x = f_to_h16((float)(re * inv_scale));
-      q.y = f_to_h16((float)(im * inv_scale));
-      ((short2*)out.data)[base + (size_t)e * out.stride] = q;
-    }
+    d.e[e / 3][e % 3].re = (double)m.e[e / 3][e % 3].re;
+    d.e[e / 3][e % 3].im = (double)m.e[e / 3][e % 3].im;
   }
+  store_link<PO, double>(out, pb, mu, cb, d, inv_scale);
 }
 
 template <int PI, int PO>
@@ -335,42 +318,6 @@ int convert_vec(const VecField& in, int pin, VecField& out, int pout) {
 // One thread per output site; the epilogue variants fuse the CG algebra.
 // ===========================================================================
 template <int P>
-__device__ __forceinline__ void dslash_site(const Geom& g, const LinkField& fat, const LinkField& lng,
-                                            const VecField& src, int cb, int pb,
-                                            Vec3<typename PrecTraits<P>::Real>& acc) {
-  typedef typename PrecTraits<P>::Real R;
-  Coord c = site_coords(g, cb, pb);
-  int po = pb ^ 1;
-  zero(acc);
-#pragma unroll
-  for (int mu = 0; mu < 4; mu++) {
-    Mat3<R> m;
-    Vec3<R> v;
-    int n;
-    // forward one hop
-    n = neighbour(g, c, cb, mu, 1);
-    load_link<P>(fat, pb, mu, cb, m);
-    load_vec<P>(src, n, v);
-    mv_acc(m, v, acc);
-    // forward three hops
-    n = neighbour(g, c, cb, mu, 3);
-    load_link<P>(lng, pb, mu, cb, m);
-    load_vec<P>(src, n, v);
-    mv_acc(m, v, acc);
-    // backward one hop: F_mu(x-mu) lives on the other parity
-    n = neighbour(g, c, cb, mu, -1);
-    load_link<P>(fat, po, mu, n, m);
-    load_vec<P>(src, n, v);
-    amv_sub(m, v, acc);
-    // backward three hops
-    n = neighbour(g, c, cb, mu, -3);
-    load_link<P>(lng, po, mu, n, m);
-    load_vec<P>(src, n, v);
-    amv_sub(m, v, acc);
-  }
-}
-
-template <int P>
 __global__ void __launch_bounds__(256) k_dslash(Geom g, LinkField fat, LinkField lng, VecField src, VecField dst,
                                                 int pb, const int* __restrict__ stop) {
   if (stop && *stop) return;
@@ -469,29 +416,12 @@ __global__ void __launch_bounds__(256) k_dslash_resid(Geom g, LinkField fat, Lin
   block_sum_store<1>(s, partials_rr);
 }
 
-// Deterministic sum of nparts block partials (fixed order per thread, fixed tree).
-__device__ __forceinline__ void sum_partials(const double* __restrict__ parts, int nblocks, int k, double& out,
-                                             double* sm) {
-  double v = 0;
-  for (int i = threadIdx.x; i < nblocks; i += blockDim.x) v += parts[i * 4 + k];
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  __syncthreads();
-  if (lane == 0) sm[wid] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double s = 0;
-    for (int w = 0; w < (int)(blockDim.x >> 6); w++) s += sm[w];
-    out = s;
-  }
-}
-
 // after a restart: rsq = actual_rsq = |resid|^2, reset the device loop state
 __global__ void k_restart_scalars(const double* __restrict__ partials_rr, int nblocks, CGScalars* sc) {
-  __shared__ double sm[16];
-  double rr = 0;
-  sum_partials(partials_rr, nblocks, 0, rr, sm);
+  double t[1];
+  reduce_records<1>(partials_rr, nblocks, t);
   if (threadIdx.x == 0) {
+    double rr = t[0];
     sc->rsq = rr;
     sc->actual_rsq = rr;
     sc->rsq_first = rr;
@@ -512,13 +442,11 @@ __global__ void k_cg_scalars(const double* __restrict__ partials, const double* 
     if (threadIdx.x == 0) sc->go = 0;
     return;
   }
-  __shared__ double sm[16];
-  double pkp = 0, c_tr = 0, c_tt = 0, oldrsq = 0;
-  sum_partials(partials, nblocks, 0, pkp, sm);
-  sum_partials(partials, nblocks, 1, c_tr, sm);
-  sum_partials(partials, nblocks, 2, c_tt, sm);
-  sum_partials(partials_rr, nblocks, 0, oldrsq, sm);
+  double d3[3], d1[1];
+  reduce_records<3>(partials, nblocks, d3);
+  reduce_records<1>(partials_rr, nblocks, d1);
   if (threadIdx.x == 0) {
+    double pkp = d3[0], c_tr = d3[1], c_tt = d3[2], oldrsq = d1[0];
     double a = -sc->rsq / pkp;
     double rsq = oldrsq + 2.0 * a * c_tr + a * a * c_tt;
     double b = rsq / oldrsq;
@@ -663,16 +591,12 @@ int vec_norm2(const VecField& v, int prec, double* out) {
 
 int vec_zero(VecField& v, int prec) {
   Context& c = ctx();
-  MH_CHECK(hipMemsetAsync(v.data, 0, 3 * (size_t)v.stride * prec_complex_bytes(prec), c.stream));
-  if (prec == P_H16 && v.norm) MH_CHECK(hipMemsetAsync(v.norm, 0, (size_t)v.stride * sizeof(float), c.stream));
+  MH_CHECK(hipMemsetAsync(v.data, 0, vec_bytes(prec, v.stride), c.stream));
   return 0;
 }
 int vec_copy(const VecField& in, VecField& out, int prec) {
   Context& c = ctx();
-  MH_CHECK(hipMemcpyAsync(out.data, in.data, 3 * (size_t)in.stride * prec_complex_bytes(prec),
-                          hipMemcpyDeviceToDevice, c.stream));
-  if (prec == P_H16)
-    MH_CHECK(hipMemcpyAsync(out.norm, in.norm, (size_t)in.stride * sizeof(float), hipMemcpyDeviceToDevice, c.stream));
+  MH_CHECK(hipMemcpyAsync(out.data, in.data, vec_bytes(prec, in.stride), hipMemcpyDeviceToDevice, c.stream));
   return 0;
 }
 
@@ -701,7 +625,7 @@ static int true_residual(const VecField& src, const VecField& x, VecField& tmp_o
   launch_dslash<P_F64>(L, x, tmp_other, pb ^ 1, nullptr);
   hipLaunchKernelGGL(k_dslash_resid<P_F64>, dim3(dslash_grid()), dim3(256), 0, c.stream, c.geom, L.fat, L.lng,
                      tmp_other, x, src, resid, p, write_p, pb, msq_x4, partials_rr());
-  hipLaunchKernelGGL(k_restart_scalars, dim3(1), dim3(256), 0, c.stream, partials_rr(), dslash_grid(), c.ws.scal);
+  hipLaunchKernelGGL(k_restart_scalars, dim3(1), dim3(1024), 0, c.stream, partials_rr(), dslash_grid(), c.ws.scal);
   MH_CHECK(hipGetLastError());
   return 0;
 }
@@ -715,7 +639,7 @@ static void enqueue_cg_iteration(const DevLinks& L, VecField& p, VecField& tmp, 
   hipLaunchKernelGGL(k_dslash<PL>, dim3(nb), dim3(256), 0, c.stream, c.geom, L.fat, L.lng, p, tmp, pb ^ 1, halt);
   hipLaunchKernelGGL((k_dslash_mdagm<PL, PP, PA>), dim3(nb), dim3(256), 0, c.stream, c.geom, L.fat, L.lng, tmp, p,
                      r, ttt, pb, msq_x4, c.ws.partials, halt);
-  hipLaunchKernelGGL(k_cg_scalars, dim3(1), dim3(256), 0, c.stream, c.ws.partials, partials_rr(), nb, c.ws.scal);
+  hipLaunchKernelGGL(k_cg_scalars, dim3(1), dim3(1024), 0, c.stream, c.ws.partials, partials_rr(), nb, c.ws.scal);
   hipLaunchKernelGGL((k_cg_update<PP, PA>), dim3(nblocks_for(c.Vh, 256)), dim3(256), 0, c.stream, x, p, r, ttt,
                      c.Vh, c.ws.scal, partials_rr());
 }

@@ -68,7 +68,8 @@ int alloc_vec(VecField& v, int prec, int stride);
 void free_vec(VecField& v);
 int alloc_links(LinkField& l, int prec, int stride);
 void free_links(LinkField& l);
-size_t prec_complex_bytes(int prec);
+size_t vec_bytes(int prec, int stride);
+size_t link_bytes(int prec, int stride);
 
 // ---- layout ----------------------------------------------------------------
 int upload_vec_milc(const double* host_aos, VecField& out, int prec, int pb);

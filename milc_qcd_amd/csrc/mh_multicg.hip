@@ -15,6 +15,7 @@
 #include <cstring>
 #include <vector>
 
+#include "mh_dslash.h"
 #include "mh_internal.h"
 
 namespace mh {
@@ -55,51 +56,10 @@ __device__ __forceinline__ void block_sum4(double (&v)[N], double* __restrict__ 
   }
 }
 
-__device__ double reduce_one(const double* __restrict__ parts, int nblocks) {
-  __shared__ double sm[16];
-  double v = 0;
-  for (int i = threadIdx.x; i < nblocks; i += blockDim.x) v += parts[i * 4];
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = v;
-  __syncthreads();
-  double s = 0;
-  if (threadIdx.x == 0)
-    for (int w = 0; w < (int)(blockDim.x >> 6); w++) s += sm[w];
-  return s;
-}
-
-// Dslash forward-declared from mh_core.hip is a template there; the multi-shift
-// solver carries its own copies of the two stencil launches.
-template <int P>
-__device__ __forceinline__ void ms_dslash_site(const Geom& g, const LinkField& fat, const LinkField& lng,
-                                               const VecField& src, int cb, int pb,
-                                               Vec3<typename PrecTraits<P>::Real>& acc) {
-  typedef typename PrecTraits<P>::Real R;
-  Coord c = site_coords(g, cb, pb);
-  int po = pb ^ 1;
-  zero(acc);
-#pragma unroll
-  for (int mu = 0; mu < 4; mu++) {
-    Mat3<R> m;
-    Vec3<R> v;
-    int n;
-    n = neighbour(g, c, cb, mu, 1);
-    load_link<P>(fat, pb, mu, cb, m);
-    load_vec<P>(src, n, v);
-    mv_acc(m, v, acc);
-    n = neighbour(g, c, cb, mu, 3);
-    load_link<P>(lng, pb, mu, cb, m);
-    load_vec<P>(src, n, v);
-    mv_acc(m, v, acc);
-    n = neighbour(g, c, cb, mu, -1);
-    load_link<P>(fat, po, mu, n, m);
-    load_vec<P>(src, n, v);
-    amv_sub(m, v, acc);
-    n = neighbour(g, c, cb, mu, -3);
-    load_link<P>(lng, po, mu, n, m);
-    load_vec<P>(src, n, v);
-    amv_sub(m, v, acc);
-  }
+__device__ __forceinline__ double reduce_one(const double* __restrict__ parts, int nblocks) {
+  double t[1] = {0.0};
+  reduce_records<1>(parts, nblocks, t);
+  return t[0];
 }
 
 template <int P>
@@ -110,7 +70,7 @@ __global__ void __launch_bounds__(256) k_ms_dslash1(Geom g, LinkField fat, LinkF
   int cb = bid * blockDim.x + threadIdx.x;
   if (cb >= g.Vh) return;
   Vec3<typename PrecTraits<P>::Real> acc;
-  ms_dslash_site<P>(g, fat, lng, src, cb, pb, acc);
+  dslash_site<P>(g, fat, lng, src, cb, pb, acc);
   store_vec<P>(dst, cb, acc);
 }
 
@@ -127,7 +87,7 @@ __global__ void __launch_bounds__(256) k_ms_dslash2(Geom g, LinkField fat, LinkF
   if (cb < g.Vh) {
     R shift0 = (R)(-sc->shifts[sc->j_low]);
     Vec3<R> acc, pv;
-    ms_dslash_site<P>(g, fat, lng, tmp, cb, pb, acc);
+    dslash_site<P>(g, fat, lng, tmp, cb, pb, acc);
     load_vec<P>(p, cb, pv);
 #pragma unroll
     for (int k = 0; k < 3; k++) {
@@ -344,10 +304,10 @@ static int multicg_run(const VecField& src64, std::vector<VecField>& x64, const 
                          d_sc);
       hipLaunchKernelGGL(k_ms_dslash2<P>, dim3(nb), dim3(256), 0, c.stream, c.geom, L.fat, L.lng, tmp, p, ttt, pb,
                          d_sc, c.ws.partials);
-      hipLaunchKernelGGL(k_ms_beta, dim3(1), dim3(256), 0, c.stream, c.ws.partials, nb, d_sc);
+      hipLaunchKernelGGL(k_ms_beta, dim3(1), dim3(1024), 0, c.stream, c.ws.partials, nb, d_sc);
       hipLaunchKernelGGL(k_ms_update<P>, dim3(nb), dim3(256), 0, c.stream, psim, pm, p, r, ttt, c.Vh, d_sc,
                          c.ws.partials);
-      hipLaunchKernelGGL(k_ms_alpha, dim3(1), dim3(256), 0, c.stream, c.ws.partials, nb, d_sc);
+      hipLaunchKernelGGL(k_ms_alpha, dim3(1), dim3(1024), 0, c.stream, c.ws.partials, nb, d_sc);
       hipLaunchKernelGGL(k_ms_p<P>, dim3(nb), dim3(256), 0, c.stream, p, r, c.Vh, d_sc);
     }
     MH_CHECK(hipGetLastError());
