@@ -43,12 +43,40 @@ def log(*a):
 
 
 def cpu_baseline(G, fp64_iters, budget_s=15.0):
-    """FP64 oracle CG (MILC's algorithm) on this host, bounded sample."""
+    """MILC's own FP64 CG (compiled reference, ks_congrad_parity_cpu) on this
+    host's cores over a bounded number of iterations; falls back to the FP64
+    oracle port when the compiled reference is absent."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import numpy as np
-    import ks_oracle as ko
-
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    import milc_ref as mr
+    if mr.available():
+        import milc_qcd_amd as mq
+        t0 = time.time()
+        mr.init(DIMS)
+        fat, lng = G.dev_read_links()
+        mr.set_fn_links(fat, lng)
+        del fat, lng
+        src = G.dev_read(0, mq.EVEN)
+        log(f"cpu baseline (MILC reference): setup {time.time() - t0:.1f}s")
+        t1 = time.perf_counter()
+        _, it, _ = mr.congrad(src, MASS, RESID, mr.EVEN, 2, 1)
+        per = (time.perf_counter() - t1) / max(it, 1)
+        k = max(2, min(200, int(budget_s / max(per, 1e-6))))
+        t1 = time.perf_counter()
+        _, it, _ = mr.congrad(src, MASS, RESID, mr.EVEN, k, 1)
+        dt = time.perf_counter() - t1
+        s_per_iter = dt / it
+        return {
+            "value": s_per_iter * fp64_iters,
+            "unit": "s/solve",
+            "cores": threads,
+            "kind": "reference",
+            "sample": (f"MILC 7.8.1 ks_congrad_parity_cpu (FP64, FEWSUMS, DBLSTORE_FN, OpenMP {threads} threads), "
+                       f"compiled from the reference sources, on the same 32^3x64 HISQ links and source: "
+                       f"{it} iterations timed ({dt:.1f} s, {s_per_iter * 1e3:.1f} ms/iter), scaled by the "
+                       f"GPU FP64 iteration count {fp64_iters}"),
+        }
+    import ks_oracle as ko
     t0 = time.time()
     L = ko.Lattice(DIMS)
     fat, lng = G.dev_read_links()

@@ -1,0 +1,32 @@
+/* ref_force.c -- exposes MILC's HISQ force level 0 (outer products) to the
+ * tests.  TEST INFRASTRUCTURE ONLY, built by Makefile.ref.
+ *
+ * fn_fermion_force_multi_hisq_smearing0 is file-static in
+ * generic_ks/fermion_force_hisq_multi.c (lines 1478-1636), so this
+ * translation unit includes that unmodified source file and adds a wrapper. */
+#include "fermion_force_hisq_multi.c"
+
+/* oprod1/oprod3: MILC link layout [i][dir][18]; coefficient 1 (the caller's
+ * 2*eps*parity factor is applied by MILC after the smearing chain) */
+void ref_force_oprod(double **multi_x, const double *res, int n, double *o1, double *o3) {
+  info_t info = INFO_ZERO;
+  Real r[64];
+  su3_matrix *fa[4], *fn[4];
+  size_t i;
+  int dir, t;
+  for (t = 0; t < n && t < 64; t++) r[t] = res[t];
+  for (dir = 0; dir < 4; dir++) {
+    fa[dir] = (su3_matrix *)malloc(sites_on_node * sizeof(su3_matrix));
+    fn[dir] = (su3_matrix *)malloc(sites_on_node * sizeof(su3_matrix));
+  }
+  fn_fermion_force_multi_hisq_smearing0(&info, 1.0, r, (su3_vector **)multi_x, n, fa, fn);
+  for (i = 0; i < sites_on_node; i++)
+    for (dir = 0; dir < 4; dir++) {
+      memcpy(o1 + (4 * i + dir) * 18, &fa[dir][i], 18 * sizeof(double));
+      memcpy(o3 + (4 * i + dir) * 18, &fn[dir][i], 18 * sizeof(double));
+    }
+  for (dir = 0; dir < 4; dir++) {
+    free(fa[dir]);
+    free(fn[dir]);
+  }
+}
