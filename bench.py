@@ -42,6 +42,20 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+class stdout_to_stderr:
+    """MILC's C code prints to fd 1; keep the benchmark's stdout to the one JSON line."""
+
+    def __enter__(self):
+        sys.stdout.flush()
+        self.saved = os.dup(1)
+        os.dup2(2, 1)
+
+    def __exit__(self, *a):
+        sys.stdout.flush()
+        os.dup2(self.saved, 1)
+        os.close(self.saved)
+
+
 def cpu_baseline(G, fp64_iters, budget_s=15.0):
     """MILC's own FP64 CG (compiled reference, ks_congrad_parity_cpu) on this
     host's cores over a bounded number of iterations; falls back to the FP64
@@ -109,6 +123,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--kernel-only", action="store_true",
+                    help="only the Dslash and copy bursts (for rocprofv3 counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); "
@@ -124,6 +140,16 @@ def main():
     for j, s in enumerate(SRC_SEEDS):
         G.dev_source(s, mq.EVEN, j)
     XM, XD = 10, 11
+    if args.kernel_only:
+        res = {}
+        for prec, name in ((0, "f64"), (1, "f32"), (2, "h16")):
+            us = G.dev_dslash_bench(prec, mq.EVEN, 20)
+            res[name] = {"usec": round(us, 2), "alg_bytes_per_launch": DSLASH_BYTES[prec] * G.Vh}
+        us = G.dev_copy_bench(1 << 30, 5)
+        res["copy"] = {"usec": round(us, 2), "bytes_read": 1 << 30, "bytes_written": 1 << 30}
+        G.close()
+        print(json.dumps({"kernel_only": res}), flush=True)
+        return
 
     for i in range(args.warmup):
         G.dev_invert(i % 10, XM, MASS, mq.EVEN, RESID, 2)
@@ -171,16 +197,20 @@ def main():
     stream = 2 * copy_bytes / us / 1e3
     log(f"dslash {dsl}; stream copy {stream:.0f} GB/s")
 
+    # HBM bytes per launch of the 16-bit Dslash from the committed rocprofv3
+    # FETCH_SIZE/WRITE_SIZE summary (copy-calibrated), or null if absent
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "dslash_h16_32x32x32x64_traffic.json")
+    tpath = os.path.join(ROOT, "profiles", "dslash_32x32x32x64_hbm_traffic.txt")
     if os.path.exists(tpath):
-        with open(tpath) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+        for line in open(tpath):
+            if line.startswith("hbm_bytes_per_launch k_dslash<2>"):
+                traffic = int(line.split("=")[1])
 
     cpu = None
     if not args.no_cpu:
         try:
-            cpu = cpu_baseline(G, int(round(sum(fp64_iters) / len(fp64_iters))))
+            with stdout_to_stderr():
+                cpu = cpu_baseline(G, int(round(sum(fp64_iters) / len(fp64_iters))))
         except Exception as e:  # the CPU leg must not hide the GPU result
             log(f"cpu baseline failed: {e!r}")
     G.close()
