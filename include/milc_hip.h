@@ -123,13 +123,30 @@ int milc_hip_load_hisq_links(int precision, const void* thin, void* fatlink, voi
  * then the HISQ fat/Naik links, kept on the device as the current links. */
 int milc_hip_dev_gauge_hisq(uint64_t seed, double scale, int* n_svd);
 /* Gaussian even/odd source (<re^2> = 1/2, grsource_imp.c:169-195) into device
- * slot `slot` (0..15). */
+ * slot `slot` (0..63). */
 int milc_hip_dev_source(uint64_t seed, int parity, int slot);
 /* Solve on device slots: x[slot_x] = (M^dag M)^-1 b[slot_b]; returns
  * iterations; stats[0..5] = final |r|/|b|, seconds, restarts/reliable
  * updates, converged, iterations, final_rsq. */
 int milc_hip_dev_invert(int slot_b, int slot_x, double mass, int parity, double resid, int mixed_precision,
                         int max_iter, double* stats);
+/* RHMC fermion action (ks_ratinv + ks_rateval + FP64 sum, d_action_rhmc.c
+ * fermion_action): action = sum_even |residues[0] phi + sum_k residues[k] x_k|^2,
+ * x_k = (M^dag M + roots[k] - 4m^2)^-1 phi (roots already include 4m^2, k = 1..order).
+ * mixed_precision 0 = FP64 multi-shift, 1 = FP32 multi-shift + mixed per-shift
+ * refinement.  Returns the total iterations. */
+int milc_hip_dev_rhmc_action(int slot_phi, int order, const double* roots, const double* residues, double resid,
+                             int mixed_precision, double* action);
+/* Level-0 fermion force of S = sum_t residues[t] phi^dag (M^dag M + offsets[t] - 4m^2)^-1 phi
+ * on the current device links (update_h_rhmc.c:77-107 up to the force call):
+ * X_t by multi-shift, Z_t = (X_t even, D_oe X_t odd), then oprod1/oprod3 as in
+ * milc_hip_compute_oprod with coeff[t] = (residues[t], residues[t]).  *S gets
+ * the action value; oprod1/oprod3 (MILC layout [i][dir][18], may be NULL) the
+ * outer products.  dS/dF_mu(x) = 2 eps(x) O1_mu(x)^dag (eps = +1 even, -1 odd). */
+int milc_hip_dev_force_level0(int slot_phi, int n, const double* offsets, const double* residues, double resid,
+                              int mixed_precision, double* S, double* oprod1, double* oprod3);
+/* Make MILC-layout host fat/long links (double) the current device links. */
+int milc_hip_dev_load_links(const double* fat, const double* lng);
 /* Copy a device slot to / from a MILC-layout host field (double). */
 int milc_hip_dev_read(int slot, int parity, double* host);
 int milc_hip_dev_write(int slot, int parity, const double* host);

@@ -17,7 +17,7 @@ using namespace mh;
 
 namespace {
 
-const int kSlots = 16;
+const int kSlots = 64;
 VecField g_slots[kSlots];
 
 int parity_from_evenodd(int evenodd) {
@@ -640,5 +640,77 @@ extern "C" int milc_hip_dev_diff(int slot_a, int slot_b, double* rel) {
     n += h[2 * i + 1];
   }
   *rel = n > 0 ? std::sqrt(d / n) : 0.0;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// RHMC compositions on device slots
+// ---------------------------------------------------------------------------
+namespace mh {
+int rhmc_action(const VecField& phi, int order, const double* roots, const double* residues, double resid, int mixed,
+                double* action, int* iters);
+int rhmc_force_level0(const VecField& phi, int n, const double* offsets, const double* residues, double resid,
+                      int mixed, double* S, LinkField& o1, LinkField& o3, int* iters);
+}
+
+extern "C" int milc_hip_dev_rhmc_action(int slot_phi, int order, const double* roots, const double* residues,
+                                        double resid, int mixed_precision, double* action) {
+  Context& c = ctx();
+  if (!c.initialised || slot_phi < 0 || slot_phi >= kSlots || !g_slots[slot_phi].data) return -1;
+  if (order <= 0 || order > 32) return -2;
+  for (int k = 1; k <= order; k++)
+    if (!(roots[k] > 0)) return -2;
+  int rc;
+  if (!c.links[P_F64].valid) return -3;
+  if (mixed_precision && (rc = ensure_links(P_F32))) return rc;
+  int iters = 0;
+  rc = rhmc_action(g_slots[slot_phi], order, roots, residues, resid, mixed_precision ? 1 : 0, action, &iters);
+  return rc ? rc : iters;
+}
+
+extern "C" int milc_hip_dev_force_level0(int slot_phi, int n, const double* offsets, const double* residues,
+                                         double resid, int mixed_precision, double* S, double* oprod1,
+                                         double* oprod3) {
+  Context& c = ctx();
+  if (!c.initialised || slot_phi < 0 || slot_phi >= kSlots || !g_slots[slot_phi].data) return -1;
+  if (n <= 0 || n > 32) return -2;
+  for (int k = 0; k < n; k++)
+    if (!(offsets[k] > 0)) return -2;
+  int rc;
+  if (!c.links[P_F64].valid) return -3;
+  if (mixed_precision && (rc = ensure_links(P_F32))) return rc;
+  LinkField o1, o3;
+  if ((rc = alloc_links(o1, P_F64, c.stride)) || (rc = alloc_links(o3, P_F64, c.stride))) return rc;
+  int iters = 0;
+  if ((rc = rhmc_force_level0(g_slots[slot_phi], n, offsets, residues, resid, mixed_precision ? 1 : 0, S, o1, o3,
+                              &iters)))
+    return rc;
+  if (oprod1 || oprod3) {
+    size_t nn = (size_t)2 * c.Vh * 4 * 18;
+    double* stage = nullptr;
+    if ((rc = dev_alloc((void**)&stage, nn * sizeof(double)))) return rc;
+    if (oprod1) {
+      if ((rc = links_to_milc_dev(o1, P_F64, stage))) return rc;
+      MH_CHECK(hipMemcpyAsync(oprod1, stage, nn * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+      MH_CHECK(hipStreamSynchronize(c.stream));
+    }
+    if (oprod3) {
+      if ((rc = links_to_milc_dev(o3, P_F64, stage))) return rc;
+      MH_CHECK(hipMemcpyAsync(oprod3, stage, nn * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+      MH_CHECK(hipStreamSynchronize(c.stream));
+    }
+    dev_free(stage);
+  }
+  free_links(o1);
+  free_links(o3);
+  return iters;
+}
+
+extern "C" int milc_hip_dev_load_links(const double* fat, const double* lng) {
+  Context& c = ctx();
+  if (!c.initialised) return -1;
+  int rc = upload_links(2, fat, lng);
+  if (rc) return rc;
+  c.link_key = c.link_key2 = nullptr;  // host-array calls re-upload their own links
   return 0;
 }

@@ -44,6 +44,7 @@ EXPORTS = [
     "milc_hip_load_hisq_links", "milc_hip_dev_gauge_hisq", "milc_hip_dev_source",
     "milc_hip_dev_invert", "milc_hip_dev_read", "milc_hip_dev_write", "milc_hip_dev_read_links",
     "milc_hip_dev_dslash_bench", "milc_hip_dev_copy_bench", "milc_hip_dev_diff",
+    "milc_hip_dev_rhmc_action", "milc_hip_dev_force_level0", "milc_hip_dev_load_links",
 ]
 
 
@@ -77,6 +78,9 @@ def load_library(path=LIB_PATH):
     L.milc_hip_dev_dslash_bench.argtypes = [I, I, I, ctypes.POINTER(D)]
     L.milc_hip_dev_copy_bench.argtypes = [ctypes.c_size_t, I, ctypes.POINTER(D)]
     L.milc_hip_dev_diff.argtypes = [I, I, ctypes.POINTER(D)]
+    L.milc_hip_dev_rhmc_action.argtypes = [I, I, ctypes.POINTER(D), ctypes.POINTER(D), D, I, ctypes.POINTER(D)]
+    L.milc_hip_dev_force_level0.argtypes = [I, I, ctypes.POINTER(D), ctypes.POINTER(D), D, I, ctypes.POINTER(D), P, P]
+    L.milc_hip_dev_load_links.argtypes = [P, P]
     _lib = L
     return L
 
@@ -318,3 +322,37 @@ class Lattice:
         r = ctypes.c_double(0)
         _check(self.lib.milc_hip_dev_diff(slot_a, slot_b, ctypes.byref(r)), "milc_hip_dev_diff")
         return r.value
+
+    def dev_load_links(self, fat, lng):
+        """Make host fat/long links (MILC layout) the current device links."""
+        self._activate()
+        fat = np.ascontiguousarray(fat, dtype=np.float64)
+        lng = np.ascontiguousarray(lng, dtype=np.float64)
+        _check(self.lib.milc_hip_dev_load_links(_p(fat), _p(lng)), "milc_hip_dev_load_links")
+        self._fresh = True
+
+    def dev_rhmc_action(self, slot_phi, roots, residues, resid, mixed=0):
+        """fermion_action: sum_even |a0 phi + sum_k a_k x_k|^2 (roots[0] unused)."""
+        self._activate()
+        order = len(roots) - 1
+        r = (ctypes.c_double * (order + 1))(*roots)
+        a = (ctypes.c_double * (order + 1))(*residues)
+        out = ctypes.c_double(0)
+        it = _check(self.lib.milc_hip_dev_rhmc_action(slot_phi, order, r, a, resid, mixed, ctypes.byref(out)),
+                    "milc_hip_dev_rhmc_action")
+        return out.value, it
+
+    def dev_force_level0(self, slot_phi, offsets, residues, resid, mixed=0, want_oprod=True):
+        """(S, O1, O3, iterations) of S = sum_t r_t phi^dag (M^dag M + sigma_t)^-1 phi."""
+        self._activate()
+        n = len(offsets)
+        o = (ctypes.c_double * n)(*offsets)
+        r = (ctypes.c_double * n)(*residues)
+        S = ctypes.c_double(0)
+        o1 = np.zeros((self.V, 4, 3, 3, 2)) if want_oprod else None
+        o3 = np.zeros((self.V, 4, 3, 3, 2)) if want_oprod else None
+        it = _check(self.lib.milc_hip_dev_force_level0(slot_phi, n, o, r, resid, mixed, ctypes.byref(S),
+                                                       _p(o1) if want_oprod else None,
+                                                       _p(o3) if want_oprod else None),
+                    "milc_hip_dev_force_level0")
+        return S.value, o1, o3, it
