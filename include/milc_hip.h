@@ -107,6 +107,25 @@ int milc_hip_dslash(int external_precision, int internal_precision, milc_hip_inv
  * fields, overwritten. */
 int milc_hip_compute_oprod(int precision, int num_terms, double** coeff, void** quark_field, void* oprod[2]);
 
+/* ---- FN (fat + Naik) fermion force on the thin links ------------------------ */
+/* Replaces qudaAsqtadForce (generic_ks/fermion_force_asqtad_gpu.c:133-136):
+ * the asqtad path chain from the outer products to the momentum.
+ * act_path_coeff = {one_link, naik, three_staple, five_staple, seven_staple,
+ * lepage} (ks_action_paths.act_path_coeff); one_link_oprod[dir] /
+ * naik_oprod[dir] = su3_matrix[V] outer products that already carry 2 eps
+ * and the term weights (fermion_force_asqtad_gpu.c:53-60); gauge = thin links
+ * [i][dir][18] with the KS phases in.  momentum = anti_hermitmat [i][dir]
+ * packed as 10 Reals (m01, m02, m12 complex; m00im, m11im, m22im; space);
+ * the force is ADDED to it.  precision 1 = float, 2 = double for every array. */
+int milc_hip_asqtad_force(int precision, const double act_path_coeff[6], const void* const one_link_oprod[4],
+                          const void* const naik_oprod[4], const void* gauge, void* momentum);
+/* Replaces fermion_force_fn_multi (generic_ks/fermion_force_fn_multi.c:81-313,
+ * dispatcher fermion_force_multi.c:29-45): multi_x[t] = X_t on all sites
+ * (Real su3_vector[V]), residues[t] (double), eps; outer products, path chain
+ * and mom += TA(2 eps U G) in one call, arrays as in milc_hip_asqtad_force. */
+int milc_hip_fermion_force_fn_multi(int precision, double eps, const double* residues, void** multi_x, int nterms,
+                                    const double act_path_coeff[6], const void* gauge, void* momentum);
+
 /* ---- link construction ----------------------------------------------------- */
 /* Replaces qudaLoadUnitarizedLink + qudaLoadKSLink for the HISQ U(3) action
  * (generic_ks/fermion_links_hisq_load_milc.c:546-702,

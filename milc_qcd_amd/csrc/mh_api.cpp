@@ -160,6 +160,7 @@ int milc_hip_finalize(void) {
     }
   }
   for (int s = 0; s < kSlots; s++) free_vec(g_slots[s]);
+  fn_table_release();
   dev_free(c.ws.partials);
   dev_free(c.ws.scal);
   if (c.ws.scal_host) (void)hipHostFree(c.ws.scal_host);
@@ -713,4 +714,135 @@ extern "C" int milc_hip_dev_load_links(const double* fat, const double* lng) {
   if (rc) return rc;
   c.link_key = c.link_key2 = nullptr;  // host-array calls re-upload their own links
   return 0;
+}
+
+// ---------------------------------------------------------------------------
+// FN (fat + Naik) fermion force on the thin links
+// ---------------------------------------------------------------------------
+namespace mh {
+int fn_force_from_oprod(const LinkField& U, const LinkField& o1, const LinkField& o3, const double coeff[6],
+                        double scale, double* mom_dev);
+}
+
+namespace {
+// MILC AoS link array [i][dir][18] (float or double) -> FP64 device links
+int host_links_in(int ext_prec, const void* h, LinkField& out, double* stage) {
+  Context& c = ctx();
+  size_t n = (size_t)2 * c.Vh * 4 * 18;
+  std::vector<double> conv;
+  const double* src = (const double*)h;
+  if (ext_prec == 1) {
+    conv.resize(n);
+    for (size_t i = 0; i < n; i++) conv[i] = ((const float*)h)[i];
+    src = conv.data();
+  }
+  MH_CHECK(hipMemcpyAsync(stage, src, n * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  int rc = links_from_milc_dev(stage, out, P_F64);
+  MH_CHECK(hipStreamSynchronize(c.stream));
+  return rc;
+}
+
+// four per-direction su3_matrix[V] arrays -> FP64 device links
+int host_dirs_in(int ext_prec, const void* const dirs[4], LinkField& out, double* stage) {
+  Context& c = ctx();
+  size_t V = (size_t)2 * c.Vh;
+  std::vector<double> h(V * 4 * 18);
+  for (int mu = 0; mu < 4; mu++)
+    for (size_t i = 0; i < V; i++)
+      for (int e = 0; e < 18; e++)
+        h[18 * (4 * i + mu) + e] =
+            ext_prec == 2 ? ((const double*)dirs[mu])[18 * i + e] : ((const float*)dirs[mu])[18 * i + e];
+  return host_links_in(2, h.data(), out, stage);
+}
+
+// mom[(4 i + dir) * 10 + k] += increments (device, double) in the caller's precision
+int add_momentum(int ext_prec, const double* dev, void* mom) {
+  Context& c = ctx();
+  size_t n = (size_t)2 * c.Vh * 4 * 10;
+  std::vector<double> h(n);
+  MH_CHECK(hipMemcpyAsync(h.data(), dev, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  MH_CHECK(hipStreamSynchronize(c.stream));
+  if (ext_prec == 2) {
+    double* m = (double*)mom;
+    for (size_t i = 0; i < n; i++) m[i] += h[i];
+  } else {
+    float* m = (float*)mom;
+    for (size_t i = 0; i < n; i++) m[i] += (float)h[i];
+  }
+  return 0;
+}
+
+struct ForceBuffers {
+  LinkField U{}, o1{}, o3{};
+  double* stage = nullptr;
+  double* mom = nullptr;
+  ~ForceBuffers() {
+    free_links(U);
+    free_links(o1);
+    free_links(o3);
+    dev_free(stage);
+    dev_free(mom);
+  }
+  int alloc() {
+    Context& c = ctx();
+    int rc;
+    if ((rc = alloc_links(U, P_F64, c.stride)) || (rc = alloc_links(o1, P_F64, c.stride)) ||
+        (rc = alloc_links(o3, P_F64, c.stride)))
+      return rc;
+    if ((rc = dev_alloc((void**)&stage, (size_t)2 * c.Vh * 4 * 18 * sizeof(double)))) return rc;
+    return dev_alloc((void**)&mom, (size_t)2 * c.Vh * 4 * 10 * sizeof(double));
+  }
+};
+}  // namespace
+
+extern "C" int milc_hip_asqtad_force(int precision, const double act_path_coeff[6], const void* const one_link_oprod[4],
+                                     const void* const naik_oprod[4], const void* gauge, void* momentum) {
+  Context& c = ctx();
+  if (!c.initialised) return -1;
+  if ((precision != 1 && precision != 2) || !act_path_coeff || !one_link_oprod || !naik_oprod || !gauge || !momentum)
+    return -2;
+  ForceBuffers b;
+  int rc;
+  if ((rc = b.alloc())) return rc;
+  if ((rc = host_links_in(precision, gauge, b.U, b.stage)) || (rc = host_dirs_in(precision, one_link_oprod, b.o1, b.stage)) ||
+      (rc = host_dirs_in(precision, naik_oprod, b.o3, b.stage)))
+    return rc;
+  if ((rc = fn_force_from_oprod(b.U, b.o1, b.o3, act_path_coeff, 1.0, b.mom))) return rc;
+  return add_momentum(precision, b.mom, momentum);
+}
+
+extern "C" int milc_hip_fermion_force_fn_multi(int precision, double eps, const double* residues, void** multi_x,
+                                               int nterms, const double act_path_coeff[6], const void* gauge,
+                                               void* momentum) {
+  Context& c = ctx();
+  if (!c.initialised) return -1;
+  if ((precision != 1 && precision != 2) || nterms <= 0 || nterms > 32 || !residues || !multi_x || !act_path_coeff ||
+      !gauge || !momentum)
+    return -2;
+  ForceBuffers b;
+  int rc;
+  if ((rc = b.alloc())) return rc;
+  std::vector<VecField> xe(nterms), xo(nterms);
+  std::vector<double> r(residues, residues + nterms);
+  auto release = [&]() {
+    for (int t = 0; t < nterms; t++) {
+      free_vec(xe[t]);
+      free_vec(xo[t]);
+    }
+  };
+  for (int t = 0; t < nterms; t++) {
+    if ((rc = alloc_vec(xe[t], P_F64, c.stride)) || (rc = alloc_vec(xo[t], P_F64, c.stride)) ||
+        (rc = host_vec_in(precision, multi_x[t], xe[t], 0)) || (rc = host_vec_in(precision, multi_x[t], xo[t], 1))) {
+      release();
+      return rc;
+    }
+  }
+  if ((rc = host_links_in(precision, gauge, b.U, b.stage)) || (rc = force_oprod_coeff(xe, xo, r, r, b.o1, b.o3)) ||
+      (rc = fn_force_from_oprod(b.U, b.o1, b.o3, act_path_coeff, 2.0 * eps, b.mom))) {
+    release();
+    return rc;
+  }
+  rc = add_momentum(precision, b.mom, momentum);
+  release();
+  return rc;
 }

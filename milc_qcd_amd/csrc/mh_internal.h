@@ -116,6 +116,8 @@ int multicg_solve(const VecField& src, std::vector<VecField>& x, const std::vect
 // ---- force outer products ---------------------------------------------------
 int force_oprod(const std::vector<VecField>& xe, const std::vector<VecField>& xo,
                 const std::vector<double>& residues, LinkField& o1, LinkField& o3);
+// device path table of the FN force (freed with the context)
+void fn_table_release();
 
 // ---- links ------------------------------------------------------------------
 int gauge_random_dev(uint64_t seed, double scale, LinkField& U);

@@ -45,6 +45,7 @@ EXPORTS = [
     "milc_hip_dev_invert", "milc_hip_dev_read", "milc_hip_dev_write", "milc_hip_dev_read_links",
     "milc_hip_dev_dslash_bench", "milc_hip_dev_copy_bench", "milc_hip_dev_diff",
     "milc_hip_dev_rhmc_action", "milc_hip_dev_force_level0", "milc_hip_dev_load_links",
+    "milc_hip_asqtad_force", "milc_hip_fermion_force_fn_multi",
 ]
 
 
@@ -81,6 +82,9 @@ def load_library(path=LIB_PATH):
     L.milc_hip_dev_rhmc_action.argtypes = [I, I, ctypes.POINTER(D), ctypes.POINTER(D), D, I, ctypes.POINTER(D)]
     L.milc_hip_dev_force_level0.argtypes = [I, I, ctypes.POINTER(D), ctypes.POINTER(D), D, I, ctypes.POINTER(D), P, P]
     L.milc_hip_dev_load_links.argtypes = [P, P]
+    L.milc_hip_asqtad_force.argtypes = [I, ctypes.POINTER(D), ctypes.POINTER(P), ctypes.POINTER(P), P, P]
+    L.milc_hip_fermion_force_fn_multi.argtypes = [I, D, ctypes.POINTER(D), ctypes.POINTER(P), I,
+                                                  ctypes.POINTER(D), P, P]
     _lib = L
     return L
 
@@ -267,6 +271,35 @@ class Lattice:
         op = (ctypes.c_void_p * 2)(ctypes.cast(d1, ctypes.c_void_p), ctypes.cast(d3, ctypes.c_void_p))
         _check(self.lib.milc_hip_compute_oprod(2, n, cptr, qptr, op), "milc_hip_compute_oprod")
         return np.stack(o1, axis=1), np.stack(o3, axis=1)
+
+    def fermion_force_fn_multi(self, eps, residues, multi_x, act_path_coeff, links, mom):
+        """fermion_force_fn_multi (generic_ks/fermion_force_fn_multi.c:81-313):
+        mom (V, 4, 10) packed anti_hermitmat, updated in place by TA(2 eps U G);
+        links = thin links (V, 4, 3, 3, 2) with the KS phases in; multi_x[t] (V, 3, 2)."""
+        self._activate()
+        n = len(multi_x)
+        xs = [np.ascontiguousarray(x, dtype=np.float64) for x in multi_x]
+        qptr = (ctypes.c_void_p * n)(*[x.ctypes.data for x in xs])
+        res = (ctypes.c_double * n)(*residues)
+        c = (ctypes.c_double * 6)(*act_path_coeff)
+        U = np.ascontiguousarray(links, dtype=np.float64)
+        _check(self.lib.milc_hip_fermion_force_fn_multi(2, eps, res, qptr, n, c, _p(U), _p(mom)),
+               "milc_hip_fermion_force_fn_multi")
+        return mom
+
+    def asqtad_force(self, act_path_coeff, one_link_oprod, naik_oprod, links, mom):
+        """qudaAsqtadForce (generic_ks/fermion_force_asqtad_gpu.c:133-136): outer
+        products (V, 4, 3, 3, 2) already scaled by 2 eps and the term weights;
+        mom (V, 4, 10) updated in place."""
+        self._activate()
+        o1 = [np.ascontiguousarray(one_link_oprod[:, mu]) for mu in range(4)]
+        o3 = [np.ascontiguousarray(naik_oprod[:, mu]) for mu in range(4)]
+        d1 = (ctypes.c_void_p * 4)(*[a.ctypes.data for a in o1])
+        d3 = (ctypes.c_void_p * 4)(*[a.ctypes.data for a in o3])
+        c = (ctypes.c_double * 6)(*act_path_coeff)
+        U = np.ascontiguousarray(links, dtype=np.float64)
+        _check(self.lib.milc_hip_asqtad_force(2, c, d1, d3, _p(U), _p(mom)), "milc_hip_asqtad_force")
+        return mom
 
     # ---------------------------------------------------------------- device-resident API
     def dev_gauge_hisq(self, seed, scale=0.4):

@@ -850,3 +850,135 @@ void kso_force_oprod(const kso_geom *g, const double *const *multi_x, const doub
     }
   }
 }
+
+/* ------------------------------------------------------------------------ */
+/* FN (fat + Naik) fermion force on the thin links                           */
+/* (generic_ks/fermion_force_fn_multi.c:81-313, eo_fermion_force_multi)       */
+/*                                                                           */
+/* The fat link is F_mu(x) = sum_p c_p P_p(x), a sum over the action's paths  */
+/* from x to x+mu (rotations and reflections of the basic paths of            */
+/* generic_ks/imp_actions/asqtad_action.h), and the Naik link is              */
+/* L_mu(x) = c_N U_mu(x) U_mu(x+mu) U_mu(x+2mu).  With                        */
+/* dS = sum 2 Re Tr[dF_mu(x) eps(x) O1_mu(x) + dL_mu(x) eps(x) O3_mu(x)]        */
+/* (O1/O3 = kso_force_oprod) every link of every path gets                   */
+/* M_j = (L_j+1 ... L_n-1) eps(x) O (L_0 ... L_j-1): a forward link U         */
+/* collects c_p M_j, a backward link U^dag collects c_p M_j^dag, into G.     */
+/* The momentum is mom_mu(y) = TA(2 eps_md U_mu(y) G_mu(y)), TA = traceless   */
+/* anti-Hermitian part (libraries/make_ahmat.c).                              */
+/* Pinned by MILC's golden binary_samples/colormatrix.mom.sample.l8.          */
+/* ------------------------------------------------------------------------ */
+typedef struct { int n; int d[7]; double c; int naik; } kso_path;
+
+static int add_path(kso_path *t, int k, int n, const int *d, double c, int naik) {
+  t[k].n = n; t[k].c = c; t[k].naik = naik;
+  for (int i = 0; i < n; i++) t[k].d[i] = d[i];
+  return k + 1;
+}
+/* signed direction: s*nu with s=+1 -> nu, s=-1 -> OPP_DIR(nu) = 7-nu */
+static int sdir(int nu, int s) { return s > 0 ? nu : 7 - nu; }
+
+/* all paths of direction mu for coefficients {one_link, naik, 3-staple,
+ * 5-staple, 7-staple, Lepage}; returns the count */
+static int fn_paths(int mu, const double coeff[6], kso_path *t) {
+  int k = 0, d[7];
+  d[0] = mu; k = add_path(t, k, 1, d, coeff[0], 0);
+  d[0] = d[1] = d[2] = mu; k = add_path(t, k, 3, d, coeff[1], 1);
+  for (int nu = 0; nu < 4; nu++) {
+    if (nu == mu) continue;
+    for (int s = -1; s <= 1; s += 2) {
+      d[0] = sdir(nu, s); d[1] = mu; d[2] = sdir(nu, -s); k = add_path(t, k, 3, d, coeff[2], 0);
+      d[0] = d[1] = sdir(nu, s); d[2] = mu; d[3] = d[4] = sdir(nu, -s); k = add_path(t, k, 5, d, coeff[5], 0);
+      for (int rho = 0; rho < 4; rho++) {
+        if (rho == mu || rho == nu) continue;
+        for (int s2 = -1; s2 <= 1; s2 += 2) {
+          d[0] = sdir(nu, s); d[1] = sdir(rho, s2); d[2] = mu; d[3] = sdir(rho, -s2); d[4] = sdir(nu, -s);
+          k = add_path(t, k, 5, d, coeff[3], 0);
+          for (int sig = 0; sig < 4; sig++) {
+            if (sig == mu || sig == nu || sig == rho) continue;
+            for (int s3 = -1; s3 <= 1; s3 += 2) {
+              d[0] = sdir(nu, s); d[1] = sdir(rho, s2); d[2] = sdir(sig, s3); d[3] = mu;
+              d[4] = sdir(sig, -s3); d[5] = sdir(rho, -s2); d[6] = sdir(nu, -s);
+              k = add_path(t, k, 7, d, coeff[4], 0);
+            }
+          }
+        }
+      }
+    }
+  }
+  return k;
+}
+
+int kso_fn_paths(int mu, const double coeff[6], int *dirs, double *c, int *len, int *naik) {
+  kso_path t[128];
+  int n = fn_paths(mu, coeff, t);
+  for (int p = 0; p < n; p++) {
+    len[p] = t[p].n; c[p] = t[p].c; naik[p] = t[p].naik;
+    for (int i = 0; i < 7; i++) dirs[7 * p + i] = i < t[p].n ? t[p].d[i] : -1;
+  }
+  return n;
+}
+
+void kso_fn_force(const kso_geom *g, const double coeff[6], const double *U, const double *const *multi_x,
+                  const double *residues, int nterms, double eps, double *mom) {
+  const size_t V = g->V;
+  double *o1 = (double *)malloc(sizeof(double) * 18 * 4 * V);
+  double *o3 = (double *)malloc(sizeof(double) * 18 * 4 * V);
+  double *G = (double *)calloc(18 * 4 * V, sizeof(double));
+  kso_force_oprod(g, multi_x, residues, nterms, o1, o3);
+  for (int mu = 0; mu < 4; mu++) {
+    kso_path t[128];
+    int np = fn_paths(mu, coeff, t);
+    for (int p = 0; p < np; p++) {
+      const kso_path *P = &t[p];
+      /* serial over sites: contributions scatter to neighbours */
+      for (size_t x = 0; x < V; x++) {
+        size_t y[8];
+        double L[7][18], pre[8][18], suf[8][18], Ob[18];
+        y[0] = x;
+        for (int j = 0; j < P->n; j++) {
+          int d = P->d[j];
+          if (d < 4) { memcpy(L[j], U + (4 * y[j] + d) * 18, sizeof L[j]); y[j + 1] = NB(g, d, y[j]); }
+          else { int dp = 7 - d; y[j + 1] = NB(g, d, y[j]); m_adj(U + (4 * y[j + 1] + dp) * 18, L[j]); }
+        }
+        /* prefix products pre[j] = L_0..L_j-1, suffix suf[j] = L_j+1..L_n-1 */
+        memset(pre[0], 0, sizeof pre[0]);
+        RE(pre[0], 0, 0) = RE(pre[0], 1, 1) = RE(pre[0], 2, 2) = 1.0;
+        for (int j = 0; j < P->n; j++) m_nn(pre[j], L[j], pre[j + 1]);
+        memset(suf[P->n - 1], 0, sizeof suf[0]);
+        RE(suf[P->n - 1], 0, 0) = RE(suf[P->n - 1], 1, 1) = RE(suf[P->n - 1], 2, 2) = 1.0;
+        for (int j = P->n - 2; j >= 0; j--) m_nn(L[j + 1], suf[j + 1], suf[j]);
+        double ex = x < g->Vh ? 1.0 : -1.0;
+        const double *O = (P->naik ? o3 : o1) + (4 * x + mu) * 18;
+        for (int k = 0; k < 18; k++) Ob[k] = ex * O[k];
+        for (int j = 0; j < P->n; j++) {
+          double A[18], M[18];
+          m_nn(suf[j], Ob, A);
+          m_nn(A, pre[j], M);
+          int d = P->d[j];
+          if (d < 4) m_axpy(G + (4 * y[j] + d) * 18, P->c, M);
+          else {
+            double Md[18];
+            m_adj(M, Md);
+            m_axpy(G + (4 * y[j + 1] + (7 - d)) * 18, P->c, Md);
+          }
+        }
+      }
+    }
+  }
+#pragma omp parallel for
+  for (size_t i = 0; i < 4 * V; i++) {
+    double UG[18], m[18];
+    m_nn(U + 18 * i, G + 18 * i, UG);
+    for (int k = 0; k < 18; k++) UG[k] *= 2.0 * eps;
+    /* traceless anti-Hermitian part, stored as a full 3x3 matrix */
+    double tr = (IM(UG, 0, 0) + IM(UG, 1, 1) + IM(UG, 2, 2)) / 3.0;
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++) {
+        RE(m, a, b) = 0.5 * (RE(UG, a, b) - RE(UG, b, a));
+        IM(m, a, b) = 0.5 * (IM(UG, a, b) + IM(UG, b, a));
+      }
+    for (int a = 0; a < 3; a++) IM(m, a, a) -= tr;
+    memcpy(mom + 18 * i, m, sizeof m);
+  }
+  free(o1); free(o3); free(G);
+}

@@ -93,6 +93,13 @@ int kso_multicg_offset(const kso_geom *g, const double *fat, const double *lng,
 void kso_force_oprod(const kso_geom *g, const double *const *multi_x, const double *residues,
                      int nterms, double *oprod1, double *oprod3);
 
+/* FN (fat + Naik) thin-link force, fermion_force_fn_multi: mom = TA(2 eps U G),
+ * stored as full anti-Hermitian 3x3 matrices [V][4][18]. */
+void kso_fn_force(const kso_geom *g, const double coeff[6], const double *U, const double *const *multi_x,
+                  const double *residues, int nterms, double eps, double *mom);
+/* the force's path table for direction mu (dirs[7*p+i], -1 padded) */
+int kso_fn_paths(int mu, const double coeff[6], int *dirs, double *c, int *len, int *naik);
+
 /* --- reductions and helpers used by tests ---------------------------------- */
 double kso_norm2(const kso_geom *g, const double *v, int parity);
 void kso_mdagm(const kso_geom *g, const double *fat, const double *lng, double offset,

@@ -65,6 +65,7 @@ def lib():
         L.kso_congrad.argtypes = [G, P, P, P, P, ctypes.POINTER(Qic), ctypes.c_double]
         L.kso_multicg_offset.argtypes = [G, P, P, P, P, P, ctypes.c_int, ctypes.POINTER(Qic)]
         L.kso_force_oprod.argtypes = [G, P, P, ctypes.c_int, P, P]
+        L.kso_fn_force.argtypes = [G, P, P, P, P, ctypes.c_int, ctypes.c_double, P]
         L.kso_norm2.argtypes = [G, P, ctypes.c_int]
         L.kso_norm2.restype = ctypes.c_double
         L.kso_mdagm.argtypes = [G, P, P, ctypes.c_double, P, P, P, ctypes.c_int]
@@ -195,6 +196,16 @@ class Lattice:
         it = lib().kso_multicg_offset(self.g, _p(fat), _p(lng), _p(src), ptrs, _p(offs), n, qics)
         return psim, it, list(qics)
 
+    def fn_force(self, coeff, U_phased, multi_x, residues, eps=1.0):
+        """fermion_force_fn_multi: TA(2 eps U G) per link, full 3x3 matrices (V, 4, 3, 3, 2)."""
+        n = len(multi_x)
+        ptrs = (ctypes.c_void_p * n)(*[np.ascontiguousarray(x).ctypes.data for x in multi_x])
+        res = np.ascontiguousarray(residues, dtype=np.float64)
+        c = np.ascontiguousarray(coeff, dtype=np.float64)
+        mom = self.new_links()
+        lib().kso_fn_force(self.g, _p(c), _p(np.ascontiguousarray(U_phased)), ptrs, _p(res), n, eps, _p(mom))
+        return mom
+
     def force_oprod(self, multi_x, residues):
         n = len(multi_x)
         ptrs = (ctypes.c_void_p * n)(*[np.ascontiguousarray(x).ctypes.data for x in multi_x])
@@ -222,4 +233,12 @@ def from_complex(c):
 
 # HISQ level-2 free-field coefficients: fat = 9/8, Naik = -1/24 (see kso_hisq_links)
 HISQ_FREE_C1 = 9.0 / 8.0
+
+
+def asqtad_coeffs(u0):
+    """act_path_coeff {one_link, naik, 3-staple, 5-staple, 7-staple, Lepage} of
+    generic_ks/imp_actions/asqtad_action.h with tadpole factors u0^-(length-1)."""
+    u2 = u0 * u0
+    return [5.0 / 8.0, -1.0 / 24.0 / u2, -1.0 / 16.0 / u2, 1.0 / 64.0 / u2 ** 2, -1.0 / 384.0 / u2 ** 3,
+            -1.0 / 16.0 / u2 ** 2]
 HISQ_FREE_C3 = -1.0 / 24.0

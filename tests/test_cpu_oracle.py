@@ -74,6 +74,30 @@ def test_golden_m_inverse(milc_l8):
     assert rel(x, ans) < 1e-6
 
 
+def test_golden_fn_force(milc_l8):
+    """fermion_force_fn_multi with the asqtad path table (u0 = 0.502), eps = 1,
+    one term r = 1, X = ksvector.src: MILC's colormatrix.mom.sample.l8
+    (ks_imp_utilities/test/check_fermion_force; FP32 file)."""
+    L, U, lex = milc_l8
+    src = golden("ksvector_src_l8.npy", lex)
+    ans = golden("colormatrix_mom_l8.npy", lex)
+    mom = L.fn_force(ko.asqtad_coeffs(0.502), L.rephase(U), [src], [1.0], eps=1.0)
+    assert rel(mom, ans) < 1e-6
+
+
+def test_fn_force_linear_in_residues_and_eps(milc_l8):
+    """the force is linear in eps and in each term's residue"""
+    L, U, lex = milc_l8
+    Up = L.rephase(U)
+    c = ko.asqtad_coeffs(0.502)
+    a = golden("ksvector_src_l8.npy", lex)
+    b = L.gaussian_vector(77, ko.EVENANDODD)
+    fa = L.fn_force(c, Up, [a], [1.0], eps=1.0)
+    fb = L.fn_force(c, Up, [b], [1.0], eps=1.0)
+    fab = L.fn_force(c, Up, [a, b], [0.3, -1.7], eps=0.25)
+    assert rel(fab, 0.25 * (0.3 * fa - 1.7 * fb)) < 1e-13
+
+
 # --------------------------------------------------------------------------
 # closed forms and symmetries
 # --------------------------------------------------------------------------
