@@ -16,6 +16,11 @@ kernel (the 16-bit Dslash) and the CPU baseline (the FP64 oracle CG timed on
 this host's cores over a bounded number of iterations).
 
 Usage: python bench.py [--gpus 1] [--steps 10] [--warmup 1] [--no-cpu]
+       python bench.py --workload multishift   (batched RHMC multi-shift: 24^3 x 48,
+           m = 0.02, 12 shifts sigma_i = 1e-4 10^(4i/11), 16 sources, per-shift
+           FP64 true residual 1e-9; prints its own JSON line)
+       python bench.py --workload hmc   (one RHMC force + FP64 action on 48^3 x 96,
+           m = 0.01, 12 shifts as above, residues 0.1 (i+1); prints its own JSON line)
 """
 import argparse
 import json
@@ -117,6 +122,163 @@ def cpu_baseline(G, fp64_iters, budget_s=15.0):
     }
 
 
+MS_DIMS = (24, 24, 24, 48)
+MS_MASS = 0.02
+MS_SIGMAS = [1e-4 * 10 ** (4 * i / 11) for i in range(12)]
+MS_GAUGE_SEED = 1002
+MS_SRC_SEEDS = [2200 + j for j in range(16)]
+MS_RESID = 1e-9        # per-shift FP64 true residual
+MS_STAGE_RESID = 5e-6  # FP32 multi-shift stage (base system), then per-shift refinement
+
+
+def multishift_workload(args):
+    """16 sources x 12 shifts in one batched solve (FP32 multi-shift with FP64
+    recurrences and deferred shift updates, captured window graph, autotuned
+    batched stencil; per-shift mixed refinement), against 16 one-source solves
+    of the same solver family and an FP64 reference solve of source 0."""
+    import milc_qcd_amd as mq
+    offs = [4 * MS_MASS ** 2 + s for s in MS_SIGMAS]
+    n = len(offs)
+    G = mq.Lattice(MS_DIMS)
+    G.dev_gauge_hisq(MS_GAUGE_SEED, 0.4)
+    for j, sd in enumerate(MS_SRC_SEEDS):
+        G.dev_source(sd, mq.EVEN, j)
+    Vh = G.Vh
+    # autotuner: every variant of the 16-rhs FP32 stencil, timed
+    nvar, chosen, us_best, desc = G.dev_block_tune(1, 16)
+    variants = []
+    for v in range(nvar):
+        us = G.dev_block_dslash_bench(1, 16, v, 20)
+        variants.append({"variant": v, "usec": round(us, 2)})
+    bytes_site = 16 * 72 + 16 * (24 + 24)
+    log(f"batched stencil: {nvar} variants, chosen {chosen} ({desc}) {us_best:.1f} us")
+    for i in range(args.warmup):
+        G.dev_multishift_block(list(range(16)), offs, MS_STAGE_RESID, MS_RESID)
+    t1 = time.perf_counter()
+    for i in range(args.steps):
+        st = G.dev_multishift_block(list(range(16)), offs, MS_STAGE_RESID, MS_RESID, out_slot0=16, out_nsrc=1)
+        if not st["converged"]:
+            raise RuntimeError(f"batched multi-shift did not converge: {st}")
+    t_batch = (time.perf_counter() - t1) / args.steps
+    log(f"batched: {t_batch:.3f} s per 16-source solve; {st}")
+    # FP64 reference of source 0 and the batched solutions' error against it
+    it64, t64, _ = G.dev_multishift(0, offs, 1e-12, 0, out_slot0=40)
+    errs = [G.dev_diff(16 + j, 40 + j) for j in range(n)]
+    # one-source solves (same FP32 + refinement scheme), all 16 sources
+    t1 = time.perf_counter()
+    seq_iters = []
+    for j in range(16):
+        it, _, conv = G.dev_multishift(j, offs, MS_RESID, 1)
+        seq_iters.append(it)
+    t_seq = time.perf_counter() - t1
+    us_f32 = G.dev_dslash_bench(1, mq.EVEN, 20)
+    G.close()
+    out = {
+        "metric": "batched multi-shift RHMC solve: 16 sources x 12 shifts to per-shift FP64 true residual 1e-9",
+        "value": round(t_batch, 4),
+        "unit": "s/solve (16 sources x 12 shifts)",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_batch * 1e3, 2),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 multi-shift + f64 recurrences, f32/f64 mixed refinement",
+        "data": "synthetic",
+        "config": {"workload": "hisq_multishift_batched_24x24x24x48_m0.02_12shifts_16src", "lattice": list(MS_DIMS),
+                   "mass": MS_MASS, "sigmas": MS_SIGMAS, "sources": 16, "gauge_seed": MS_GAUGE_SEED,
+                   "target_true_rel_resid": MS_RESID, "multishift_stage_resid": MS_STAGE_RESID},
+        "multishift_iters": st["ms_iters"],
+        "multishift_seconds": round(st["ms_seconds"], 4),
+        "refine_seconds": round(st["refine_seconds"], 4),
+        "refine_iters_per_shift": st["refine_iters"],
+        "true_rel_resid_per_shift_max": st["true_rel"],
+        "rel_err_vs_fp64_solve_src0": max(errs),
+        "fp64_reference": {"iters": it64, "seconds": round(t64, 4)},
+        "sequential_16_one_source_solves_s": round(t_seq, 4),
+        "speedup_batched_vs_sequential": round(t_seq / t_batch, 3),
+        "sequential_iters": seq_iters,
+        "autotune": {"chosen": chosen, "desc": desc, "variants": variants},
+        "batched_dslash": {"usec": round(us_best, 2), "alg_bytes_per_site": bytes_site,
+                           "GBps": round(bytes_site * Vh / us_best / 1e3, 1),
+                           "usec_per_rhs": round(us_best / 16, 2),
+                           "single_rhs_f32_dslash_usec": round(us_f32, 2)},
+        "roofline": {"bound": "hbm", "achieved": round(bytes_site * Vh / us_best / 1e3, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(bytes_site * Vh / us_best / 1e3 / HBM_PEAK_GBS, 4),
+                     "traffic": None, "kernel": f"batched stencil, 16 rhs ({desc})"},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
+HMC_DIMS = (48, 48, 48, 96)
+HMC_MASS = 0.01
+HMC_GAUGE_SEED = 1004
+HMC_SRC_SEED = 2400
+HMC_RESID = 1e-9
+
+
+def hmc_workload(args):
+    """One molecular-dynamics fermion force (12-shift mixed multi-shift, D_oe,
+    outer products) plus the FP64 fermion action on 48^3 x 96; the force is
+    compared with the same computation in FP64 on the device."""
+    import milc_qcd_amd as mq
+    offs = [4 * HMC_MASS ** 2 + s for s in MS_SIGMAS]
+    res = [0.1 * (i + 1) for i in range(len(offs))]
+    G = mq.Lattice(HMC_DIMS)
+    t0 = time.time()
+    G.dev_gauge_hisq(HMC_GAUGE_SEED, 0.4)
+    log(f"48^3x96 links: {time.time() - t0:.1f}s")
+    G.dev_source(HMC_SRC_SEED, mq.EVEN, 0)
+    Vh = G.Vh
+    dsl = {}
+    for prec, name in ((0, "f64"), (1, "f32"), (2, "h16")):
+        us = G.dev_dslash_bench(prec, mq.EVEN, 10)
+        dsl[name] = {"usec": round(us, 1), "GBps": round(DSLASH_BYTES[prec] * Vh / us / 1e3, 1),
+                     "frac_peak": round(DSLASH_BYTES[prec] * Vh / us / 1e3 / HBM_PEAK_GBS, 4)}
+    log(f"dslash {dsl}")
+    steps = []
+    for i in range(args.warmup + args.steps):
+        st = G.dev_hmc_step(0, offs, res, HMC_RESID, mixed=1, compare_fp64=(i == args.warmup))
+        log(f"step {i}: {st}")
+        if i >= args.warmup:
+            steps.append(st)
+    G.close()
+    t_step = sum(s["force_seconds"] + s["action_seconds"] for s in steps) / len(steps)
+    cmp = [s for s in steps if s["rel_err_O1"] >= 0][0]
+    out = {
+        "metric": "RHMC force + FP64 action time per MD step (48^3x96 HISQ, 12-shift mixed multi-shift)",
+        "value": round(t_step, 4),
+        "unit": "s/step",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_step * 1e3, 2),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 multi-shift + f64 recurrences, mixed refinement, f64 force and action",
+        "data": "synthetic",
+        "config": {"workload": "hisq_rhmc_force_action_48x48x48x96_m0.01_12shifts", "lattice": list(HMC_DIMS),
+                   "mass": HMC_MASS, "sigmas": MS_SIGMAS, "residues": res, "gauge_seed": HMC_GAUGE_SEED,
+                   "source_seed": HMC_SRC_SEED, "force_resid": HMC_RESID},
+        "force_seconds": round(sum(s["force_seconds"] for s in steps) / len(steps), 4),
+        "action_seconds": round(sum(s["action_seconds"] for s in steps) / len(steps), 4),
+        "force_iters": steps[-1]["force_iters"],
+        "action_iters": steps[-1]["action_iters"],
+        "action": steps[-1]["action"],
+        "force_rel_err_vs_fp64": {"O1": cmp["rel_err_O1"], "O3": cmp["rel_err_O3"]},
+        "fp64_force_seconds": round(cmp["fp64_force_seconds"], 4),
+        "dslash": dsl,
+        "roofline": {"bound": "hbm", "achieved": dsl["f32"]["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": dsl["f32"]["frac_peak"], "traffic": None,
+                     "kernel": "k_dslash<P_F32> (the multi-shift bulk, 1200 B/site alg.)"},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -125,11 +287,18 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--kernel-only", action="store_true",
                     help="only the Dslash and copy bursts (for rocprofv3 counter runs)")
+    ap.add_argument("--workload", choices=["cg", "multishift", "hmc"], default="cg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); "
               "the multi-GPU domain decomposition is a design, not a measured path", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "multishift":
+        multishift_workload(args)
+        return
+    if args.workload == "hmc":
+        hmc_workload(args)
+        return
 
     import milc_qcd_amd as mq
 

@@ -126,6 +126,32 @@ int milc_hip_asqtad_force(int precision, const double act_path_coeff[6], const v
 int milc_hip_fermion_force_fn_multi(int precision, double eps, const double* residues, void** multi_x, int nterms,
                                     const double act_path_coeff[6], const void* gauge, void* momentum);
 
+/* ---- many right-hand sides ------------------------------------------------- */
+/* Block CG: replaces ks_congrad_block_parity_gpu (generic_ks/d_congrad5_fn_gpu.c:174-176):
+ * nsrc sources on parity args.evenodd, solution[s] holds the initial guess on
+ * entry (d_congrad5_fn_gpu.c:104-110) and the solution on exit (only that
+ * parity's sites are written).  Batched FP32 iterations with FP64 reliable
+ * updates to the FP64 true residual |b - M^dag M x| / |b| < target_residual;
+ * final_residual[s] (may be NULL) gets each source's true residual.  Returns
+ * the total iteration count. */
+int milc_hip_invert_block(int external_precision, int nsrc, double mass, milc_hip_inv_args args,
+                          double target_residual, const void* fatlink, const void* longlink, void* const* source,
+                          void* const* solution, double* final_residual, int* num_iters);
+/* Multi-shift over many sources: the batched form of ks_multicg_offset_field_gpu
+ * (generic_ks/ks_multicg_offset_gpu.c:38-45), one call for the sources that
+ * MILC solves one after another (the pseudofermion loop of
+ * ks_imp_rhmc/update_h_rhmc.c:64-84, source loops of ks_spectrum).
+ * solution[s * num_offsets + j] = (M^dag M + offsets[j] - 4m^2)^-1 source[s]
+ * (offsets include 4m^2, ks_multicg_offset.c:184-194).  The multi-shift stage
+ * runs to target_residual[0] in FP32 with FP64 recurrences; refine_residual > 0
+ * then refines every shift to that FP64 true residual (mixed-precision CG,
+ * the per-shift refinement of ks_multicg_hybrid_field, ks_multicg.c:170-236).
+ * final_residual[s * num_offsets + j] gets the true residuals (after refinement). */
+int milc_hip_multishift_invert_block(int external_precision, int nsrc, int num_offsets, const double* offsets,
+                                     milc_hip_inv_args args, const double* target_residual, double refine_residual,
+                                     const void* fatlink, const void* longlink, void* const* source,
+                                     void* const* solution, double* final_residual, int* num_iters);
+
 /* ---- link construction ----------------------------------------------------- */
 /* Replaces qudaLoadUnitarizedLink + qudaLoadKSLink for the HISQ U(3) action
  * (generic_ks/fermion_links_hisq_load_milc.c:546-702,
@@ -177,6 +203,45 @@ int milc_hip_dev_dslash_bench(int prec, int parity, int reps, double* usec);
 /* STREAM-style device copy of `bytes` bytes, `reps` times (HIP events):
  * the measured HBM reference next to the vendor peak. */
 int milc_hip_dev_copy_bench(size_t bytes, int reps, double* usec);
+/* Batched stencil autotuner: times every variant (VALU threads-per-site
+ * shapes and the MFMA 4x4x1 form) for prec (1 float, 2 double) and nrhs
+ * (4, 8, 16) right-hand sides and keeps the fastest; returns the number of
+ * variants, the chosen index, its time and a description. */
+int milc_hip_dev_block_tune(int prec, int nrhs, int* variant, double* usec, char* desc, int ndesc);
+/* Time `reps` batched stencil applications of one variant (HIP events). */
+int milc_hip_dev_block_dslash_bench(int prec, int nrhs, int variant, int reps, double* usec);
+/* dst_slots[s] = D src_slots[s] on out_parity for s < nsrc <= nrhs, through
+ * the batched stencil (variant -1: the autotuned one). */
+int milc_hip_dev_block_dslash(int prec, int nrhs, int variant, int nsrc, const int* src_slots, int out_parity,
+                              const int* dst_slots);
+/* Batched multi-shift on device slots (see milc_hip_multishift_invert_block);
+ * solutions of the first out_nsrc sources to slots out_slot0 + s * num_offsets + j
+ * when out_slot0 >= 0.
+ * stats[0..5] = multi-shift iterations, multi-shift seconds, refinement
+ * seconds, converged, stencil variant, its time (us); stats[6 + j] = max true
+ * residual of shift j, stats[6 + num_offsets + j] = its refinement iterations.
+ * Returns the multi-shift iteration count. */
+int milc_hip_dev_multishift_block(int nsrc, const int* src_slots, int num_offsets, const double* offsets,
+                                  double resid, double refine_resid, int parity, int max_iters, int out_slot0,
+                                  int out_nsrc, double* stats);
+/* One source on device slots, as milc_hip_multishift_invert does for host
+ * arrays: FP64 multi-shift (mixed_precision 0) or FP32 multi-shift + mixed
+ * per-shift refinement (1) to the true residual resid; solutions to slots
+ * out_slot0 + j.  stats[0..2] = iterations, seconds, converged. */
+int milc_hip_dev_multishift(int slot_src, int parity, int num_offsets, const double* offsets, double resid,
+                            int mixed_precision, int out_slot0, double* stats);
+/* One RHMC force evaluation on the current device links and the FP64 action
+ * (the fermion part of update_h_rhmc.c:77-107 at level 0 and d_action_rhmc.c):
+ * X_t = (M^dag M + offsets[t] - 4m^2)^-1 phi (mixed_precision 1: FP32
+ * multi-shift + mixed refinement to resid; 0: FP64), outer products O1/O3 with
+ * coefficients residues[t], then S_act = sum_even |phi + sum_t residues[t] X_t|^2
+ * in FP64 (target min(resid, 1e-10)).  compare_fp64 != 0 repeats the force
+ * in FP64 (target min(resid, 1e-12)) and reports the relative difference.
+ * stats[0..8] = force seconds, force iterations, sum_t r_t Re<phi, X_t>, action,
+ * action seconds, action iterations, rel. error O1, rel. error O3 (-1 if not
+ * compared), FP64 force seconds. */
+int milc_hip_dev_hmc_step(int slot_phi, int n, const double* offsets, const double* residues, double resid,
+                          int mixed_precision, int compare_fp64, double* stats);
 /* |a - b| / |b| over the parity half held in two device slots. */
 int milc_hip_dev_diff(int slot_a, int slot_b, double* rel);
 

@@ -60,6 +60,7 @@ int upload_links(int ext_prec, const void* fat, const void* lng) {
   dev_free(stage);
   L.prec = P_F64;
   L.valid = true;
+  c.links_version++;
   return 0;
 }
 
@@ -161,6 +162,7 @@ int milc_hip_finalize(void) {
   }
   for (int s = 0; s < kSlots; s++) free_vec(g_slots[s]);
   fn_table_release();
+  batch_release();
   dev_free(c.ws.partials);
   dev_free(c.ws.scal);
   if (c.ws.scal_host) (void)hipHostFree(c.ws.scal_host);
@@ -376,6 +378,7 @@ extern "C" int milc_hip_dev_gauge_hisq(uint64_t seed, double scale, int* n_svd) 
   free_links(U);
   L.prec = P_F64;
   L.valid = true;
+  c.links_version++;
   return 0;
 }
 
@@ -845,4 +848,432 @@ extern "C" int milc_hip_fermion_force_fn_multi(int precision, double eps, const 
   rc = add_momentum(precision, b.mom, momentum);
   release();
   return rc;
+}
+
+// ---------------------------------------------------------------------------
+// many right-hand sides: batched stencil, multi-shift and CG
+// ---------------------------------------------------------------------------
+#include "mh_batch.h"
+
+namespace {
+int batch_width(int nsrc) { return nsrc <= 4 ? 4 : (nsrc <= 8 ? 8 : 16); }
+
+struct BBuf {
+  void* p = nullptr;
+  BBuf() = default;
+  BBuf(const BBuf&) = delete;
+  BBuf& operator=(const BBuf&) = delete;
+  ~BBuf() { dev_free(p); }
+};
+
+bool slot_ok(int s, bool need_data) {
+  return s >= 0 && s < kSlots && (!need_data || g_slots[s].data != nullptr);
+}
+
+int ensure_slot(int s) {
+  Context& c = ctx();
+  if (!g_slots[s].data) return alloc_vec(g_slots[s], P_F64, c.stride);
+  return 0;
+}
+}  // namespace
+
+extern "C" int milc_hip_dev_block_tune(int prec, int nrhs, int* variant, double* usec, char* desc, int ndesc) {
+  Context& c = ctx();
+  if (!c.initialised || !c.links[P_F64].valid) return -1;
+  int ip = prec == 2 ? P_F64 : P_F32;
+  if (ip == P_F32 && ensure_links(P_F32)) return -3;
+  int v;
+  double us;
+  int rc = btune(ip, nrhs, &v, &us);
+  if (rc) return rc;
+  if (variant) *variant = v;
+  if (usec) *usec = us;
+  if (desc && ndesc > 0) bdescribe(ip, nrhs, v, desc, ndesc);
+  return (int)bvariants(ip, nrhs).size();
+}
+
+extern "C" int milc_hip_dev_block_dslash_bench(int prec, int nrhs, int variant, int reps, double* usec) {
+  Context& c = ctx();
+  if (!c.initialised || !c.links[P_F64].valid) return -1;
+  int ip = prec == 2 ? P_F64 : P_F32;
+  if (ip == P_F32 && ensure_links(P_F32)) return -3;
+  return btime_variant(ip, nrhs, variant, reps, usec);
+}
+
+extern "C" int milc_hip_dev_block_dslash(int prec, int nrhs, int variant, int nsrc, const int* src_slots,
+                                         int out_parity, const int* dst_slots) {
+  Context& c = ctx();
+  if (!c.initialised || !c.links[P_F64].valid) return -1;
+  if (nsrc <= 0 || nsrc > nrhs || (nrhs != 4 && nrhs != 8 && nrhs != 16)) return -2;
+  int ip = prec == 2 ? P_F64 : P_F32;
+  int rc;
+  if (ip == P_F32 && (rc = ensure_links(P_F32))) return rc;
+  for (int s = 0; s < nsrc; s++)
+    if (!slot_ok(src_slots[s], true) || !slot_ok(dst_slots[s], false)) return -2;
+  BBuf a, b;
+  if ((rc = balloc(&a.p, ip, nrhs)) || (rc = balloc(&b.p, ip, nrhs))) return rc;
+  for (int s = 0; s < nsrc; s++)
+    if ((rc = bpack(g_slots[src_slots[s]], a.p, ip, nrhs, s))) return rc;
+  BDOp op;
+  op.prec = ip;
+  op.nr = nrhs;
+  op.kind = BK_STORE;
+  op.pb = pbit(out_parity);
+  op.src = a.p;
+  op.dst = b.p;
+  op.variant = variant;
+  if ((rc = bdslash(op, nullptr))) return rc;
+  for (int s = 0; s < nsrc; s++) {
+    if ((rc = ensure_slot(dst_slots[s]))) return rc;
+    if ((rc = bunpack(b.p, ip, nrhs, s, g_slots[dst_slots[s]]))) return rc;
+  }
+  MH_CHECK(hipStreamSynchronize(c.stream));
+  return 0;
+}
+
+extern "C" int milc_hip_dev_multishift_block(int nsrc, const int* src_slots, int num_offsets, const double* offsets,
+                                             double resid, double refine_resid, int parity, int max_iters,
+                                             int out_slot0, int out_nsrc, double* stats) {
+  Context& c = ctx();
+  if (!c.initialised || !c.links[P_F64].valid) return -1;
+  if (nsrc <= 0 || nsrc > BNR_MAX || num_offsets <= 0 || num_offsets > BMS_MAX) return -2;
+  if (parity != EVEN && parity != ODD) return -2;
+  for (int s = 0; s < nsrc; s++)
+    if (!slot_ok(src_slots[s], true)) return -2;
+  if (out_slot0 >= 0 && (out_nsrc < 0 || out_nsrc > nsrc || out_slot0 + out_nsrc * num_offsets > kSlots)) return -2;
+  int rc;
+  if ((rc = ensure_links(P_F32))) return rc;
+  const int nr = batch_width(nsrc);
+  BBuf b;
+  std::vector<BBuf> x(num_offsets);
+  if ((rc = balloc(&b.p, P_F64, nr))) return rc;
+  std::vector<void*> xp(num_offsets);
+  for (int j = 0; j < num_offsets; j++) {
+    if ((rc = balloc(&x[j].p, P_F64, nr))) return rc;
+    xp[j] = x[j].p;
+  }
+  for (int s = 0; s < nsrc; s++)
+    if ((rc = bpack(g_slots[src_slots[s]], b.p, P_F64, nr, s))) return rc;
+  BMSParams prm;
+  prm.parity = parity;
+  prm.max_iters = max_iters > 0 ? max_iters : 20000;
+  prm.resid = resid;
+  prm.refine_resid = refine_resid;
+  prm.delta = 0.1;
+  BMSStats st;
+  std::vector<double> offs(offsets, offsets + num_offsets);
+  if ((rc = bmultishift(b.p, nr, offs, prm, xp, st))) return rc;
+  if (out_slot0 >= 0) {
+    for (int s = 0; s < out_nsrc; s++)
+      for (int j = 0; j < num_offsets; j++) {
+        int slot = out_slot0 + s * num_offsets + j;
+        if ((rc = ensure_slot(slot)) || (rc = bunpack(x[j].p, P_F64, nr, s, g_slots[slot]))) return rc;
+      }
+    MH_CHECK(hipStreamSynchronize(c.stream));
+  }
+  if (stats) {
+    int v;
+    double us = 0;
+    btune(P_F32, nr, &v, &us);
+    stats[0] = st.ms_iters;
+    stats[1] = st.ms_seconds;
+    stats[2] = st.refine_seconds;
+    stats[3] = st.converged;
+    stats[4] = v;
+    stats[5] = us;
+    for (int j = 0; j < num_offsets; j++) {
+      stats[6 + j] = st.true_rel[j];
+      stats[6 + num_offsets + j] = st.refine_iters[j];
+    }
+  }
+  return st.ms_iters;
+}
+
+extern "C" int milc_hip_multishift_invert_block(int external_precision, int nsrc, int num_offsets,
+                                                const double* offsets, milc_hip_inv_args args,
+                                                const double* target_residual, double refine_residual,
+                                                const void* fatlink, const void* longlink, void* const* source,
+                                                void* const* solution, double* final_residual, int* num_iters) {
+  Context& c = ctx();
+  if (!c.initialised) return -1;
+  if ((external_precision != 1 && external_precision != 2) || nsrc <= 0 || num_offsets <= 0 ||
+      num_offsets > BMS_MAX || !offsets || !target_residual || !source || !solution)
+    return -2;
+  int parity = parity_from_evenodd(args.evenodd);
+  if (parity < 0) return -2;
+  int rc;
+  if ((rc = refresh_links(external_precision, fatlink, longlink, num_iters))) return rc;
+  if ((rc = ensure_links(P_F32))) return rc;
+  const int pb = pbit(parity);
+  int total = 0;
+  for (int s0 = 0; s0 < nsrc; s0 += BNR_MAX) {
+    int ns = std::min(BNR_MAX, nsrc - s0);
+    int nr = batch_width(ns);
+    BBuf b;
+    std::vector<BBuf> x(num_offsets);
+    std::vector<void*> xp(num_offsets);
+    if ((rc = balloc(&b.p, P_F64, nr))) return rc;
+    for (int j = 0; j < num_offsets; j++) {
+      if ((rc = balloc(&x[j].p, P_F64, nr))) return rc;
+      xp[j] = x[j].p;
+    }
+    VecField v;
+    if ((rc = alloc_vec(v, P_F64, c.stride))) return rc;
+    for (int s = 0; s < ns; s++) {
+      if ((rc = host_vec_in(external_precision, source[s0 + s], v, pb)) || (rc = bpack(v, b.p, P_F64, nr, s))) {
+        free_vec(v);
+        return rc;
+      }
+    }
+    BMSParams prm;
+    prm.parity = parity;
+    prm.max_iters = args.max_iter > 0 ? args.max_iter : 20000;
+    prm.resid = target_residual[0];
+    prm.refine_resid = refine_residual;
+    prm.delta = 0.1;
+    BMSStats st;
+    std::vector<double> offs(offsets, offsets + num_offsets);
+    if ((rc = bmultishift(b.p, nr, offs, prm, xp, st))) {
+      free_vec(v);
+      return rc;
+    }
+    for (int s = 0; s < ns; s++)
+      for (int j = 0; j < num_offsets; j++) {
+        if ((rc = bunpack(x[j].p, P_F64, nr, s, v)) ||
+            (rc = host_vec_out(external_precision, v, pb, solution[(size_t)(s0 + s) * num_offsets + j]))) {
+          free_vec(v);
+          return rc;
+        }
+        if (final_residual) final_residual[(size_t)(s0 + s) * num_offsets + j] = st.true_rel_r[j][s];
+      }
+    free_vec(v);
+    int it = st.ms_iters;
+    for (int j = 0; j < num_offsets; j++) it += st.refine_iters[j];
+    total += it;
+  }
+  if (num_iters) *num_iters = total;
+  return total;
+}
+
+extern "C" int milc_hip_invert_block(int external_precision, int nsrc, double mass, milc_hip_inv_args args,
+                                     double target_residual, const void* fatlink, const void* longlink,
+                                     void* const* source, void* const* solution, double* final_residual,
+                                     int* num_iters) {
+  Context& c = ctx();
+  if (!c.initialised) return -1;
+  if ((external_precision != 1 && external_precision != 2) || nsrc <= 0 || !source || !solution) return -2;
+  int parity = parity_from_evenodd(args.evenodd);
+  if (parity < 0) return -2;
+  int rc;
+  if ((rc = refresh_links(external_precision, fatlink, longlink, num_iters))) return rc;
+  if ((rc = ensure_links(P_F32))) return rc;
+  const int pb = pbit(parity);
+  int total = 0;
+  for (int s0 = 0; s0 < nsrc; s0 += BNR_MAX) {
+    int ns = std::min(BNR_MAX, nsrc - s0);
+    int nr = batch_width(ns);
+    BBuf b, x;
+    if ((rc = balloc(&b.p, P_F64, nr)) || (rc = balloc(&x.p, P_F64, nr))) return rc;
+    VecField v;
+    if ((rc = alloc_vec(v, P_F64, c.stride))) return rc;
+    for (int s = 0; s < ns; s++) {
+      // t_dest is the initial guess (d_congrad5_fn_gpu.c:104-110)
+      if ((rc = host_vec_in(external_precision, source[s0 + s], v, pb)) || (rc = bpack(v, b.p, P_F64, nr, s)) ||
+          (rc = host_vec_in(external_precision, solution[s0 + s], v, pb)) || (rc = bpack(v, x.p, P_F64, nr, s))) {
+        free_vec(v);
+        return rc;
+      }
+    }
+    double off[BNR_MAX];
+    for (int r = 0; r < BNR_MAX; r++) off[r] = 4.0 * mass * mass;
+    BCGParams cp;
+    cp.parity = parity;
+    cp.max_iters = args.max_iter > 0 ? args.max_iter : 20000;
+    cp.resid = target_residual;
+    cp.delta = 0.1;
+    BCGStats cs;
+    if ((rc = bcg_mixed(b.p, x.p, nr, off, cp, cs))) {
+      free_vec(v);
+      return rc;
+    }
+    for (int s = 0; s < ns; s++) {
+      if ((rc = bunpack(x.p, P_F64, nr, s, v)) ||
+          (rc = host_vec_out(external_precision, v, pb, solution[s0 + s]))) {
+        free_vec(v);
+        return rc;
+      }
+      if (final_residual) final_residual[s0 + s] = cs.true_rel[s];
+    }
+    free_vec(v);
+    total += cs.iters;
+  }
+  if (num_iters) *num_iters = total;
+  return total;
+}
+
+namespace mh {
+int multishift_dev(const VecField& src, int n, const double* offsets, double resid, int mixed, int parity,
+                   std::vector<VecField>& x, SolveStats& st);
+}
+
+extern "C" int milc_hip_dev_multishift(int slot_src, int parity, int num_offsets, const double* offsets, double resid,
+                                       int mixed_precision, int out_slot0, double* stats) {
+  Context& c = ctx();
+  if (!c.initialised || !c.links[P_F64].valid) return -1;
+  if (!slot_ok(slot_src, true) || num_offsets <= 0 || num_offsets > 32 || (parity != EVEN && parity != ODD)) return -2;
+  if (out_slot0 >= 0 && out_slot0 + num_offsets > kSlots) return -2;
+  int rc;
+  if (mixed_precision && (rc = ensure_links(P_F32))) return rc;
+  std::vector<VecField> x;
+  SolveStats st;
+  rc = multishift_dev(g_slots[slot_src], num_offsets, offsets, resid, mixed_precision ? 1 : 0, parity, x, st);
+  if (!rc && out_slot0 >= 0)
+    for (int j = 0; j < num_offsets && !rc; j++)
+      if (!(rc = ensure_slot(out_slot0 + j))) rc = vec_copy(x[j], g_slots[out_slot0 + j], P_F64);
+  MH_CHECK(hipStreamSynchronize(c.stream));
+  for (auto& v : x) free_vec(v);
+  if (rc) return rc;
+  if (stats) {
+    stats[0] = st.iters;
+    stats[1] = st.seconds;
+    stats[2] = st.converged;
+  }
+  return st.iters;
+}
+
+// ---------------------------------------------------------------------------
+// one RHMC molecular-dynamics force evaluation plus the FP64 action, on device
+// ---------------------------------------------------------------------------
+namespace {
+// |a - b|^2 and |b|^2 over the valid sites of two FP64 link fields
+__global__ void k_link_diff(const double2* __restrict__ a, const double2* __restrict__ b, int stride, int Vh,
+                            size_t n, double* __restrict__ out) {
+  double s0 = 0, s1 = 0;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    if ((int)(i % stride) >= Vh) continue;
+    double2 x = a[i], y = b[i];
+    double dr = x.x - y.x, di = x.y - y.y;
+    s0 += dr * dr + di * di;
+    s1 += y.x * y.x + y.y * y.y;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    s0 += __shfl_xor(s0, off, 64);
+    s1 += __shfl_xor(s1, off, 64);
+  }
+  __shared__ double sm[2][16];
+  if ((threadIdx.x & 63) == 0) {
+    sm[0][threadIdx.x >> 6] = s0;
+    sm[1][threadIdx.x >> 6] = s1;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t0 = 0, t1 = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) {
+      t0 += sm[0][w];
+      t1 += sm[1][w];
+    }
+    out[2 * blockIdx.x] = t0;
+    out[2 * blockIdx.x + 1] = t1;
+  }
+}
+
+int link_rel_diff(const LinkField& a, const LinkField& b, double* rel) {
+  Context& c = ctx();
+  const int nb = 1024;
+  size_t n = (size_t)72 * a.stride;
+  hipLaunchKernelGGL(k_link_diff, dim3(nb), dim3(256), 0, c.stream, (const double2*)a.data, (const double2*)b.data,
+                     a.stride, c.Vh, n, c.ws.partials);
+  MH_CHECK(hipGetLastError());
+  std::vector<double> h(2 * nb);
+  MH_CHECK(hipMemcpyAsync(h.data(), c.ws.partials, h.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  MH_CHECK(hipStreamSynchronize(c.stream));
+  double d = 0, m = 0;
+  for (int i = 0; i < nb; i++) {
+    d += h[2 * i];
+    m += h[2 * i + 1];
+  }
+  *rel = m > 0 ? std::sqrt(d / m) : 0.0;
+  return 0;
+}
+}  // namespace
+
+extern "C" int milc_hip_dev_hmc_step(int slot_phi, int n, const double* offsets, const double* residues,
+                                     double resid, int mixed_precision, int compare_fp64, double* stats) {
+  Context& c = ctx();
+  if (!c.initialised || !slot_ok(slot_phi, true) || !c.links[P_F64].valid) return -1;
+  if (n <= 0 || n > 32 || !stats) return -2;
+  for (int k = 0; k < n; k++)
+    if (!(offsets[k] > 0)) return -2;
+  int rc;
+  if (mixed_precision && (rc = ensure_links(P_F32))) return rc;
+  LinkField o1{}, o3{}, r1{}, r3{};
+  auto release = [&]() {
+    free_links(o1);
+    free_links(o3);
+    free_links(r1);
+    free_links(r3);
+  };
+  if ((rc = alloc_links(o1, P_F64, c.stride)) || (rc = alloc_links(o3, P_F64, c.stride))) {
+    release();
+    return rc;
+  }
+  // force: multi-shift, D_oe, outer products (update_h_rhmc.c:77-107 with level 0)
+  auto t0 = std::chrono::steady_clock::now();
+  double S = 0;
+  int iters = 0;
+  rc = rhmc_force_level0(g_slots[slot_phi], n, offsets, residues, resid, mixed_precision ? 1 : 0, &S, o1, o3, &iters);
+  (void)hipStreamSynchronize(c.stream);
+  double t_force = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (rc) {
+    release();
+    return rc;
+  }
+  // FP64 action sum_even |a0 phi + sum_k a_k x_k|^2 (d_action_rhmc.c), a0 = 1
+  std::vector<double> roots(n + 1), res(n + 1);
+  roots[0] = 0;
+  res[0] = 1.0;
+  for (int k = 0; k < n; k++) {
+    roots[k + 1] = offsets[k];
+    res[k + 1] = residues[k];
+  }
+  t0 = std::chrono::steady_clock::now();
+  double action = 0;
+  int it_act = 0;
+  rc = rhmc_action(g_slots[slot_phi], n, roots.data(), res.data(), std::min(resid, 1e-10), 0, &action, &it_act);
+  (void)hipStreamSynchronize(c.stream);
+  double t_action = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (rc) {
+    release();
+    return rc;
+  }
+  double e1 = -1, e3 = -1, t_ref = 0;
+  if (compare_fp64) {
+    if ((rc = alloc_links(r1, P_F64, c.stride)) || (rc = alloc_links(r3, P_F64, c.stride))) {
+      release();
+      return rc;
+    }
+    double S2;
+    int it2;
+    t0 = std::chrono::steady_clock::now();
+    rc = rhmc_force_level0(g_slots[slot_phi], n, offsets, residues, std::min(resid, 1e-12), 0, &S2, r1, r3, &it2);
+    (void)hipStreamSynchronize(c.stream);
+    t_ref = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (!rc) rc = link_rel_diff(o1, r1, &e1);
+    if (!rc) rc = link_rel_diff(o3, r3, &e3);
+    if (rc) {
+      release();
+      return rc;
+    }
+  }
+  release();
+  stats[0] = t_force;
+  stats[1] = iters;
+  stats[2] = S;
+  stats[3] = action;
+  stats[4] = t_action;
+  stats[5] = it_act;
+  stats[6] = e1;
+  stats[7] = e3;
+  stats[8] = t_ref;
+  return 0;
 }

@@ -53,6 +53,7 @@ struct Context {
   const void* link_key = nullptr;
   const void* link_key2 = nullptr;
   DevLinks links[3];   // by Prec
+  unsigned links_version = 0;  // bumped whenever the FP64 links change
   double* staging = nullptr;   // FP64 AoS staging buffer, 4*V*18 doubles
   size_t staging_elems = 0;
   int block = 256;
@@ -118,6 +119,8 @@ int force_oprod(const std::vector<VecField>& xe, const std::vector<VecField>& xo
                 const std::vector<double>& residues, LinkField& o1, LinkField& o3);
 // device path table of the FN force (freed with the context)
 void fn_table_release();
+// batched-solver caches (freed with the context)
+void batch_release();
 
 // ---- links ------------------------------------------------------------------
 int gauge_random_dev(uint64_t seed, double scale, LinkField& U);

@@ -122,6 +122,17 @@ static int rational_solve(const VecField& phi, int n, const double* offsets, dou
   return 0;
 }
 
+// Multi-shift solve of one source on the current links (FP64, or FP32 bulk +
+// mixed refinement); x is allocated here.
+int multishift_dev(const VecField& src, int n, const double* offsets, double resid, int mixed, int parity,
+                   std::vector<VecField>& x, SolveStats& st) {
+  auto t0 = std::chrono::steady_clock::now();
+  int rc = rational_solve(src, n, offsets, resid, mixed, parity, x, st);
+  (void)hipStreamSynchronize(ctx().stream);
+  st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return rc;
+}
+
 int rhmc_action(const VecField& phi, int order, const double* roots, const double* residues, double resid, int mixed,
                 double* action, int* iters) {
   Context& c = ctx();

@@ -46,6 +46,9 @@ EXPORTS = [
     "milc_hip_dev_dslash_bench", "milc_hip_dev_copy_bench", "milc_hip_dev_diff",
     "milc_hip_dev_rhmc_action", "milc_hip_dev_force_level0", "milc_hip_dev_load_links",
     "milc_hip_asqtad_force", "milc_hip_fermion_force_fn_multi",
+    "milc_hip_invert_block", "milc_hip_multishift_invert_block", "milc_hip_dev_block_tune",
+    "milc_hip_dev_block_dslash_bench", "milc_hip_dev_block_dslash", "milc_hip_dev_multishift_block",
+    "milc_hip_dev_multishift", "milc_hip_dev_hmc_step",
 ]
 
 
@@ -85,6 +88,17 @@ def load_library(path=LIB_PATH):
     L.milc_hip_asqtad_force.argtypes = [I, ctypes.POINTER(D), ctypes.POINTER(P), ctypes.POINTER(P), P, P]
     L.milc_hip_fermion_force_fn_multi.argtypes = [I, D, ctypes.POINTER(D), ctypes.POINTER(P), I,
                                                   ctypes.POINTER(D), P, P]
+    PI = ctypes.POINTER(I)
+    PD = ctypes.POINTER(D)
+    L.milc_hip_invert_block.argtypes = [I, I, D, InvArgs, D, P, P, ctypes.POINTER(P), ctypes.POINTER(P), PD, PI]
+    L.milc_hip_multishift_invert_block.argtypes = [I, I, I, PD, InvArgs, PD, D, P, P, ctypes.POINTER(P),
+                                                   ctypes.POINTER(P), PD, PI]
+    L.milc_hip_dev_block_tune.argtypes = [I, I, PI, PD, ctypes.c_char_p, I]
+    L.milc_hip_dev_block_dslash_bench.argtypes = [I, I, I, I, PD]
+    L.milc_hip_dev_block_dslash.argtypes = [I, I, I, I, PI, I, PI]
+    L.milc_hip_dev_multishift_block.argtypes = [I, PI, I, PD, D, D, I, I, I, I, PD]
+    L.milc_hip_dev_multishift.argtypes = [I, I, I, PD, D, I, I, PD]
+    L.milc_hip_dev_hmc_step.argtypes = [I, I, PD, PD, D, I, I, PD]
     _lib = L
     return L
 
@@ -231,6 +245,62 @@ class Lattice:
         qic.converged = int(res.value < qic.resid)
         return it
 
+    def ks_congrad_field(self, src, dest, qic: QuarkInvertControl, mass):
+        """EVEN / ODD / EVENANDODD dispatch of generic_ks/d_congrad5_fn.c:16-36:
+        EVENANDODD solves the even, then the odd sites; qic.parity is restored."""
+        parity = qic.parity
+        iters = 0
+        if parity in (EVEN, EVENANDODD):
+            qic.parity = EVEN
+            iters += self.ks_congrad_parity(src, dest, qic, mass)
+        if parity in (ODD, EVENANDODD):
+            qic.parity = ODD
+            iters += self.ks_congrad_parity(src, dest, qic, mass)
+        qic.parity = parity
+        return iters
+
+    def ks_congrad_block_parity(self, srcs, dests, qic: QuarkInvertControl, mass):
+        """ks_congrad_block_parity_gpu (d_congrad5_fn_gpu.c:174-176) through the
+        batched solver: dests[s] are initial guesses, overwritten on qic.parity.
+        Returns (iterations, true residual per source)."""
+        self._activate()
+        n = len(srcs)
+        srcs = [np.ascontiguousarray(x, dtype=np.float64) for x in srcs]
+        for d in dests:
+            if d.dtype != np.float64 or not d.flags.c_contiguous:
+                raise MilcHipError("dests must be C-contiguous float64 arrays")
+        sp = (ctypes.c_void_p * n)(*[x.ctypes.data for x in srcs])
+        dp = (ctypes.c_void_p * n)(*[d.ctypes.data for d in dests])
+        fres = (ctypes.c_double * n)()
+        args = InvArgs(qic.max * qic.nrestart, _evenodd(qic.parity), 1, 0.0, 1.0, qic.max)
+        it = _check(self.lib.milc_hip_invert_block(2, n, mass, args, qic.resid, _p(self.fat), _p(self.lng), sp, dp,
+                                                   fres, ctypes.byref(self._refresh_flag())),
+                    "milc_hip_invert_block")
+        qic.final_iters = it
+        qic.final_rsq = max(fres) ** 2
+        qic.converged = int(max(fres) < qic.resid)
+        return it, list(fres)
+
+    def ks_multicg_offset_block(self, srcs, offsets, parity, resid, refine_resid, max_iter=20000):
+        """Multi-shift for several sources at once; returns (psim[s][j], true
+        residual [s][j], iterations)."""
+        self._activate()
+        ns, n = len(srcs), len(offsets)
+        srcs = [np.ascontiguousarray(x, dtype=np.float64) for x in srcs]
+        psim = [[np.zeros_like(srcs[0]) for _ in range(n)] for _ in range(ns)]
+        sp = (ctypes.c_void_p * ns)(*[x.ctypes.data for x in srcs])
+        xp = (ctypes.c_void_p * (ns * n))(*[psim[s][j].ctypes.data for s in range(ns) for j in range(n)])
+        offs = (ctypes.c_double * n)(*offsets)
+        tr = (ctypes.c_double * 1)(resid)
+        fres = (ctypes.c_double * (ns * n))()
+        args = InvArgs(max_iter, _evenodd(parity), 1, 0.0, 1.0, 0)
+        it = _check(self.lib.milc_hip_multishift_invert_block(2, ns, n, offs, args, tr, refine_resid, _p(self.fat),
+                                                              _p(self.lng), sp, xp, fres,
+                                                              ctypes.byref(self._refresh_flag())),
+                    "milc_hip_multishift_invert_block")
+        res = [[fres[s * n + j] for j in range(n)] for s in range(ns)]
+        return psim, res, it
+
     def ks_multicg_offset_field(self, src, ksp, qic, mixed=0, prec=2):
         """Multi-shift CG; returns the list of solutions psim[j]."""
         self._activate()
@@ -319,6 +389,65 @@ class Lattice:
                     "milc_hip_dev_invert")
         return it, {"resid": st[0], "seconds": st[1], "reliable_updates": int(st[2]), "converged": int(st[3]),
                     "iters": int(st[4]), "final_rsq": st[5]}
+
+    def dev_block_tune(self, prec, nrhs):
+        """(number of variants, chosen variant, its time in us, description)"""
+        self._activate()
+        v, us = ctypes.c_int(0), ctypes.c_double(0)
+        buf = ctypes.create_string_buffer(256)
+        n = _check(self.lib.milc_hip_dev_block_tune(prec, nrhs, ctypes.byref(v), ctypes.byref(us), buf, 256),
+                   "milc_hip_dev_block_tune")
+        return n, v.value, us.value, buf.value.decode()
+
+    def dev_block_dslash_bench(self, prec, nrhs, variant, reps):
+        self._activate()
+        us = ctypes.c_double(0)
+        _check(self.lib.milc_hip_dev_block_dslash_bench(prec, nrhs, variant, reps, ctypes.byref(us)),
+               "milc_hip_dev_block_dslash_bench")
+        return us.value
+
+    def dev_block_dslash(self, prec, nrhs, variant, src_slots, out_parity, dst_slots):
+        self._activate()
+        n = len(src_slots)
+        a = (ctypes.c_int * n)(*src_slots)
+        b = (ctypes.c_int * n)(*dst_slots)
+        _check(self.lib.milc_hip_dev_block_dslash(prec, nrhs, variant, n, a, out_parity, b), "milc_hip_dev_block_dslash")
+
+    def dev_multishift_block(self, src_slots, offsets, resid, refine_resid, parity=EVEN, max_iters=20000,
+                             out_slot0=-1, out_nsrc=None):
+        self._activate()
+        ns, n = len(src_slots), len(offsets)
+        a = (ctypes.c_int * ns)(*src_slots)
+        offs = (ctypes.c_double * n)(*offsets)
+        st = (ctypes.c_double * (6 + 2 * n))()
+        on = ns if out_nsrc is None else out_nsrc
+        it = _check(self.lib.milc_hip_dev_multishift_block(ns, a, n, offs, resid, refine_resid, parity, max_iters,
+                                                           out_slot0, on, st), "milc_hip_dev_multishift_block")
+        return {"ms_iters": int(st[0]), "ms_seconds": st[1], "refine_seconds": st[2], "converged": int(st[3]),
+                "variant": int(st[4]), "variant_usec": st[5], "true_rel": list(st[6:6 + n]),
+                "refine_iters": [int(x) for x in st[6 + n:6 + 2 * n]], "iters": it}
+
+    def dev_multishift(self, slot, offsets, resid, mixed, parity=EVEN, out_slot0=-1):
+        """one source: (iterations, seconds, converged)"""
+        self._activate()
+        n = len(offsets)
+        offs = (ctypes.c_double * n)(*offsets)
+        st = (ctypes.c_double * 3)()
+        it = _check(self.lib.milc_hip_dev_multishift(slot, parity, n, offs, resid, mixed, out_slot0, st),
+                    "milc_hip_dev_multishift")
+        return it, st[1], int(st[2])
+
+    def dev_hmc_step(self, slot_phi, offsets, residues, resid, mixed=1, compare_fp64=True):
+        self._activate()
+        n = len(offsets)
+        offs = (ctypes.c_double * n)(*offsets)
+        res = (ctypes.c_double * n)(*residues)
+        st = (ctypes.c_double * 9)()
+        _check(self.lib.milc_hip_dev_hmc_step(slot_phi, n, offs, res, resid, mixed, int(bool(compare_fp64)), st),
+               "milc_hip_dev_hmc_step")
+        return {"force_seconds": st[0], "force_iters": int(st[1]), "S": st[2], "action": st[3],
+                "action_seconds": st[4], "action_iters": int(st[5]), "rel_err_O1": st[6], "rel_err_O3": st[7],
+                "fp64_force_seconds": st[8]}
 
     def dev_read(self, slot, parity):
         self._activate()
