@@ -918,37 +918,38 @@ int kso_fn_paths(int mu, const double coeff[6], int *dirs, double *c, int *len, 
   return n;
 }
 
-void kso_fn_force(const kso_geom *g, const double coeff[6], const double *U, const double *const *multi_x,
-                  const double *residues, int nterms, double eps, double *mom) {
+/* Adjoint of a path-table smearing X_mu(x) = sum_p c_p P_p(x): given the
+ * force on X (non-Naik paths read F1, Naik paths F3; with_eps multiplies the
+ * force at the path start x by eps(x)), add to G the force on the links L of
+ * the paths, all in the convention dS = sum 2 Re Tr[dA_mu(x) F_mu(x)]. */
+static void path_adjoint(const kso_geom *g, const double coeff[6], const double *L, const double *F1,
+                         const double *F3, int with_eps, double *G) {
   const size_t V = g->V;
-  double *o1 = (double *)malloc(sizeof(double) * 18 * 4 * V);
-  double *o3 = (double *)malloc(sizeof(double) * 18 * 4 * V);
-  double *G = (double *)calloc(18 * 4 * V, sizeof(double));
-  kso_force_oprod(g, multi_x, residues, nterms, o1, o3);
   for (int mu = 0; mu < 4; mu++) {
     kso_path t[128];
     int np = fn_paths(mu, coeff, t);
     for (int p = 0; p < np; p++) {
       const kso_path *P = &t[p];
+      if (P->c == 0.0) continue;
       /* serial over sites: contributions scatter to neighbours */
       for (size_t x = 0; x < V; x++) {
         size_t y[8];
-        double L[7][18], pre[8][18], suf[8][18], Ob[18];
+        double Lk[7][18], pre[8][18], suf[8][18], Ob[18];
         y[0] = x;
         for (int j = 0; j < P->n; j++) {
           int d = P->d[j];
-          if (d < 4) { memcpy(L[j], U + (4 * y[j] + d) * 18, sizeof L[j]); y[j + 1] = NB(g, d, y[j]); }
-          else { int dp = 7 - d; y[j + 1] = NB(g, d, y[j]); m_adj(U + (4 * y[j + 1] + dp) * 18, L[j]); }
+          if (d < 4) { memcpy(Lk[j], L + (4 * y[j] + d) * 18, sizeof Lk[j]); y[j + 1] = NB(g, d, y[j]); }
+          else { int dp = 7 - d; y[j + 1] = NB(g, d, y[j]); m_adj(L + (4 * y[j + 1] + dp) * 18, Lk[j]); }
         }
         /* prefix products pre[j] = L_0..L_j-1, suffix suf[j] = L_j+1..L_n-1 */
         memset(pre[0], 0, sizeof pre[0]);
         RE(pre[0], 0, 0) = RE(pre[0], 1, 1) = RE(pre[0], 2, 2) = 1.0;
-        for (int j = 0; j < P->n; j++) m_nn(pre[j], L[j], pre[j + 1]);
+        for (int j = 0; j < P->n; j++) m_nn(pre[j], Lk[j], pre[j + 1]);
         memset(suf[P->n - 1], 0, sizeof suf[0]);
         RE(suf[P->n - 1], 0, 0) = RE(suf[P->n - 1], 1, 1) = RE(suf[P->n - 1], 2, 2) = 1.0;
-        for (int j = P->n - 2; j >= 0; j--) m_nn(L[j + 1], suf[j + 1], suf[j]);
-        double ex = x < g->Vh ? 1.0 : -1.0;
-        const double *O = (P->naik ? o3 : o1) + (4 * x + mu) * 18;
+        for (int j = P->n - 2; j >= 0; j--) m_nn(Lk[j + 1], suf[j + 1], suf[j]);
+        double ex = (with_eps && x >= g->Vh) ? -1.0 : 1.0;
+        const double *O = (P->naik ? F3 : F1) + (4 * x + mu) * 18;
         for (int k = 0; k < 18; k++) Ob[k] = ex * O[k];
         for (int j = 0; j < P->n; j++) {
           double A[18], M[18];
@@ -965,12 +966,15 @@ void kso_fn_force(const kso_geom *g, const double coeff[6], const double *U, con
       }
     }
   }
+}
+
+/* mom = TA(scale U G) per link, full anti-Hermitian matrices (libraries/make_ahmat.c) */
+static void ta_force(const kso_geom *g, const double *U, const double *G, double scale, double *mom) {
 #pragma omp parallel for
-  for (size_t i = 0; i < 4 * V; i++) {
+  for (size_t i = 0; i < 4 * g->V; i++) {
     double UG[18], m[18];
     m_nn(U + 18 * i, G + 18 * i, UG);
-    for (int k = 0; k < 18; k++) UG[k] *= 2.0 * eps;
-    /* traceless anti-Hermitian part, stored as a full 3x3 matrix */
+    for (int k = 0; k < 18; k++) UG[k] *= scale;
     double tr = (IM(UG, 0, 0) + IM(UG, 1, 1) + IM(UG, 2, 2)) / 3.0;
     for (int a = 0; a < 3; a++)
       for (int b = 0; b < 3; b++) {
@@ -980,5 +984,168 @@ void kso_fn_force(const kso_geom *g, const double coeff[6], const double *U, con
     for (int a = 0; a < 3; a++) IM(m, a, a) -= tr;
     memcpy(mom + 18 * i, m, sizeof m);
   }
+}
+
+void kso_fn_force(const kso_geom *g, const double coeff[6], const double *U, const double *const *multi_x,
+                  const double *residues, int nterms, double eps, double *mom) {
+  const size_t V = g->V;
+  double *o1 = (double *)malloc(sizeof(double) * 18 * 4 * V);
+  double *o3 = (double *)malloc(sizeof(double) * 18 * 4 * V);
+  double *G = (double *)calloc(18 * 4 * V, sizeof(double));
+  kso_force_oprod(g, multi_x, residues, nterms, o1, o3);
+  path_adjoint(g, coeff, U, o1, o3, 1, G);
+  ta_force(g, U, G, 2.0 * eps, mom);
   free(o1); free(o3); free(G);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Derivative of the U(3) projection W = V (V^dag V)^-1/2 (the role of        */
+/* u3_unit_der_analytic, generic_ks/su3_mat_op.c:1379, and                    */
+/* fn_fermion_force_multi_hisq_reunit, fermion_force_hisq_multi.c:1877-1998). */
+/* With M = V^dag V = E diag(l) E^dag, Q = M^-1/2 and                          */
+/* dQ = E [ f(l_i,l_j) (E^dag dM E)_ij ] E^dag,                               */
+/* f(a,b) = (a^-1/2 - b^-1/2)/(a - b) = -1/(sqrt a sqrt b (sqrt a + sqrt b)),  */
+/* a force F_W (dS = 2 Re Tr[dW F_W]) becomes                                  */
+/*   F_V = Q F_W + L(K + K^dag) V^dag,  K = F_W V,  L(H) = E [f_ij H'_ij] E^dag */
+/* MILC's force filter: when min l < 5e-5 all l are raised by 5e-5            */
+/* (HISQ_FORCE_FILTER of ks_imp_rhmc/Make_template).  E from cyclic complex  */
+/* Jacobi rotations.                                                          */
+/* ------------------------------------------------------------------------ */
+#include <complex.h>
+
+static void jacobi_herm3(double complex a[3][3], double complex e[3][3], double l[3]) {
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) e[i][j] = i == j ? 1.0 : 0.0;
+  for (int sweep = 0; sweep < 50; sweep++) {
+    double off = cabs(a[0][1]) + cabs(a[0][2]) + cabs(a[1][2]);
+    double diag = fabs(creal(a[0][0])) + fabs(creal(a[1][1])) + fabs(creal(a[2][2]));
+    if (off <= 1e-18 * diag) break;
+    for (int p = 0; p < 2; p++)
+      for (int q = p + 1; q < 3; q++) {
+        double r = cabs(a[p][q]);
+        if (r == 0.0) continue;
+        double complex ph = a[p][q] / r;  /* e^{i phi} */
+        double tau = (creal(a[q][q]) - creal(a[p][p])) / (2 * r);
+        double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1 + tau * tau));
+        double c = 1 / sqrt(1 + t * t), s = t * c;
+        /* J = diag(1, e^{-i phi}) [[c, s], [-s, c]] in the (p, q) plane */
+        double complex Jpp = c, Jpq = s, Jqp = -s * conj(ph), Jqq = c * conj(ph);
+        /* a <- a J */
+        for (int k = 0; k < 3; k++) {
+          double complex akp = a[k][p], akq = a[k][q];
+          a[k][p] = akp * Jpp + akq * Jqp;
+          a[k][q] = akp * Jpq + akq * Jqq;
+        }
+        /* a <- J^dag a */
+        for (int k = 0; k < 3; k++) {
+          double complex apk = a[p][k], aqk = a[q][k];
+          a[p][k] = conj(Jpp) * apk + conj(Jqp) * aqk;
+          a[q][k] = conj(Jpq) * apk + conj(Jqq) * aqk;
+        }
+        /* e <- e J */
+        for (int k = 0; k < 3; k++) {
+          double complex ekp = e[k][p], ekq = e[k][q];
+          e[k][p] = ekp * Jpp + ekq * Jqp;
+          e[k][q] = ekp * Jpq + ekq * Jqq;
+        }
+      }
+  }
+  for (int i = 0; i < 3; i++) l[i] = creal(a[i][i]);
+}
+
+void kso_u3_adjoint(const double *Vm, const double *FW, double *FV) {
+  double complex v[3][3], f[3][3], m[3][3], e[3][3], q[3][3], k[3][3], h[3][3], t[3][3], lm[3][3];
+  double l[3], sq[3];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      v[i][j] = RE(Vm, i, j) + I * IM(Vm, i, j);
+      f[i][j] = RE(FW, i, j) + I * IM(FW, i, j);
+    }
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      m[i][j] = 0;
+      for (int c = 0; c < 3; c++) m[i][j] += conj(v[c][i]) * v[c][j];
+    }
+  jacobi_herm3(m, e, l);
+  double lmin = fmin(l[0], fmin(l[1], l[2]));
+  if (lmin < 5.0e-5)
+    for (int i = 0; i < 3; i++) l[i] += 5.0e-5;
+  for (int i = 0; i < 3; i++) sq[i] = sqrt(l[i]);
+  /* Q = E diag(1/sqrt l) E^dag */
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      q[i][j] = 0;
+      for (int c = 0; c < 3; c++) q[i][j] += e[i][c] * (1.0 / sq[c]) * conj(e[j][c]);
+    }
+  /* K = F V, H = K + K^dag */
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      k[i][j] = 0;
+      for (int c = 0; c < 3; c++) k[i][j] += f[i][c] * v[c][j];
+    }
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) h[i][j] = k[i][j] + conj(k[j][i]);
+  /* H' = E^dag H E, scaled by f(l_i, l_j), back: L = E H' E^dag */
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      t[i][j] = 0;
+      for (int c = 0; c < 3; c++) t[i][j] += h[i][c] * e[c][j];
+    }
+  double complex hp[3][3];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      double complex s2 = 0;
+      for (int c = 0; c < 3; c++) s2 += conj(e[c][i]) * t[c][j];
+      hp[i][j] = s2 * (-1.0 / (sq[i] * sq[j] * (sq[i] + sq[j])));
+    }
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      t[i][j] = 0;
+      for (int c = 0; c < 3; c++) t[i][j] += e[i][c] * hp[c][j];
+    }
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      lm[i][j] = 0;
+      for (int c = 0; c < 3; c++) lm[i][j] += t[i][c] * conj(e[j][c]);
+    }
+  /* F_V = Q F + L V^dag */
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      double complex s2 = 0;
+      for (int c = 0; c < 3; c++) s2 += q[i][c] * f[c][j] + lm[i][c] * conj(v[j][c]);
+      RE(FV, i, j) = creal(s2);
+      IM(FV, i, j) = cimag(s2);
+    }
+}
+
+/* HISQ thin-link force (fn_fermion_force_multi_hisq_wrapper_mx_cpu,
+ * generic_ks/fermion_force_hisq_multi.c:1182-1475, one Naik set):
+ * O1/O3 (level 0) -> level-2 path adjoint on W -> U(3) projection adjoint at
+ * V -> level-1 (fat7) path adjoint on U -> mom = TA(2 eps U F_U).
+ * U has the KS phases in; V and W are rebuilt here as kso_hisq_links does. */
+void kso_hisq_force(const kso_geom *g, const double *U, const double *const *multi_x, const double *residues,
+                    int nterms, double eps, double *mom) {
+  const size_t n = 4 * g->V * 18;
+  const double c1[6] = {1.0 / 8.0, 0.0, -1.0 / 8.0 * 0.5, 1.0 / 8.0 * 0.25 * 0.5,
+                        -1.0 / 8.0 * 0.125 * (1.0 / 6.0), 0.0};
+  const double c2[6] = {(1.0 / 8.0) + (2.0 * 6.0 / 16.0) + (1.0 / 8.0), -1.0 / 24.0,
+                        -1.0 / 8.0 * 0.5, 1.0 / 8.0 * 0.25 * 0.5,
+                        -1.0 / 8.0 * 0.125 * (1.0 / 6.0), -2.0 / 16.0};
+  double *Vl = (double *)malloc(sizeof(double) * n);
+  double *W = (double *)malloc(sizeof(double) * n);
+  double *o1 = (double *)malloc(sizeof(double) * n);
+  double *o3 = (double *)malloc(sizeof(double) * n);
+  double *FW = (double *)calloc(n, sizeof(double));
+  double *FV = (double *)malloc(sizeof(double) * n);
+  double *FU = (double *)calloc(n, sizeof(double));
+  kso_fat_links(g, c1, U, Vl);
+#pragma omp parallel for
+  for (size_t l = 0; l < 4 * g->V; l++) kso_u3_unitarize(Vl + 18 * l, W + 18 * l);
+  kso_force_oprod(g, multi_x, residues, nterms, o1, o3);
+  path_adjoint(g, c2, W, o1, o3, 1, FW);
+#pragma omp parallel for
+  for (size_t l = 0; l < 4 * g->V; l++) kso_u3_adjoint(Vl + 18 * l, FW + 18 * l, FV + 18 * l);
+  path_adjoint(g, c1, U, FV, FV, 0, FU);
+  ta_force(g, U, FU, 2.0 * eps, mom);
+  free(Vl); free(W); free(o1); free(o3); free(FW); free(FV); free(FU);
 }

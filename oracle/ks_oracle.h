@@ -97,6 +97,12 @@ void kso_force_oprod(const kso_geom *g, const double *const *multi_x, const doub
  * stored as full anti-Hermitian 3x3 matrices [V][4][18]. */
 void kso_fn_force(const kso_geom *g, const double coeff[6], const double *U, const double *const *multi_x,
                   const double *residues, int nterms, double eps, double *mom);
+/* HISQ thin-link force (level 0 -> level-2 adjoint on W -> U(3) projection
+ * adjoint -> fat7 adjoint on U): mom = TA(2 eps U F_U), full matrices. */
+void kso_hisq_force(const kso_geom *g, const double *U, const double *const *multi_x,
+                    const double *residues, int nterms, double eps, double *mom);
+/* force through W = V (V^dag V)^-1/2 at one link: F_V from F_W */
+void kso_u3_adjoint(const double *V, const double *FW, double *FV);
 /* the force's path table for direction mu (dirs[7*p+i], -1 padded) */
 int kso_fn_paths(int mu, const double coeff[6], int *dirs, double *c, int *len, int *naik);
 

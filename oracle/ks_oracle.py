@@ -66,6 +66,8 @@ def lib():
         L.kso_multicg_offset.argtypes = [G, P, P, P, P, P, ctypes.c_int, ctypes.POINTER(Qic)]
         L.kso_force_oprod.argtypes = [G, P, P, ctypes.c_int, P, P]
         L.kso_fn_force.argtypes = [G, P, P, P, P, ctypes.c_int, ctypes.c_double, P]
+        L.kso_hisq_force.argtypes = [G, P, P, P, ctypes.c_int, ctypes.c_double, P]
+        L.kso_u3_adjoint.argtypes = [P, P, P]
         L.kso_norm2.argtypes = [G, P, ctypes.c_int]
         L.kso_norm2.restype = ctypes.c_double
         L.kso_mdagm.argtypes = [G, P, P, ctypes.c_double, P, P, P, ctypes.c_int]
@@ -206,6 +208,16 @@ class Lattice:
         lib().kso_fn_force(self.g, _p(c), _p(np.ascontiguousarray(U_phased)), ptrs, _p(res), n, eps, _p(mom))
         return mom
 
+    def hisq_force(self, U_phased, multi_x, residues, eps=1.0):
+        """HISQ thin-link force TA(2 eps U F_U), full 3x3 matrices (V, 4, 3, 3, 2)."""
+        n = len(multi_x)
+        xs = [np.ascontiguousarray(x, dtype=np.float64) for x in multi_x]
+        ptrs = (ctypes.c_void_p * n)(*[x.ctypes.data for x in xs])
+        res = np.ascontiguousarray(residues, dtype=np.float64)
+        mom = self.new_links()
+        lib().kso_hisq_force(self.g, _p(np.ascontiguousarray(U_phased)), ptrs, _p(res), n, eps, _p(mom))
+        return mom
+
     def force_oprod(self, multi_x, residues):
         n = len(multi_x)
         ptrs = (ctypes.c_void_p * n)(*[np.ascontiguousarray(x).ctypes.data for x in multi_x])
@@ -220,6 +232,15 @@ def u3_unitarize(V):
     W = np.zeros_like(V)
     svd = lib().kso_u3_unitarize(_p(V), _p(W))
     return W, svd
+
+
+def u3_adjoint(V, FW):
+    """force through the U(3) projection at one link: F_V from F_W ((3,3,2) arrays)"""
+    V = np.ascontiguousarray(V, dtype=np.float64).reshape(3, 3, 2)
+    FW = np.ascontiguousarray(FW, dtype=np.float64).reshape(3, 3, 2)
+    FV = np.zeros_like(V)
+    lib().kso_u3_adjoint(_p(V), _p(FW), _p(FV))
+    return FV
 
 
 def to_complex(a):

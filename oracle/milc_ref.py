@@ -38,6 +38,7 @@ def lib():
         L.ref_congrad.argtypes = [P, P, I, D, I, I, D, ctypes.POINTER(D), ctypes.POINTER(I), ctypes.POINTER(I)]
         L.ref_multicg.argtypes = [P, ctypes.POINTER(P), P, I, I, I, D, ctypes.POINTER(D)]
         L.ref_force_oprod.argtypes = [ctypes.POINTER(P), P, I, P, P]
+        L.ref_hisq_force.argtypes = [ctypes.POINTER(P), P, I, ctypes.c_double, P]
         _lib = L
     return _lib
 
@@ -112,3 +113,17 @@ def force_oprod(multi_x, residues, V):
     o3 = np.zeros((V, 4, 3, 3, 2))
     lib().ref_force_oprod(ptrs, _p(res), n, _p(o1), _p(o3))
     return o1, o3
+
+
+def hisq_force(multi_x, residues, eps, V):
+    """MILC's HISQ thin-link force on the links of the last hisq_links call;
+    returns the momentum added to zero, (V, 4, 10) packed anti_hermitmat."""
+    n = len(multi_x)
+    xs = [np.ascontiguousarray(x, dtype=np.float64) for x in multi_x]
+    ptrs = (ctypes.c_void_p * n)(*[x.ctypes.data for x in xs])
+    res = np.ascontiguousarray(residues, dtype=np.float64)
+    mom = np.zeros((V, 4, 10))
+    rc = lib().ref_hisq_force(ptrs, _p(res), n, eps, _p(mom))
+    if rc != 0:
+        raise RuntimeError("ref_hisq_force: no HISQ links (call hisq_links first)")
+    return mom

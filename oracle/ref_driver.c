@@ -143,3 +143,6 @@ int ref_multicg(const double *src, double **psim, const double *offsets, int n, 
   if (final_rsq) *final_rsq = qic[0].final_rsq;
   return it;
 }
+
+/* the HISQ fermion links built by ref_hisq_links (for the force entry point) */
+fermion_links_t *ref_get_fl(void) { return g_fl; }

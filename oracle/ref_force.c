@@ -30,3 +30,27 @@ void ref_force_oprod(double **multi_x, const double *res, int n, double *o1, dou
     free(fn[dir]);
   }
 }
+
+fermion_links_t *ref_get_fl(void);
+
+/* MILC's full HISQ thin-link force (eo_fermion_force_multi ->
+ * fn_fermion_force_multi_hisq_wrapper_mx_cpu with KS_MULTIFF = FNMAT and
+ * HISQ_FF_MULTI_WRAPPER) on the links of the last ref_hisq_links call.
+ * multi_x[t] = X_t on all sites, one Naik set; mom: the anti_hermitmat
+ * added to a zero momentum, [i][dir][10] doubles. */
+int ref_hisq_force(double **multi_x, const double *res, int n, double eps, double *mom) {
+  fermion_links_t *fl = ref_get_fl();
+  Real r[64];
+  size_t i;
+  int dir, t;
+  if (!fl || n > 64) return -1;
+  for (t = 0; t < n; t++) r[t] = res[t];
+  n_order_naik_total = n;
+  n_orders_naik[0] = n;
+  for (i = 0; i < sites_on_node; i++)
+    for (dir = 0; dir < 4; dir++) memset(&lattice[i].mom[dir], 0, sizeof(anti_hermitmat));
+  eo_fermion_force_multi(eps, r, (su3_vector **)multi_x, n, MILC_PRECISION, fl);
+  for (i = 0; i < sites_on_node; i++)
+    for (dir = 0; dir < 4; dir++) memcpy(mom + (4 * i + dir) * 10, &lattice[i].mom[dir], 10 * sizeof(double));
+  return 0;
+}

@@ -319,3 +319,110 @@ def test_force_oprod_against_finite_differences(hisq44):
         Oc = ko.to_complex(O)[i, mu]
         pred = 2 * eps * (np.conj(Oc[b, a]).real if part == 0 else np.conj(Oc[b, a]).imag)
         assert abs(fd - pred) < 1e-6 * max(1.0, abs(pred)), (trial, fd, pred)
+
+
+# --------------------------------------------------------------------------
+# thin-link forces: chain rule checked by central differences of the action
+# --------------------------------------------------------------------------
+def test_u3_adjoint_finite_differences():
+    """2 Re Tr[dW F_W] = 2 Re Tr[dV F_V] for W = V (V^dag V)^-1/2, by central
+    differences of the oracle's own projection in all 18 real directions."""
+    rng = np.random.default_rng(31)
+    for trial in range(4):
+        Vc = 0.6 * np.eye(3) + 0.25 * (rng.normal(size=(3, 3)) + 1j * rng.normal(size=(3, 3)))
+        if trial == 3:  # nearly degenerate singular values
+            Vc = 0.8 * np.eye(3) + 1e-4 * (rng.normal(size=(3, 3)) + 1j * rng.normal(size=(3, 3)))
+        Fc = rng.normal(size=(3, 3)) + 1j * rng.normal(size=(3, 3))
+        V, F = ko.from_complex(Vc), ko.from_complex(Fc)
+        FV = ko.to_complex(ko.u3_adjoint(V, F))
+        h = 1e-6
+        for a in range(3):
+            for b in range(3):
+                for part in (0, 1):
+                    dp, dm = V.copy(), V.copy()
+                    dp[a, b, part] += h
+                    dm[a, b, part] -= h
+                    Wp = ko.to_complex(ko.u3_unitarize(dp)[0])
+                    Wm = ko.to_complex(ko.u3_unitarize(dm)[0])
+                    fd = 2 * np.trace((Wp - Wm) / (2 * h) @ Fc).real
+                    dV = np.zeros((3, 3), complex)
+                    dV[a, b] = 1.0 if part == 0 else 1j
+                    pred = 2 * np.trace(dV @ FV).real
+                    assert abs(fd - pred) < 1e-7 * max(1.0, abs(pred)), (trial, a, b, part, fd, pred)
+
+
+def _su3_generator(rng):
+    A = rng.normal(size=(3, 3)) + 1j * rng.normal(size=(3, 3))
+    T = (A + A.conj().T) / 2
+    return T - np.trace(T) / 3 * np.eye(3)
+
+
+def _expm_herm(T, t):
+    w, v = np.linalg.eigh(T)
+    return (v * np.exp(1j * t * w)) @ v.conj().T
+
+
+def _thin_link_fd(L, U, links_fn, force_fn, seed, ntrial=4):
+    """S(U) = sum_t r_t Re<phi, (M^dag M + sigma_t)^-1 phi> on even sites; for
+    U_mu(x) -> exp(i t T) U_mu(x): dS/dt = Re Tr[i T mom_mu(x)] / eps."""
+    m = 0.1
+    sig = [0.02, 0.3]
+    res = [0.4, 0.9]
+    offs = [4 * m * m + s for s in sig]
+    phi = L.gaussian_vector(41, ko.EVEN)
+    E, O = L.parity_slice(ko.EVEN), L.parity_slice(ko.ODD)
+
+    def solve(Ul):
+        fat, lng = links_fn(Ul)
+        xs, _, _ = L.multicg_offset(fat, lng, phi, offs, 1e-14, parity=ko.EVEN, max_iters=4000, nrestart=1)
+        return fat, lng, xs
+
+    def action(Ul):
+        _, _, xs = solve(Ul)
+        return sum(r * float(np.sum(phi[E] * x[E])) for r, x in zip(res, xs))
+
+    fat, lng, xs = solve(U)
+    Z = []
+    for x in xs:
+        z = x.copy()
+        z[O] = L.dslash(fat, lng, x, ko.ODD)[O]
+        Z.append(z)
+    eps = 0.7
+    mom = ko.to_complex(force_fn(U, Z, res, eps))
+    rng = np.random.default_rng(seed)
+    h = 1e-5
+    worst = 0.0
+    for trial in range(ntrial):
+        i = int(rng.integers(L.V))
+        mu = int(rng.integers(4))
+        T = _su3_generator(rng)
+        Uc = ko.to_complex(U)
+        vals = []
+        for sg in (1, -1):
+            Up = Uc.copy()
+            Up[i, mu] = _expm_herm(T, sg * h) @ Uc[i, mu]
+            vals.append(action(ko.from_complex(Up)))
+        fd = (vals[0] - vals[1]) / (2 * h)
+        pred = np.trace(1j * T @ mom[i, mu]).real / eps
+        worst = max(worst, abs(fd - pred) / max(1e-3, abs(pred)))
+        assert abs(fd - pred) < 2e-6 * max(1.0, abs(pred)), (trial, i, mu, fd, pred)
+    return worst
+
+
+def test_fn_force_thin_link_finite_differences():
+    """asqtad links (u0 = 0.9) and kso_fn_force against central differences
+    of the action through the link construction."""
+    L = ko.Lattice((4, 4, 4, 4))
+    U = L.rephase(L.gauge_random(77, 0.3))
+    c = ko.asqtad_coeffs(0.9)
+    _thin_link_fd(L, U, lambda Ul: L.asqtad_links(Ul, 0.9),
+                  lambda Ul, Z, r, e: L.fn_force(c, Ul, Z, r, eps=e), seed=5)
+
+
+def test_hisq_force_thin_link_finite_differences():
+    """HISQ chain (fat7 -> U(3) projection -> level 2 + Naik) and
+    kso_hisq_force against central differences of the action."""
+    L = ko.Lattice((4, 4, 4, 4))
+    U = L.rephase(L.gauge_random(78, 0.3))
+    _thin_link_fd(L, U, lambda Ul: L.hisq_links(Ul)[:2],
+                  lambda Ul, Z, r, e: L.hisq_force(Ul, Z, r, eps=e), seed=6)

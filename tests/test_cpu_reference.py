@@ -76,3 +76,31 @@ def test_force_oprod_matches_milc_smearing0(setup):
     o1a, o3a = mr.force_oprod(xs, res, V)
     o1b, o3b = L.force_oprod(xs, res)
     assert rel(o1b, o1a) < 1e-15 and rel(o3b, o3a) < 1e-15
+
+
+def _pack_ah(F):
+    out = np.zeros(F.shape[:2] + (10,))
+    out[..., 0:2] = F[..., 0, 1, :]
+    out[..., 2:4] = F[..., 0, 2, :]
+    out[..., 4:6] = F[..., 1, 2, :]
+    out[..., 6] = F[..., 0, 0, 1]
+    out[..., 7] = F[..., 1, 1, 1]
+    out[..., 8] = F[..., 2, 2, 1]
+    return out
+
+
+def test_hisq_force_matches_milc(setup):
+    """The full HISQ thin-link force (eo_fermion_force_multi, FNMAT wrapper):
+    level 0, level-2 smearing on W, U(3) reunitarisation derivative, fat7
+    smearing on U, TA(2 eps U F) -- the oracle against MILC's own code."""
+    L, V, fat, lng, f_ref, l_ref = setup
+    mr.hisq_links(V)  # MILC's own V, W, U auxiliary links for the force
+    U = L.gauge_random(1000)
+    xs = [L.gaussian_vector(200 + t, ko.EVENANDODD) for t in range(3)]
+    res = [0.25, -0.8, 1.3]
+    eps = 0.35
+    m_milc = mr.hisq_force(xs, res, eps, V)
+    m_orc = _pack_ah(L.hisq_force(L.rephase(U), xs, res, eps=eps))
+    e = rel(m_orc, m_milc)
+    print(f"HISQ force oracle vs MILC: {e:.2e}")
+    assert e < 1e-12
