@@ -128,7 +128,42 @@ MS_SIGMAS = [1e-4 * 10 ** (4 * i / 11) for i in range(12)]
 MS_GAUGE_SEED = 1002
 MS_SRC_SEEDS = [2200 + j for j in range(16)]
 MS_RESID = 1e-9        # per-shift FP64 true residual
-MS_STAGE_RESID = 5e-6  # FP32 multi-shift stage (base system), then per-shift refinement
+MS_STAGE_RESID = 1e-7  # FP32 multi-shift stage (base system), then per-shift refinement
+
+
+def cpu_multishift_baseline(G, dims, offs, fp64_iters, nsrc, budget_s=15.0):
+    """MILC's own ks_multicg_offset_field_cpu (compiled reference, FP64) on the
+    host cores, same links and source 0, over a bounded number of iterations,
+    scaled to fp64_iters iterations per source and nsrc sources."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import milc_ref as mr
+    if not mr.available():
+        return None
+    import milc_qcd_amd as mq
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    mr.init(dims)
+    fat, lng = G.dev_read_links()
+    mr.set_fn_links(fat, lng)
+    del fat, lng
+    src = G.dev_read(0, mq.EVEN)
+    t1 = time.perf_counter()
+    _, it, _ = mr.multicg(src, offs, 1e-30, mr.EVEN, 3)
+    per = (time.perf_counter() - t1) / max(it, 1)
+    k = max(3, min(300, int(budget_s / max(per, 1e-6))))
+    t1 = time.perf_counter()
+    _, it, _ = mr.multicg(src, offs, 1e-30, mr.EVEN, k)
+    dt = time.perf_counter() - t1
+    s_per_iter = dt / it
+    return {
+        "value": s_per_iter * fp64_iters * nsrc,
+        "unit": "s/solve",
+        "cores": threads,
+        "kind": "reference",
+        "sample": (f"MILC 7.8.1 ks_multicg_offset_field_cpu (FP64, {len(offs)} shifts, OpenMP {threads} threads), "
+                   f"compiled from the reference sources, same {'x'.join(map(str, dims))} HISQ links and source 0: "
+                   f"{it} iterations timed ({dt:.1f} s, {s_per_iter * 1e3:.1f} ms/iter), scaled by the GPU FP64 "
+                   f"multi-shift iteration count {fp64_iters} to the target and by {nsrc} sources"),
+    }
 
 
 def multishift_workload(args):
@@ -149,7 +184,7 @@ def multishift_workload(args):
     variants = []
     for v in range(nvar):
         us = G.dev_block_dslash_bench(1, 16, v, 20)
-        variants.append({"variant": v, "usec": round(us, 2)})
+        variants.append({"variant": v, "usec": round(us, 2), "desc": G.dev_block_describe(1, 16, v)})
     bytes_site = 16 * 72 + 16 * (24 + 24)
     log(f"batched stencil: {nvar} variants, chosen {chosen} ({desc}) {us_best:.1f} us")
     for i in range(args.warmup):
@@ -164,6 +199,7 @@ def multishift_workload(args):
     # FP64 reference of source 0 and the batched solutions' error against it
     it64, t64, _ = G.dev_multishift(0, offs, 1e-12, 0, out_slot0=40)
     errs = [G.dev_diff(16 + j, 40 + j) for j in range(n)]
+    it64_target, _, _ = G.dev_multishift(0, offs, MS_RESID, 0)
     # one-source solves (same FP32 + refinement scheme), all 16 sources
     t1 = time.perf_counter()
     seq_iters = []
@@ -172,6 +208,13 @@ def multishift_workload(args):
         seq_iters.append(it)
     t_seq = time.perf_counter() - t1
     us_f32 = G.dev_dslash_bench(1, mq.EVEN, 20)
+    cpu = None
+    if not args.no_cpu:
+        try:
+            with stdout_to_stderr():
+                cpu = cpu_multishift_baseline(G, MS_DIMS, offs, it64_target, 16)
+        except Exception as e:  # the CPU leg must not hide the GPU result
+            log(f"cpu baseline failed: {e!r}")
     G.close()
     out = {
         "metric": "batched multi-shift RHMC solve: 16 sources x 12 shifts to per-shift FP64 true residual 1e-9",
@@ -207,7 +250,7 @@ def multishift_workload(args):
         "roofline": {"bound": "hbm", "achieved": round(bytes_site * Vh / us_best / 1e3, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(bytes_site * Vh / us_best / 1e3 / HBM_PEAK_GBS, 4),
                      "traffic": None, "kernel": f"batched stencil, 16 rhs ({desc})"},
-        "cpu_baseline": None,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 
@@ -221,8 +264,9 @@ HMC_RESID = 1e-9
 
 def hmc_workload(args):
     """One molecular-dynamics fermion force (12-shift mixed multi-shift, D_oe,
-    outer products) plus the FP64 fermion action on 48^3 x 96; the force is
-    compared with the same computation in FP64 on the device."""
+    outer products, HISQ chain to the thin-link momentum) plus the FP64
+    fermion action on 48^3 x 96; the force is compared with the same
+    computation in FP64 on the device."""
     import milc_qcd_amd as mq
     offs = [4 * HMC_MASS ** 2 + s for s in MS_SIGMAS]
     res = [0.1 * (i + 1) for i in range(len(offs))]
@@ -240,15 +284,25 @@ def hmc_workload(args):
     log(f"dslash {dsl}")
     steps = []
     for i in range(args.warmup + args.steps):
-        st = G.dev_hmc_step(0, offs, res, HMC_RESID, mixed=1, compare_fp64=(i == args.warmup))
+        st = G.dev_hmc_step(0, offs, res, HMC_RESID, mixed=1, compare_fp64=(i == args.warmup), thin_force=True)
         log(f"step {i}: {st}")
         if i >= args.warmup:
             steps.append(st)
+    cpu = None
+    if not args.no_cpu:
+        try:
+            it64, _, _ = G.dev_multishift(0, offs, HMC_RESID, 0)
+            with stdout_to_stderr():
+                cpu = cpu_multishift_baseline(G, HMC_DIMS, offs, it64, 1)
+            if cpu:
+                cpu["sample"] += "; the multi-shift solve of the step only (the CPU force and action are not timed)"
+        except Exception as e:
+            log(f"cpu baseline failed: {e!r}")
     G.close()
-    t_step = sum(s["force_seconds"] + s["action_seconds"] for s in steps) / len(steps)
+    t_step = sum(s["force_seconds"] + s["thin_force_seconds"] + s["action_seconds"] for s in steps) / len(steps)
     cmp = [s for s in steps if s["rel_err_O1"] >= 0][0]
     out = {
-        "metric": "RHMC force + FP64 action time per MD step (48^3x96 HISQ, 12-shift mixed multi-shift)",
+        "metric": "RHMC fermion force (to thin-link momentum) + FP64 action per MD step (48^3x96 HISQ, 12 shifts)",
         "value": round(t_step, 4),
         "unit": "s/step",
         "n_gpus": 1,
@@ -264,6 +318,8 @@ def hmc_workload(args):
                    "mass": HMC_MASS, "sigmas": MS_SIGMAS, "residues": res, "gauge_seed": HMC_GAUGE_SEED,
                    "source_seed": HMC_SRC_SEED, "force_resid": HMC_RESID},
         "force_seconds": round(sum(s["force_seconds"] for s in steps) / len(steps), 4),
+        "thin_link_hisq_force_seconds": round(sum(s["thin_force_seconds"] for s in steps) / len(steps), 4),
+        "thin_link_momentum_rel_err_vs_fp64": cmp["rel_err_mom"],
         "action_seconds": round(sum(s["action_seconds"] for s in steps) / len(steps), 4),
         "force_iters": steps[-1]["force_iters"],
         "action_iters": steps[-1]["action_iters"],
@@ -274,7 +330,7 @@ def hmc_workload(args):
         "roofline": {"bound": "hbm", "achieved": dsl["f32"]["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": dsl["f32"]["frac_peak"], "traffic": None,
                      "kernel": "k_dslash<P_F32> (the multi-shift bulk, 1200 B/site alg.)"},
-        "cpu_baseline": None,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 

@@ -126,6 +126,26 @@ int milc_hip_asqtad_force(int precision, const double act_path_coeff[6], const v
 int milc_hip_fermion_force_fn_multi(int precision, double eps, const double* residues, void** multi_x, int nterms,
                                     const double act_path_coeff[6], const void* gauge, void* momentum);
 
+/* HISQ thin-link force.  Replaces qudaHisqForce (generic_ks/fermion_force_hisq_multi.c:2266-2268,
+ * fn_fermion_force_multi_hisq_wrapper_mx_gpu): coeff[t] = {one-hop, three-hop}
+ * weights of term t (MILC passes 2 r_t and naik * 2 r_t), quark_field[t] = X_t on
+ * all sites; level2_coeff / fat7_coeff = the six path coefficients of the two
+ * smearing levels (write_path_coeffs_to_array order); w_link, v_link, u_link =
+ * HISQ W (U(3)-projected), V (fat7) and thin links, [i][dir][18], phases in.
+ * Level 0 outer products -> level-2 adjoint on W (the three-hop product enters
+ * the Naik path with unit weight) -> derivative of W = V (V^dag V)^-1/2 ->
+ * fat7 adjoint on U; TA(eps U F) is ADDED to momentum ([i][dir][10]).
+ * num_naik_terms must be 0 (extra Naik-epsilon sets are rejected, -5). */
+int milc_hip_hisq_force(int precision, int num_terms, int num_naik_terms, double eps, double** coeff,
+                        void** quark_field, const double level2_coeff[6], const double fat7_coeff[6],
+                        const void* w_link, const void* v_link, const void* u_link, void* momentum);
+/* The same force in the form of eo_fermion_force_multi for the HISQ action
+ * (fermion_force_hisq_multi.c:164-209 -> ..._wrapper_mx_cpu :1182-1475):
+ * residues[t], multi_x[t] on all sites, thin links with phases in; V and W are
+ * rebuilt on the device; TA(2 eps U F) is ADDED to momentum. */
+int milc_hip_eo_fermion_force_multi(int precision, double eps, const double* residues, void** multi_x, int nterms,
+                                    const void* thin_links, void* momentum);
+
 /* ---- many right-hand sides ------------------------------------------------- */
 /* Block CG: replaces ks_congrad_block_parity_gpu (generic_ks/d_congrad5_fn_gpu.c:174-176):
  * nsrc sources on parity args.evenodd, solution[s] holds the initial guess on
@@ -208,6 +228,9 @@ int milc_hip_dev_copy_bench(size_t bytes, int reps, double* usec);
  * (4, 8, 16) right-hand sides and keeps the fastest; returns the number of
  * variants, the chosen index, its time and a description. */
 int milc_hip_dev_block_tune(int prec, int nrhs, int* variant, double* usec, char* desc, int ndesc);
+/* Description of stencil variant `variant` (threads per site, rhs per thread or
+ * MFMA, block size, site loop, site-order block shape); returns the count. */
+int milc_hip_dev_block_describe(int prec, int nrhs, int variant, char* desc, int ndesc);
 /* Time `reps` batched stencil applications of one variant (HIP events). */
 int milc_hip_dev_block_dslash_bench(int prec, int nrhs, int variant, int reps, double* usec);
 /* dst_slots[s] = D src_slots[s] on out_parity for s < nsrc <= nrhs, through
@@ -231,17 +254,20 @@ int milc_hip_dev_multishift_block(int nsrc, const int* src_slots, int num_offset
 int milc_hip_dev_multishift(int slot_src, int parity, int num_offsets, const double* offsets, double resid,
                             int mixed_precision, int out_slot0, double* stats);
 /* One RHMC force evaluation on the current device links and the FP64 action
- * (the fermion part of update_h_rhmc.c:77-107 at level 0 and d_action_rhmc.c):
+ * (the fermion part of update_h_rhmc.c:77-107 and d_action_rhmc.c):
  * X_t = (M^dag M + offsets[t] - 4m^2)^-1 phi (mixed_precision 1: FP32
  * multi-shift + mixed refinement to resid; 0: FP64), outer products O1/O3 with
- * coefficients residues[t], then S_act = sum_even |phi + sum_t residues[t] X_t|^2
- * in FP64 (target min(resid, 1e-10)).  compare_fp64 != 0 repeats the force
- * in FP64 (target min(resid, 1e-12)) and reports the relative difference.
- * stats[0..8] = force seconds, force iterations, sum_t r_t Re<phi, X_t>, action,
- * action seconds, action iterations, rel. error O1, rel. error O3 (-1 if not
- * compared), FP64 force seconds. */
+ * coefficients residues[t]; thin_force != 0 (links built by
+ * milc_hip_dev_gauge_hisq) carries them through the HISQ chain to the
+ * thin-link momentum TA(2 U F_U); then S_act = sum_even |phi + sum_t
+ * residues[t] X_t|^2 in FP64 (target min(resid, 1e-10)).  compare_fp64 != 0
+ * repeats the force in FP64 (target min(resid, 1e-12)) and reports relative
+ * differences.  stats[0..10] = force seconds, force iterations,
+ * sum_t r_t Re<phi, X_t>, action, action seconds, action iterations, rel.
+ * error O1, rel. error O3, FP64 force seconds, thin-link force seconds,
+ * rel. error of the thin-link momentum (-1 when not computed). */
 int milc_hip_dev_hmc_step(int slot_phi, int n, const double* offsets, const double* residues, double resid,
-                          int mixed_precision, int compare_fp64, double* stats);
+                          int mixed_precision, int compare_fp64, int thin_force, double* stats);
 /* |a - b| / |b| over the parity half held in two device slots. */
 int milc_hip_dev_diff(int slot_a, int slot_b, double* rel);
 

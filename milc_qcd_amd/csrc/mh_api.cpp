@@ -47,6 +47,10 @@ int upload_links(int ext_prec, const void* fat, const void* lng) {
       c.links[p].valid = false;
     }
   }
+  if (c.thin_valid) {  // the host supplied fat/long links: no thin links on the device
+    free_links(c.thin);
+    c.thin_valid = false;
+  }
   DevLinks& L = c.links[P_F64];
   int rc;
   if ((rc = alloc_links(L.fat, P_F64, c.stride)) || (rc = alloc_links(L.lng, P_F64, c.stride))) return rc;
@@ -163,6 +167,8 @@ int milc_hip_finalize(void) {
   for (int s = 0; s < kSlots; s++) free_vec(g_slots[s]);
   fn_table_release();
   batch_release();
+  if (c.thin_valid) free_links(c.thin);
+  c.thin_valid = false;
   dev_free(c.ws.partials);
   dev_free(c.ws.scal);
   if (c.ws.scal_host) (void)hipHostFree(c.ws.scal_host);
@@ -375,7 +381,9 @@ extern "C" int milc_hip_dev_gauge_hisq(uint64_t seed, double scale, int* n_svd) 
   if ((rc = alloc_links(L.fat, P_F64, c.stride)) || (rc = alloc_links(L.lng, P_F64, c.stride))) return rc;
   if ((rc = gauge_random_dev(seed, scale, U)) || (rc = rephase_dev(U))) return rc;
   if ((rc = hisq_links_dev(U, L.fat, L.lng, nullptr, n_svd))) return rc;
-  free_links(U);
+  if (c.thin_valid) free_links(c.thin);
+  c.thin = U;  // kept for the thin-link force
+  c.thin_valid = true;
   L.prec = P_F64;
   L.valid = true;
   c.links_version++;
@@ -892,6 +900,16 @@ extern "C" int milc_hip_dev_block_tune(int prec, int nrhs, int* variant, double*
   return (int)bvariants(ip, nrhs).size();
 }
 
+extern "C" int milc_hip_dev_block_describe(int prec, int nrhs, int variant, char* desc, int ndesc) {
+  Context& c = ctx();
+  if (!c.initialised || !desc || ndesc <= 0) return -1;
+  int ip = prec == 2 ? P_F64 : P_F32;
+  int n = (int)bvariants(ip, nrhs).size();
+  if (variant < 0 || variant >= n) return -2;
+  bdescribe(ip, nrhs, variant, desc, ndesc);
+  return n;
+}
+
 extern "C" int milc_hip_dev_block_dslash_bench(int prec, int nrhs, int variant, int reps, double* usec) {
   Context& c = ctx();
   if (!c.initialised || !c.links[P_F64].valid) return -1;
@@ -1177,6 +1195,53 @@ __global__ void k_link_diff(const double2* __restrict__ a, const double2* __rest
   }
 }
 
+// |a - b| / |b| over n doubles
+__global__ void k_flat_diff(const double* __restrict__ a, const double* __restrict__ b, size_t n,
+                            double* __restrict__ out) {
+  double s0 = 0, s1 = 0;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    double d = a[i] - b[i];
+    s0 += d * d;
+    s1 += b[i] * b[i];
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    s0 += __shfl_xor(s0, off, 64);
+    s1 += __shfl_xor(s1, off, 64);
+  }
+  __shared__ double sm[2][16];
+  if ((threadIdx.x & 63) == 0) {
+    sm[0][threadIdx.x >> 6] = s0;
+    sm[1][threadIdx.x >> 6] = s1;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t0 = 0, t1 = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) {
+      t0 += sm[0][w];
+      t1 += sm[1][w];
+    }
+    out[2 * blockIdx.x] = t0;
+    out[2 * blockIdx.x + 1] = t1;
+  }
+}
+
+int flat_rel_diff(const double* a, const double* b, size_t n, double* rel) {
+  Context& c = ctx();
+  const int nb = 1024;
+  hipLaunchKernelGGL(k_flat_diff, dim3(nb), dim3(256), 0, c.stream, a, b, n, c.ws.partials);
+  MH_CHECK(hipGetLastError());
+  std::vector<double> h(2 * nb);
+  MH_CHECK(hipMemcpyAsync(h.data(), c.ws.partials, h.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  MH_CHECK(hipStreamSynchronize(c.stream));
+  double d = 0, m = 0;
+  for (int i = 0; i < nb; i++) {
+    d += h[2 * i];
+    m += h[2 * i + 1];
+  }
+  *rel = m > 0 ? std::sqrt(d / m) : 0.0;
+  return 0;
+}
+
 int link_rel_diff(const LinkField& a, const LinkField& b, double* rel) {
   Context& c = ctx();
   const int nb = 1024;
@@ -1198,21 +1263,29 @@ int link_rel_diff(const LinkField& a, const LinkField& b, double* rel) {
 }  // namespace
 
 extern "C" int milc_hip_dev_hmc_step(int slot_phi, int n, const double* offsets, const double* residues,
-                                     double resid, int mixed_precision, int compare_fp64, double* stats) {
+                                     double resid, int mixed_precision, int compare_fp64, int thin_force,
+                                     double* stats) {
   Context& c = ctx();
   if (!c.initialised || !slot_ok(slot_phi, true) || !c.links[P_F64].valid) return -1;
   if (n <= 0 || n > 32 || !stats) return -2;
+  if (thin_force && !c.thin_valid) return -3;  // needs links built on the device
   for (int k = 0; k < n; k++)
     if (!(offsets[k] > 0)) return -2;
   int rc;
   if (mixed_precision && (rc = ensure_links(P_F32))) return rc;
-  LinkField o1{}, o3{}, r1{}, r3{};
+  LinkField o1{}, o3{}, r1{}, r3{}, V{}, W{};
+  double *mom = nullptr, *mom_ref = nullptr;
   auto release = [&]() {
     free_links(o1);
     free_links(o3);
     free_links(r1);
     free_links(r3);
+    free_links(V);
+    free_links(W);
+    dev_free(mom);
+    dev_free(mom_ref);
   };
+  const size_t nmom = (size_t)2 * c.Vh * 4 * 10;
   if ((rc = alloc_links(o1, P_F64, c.stride)) || (rc = alloc_links(o3, P_F64, c.stride))) {
     release();
     return rc;
@@ -1227,6 +1300,23 @@ extern "C" int milc_hip_dev_hmc_step(int slot_phi, int n, const double* offsets,
   if (rc) {
     release();
     return rc;
+  }
+  // thin-link HISQ force from the outer products (eps = 1: mom = TA(2 U F_U))
+  double t_thin = 0;
+  if (thin_force) {
+    if ((rc = alloc_links(V, P_F64, c.stride)) || (rc = alloc_links(W, P_F64, c.stride)) ||
+        (rc = dev_alloc((void**)&mom, nmom * sizeof(double))) || (rc = hisq_aux_dev(c.thin, V, W))) {
+      release();
+      return rc;
+    }
+    t0 = std::chrono::steady_clock::now();
+    rc = hisq_force_from_oprod(c.thin, V, W, o1, o3, kHisqLevel2, kHisqLevel1, 2.0, kForceFilter, mom);
+    (void)hipStreamSynchronize(c.stream);
+    t_thin = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (rc) {
+      release();
+      return rc;
+    }
   }
   // FP64 action sum_even |a0 phi + sum_k a_k x_k|^2 (d_action_rhmc.c), a0 = 1
   std::vector<double> roots(n + 1), res(n + 1);
@@ -1246,7 +1336,7 @@ extern "C" int milc_hip_dev_hmc_step(int slot_phi, int n, const double* offsets,
     release();
     return rc;
   }
-  double e1 = -1, e3 = -1, t_ref = 0;
+  double e1 = -1, e3 = -1, em = -1, t_ref = 0;
   if (compare_fp64) {
     if ((rc = alloc_links(r1, P_F64, c.stride)) || (rc = alloc_links(r3, P_F64, c.stride))) {
       release();
@@ -1260,6 +1350,11 @@ extern "C" int milc_hip_dev_hmc_step(int slot_phi, int n, const double* offsets,
     t_ref = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (!rc) rc = link_rel_diff(o1, r1, &e1);
     if (!rc) rc = link_rel_diff(o3, r3, &e3);
+    if (!rc && thin_force) {
+      if (!(rc = dev_alloc((void**)&mom_ref, nmom * sizeof(double))))
+        rc = hisq_force_from_oprod(c.thin, V, W, r1, r3, kHisqLevel2, kHisqLevel1, 2.0, kForceFilter, mom_ref);
+      if (!rc) rc = flat_rel_diff(mom, mom_ref, nmom, &em);
+    }
     if (rc) {
       release();
       return rc;
@@ -1275,5 +1370,107 @@ extern "C" int milc_hip_dev_hmc_step(int slot_phi, int n, const double* offsets,
   stats[6] = e1;
   stats[7] = e3;
   stats[8] = t_ref;
+  stats[9] = t_thin;
+  stats[10] = em;
   return 0;
+}
+
+// ---------------------------------------------------------------------------
+// HISQ fermion force on the thin links
+// ---------------------------------------------------------------------------
+namespace {
+struct HisqBuffers : ForceBuffers {
+  LinkField V{}, W{};
+  ~HisqBuffers() {
+    free_links(V);
+    free_links(W);
+  }
+  int alloc_vw() {
+    Context& c = ctx();
+    int rc;
+    if ((rc = alloc_links(V, P_F64, c.stride)) || (rc = alloc_links(W, P_F64, c.stride))) return rc;
+    return 0;
+  }
+};
+
+int quark_fields_in(int precision, void* const* multi_x, int nterms, std::vector<VecField>& xe,
+                    std::vector<VecField>& xo) {
+  Context& c = ctx();
+  int rc;
+  xe.assign(nterms, VecField{});
+  xo.assign(nterms, VecField{});
+  for (int t = 0; t < nterms; t++)
+    if ((rc = alloc_vec(xe[t], P_F64, c.stride)) || (rc = alloc_vec(xo[t], P_F64, c.stride)) ||
+        (rc = host_vec_in(precision, multi_x[t], xe[t], 0)) || (rc = host_vec_in(precision, multi_x[t], xo[t], 1)))
+      return rc;
+  return 0;
+}
+void release_fields(std::vector<VecField>& v) {
+  for (auto& f : v) free_vec(f);
+}
+}  // namespace
+
+extern "C" int milc_hip_hisq_force(int precision, int num_terms, int num_naik_terms, double eps, double** coeff,
+                                   void** quark_field, const double level2_coeff[6], const double fat7_coeff[6],
+                                   const void* w_link, const void* v_link, const void* u_link, void* momentum) {
+  Context& c = ctx();
+  if (!c.initialised) return -1;
+  if ((precision != 1 && precision != 2) || num_terms <= 0 || num_terms > 32 || !coeff || !quark_field ||
+      !level2_coeff || !fat7_coeff || !w_link || !v_link || !u_link || !momentum)
+    return -2;
+  if (num_naik_terms != 0) return -5;  // extra Naik-epsilon sets (charm) are not supported
+  HisqBuffers b;
+  int rc;
+  if ((rc = b.alloc()) || (rc = b.alloc_vw())) return rc;
+  if ((rc = host_links_in(precision, u_link, b.U, b.stage)) || (rc = host_links_in(precision, v_link, b.V, b.stage)) ||
+      (rc = host_links_in(precision, w_link, b.W, b.stage)))
+    return rc;
+  std::vector<VecField> xe, xo;
+  std::vector<double> c1(num_terms), c3(num_terms);
+  for (int t = 0; t < num_terms; t++) {
+    c1[t] = coeff[t][0];
+    c3[t] = coeff[t][1];
+  }
+  if ((rc = quark_fields_in(precision, quark_field, num_terms, xe, xo)) ||
+      (rc = force_oprod_coeff(xe, xo, c1, c3, b.o1, b.o3))) {
+    release_fields(xe);
+    release_fields(xo);
+    return rc;
+  }
+  // the three-hop coefficients already carry the Naik weight
+  // (fermion_force_hisq_multi.c:2190-2193): unit weight on the Naik path
+  double l2[6];
+  for (int i = 0; i < 6; i++) l2[i] = level2_coeff[i];
+  l2[1] = 1.0;
+  rc = hisq_force_from_oprod(b.U, b.V, b.W, b.o1, b.o3, l2, fat7_coeff, eps, kForceFilter, b.mom);
+  release_fields(xe);
+  release_fields(xo);
+  if (rc) return rc;
+  return add_momentum(precision, b.mom, momentum);
+}
+
+extern "C" int milc_hip_eo_fermion_force_multi(int precision, double eps, const double* residues, void** multi_x,
+                                               int nterms, const void* thin_links, void* momentum) {
+  Context& c = ctx();
+  if (!c.initialised) return -1;
+  if ((precision != 1 && precision != 2) || nterms <= 0 || nterms > 32 || !residues || !multi_x || !thin_links ||
+      !momentum)
+    return -2;
+  HisqBuffers b;
+  int rc;
+  if ((rc = b.alloc()) || (rc = b.alloc_vw())) return rc;
+  if ((rc = host_links_in(precision, thin_links, b.U, b.stage)) || (rc = hisq_aux_dev(b.U, b.V, b.W))) return rc;
+  std::vector<VecField> xe, xo;
+  std::vector<double> r(residues, residues + nterms);
+  if ((rc = quark_fields_in(precision, multi_x, nterms, xe, xo)) ||
+      (rc = force_oprod_coeff(xe, xo, r, r, b.o1, b.o3))) {
+    release_fields(xe);
+    release_fields(xo);
+    return rc;
+  }
+  rc = hisq_force_from_oprod(b.U, b.V, b.W, b.o1, b.o3, kHisqLevel2, kHisqLevel1, 2.0 * eps, kForceFilter, b.mom);
+  release_fields(xe);
+  release_fields(xo);
+  if (rc) return rc;
+  return add_momentum(precision, b.mom, momentum);
 }

@@ -44,7 +44,7 @@ struct BDOp {
 // A stencil variant: VALU (tps threads per site, nr/tps right-hand sides per
 // thread) or MFMA (v_mfma_f32_4x4x1f32, loop sites groups per wave).
 struct BVariant {
-  int mfma, tps, block, loop;
+  int mfma, tps, block, loop, order;  // order: site-order block shape index (0 = checkerboard order)
 };
 
 size_t bvec_bytes(int prec, int nr);

@@ -40,7 +40,7 @@ __global__ void __launch_bounds__(256) k_bds_valu(Geom g, LinkField fat, LinkFie
                                                   typename BR<P>::T* __restrict__ dst,
                                                   const typename BR<P>::T* __restrict__ pv,
                                                   const typename BR<P>::T* __restrict__ rv,
-                                                  BCoef<typename BR<P>::T> coef, int pb, int loop,
+                                                  BCoef<typename BR<P>::T> coef, int pb, int loop, BOrder ord,
                                                   double* __restrict__ partials, int pstride,
                                                   const int* __restrict__ halt_all) {
   typedef typename BR<P>::T R;
@@ -57,10 +57,11 @@ __global__ void __launch_bounds__(256) k_bds_valu(Geom g, LinkField fat, LinkFie
     for (int i = 0; i < RPT; i++) d[v][i] = 0.0;
   const int po = pb ^ 1;
   for (int it = 0; it < loop; it++) {
-    int cb = it * per_pass + gid / TPS;
-    bool valid = cb < g.Vh;
-    if (!valid) cb = g.Vh - 1;
-    Coord c = site_coords(g, cb, pb);
+    int li = it * per_pass + gid / TPS;
+    bool valid = li < g.Vh;
+    if (!valid) li = g.Vh - 1;
+    Coord c;
+    const int cb = border_site(g, ord, li, pb, c);
     Vec3<R> acc[RPT];
 #pragma unroll
     for (int i = 0; i < RPT; i++) zero(acc[i]);
@@ -139,8 +140,9 @@ template <int NR, int KIND>
 __global__ void __launch_bounds__(256) k_bds_mfma(Geom g, const float* __restrict__ W,
                                                   const float* __restrict__ src, float* __restrict__ dst,
                                                   const float* __restrict__ pv, const float* __restrict__ rv,
-                                                  BCoef<float> coef, int pb, int loop, double* __restrict__ partials,
-                                                  int pstride, const int* __restrict__ halt_all) {
+                                                  BCoef<float> coef, int pb, int loop, BOrder ord,
+                                                  double* __restrict__ partials, int pstride,
+                                                  const int* __restrict__ halt_all) {
   if (halt_all && *halt_all) return;
   constexpr int LPS = 2 * NR;   // lanes per site
   constexpr int SPW = 64 / LPS; // sites per wavefront and group
@@ -158,10 +160,11 @@ __global__ void __launch_bounds__(256) k_bds_mfma(Geom g, const float* __restric
 #pragma unroll
   for (int v = 0; v < NV; v++) d[v] = 0.0;
   for (int it = 0; it < loop; it++) {
-    int cb = (it * nwaves + wave) * SPW + lane / LPS;
-    bool valid = cb < g.Vh;
-    if (!valid) cb = g.Vh - 1;
-    Coord c = site_coords(g, cb, pb);
+    int li = (it * nwaves + wave) * SPW + lane / LPS;
+    bool valid = li < g.Vh;
+    if (!valid) li = g.Vh - 1;
+    Coord c;
+    const int cb = border_site(g, ord, li, pb, c);
     mf4 a1 = {0.f, 0.f, 0.f, 0.f}, a2 = {0.f, 0.f, 0.f, 0.f};
     const float* wrow = W + ((size_t)(pb * 16) * 3 + row) * plane + (size_t)cb * 6;
 #pragma unroll 4
@@ -304,6 +307,13 @@ struct BWork {
 BWork g_bw;
 
 int nr_index(int nr) { return nr == 4 ? 0 : (nr == 8 ? 1 : (nr == 16 ? 2 : -1)); }
+
+// site-order block shapes offered to the tuner (index 0: checkerboard order)
+const int kOrders[4][4] = {{0, 0, 0, 0}, {8, 8, 8, 4}, {8, 4, 4, 4}, {4, 4, 4, 4}};
+BOrder order_of(int k) {
+  if (k <= 0 || k > 3) return BOrder{};
+  return make_border(ctx().geom, kOrders[k][0], kOrders[k][1], kOrders[k][2], kOrders[k][3]);
+}
 int prec_index(int prec) { return prec == P_F64 ? 0 : (prec == P_F32 ? 1 : -1); }
 
 int ensure_partials() {
@@ -332,13 +342,14 @@ int loop_for(long blocks_at_loop1, int loop_min) {
 }
 
 template <int P, int NR, int TPS>
-int launch_valu(const BDOp& op, int block, int loop_min, int* nb_out) {
+int launch_valu(const BDOp& op, int block, int loop_min, int order, int* nb_out) {
   typedef typename BR<P>::T R;
   Context& c = ctx();
   long threads1 = (long)c.Vh * TPS;
   long blocks1 = (threads1 + block - 1) / block;
   int loop = loop_for(blocks1, loop_min);
   int nb = (int)((blocks1 + loop - 1) / loop);
+  const BOrder ord = order_of(order);
   BCoef<R> cf;
   for (int r = 0; r < BNR_MAX; r++) cf.c[r] = (R)op.coef[r];
   const DevLinks& L = c.links[P];
@@ -349,20 +360,20 @@ int launch_valu(const BDOp& op, int block, int loop_min, int* nb_out) {
   const R* rv = (const R*)op.r;
   if (op.kind == BK_STORE)
     hipLaunchKernelGGL((k_bds_valu<P, NR, TPS, BK_STORE>), dim3(nb), dim3(block), 0, c.stream, c.geom, L.fat, L.lng,
-                       src, dst, pv, rv, cf, op.pb, loop, g_bw.partials, kBlockCap, op.halt_all);
+                       src, dst, pv, rv, cf, op.pb, loop, ord, g_bw.partials, kBlockCap, op.halt_all);
   else if (op.kind == BK_MS)
     hipLaunchKernelGGL((k_bds_valu<P, NR, TPS, BK_MS>), dim3(nb), dim3(block), 0, c.stream, c.geom, L.fat, L.lng, src,
-                       dst, pv, rv, cf, op.pb, loop, g_bw.partials, kBlockCap, op.halt_all);
+                       dst, pv, rv, cf, op.pb, loop, ord, g_bw.partials, kBlockCap, op.halt_all);
   else
     hipLaunchKernelGGL((k_bds_valu<P, NR, TPS, BK_CG>), dim3(nb), dim3(block), 0, c.stream, c.geom, L.fat, L.lng, src,
-                       dst, pv, rv, cf, op.pb, loop, g_bw.partials, kBlockCap, op.halt_all);
+                       dst, pv, rv, cf, op.pb, loop, ord, g_bw.partials, kBlockCap, op.halt_all);
   MH_CHECK(hipGetLastError());
   if (nb_out) *nb_out = nb;
   return 0;
 }
 
 template <int NR>
-int launch_mfma(const BDOp& op, int block, int loop_min, int* nb_out) {
+int launch_mfma(const BDOp& op, int block, int loop_min, int order, int* nb_out) {
   Context& c = ctx();
   int rc = ensure_w();
   if (rc) return rc;
@@ -374,6 +385,7 @@ int launch_mfma(const BDOp& op, int block, int loop_min, int* nb_out) {
   // every site group of every wave must exist: round the wave count up
   long waves = (waves1 + loop - 1) / loop;
   int nb = (int)((waves + wpb - 1) / wpb);
+  const BOrder ord = order_of(order);
   BCoef<float> cf;
   for (int r = 0; r < BNR_MAX; r++) cf.c[r] = (float)op.coef[r];
   const float* src = (const float*)op.src;
@@ -382,13 +394,13 @@ int launch_mfma(const BDOp& op, int block, int loop_min, int* nb_out) {
   const float* rv = (const float*)op.r;
   if (op.kind == BK_STORE)
     hipLaunchKernelGGL((k_bds_mfma<NR, BK_STORE>), dim3(nb), dim3(block), 0, c.stream, c.geom, g_bw.W, src, dst, pv,
-                       rv, cf, op.pb, loop, g_bw.partials, kBlockCap, op.halt_all);
+                       rv, cf, op.pb, loop, ord, g_bw.partials, kBlockCap, op.halt_all);
   else if (op.kind == BK_MS)
     hipLaunchKernelGGL((k_bds_mfma<NR, BK_MS>), dim3(nb), dim3(block), 0, c.stream, c.geom, g_bw.W, src, dst, pv, rv,
-                       cf, op.pb, loop, g_bw.partials, kBlockCap, op.halt_all);
+                       cf, op.pb, loop, ord, g_bw.partials, kBlockCap, op.halt_all);
   else
     hipLaunchKernelGGL((k_bds_mfma<NR, BK_CG>), dim3(nb), dim3(block), 0, c.stream, c.geom, g_bw.W, src, dst, pv, rv,
-                       cf, op.pb, loop, g_bw.partials, kBlockCap, op.halt_all);
+                       cf, op.pb, loop, ord, g_bw.partials, kBlockCap, op.halt_all);
   MH_CHECK(hipGetLastError());
   if (nb_out) *nb_out = nb;
   return 0;
@@ -399,15 +411,15 @@ int launch_valu_tps(const BDOp& op, const BVariant& v, int* nb) {
   constexpr int RMAX = P == P_F64 ? 4 : 8;  // accumulators per thread (more spill)
   switch (v.tps) {
     case 1:
-      if constexpr (NR <= RMAX) return launch_valu<P, NR, 1>(op, v.block, v.loop, nb);
+      if constexpr (NR <= RMAX) return launch_valu<P, NR, 1>(op, v.block, v.loop, v.order, nb);
       break;
     case 2:
-      if constexpr (NR / 2 <= RMAX) return launch_valu<P, NR, 2>(op, v.block, v.loop, nb);
+      if constexpr (NR / 2 <= RMAX) return launch_valu<P, NR, 2>(op, v.block, v.loop, v.order, nb);
       break;
     case 4:
-      return launch_valu<P, NR, 4>(op, v.block, v.loop, nb);
+      return launch_valu<P, NR, 4>(op, v.block, v.loop, v.order, nb);
     case 8:
-      if constexpr (NR >= 8) return launch_valu<P, NR, 8>(op, v.block, v.loop, nb);
+      if constexpr (NR >= 8) return launch_valu<P, NR, 8>(op, v.block, v.loop, v.order, nb);
       break;
   }
   return -2;
@@ -416,9 +428,9 @@ int launch_valu_tps(const BDOp& op, const BVariant& v, int* nb) {
 int launch_variant(const BDOp& op, const BVariant& v, int* nb) {
   if (v.mfma) {
     if (op.prec != P_F32) return -2;
-    if (op.nr == 4) return launch_mfma<4>(op, v.block, v.loop, nb);
-    if (op.nr == 8) return launch_mfma<8>(op, v.block, v.loop, nb);
-    return launch_mfma<16>(op, v.block, v.loop, nb);
+    if (op.nr == 4) return launch_mfma<4>(op, v.block, v.loop, v.order, nb);
+    if (op.nr == 8) return launch_mfma<8>(op, v.block, v.loop, v.order, nb);
+    return launch_mfma<16>(op, v.block, v.loop, v.order, nb);
   }
   if (op.prec == P_F64) {
     if (op.nr == 4) return launch_valu_tps<P_F64, 4>(op, v, nb);
@@ -447,16 +459,25 @@ int balloc(void** p, int prec, int nr) {
 std::vector<BVariant> bvariants(int prec, int nr) {
   std::vector<BVariant> v;
   int rpt_max = prec == P_F64 ? 4 : 8;
-  for (int tps : {1, 2, 4, 8}) {
-    if (tps > nr || nr / tps > rpt_max) continue;
-    for (int block : {128, 256})
-      for (int loop : {1, 2})
-        v.push_back(BVariant{0, tps, block, loop});
-  }
+  std::vector<int> orders = {0};
+  for (int k = 1; k < 4; k++)
+    if (order_of(k).on) orders.push_back(k);
+  for (int order : orders)
+    for (int tps : {1, 2, 4, 8}) {
+      if (tps > nr || nr / tps > rpt_max) continue;
+      for (int block : {128, 256})
+        for (int loop : {1, 2}) {
+          if (order && loop > 1) continue;
+          v.push_back(BVariant{0, tps, block, loop, order});
+        }
+    }
   if (prec == P_F32)
-    for (int block : {128, 256})
-      for (int loop : {1, 2, 4})
-        v.push_back(BVariant{1, 0, block, loop});
+    for (int order : orders)
+      for (int block : {128, 256})
+        for (int loop : {1, 2, 4}) {
+          if (order && loop > 2) continue;
+          v.push_back(BVariant{1, 0, block, loop, order});
+        }
   return v;
 }
 
@@ -467,11 +488,17 @@ void bdescribe(int prec, int nr, int variant, char* buf, int n) {
     return;
   }
   const BVariant& v = vs[variant];
-  if (v.mfma)
-    snprintf(buf, n, "mfma_f32_4x4x1 nr=%d sites/wave=%d block=%d loop=%d", nr, 32 / nr, v.block, v.loop);
+  char ord[48];
+  if (v.order)
+    snprintf(ord, sizeof ord, "site blocks %dx%dx%dx%d", kOrders[v.order][0], kOrders[v.order][1],
+             kOrders[v.order][2], kOrders[v.order][3]);
   else
-    snprintf(buf, n, "valu %s nr=%d threads/site=%d rhs/thread=%d block=%d loop=%d", prec == P_F64 ? "f64" : "f32", nr,
-             v.tps, nr / v.tps, v.block, v.loop);
+    snprintf(ord, sizeof ord, "checkerboard order");
+  if (v.mfma)
+    snprintf(buf, n, "mfma_f32_4x4x1 nr=%d sites/wave=%d block=%d loop=%d %s", nr, 32 / nr, v.block, v.loop, ord);
+  else
+    snprintf(buf, n, "valu %s nr=%d threads/site=%d rhs/thread=%d block=%d loop=%d %s", prec == P_F64 ? "f64" : "f32",
+             nr, v.tps, nr / v.tps, v.block, v.loop, ord);
 }
 
 int bpartials(double** p, int* pstride) {

@@ -429,32 +429,6 @@ __global__ void __launch_bounds__(256) k_bcg_update(float* __restrict__ x, float
 // ---------------------------------------------------------------------------
 namespace {
 
-struct Graph {
-  hipGraph_t g = nullptr;
-  hipGraphExec_t e = nullptr;
-  Graph() = default;
-  Graph(const Graph&) = delete;
-  Graph& operator=(const Graph&) = delete;
-  ~Graph() {
-    if (e) (void)hipGraphExecDestroy(e);
-    if (g) (void)hipGraphDestroy(g);
-  }
-};
-
-// Record the kernels that `enqueue` puts on the context stream into a graph.
-int capture(Graph& gr, const std::function<int()>& enqueue) {
-  Context& c = ctx();
-  MH_CHECK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
-  int rc = enqueue();
-  hipGraph_t g = nullptr;
-  hipError_t e = hipStreamEndCapture(c.stream, &g);
-  gr.g = g;
-  if (rc) return rc;
-  MH_CHECK(e);
-  MH_CHECK(hipGraphInstantiate(&gr.e, gr.g, nullptr, nullptr, 0));
-  return 0;
-}
-
 struct DevBuf {
   void* p = nullptr;
   DevBuf() = default;
@@ -630,8 +604,8 @@ int bmultishift_nr(const void* b64, const std::vector<double>& offsets, const BM
     return 0;
   };
   auto t0 = std::chrono::steady_clock::now();
-  Graph gr;
-  if ((rc = capture(gr, window))) return rc;
+  HipGraph gr;
+  if ((rc = capture_graph(gr, window))) return rc;
   BMSG* hgp = nullptr;
   MH_CHECK(hipHostMalloc((void**)&hgp, sizeof(BMSG), hipHostMallocDefault));
   *hgp = hg;
@@ -757,7 +731,7 @@ int bcg_mixed_nr(const void* b64, void* x64, const double* offsets, const BCGPar
     return 0;
   };
   auto t0 = std::chrono::steady_clock::now();
-  Graph gr;
+  HipGraph gr;
   BCGG* hg = nullptr;
   MH_CHECK(hipHostMalloc((void**)&hg, sizeof(BCGG), hipHostMallocDefault));
   int converged[BNR_MAX] = {};
@@ -798,7 +772,7 @@ int bcg_mixed_nr(const void* b64, void* x64, const double* offsets, const BCGPar
     if ((rc = breduce(NR, rg.nb, dsum_rr))) break;
     if (!first) nru++;
     first = false;
-    if (!gr.e && (rc = capture(gr, iterations))) break;
+    if (!gr.e && (rc = capture_graph(gr, iterations))) break;
     for (;;) {
       if ((rc = (int)hipGraphLaunch(gr.e, c.stream))) break;
       if ((rc = (int)hipMemcpyAsync(hg, dg, sizeof(BCGG), hipMemcpyDeviceToHost, c.stream))) break;

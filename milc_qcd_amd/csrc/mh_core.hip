@@ -20,6 +20,24 @@ Context& ctx() { return g_ctx; }
 size_t vec_bytes(int prec, int stride) { return (size_t)vec_site_bytes(prec) * stride; }
 size_t link_bytes(int prec, int stride) { return (size_t)8 * link_site_bytes(prec) * stride; }
 
+HipGraph::~HipGraph() {
+  if (e) (void)hipGraphExecDestroy(e);
+  if (g) (void)hipGraphDestroy(g);
+}
+
+int capture_graph(HipGraph& gr, const std::function<int()>& enqueue) {
+  Context& c = ctx();
+  MH_CHECK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+  int rc = enqueue();
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(c.stream, &g);
+  gr.g = g;
+  if (rc) return rc;
+  MH_CHECK(e);
+  MH_CHECK(hipGraphInstantiate(&gr.e, gr.g, nullptr, nullptr, 0));
+  return 0;
+}
+
 int dev_alloc(void** p, size_t bytes) {
   *p = nullptr;
   if (bytes == 0) return 0;

@@ -154,12 +154,20 @@ __device__ __forceinline__ int cb_of(const Geom& g, const int* p, int& pb) {
 }
 __device__ __forceinline__ int wrap(int v, int L) { return v < 0 ? v + L : (v >= L ? v - L : v); }
 
-// mom(y, nu) += TA(scale U_nu(y) G_nu(y)), written as MILC's packed
-// anti_hermitmat (m01, m02, m12 complex; m00im, m11im, m22im; space) at
-// MILC index 4 i + nu, i = node_index (even sites first).
-__global__ void __launch_bounds__(128) k_fn_force(Geom g, LinkField U, LinkField O1, LinkField O3,
-                                                  const FnPath* __restrict__ paths, const FnEntry* __restrict__ ent,
-                                                  int4 e_lo, int4 e_hi, double scale, double* __restrict__ mom) {
+// Gather form of the path-table adjoint.  Thread = target link (y, nu):
+//   G_nu(y) = sum over (path p, position j) whose link is (y, nu) of
+//             c_p M_j (forward link) or c_p M_j^dag (backward link),
+//   M_j = (L_j+1 ... L_n-1) F(x) (L_0 ... L_j-1),  F = F3 on Naik paths else F1,
+// F(x) multiplied by eps(x) when with_eps (the level-0 force is eps O).
+// MODE 0 writes G; MODE 1 adds TA(scale U_nu(y) G_nu(y)) to MILC's packed
+// anti_hermitmat (m01, m02, m12 complex; m00im, m11im, m22im; space) at MILC
+// index 4 i + nu, i = node_index (even sites first).
+template <int MODE>
+__global__ void __launch_bounds__(128) k_path_adjoint(Geom g, LinkField Lk, LinkField F1, LinkField F3, int with_eps,
+                                                      const FnPath* __restrict__ paths,
+                                                      const FnEntry* __restrict__ ent, int4 e_lo, int4 e_hi,
+                                                      LinkField Gout, LinkField U, double scale,
+                                                      double* __restrict__ mom) {
   int cb = blockIdx.x * blockDim.x + threadIdx.x;
   int pm = blockIdx.y;
   if (cb >= g.Vh) return;
@@ -197,7 +205,7 @@ __global__ void __launch_bounds__(128) k_fn_force(Geom g, LinkField U, LinkField
         if (q == ld) pos[q] = wrap(pos[q] + step, g.X[q]);
       if (back) lcb = cb_of(g, pos, lpb);
       if (k == jt) continue;
-      mload(U, lpb, ld, lcb, L);
+      mload(Lk, lpb, ld, lcb, L);
       if (back) madj(L);
       if (k < jt) {
         mmul(pre, L, T);
@@ -208,8 +216,8 @@ __global__ void __launch_bounds__(128) k_fn_force(Geom g, LinkField U, LinkField
       }
     }
     Mat3<double> Ob, A, M;
-    mload(P->naik ? O3 : O1, xpb, E->mu, xcb, Ob);
-    double w = (xpb ? -1.0 : 1.0) * P->c;
+    mload(P->naik ? F3 : F1, xpb, E->mu, xcb, Ob);
+    double w = ((with_eps && xpb) ? -1.0 : 1.0) * P->c;
     mmul(suf, Ob, A);
     mmul(A, pre, M);
     if (E->adj) madj(M);
@@ -220,6 +228,12 @@ __global__ void __launch_bounds__(128) k_fn_force(Geom g, LinkField U, LinkField
         G.e[i][j].re += w * M.e[i][j].re;
         G.e[i][j].im += w * M.e[i][j].im;
       }
+  }
+  if (MODE == 0) {
+    double2* out = (double2*)Gout.data + (size_t)(pm * 9) * Gout.stride + cb;
+#pragma unroll
+    for (int e = 0; e < 9; e++) out[(size_t)e * Gout.stride] = make_double2(G.e[e / 3][e % 3].re, G.e[e / 3][e % 3].im);
+    return;
   }
   Mat3<double> Uy, X;
   mload(U, pb, nu, cb, Uy);
@@ -240,7 +254,167 @@ __global__ void __launch_bounds__(128) k_fn_force(Geom g, LinkField U, LinkField
   o[9] = 0.0;
 }
 
-// Path table of an action, built once per coefficient set and kept on the device.
+// ---------------------------------------------------------------------------
+// Force through the U(3) projection W = V (V^dag V)^-1/2 (the role of
+// u3_unit_der_analytic, generic_ks/su3_mat_op.c:1379, in
+// fn_fermion_force_multi_hisq_reunit, fermion_force_hisq_multi.c:1877-1998):
+// with M = V^dag V = E diag(l) E^dag, Q = M^-1/2, K = F_W V,
+//   F_V = Q F_W + E [ f(l_i, l_j) (E^dag (K + K^dag) E)_ij ] E^dag V^dag,
+//   f(a, b) = -1 / (sqrt a sqrt b (sqrt a + sqrt b))   (divided difference of
+//   x^-1/2, exact also at a = b).
+// When min l < filter all l are raised by filter (MILC's HISQ_FORCE_FILTER).
+// E from cyclic complex Jacobi rotations in FP64.
+// ---------------------------------------------------------------------------
+struct C2 {
+  double re, im;
+};
+__device__ __forceinline__ C2 cm(C2 a, C2 b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+__device__ __forceinline__ C2 cmc(C2 a, C2 b) { return {a.re * b.re + a.im * b.im, a.im * b.re - a.re * b.im}; }  // a conj(b)
+__device__ __forceinline__ C2 cca(C2 a, C2 b) { return {a.re * b.re + a.im * b.im, a.re * b.im - a.im * b.re}; }  // conj(a) b
+__device__ __forceinline__ C2 cadd(C2 a, C2 b) { return {a.re + b.re, a.im + b.im}; }
+
+__global__ void __launch_bounds__(128) k_u3_adjoint(Geom g, LinkField Vf, LinkField FW, LinkField FV, double filter) {
+  int cb = blockIdx.x * blockDim.x + threadIdx.x;
+  int pm = blockIdx.y;
+  if (cb >= g.Vh) return;
+  int pb = pm >> 2, mu = pm & 3;
+  Mat3<double> Vm, Fm;
+  mload(Vf, pb, mu, cb, Vm);
+  mload(FW, pb, mu, cb, Fm);
+  C2 v[3][3], f[3][3], a[3][3], e[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      v[i][j] = {Vm.e[i][j].re, Vm.e[i][j].im};
+      f[i][j] = {Fm.e[i][j].re, Fm.e[i][j].im};
+      e[i][j] = {i == j ? 1.0 : 0.0, 0.0};
+    }
+  // a = V^dag V
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      C2 s = {0, 0};
+#pragma unroll
+      for (int k = 0; k < 3; k++) s = cadd(s, cca(v[k][i], v[k][j]));
+      a[i][j] = s;
+    }
+  for (int sweep = 0; sweep < 8; sweep++) {
+#pragma unroll
+    for (int pq = 0; pq < 3; pq++) {
+      const int p = pq == 2 ? 1 : 0, q = pq == 0 ? 1 : 2;
+      double r = sqrt(a[p][q].re * a[p][q].re + a[p][q].im * a[p][q].im);
+      if (r == 0.0) continue;
+      C2 ph = {a[p][q].re / r, a[p][q].im / r};
+      double tau = (a[q][q].re - a[p][p].re) / (2 * r);
+      double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1 + tau * tau));
+      double c = 1 / sqrt(1 + t * t), s = t * c;
+      // J = diag(1, e^{-i phi}) [[c, s], [-s, c]] in the (p, q) plane
+      C2 Jpp = {c, 0}, Jpq = {s, 0}, Jqp = {-s * ph.re, s * ph.im}, Jqq = {c * ph.re, -c * ph.im};
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        C2 akp = a[k][p], akq = a[k][q];
+        a[k][p] = cadd(cm(akp, Jpp), cm(akq, Jqp));
+        a[k][q] = cadd(cm(akp, Jpq), cm(akq, Jqq));
+      }
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        C2 apk = a[p][k], aqk = a[q][k];
+        a[p][k] = cadd(cca(Jpp, apk), cca(Jqp, aqk));
+        a[q][k] = cadd(cca(Jpq, apk), cca(Jqq, aqk));
+      }
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        C2 ekp = e[k][p], ekq = e[k][q];
+        e[k][p] = cadd(cm(ekp, Jpp), cm(ekq, Jqp));
+        e[k][q] = cadd(cm(ekp, Jpq), cm(ekq, Jqq));
+      }
+    }
+  }
+  double l[3] = {a[0][0].re, a[1][1].re, a[2][2].re};
+  double lmin = fmin(l[0], fmin(l[1], l[2]));
+  if (lmin < filter) {
+    l[0] += filter;
+    l[1] += filter;
+    l[2] += filter;
+  }
+  double sq[3] = {sqrt(l[0]), sqrt(l[1]), sqrt(l[2])};
+  // H = K + K^dag, K = F V; H' = E^dag H E scaled by f; then E H' E^dag
+  C2 k[3][3], t[3][3], hp[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      C2 s = {0, 0};
+#pragma unroll
+      for (int c = 0; c < 3; c++) s = cadd(s, cm(f[i][c], v[c][j]));
+      k[i][j] = s;
+    }
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      // (K + K^dag) E
+      C2 s = {0, 0};
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        C2 h = {k[i][c].re + k[c][i].re, k[i][c].im - k[c][i].im};
+        s = cadd(s, cm(h, e[c][j]));
+      }
+      t[i][j] = s;
+    }
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      C2 s = {0, 0};
+#pragma unroll
+      for (int c = 0; c < 3; c++) s = cadd(s, cca(e[c][i], t[c][j]));
+      double fij = -1.0 / (sq[i] * sq[j] * (sq[i] + sq[j]));
+      hp[i][j] = {s.re * fij, s.im * fij};
+    }
+  // t = E H'
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      C2 s = {0, 0};
+#pragma unroll
+      for (int c = 0; c < 3; c++) s = cadd(s, cm(e[i][c], hp[c][j]));
+      t[i][j] = s;
+    }
+  // L = t E^dag (into hp), Q = E diag(1/sq) E^dag (into k)
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      C2 s = {0, 0}, qv = {0, 0};
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        s = cadd(s, cmc(t[i][c], e[j][c]));
+        C2 ec = cmc(e[i][c], e[j][c]);
+        qv.re += ec.re / sq[c];
+        qv.im += ec.im / sq[c];
+      }
+      hp[i][j] = s;
+      k[i][j] = qv;
+    }
+  // F_V = Q F + L V^dag
+  double2* out = (double2*)FV.data + (size_t)(pm * 9) * FV.stride + cb;
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      C2 s = {0, 0};
+#pragma unroll
+      for (int c = 0; c < 3; c++) s = cadd(s, cadd(cm(k[i][c], f[c][j]), cmc(hp[i][c], v[j][c])));
+      out[(size_t)(3 * i + j) * FV.stride] = make_double2(s.re, s.im);
+    }
+}
+
+// Path tables, built once per coefficient set and kept on the device (a few
+// sets: asqtad, HISQ level 2, HISQ level 1).
 struct FnTable {
   double coeff[6] = {0, 0, 0, 0, 0, 0};
   FnPath* paths = nullptr;
@@ -248,44 +422,74 @@ struct FnTable {
   int start[5] = {0, 0, 0, 0, 0};
   bool valid = false;
 };
-static FnTable g_table;
+static FnTable g_tables[4];
+static int g_next_table = 0;
 
 static int fn_table(const double coeff[6], FnTable*& out) {
   Context& c = ctx();
-  bool same = g_table.valid;
-  for (int i = 0; i < 6 && same; i++) same = g_table.coeff[i] == coeff[i];
-  if (!same) {
-    if (g_table.valid) {
-      dev_free(g_table.paths);
-      dev_free(g_table.ent);
-      g_table.valid = false;
+  for (auto& t : g_tables) {
+    if (!t.valid) continue;
+    bool same = true;
+    for (int i = 0; i < 6 && same; i++) same = t.coeff[i] == coeff[i];
+    if (same) {
+      out = &t;
+      return 0;
     }
-    std::vector<FnPath> paths;
-    std::vector<int> path_mu;
-    std::vector<FnEntry> ent;
-    build_paths(coeff, paths, path_mu);
-    build_entries(paths, path_mu, ent, g_table.start);
-    int rc;
-    if ((rc = dev_alloc((void**)&g_table.paths, paths.size() * sizeof(FnPath))) ||
-        (rc = dev_alloc((void**)&g_table.ent, ent.size() * sizeof(FnEntry))))
-      return rc;
-    MH_CHECK(hipMemcpyAsync(g_table.paths, paths.data(), paths.size() * sizeof(FnPath), hipMemcpyHostToDevice,
-                            c.stream));
-    MH_CHECK(hipMemcpyAsync(g_table.ent, ent.data(), ent.size() * sizeof(FnEntry), hipMemcpyHostToDevice, c.stream));
-    MH_CHECK(hipStreamSynchronize(c.stream));
-    for (int i = 0; i < 6; i++) g_table.coeff[i] = coeff[i];
-    g_table.valid = true;
   }
-  out = &g_table;
+  FnTable& t = g_tables[g_next_table];
+  g_next_table = (g_next_table + 1) % 4;
+  if (t.valid) {
+    dev_free(t.paths);
+    dev_free(t.ent);
+    t = FnTable{};
+  }
+  std::vector<FnPath> paths;
+  std::vector<int> path_mu;
+  std::vector<FnEntry> ent;
+  build_paths(coeff, paths, path_mu);
+  build_entries(paths, path_mu, ent, t.start);
+  int rc;
+  if ((rc = dev_alloc((void**)&t.paths, paths.size() * sizeof(FnPath))) ||
+      (rc = dev_alloc((void**)&t.ent, ent.size() * sizeof(FnEntry))))
+    return rc;
+  MH_CHECK(hipMemcpyAsync(t.paths, paths.data(), paths.size() * sizeof(FnPath), hipMemcpyHostToDevice, c.stream));
+  MH_CHECK(hipMemcpyAsync(t.ent, ent.data(), ent.size() * sizeof(FnEntry), hipMemcpyHostToDevice, c.stream));
+  MH_CHECK(hipStreamSynchronize(c.stream));
+  for (int i = 0; i < 6; i++) t.coeff[i] = coeff[i];
+  t.valid = true;
+  out = &t;
   return 0;
 }
 
 void fn_table_release() {
-  if (g_table.valid) {
-    dev_free(g_table.paths);
-    dev_free(g_table.ent);
+  for (auto& t : g_tables) {
+    if (t.valid) {
+      dev_free(t.paths);
+      dev_free(t.ent);
+    }
+    t = FnTable{};
   }
-  g_table = FnTable{};
+  g_next_table = 0;
+}
+
+static int path_adjoint_launch(int mode, const LinkField& Lk, const LinkField& F1, const LinkField& F3, int with_eps,
+                               const double coeff[6], const LinkField& Gout, const LinkField& U, double scale,
+                               double* mom) {
+  Context& c = ctx();
+  FnTable* t;
+  int rc = fn_table(coeff, t);
+  if (rc) return rc;
+  int4 lo = make_int4(t->start[0], t->start[1], t->start[2], t->start[3]);
+  int4 hi = make_int4(t->start[1], t->start[2], t->start[3], t->start[4]);
+  dim3 grid((c.Vh + 127) / 128, 8);
+  if (mode == 0)
+    hipLaunchKernelGGL(k_path_adjoint<0>, grid, dim3(128), 0, c.stream, c.geom, Lk, F1, F3, with_eps, t->paths,
+                       t->ent, lo, hi, Gout, U, scale, mom);
+  else
+    hipLaunchKernelGGL(k_path_adjoint<1>, grid, dim3(128), 0, c.stream, c.geom, Lk, F1, F3, with_eps, t->paths,
+                       t->ent, lo, hi, Gout, U, scale, mom);
+  MH_CHECK(hipGetLastError());
+  return 0;
 }
 
 // Packed momentum increments (MILC AoS, 10 doubles per link) into mom_dev from
@@ -294,16 +498,36 @@ void fn_table_release() {
 // the outer products already carry 2 eps, fermion_force_asqtad_gpu.c:53-56).
 int fn_force_from_oprod(const LinkField& U, const LinkField& o1, const LinkField& o3, const double coeff[6],
                         double scale, double* mom_dev) {
+  LinkField none{nullptr, 1.0f, 0};
+  return path_adjoint_launch(1, U, o1, o3, 1, coeff, none, U, scale, mom_dev);
+}
+
+// HISQ thin-link force from the level-0 outer products (the chain of
+// fn_fermion_force_multi_hisq_wrapper_mx_cpu, fermion_force_hisq_multi.c:1182-1475):
+// F_W = level-2 adjoint on W (coeff2, eps(x) O1 / O3), F_V = U(3) projection
+// adjoint at V, then the fat7 adjoint on U (coeff1) straight into
+// mom += TA(scale U F_U).  Scratch: two FP64 link fields.
+int hisq_force_from_oprod(const LinkField& U, const LinkField& V, const LinkField& W, const LinkField& o1,
+                          const LinkField& o3, const double coeff2[6], const double coeff1[6], double scale,
+                          double filter, double* mom_dev) {
   Context& c = ctx();
-  FnTable* t;
-  int rc = fn_table(coeff, t);
-  if (rc) return rc;
-  int4 lo = make_int4(t->start[0], t->start[1], t->start[2], t->start[3]);
-  int4 hi = make_int4(t->start[1], t->start[2], t->start[3], t->start[4]);
-  hipLaunchKernelGGL(k_fn_force, dim3((c.Vh + 127) / 128, 8), dim3(128), 0, c.stream, c.geom, U, o1, o3, t->paths,
-                     t->ent, lo, hi, scale, mom_dev);
-  MH_CHECK(hipGetLastError());
-  return 0;
+  LinkField FW{}, FV{};
+  int rc;
+  if ((rc = alloc_links(FW, P_F64, c.stride)) || (rc = alloc_links(FV, P_F64, c.stride))) {
+    free_links(FW);
+    return rc;
+  }
+  LinkField none{nullptr, 1.0f, 0};
+  rc = path_adjoint_launch(0, W, o1, o3, 1, coeff2, FW, none, 0.0, nullptr);
+  if (!rc) {
+    hipLaunchKernelGGL(k_u3_adjoint, dim3((c.Vh + 127) / 128, 8), dim3(128), 0, c.stream, c.geom, V, FW, FV, filter);
+    rc = hipGetLastError() == hipSuccess ? 0 : -1;
+  }
+  if (!rc) rc = path_adjoint_launch(1, U, FV, FV, 0, coeff1, none, U, scale, mom_dev);
+  MH_CHECK(hipStreamSynchronize(c.stream));
+  free_links(FW);
+  free_links(FV);
+  return rc;
 }
 
 }  // namespace mh

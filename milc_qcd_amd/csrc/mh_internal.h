@@ -1,5 +1,6 @@
 // mh_internal.h -- host-side declarations shared by the library's translation units.
 #pragma once
+#include <functional>
 #include <vector>
 #include "mh_common.h"
 
@@ -54,6 +55,8 @@ struct Context {
   const void* link_key2 = nullptr;
   DevLinks links[3];   // by Prec
   unsigned links_version = 0;  // bumped whenever the FP64 links change
+  LinkField thin{};            // thin links (phases in) when the device built the links
+  bool thin_valid = false;
   double* staging = nullptr;   // FP64 AoS staging buffer, 4*V*18 doubles
   size_t staging_elems = 0;
   int block = 256;
@@ -61,6 +64,19 @@ struct Context {
 };
 
 Context& ctx();
+
+// A hipGraph captured from the kernels that `enqueue` puts on the context
+// stream (no allocations, memsets or synchronisations inside), replayed with
+// hipGraphLaunch.
+struct HipGraph {
+  hipGraph_t g = nullptr;
+  hipGraphExec_t e = nullptr;
+  HipGraph() = default;
+  HipGraph(const HipGraph&) = delete;
+  HipGraph& operator=(const HipGraph&) = delete;
+  ~HipGraph();
+};
+int capture_graph(HipGraph& gr, const std::function<int()>& enqueue);
 
 // device memory helpers
 int dev_alloc(void** p, size_t bytes);
@@ -117,6 +133,16 @@ int multicg_solve(const VecField& src, std::vector<VecField>& x, const std::vect
 // ---- force outer products ---------------------------------------------------
 int force_oprod(const std::vector<VecField>& xe, const std::vector<VecField>& xo,
                 const std::vector<double>& residues, LinkField& o1, LinkField& o3);
+// HISQ path coefficients (generic_ks/imp_actions/hisq/hisq_u3_action.h) in the
+// order {one_link, naik, 3-staple, 5-staple, 7-staple, Lepage}
+constexpr double kHisqLevel2[6] = {(1.0 / 8.0) + (2.0 * 6.0 / 16.0) + (1.0 / 8.0), -1.0 / 24.0, -1.0 / 8.0 * 0.5,
+                                   1.0 / 8.0 * 0.25 * 0.5, -1.0 / 8.0 * 0.125 * (1.0 / 6.0), -2.0 / 16.0};
+constexpr double kHisqLevel1[6] = {1.0 / 8.0, 0.0, -1.0 / 8.0 * 0.5, 1.0 / 8.0 * 0.25 * 0.5,
+                                   -1.0 / 8.0 * 0.125 * (1.0 / 6.0), 0.0};
+constexpr double kForceFilter = 5.0e-5;  // HISQ_FORCE_FILTER of ks_imp_rhmc/Make_template
+int hisq_force_from_oprod(const LinkField& U, const LinkField& V, const LinkField& W, const LinkField& o1,
+                          const LinkField& o3, const double coeff2[6], const double coeff1[6], double scale,
+                          double filter, double* mom_dev);
 // device path table of the FN force (freed with the context)
 void fn_table_release();
 // batched-solver caches (freed with the context)
@@ -126,6 +152,7 @@ void batch_release();
 int gauge_random_dev(uint64_t seed, double scale, LinkField& U);
 int hisq_links_dev(const LinkField& U, LinkField& fat, LinkField& lng, LinkField* W_out, int* nsvd);
 int asqtad_links_dev(const LinkField& U, double u0, LinkField& fat, LinkField& lng);
+int hisq_aux_dev(const LinkField& U, LinkField& V, LinkField& W);
 int rephase_dev(LinkField& U);
 int gaussian_vec_dev(uint64_t seed, int pb, VecField& v);
 

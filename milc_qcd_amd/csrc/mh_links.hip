@@ -473,6 +473,24 @@ int hisq_links_dev(const LinkField& U, LinkField& fat, LinkField& lng, LinkField
   return 0;
 }
 
+// V = fat7(U) and W = U(3) projection of V (the auxiliary links the HISQ force
+// needs, hisq_auxiliary_t of generic_ks/fermion_links_hisq_load_milc.c)
+int hisq_aux_dev(const LinkField& U, LinkField& V, LinkField& W) {
+  Context& c = ctx();
+  const double c1[6] = {1.0 / 8.0, 0.0, -1.0 / 8.0 * 0.5, 1.0 / 8.0 * 0.25 * 0.5, -1.0 / 8.0 * 0.125 * (1.0 / 6.0),
+                        0.0};
+  int rc;
+  if ((rc = fat_links_dev(U, c1, V))) return rc;
+  int* d_nsvd = nullptr;
+  if ((rc = dev_alloc((void**)&d_nsvd, sizeof(int)))) return rc;
+  MH_CHECK(hipMemsetAsync(d_nsvd, 0, sizeof(int), c.stream));
+  hipLaunchKernelGGL(k_u3_unitarize, dim3(nb_for(c.Vh), 8), dim3(256), 0, c.stream, c.geom, V, W, d_nsvd);
+  MH_CHECK(hipGetLastError());
+  MH_CHECK(hipStreamSynchronize(c.stream));
+  dev_free(d_nsvd);
+  return 0;
+}
+
 int asqtad_links_dev(const LinkField& U, double u0, LinkField& fat, LinkField& lng) {
   Context& c = ctx();
   double u2 = u0 * u0, u4 = u2 * u2, u6 = u4 * u2;

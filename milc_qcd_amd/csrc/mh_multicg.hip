@@ -4,11 +4,16 @@
 // Jegerlehner multi-mass CG for (-D_eo D_oe + offset_j) x_j = b with the
 // smallest offset as the base system, zeta/beta/alpha recurrences in FP64 and
 // retirement of shifts whose zeta underflows (:324-355, :429-444).
-// The recurrences run in single-block kernels on device-resident scalars, so
-// the host only checks convergence every `batch` iterations.  The update of
-// the non-base search directions pm_j (:447-453) is deferred into the next
-// iteration's solution update, so every shift's pm_j/psim_j pair is streamed
-// once per iteration.
+// The recurrences run in single-block kernels on device-resident scalars.
+// The updates of psim_j and pm_j (:390-396, :447-453) are deferred over
+// windows of MS_KW iterations: the window's residuals are kept in a ring, the
+// per-iteration beta/zeta/alpha are folded into FP64 window coefficients
+//   pm_j^(s+m) = wa_m pm_j^(s) + sum_l wc_{m,l} r^(s+l)
+//   psim_j^(s+K) = psim_j^(s) + wx pm_j^(s) + sum_l wb_l r^(s+l)
+// and one flush kernel per window streams every shift's pair once.  The scalar
+// recurrences (and so the iteration count) are those of MILC; only the
+// rounding of the vector arithmetic differs.  One window (reset, K
+// iterations, flush) is captured as a hipGraph and replayed.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -21,13 +26,16 @@
 namespace mh {
 
 #define MS_MAX 32
+#define MS_KW 8
 
 struct MSScalars {
   double zeta_i[MS_MAX], zeta_im1[MS_MAX], zeta_ip1[MS_MAX];
   double beta_i[MS_MAX], beta_im1[MS_MAX], alpha[MS_MAX];
   double shifts[MS_MAX];
+  double wa[MS_MAX], wx[MS_MAX], wc[MS_MAX][MS_KW], wb[MS_MAX][MS_KW];
   double rsq, oldrsq, pkp, source_norm, rsqstop;
-  int num_now, j_low, n, halt, iters, pending, converged, pad;
+  int num_now, j_low, n, halt, iters, converged, m, now_ws;
+  int max_iter, pad;
 };
 
 struct VecSet {
@@ -127,59 +135,100 @@ __global__ void k_ms_beta(const double* __restrict__ partials, int nblocks, MSSc
     }
   }
   sc->num_now = now;
+  // psim_j += beta_j pm_j^(s+m-1), pm_j^(s+m-1) = wa pm_j^(s) + sum_{l<m-1} wc_l r^(s+1+l)
+  int m = sc->m + 1;
+  sc->m = m;
+  for (int j = 0; j < now; j++) {
+    double b = sc->beta_i[j];
+    sc->wx[j] += b * sc->wa[j];
+    for (int l = 0; l < m - 1; l++) sc->wb[j][l] += b * sc->wc[j][l];
+  }
 }
 
-// psim_j += beta_j pm_j (pm_j first brought up to date with the deferred
-// pm_j = zeta_j r + alpha_j pm_j of the previous iteration); r += beta_low ttt
+__global__ void k_ms_reset(MSScalars* sc) {
+  if (threadIdx.x != 0) return;
+  sc->m = 0;
+  sc->now_ws = sc->num_now;
+  for (int j = 0; j < sc->n; j++) {
+    sc->wa[j] = 1.0;
+    sc->wx[j] = 0.0;
+    for (int l = 0; l < MS_KW; l++) sc->wc[j][l] = sc->wb[j][l] = 0.0;
+  }
+}
+
+// r^(m) = r^(m-1) + beta_low ttt; |r^(m)|^2 partials (ks_multicg_offset.c:394-396)
 template <int P>
-__global__ void __launch_bounds__(256) k_ms_update(VecSet psim, VecSet pm, VecField p, VecField r, VecField ttt,
-                                                   int Vh, const MSScalars* __restrict__ sc,
-                                                   double* __restrict__ partials) {
+__global__ void __launch_bounds__(256) k_ms_r(VecField rprev, VecField rcur, VecField ttt, int Vh,
+                                              const MSScalars* __restrict__ sc, double* __restrict__ partials) {
   if (sc->halt) return;
   typedef typename PrecTraits<P>::Real R;
   int cb = blockIdx.x * blockDim.x + threadIdx.x;
   double s[1] = {0};
   if (cb < Vh) {
-    int now = sc->num_now, jl = sc->j_low, pending = sc->pending;
-    Vec3<R> rv, tv, pv;
-    load_vec<P>(r, cb, rv);
+    Vec3<R> rv, tv;
+    load_vec<P>(rprev, cb, rv);
     load_vec<P>(ttt, cb, tv);
-    load_vec<P>(p, cb, pv);
-    for (int j = 0; j < now; j++) {
-      Vec3<R> d, x;
-      R b = (R)sc->beta_i[j];
-      if (j == jl) {
-        d = pv;
-      } else {
-        load_vec<P>(pm.v[j], cb, d);
-        if (pending) {
-          R z = (R)sc->zeta_i[j], a = (R)sc->alpha[j];
-#pragma unroll
-          for (int k = 0; k < 3; k++) {
-            d.c[k].re = z * rv.c[k].re + a * d.c[k].re;
-            d.c[k].im = z * rv.c[k].im + a * d.c[k].im;
-          }
-          store_vec<P>(pm.v[j], cb, d);
-        }
-      }
-      load_vec<P>(psim.v[j], cb, x);
-#pragma unroll
-      for (int k = 0; k < 3; k++) {
-        x.c[k].re = x.c[k].re + b * d.c[k].re;
-        x.c[k].im = x.c[k].im + b * d.c[k].im;
-      }
-      store_vec<P>(psim.v[j], cb, x);
-    }
-    R bl = (R)sc->beta_i[jl];
+    R bl = (R)sc->beta_i[sc->j_low];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
       rv.c[k].re = rv.c[k].re + bl * tv.c[k].re;
       rv.c[k].im = rv.c[k].im + bl * tv.c[k].im;
       s[0] += (double)rv.c[k].re * (double)rv.c[k].re + (double)rv.c[k].im * (double)rv.c[k].im;
     }
-    store_vec<P>(r, cb, rv);
+    store_vec<P>(rcur, cb, rv);
   }
   block_sum4<1>(s, partials);
+}
+
+#define MS_RING (MS_KW + 1)
+struct VecRing {
+  VecField v[MS_RING];
+};
+
+// window flush: psim_j and pm_j brought up to date from the ring of residuals;
+// ring[0] = the window's last residual
+template <int P>
+__global__ void __launch_bounds__(256) k_ms_flush(VecRing ring, VecSet psim, VecSet pm, int Vh,
+                                                  const MSScalars* __restrict__ sc) {
+  typedef typename PrecTraits<P>::Real R;
+  int cb = blockIdx.x * blockDim.x + threadIdx.x;
+  const int md = sc->m;
+  if (cb >= Vh || md == 0) return;
+  Vec3<R> rv[MS_KW];
+#pragma unroll
+  for (int l = 0; l < MS_KW; l++) {
+    if (l < md) load_vec<P>(ring.v[l + 1], cb, rv[l]);
+    else zero(rv[l]);
+  }
+  const int nn = sc->now_ws;
+  for (int j = 0; j < nn; j++) {
+    Vec3<R> d, x;
+    load_vec<P>(pm.v[j], cb, d);
+    load_vec<P>(psim.v[j], cb, x);
+    R cx = (R)sc->wx[j], ca = (R)sc->wa[j];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      R xr = x.c[k].re + cx * d.c[k].re, xi = x.c[k].im + cx * d.c[k].im;
+      R qr = ca * d.c[k].re, qi = ca * d.c[k].im;
+#pragma unroll
+      for (int l = 0; l < MS_KW; l++) {
+        R b = (R)sc->wb[j][l], cc = (R)sc->wc[j][l];
+        xr += b * rv[l].c[k].re;
+        xi += b * rv[l].c[k].im;
+        qr += cc * rv[l].c[k].re;
+        qi += cc * rv[l].c[k].im;
+      }
+      x.c[k].re = xr;
+      x.c[k].im = xi;
+      d.c[k].re = qr;
+      d.c[k].im = qi;
+    }
+    store_vec<P>(psim.v[j], cb, x);
+    store_vec<P>(pm.v[j], cb, d);
+  }
+#pragma unroll
+  for (int l = 0; l < MS_KW; l++)
+    if (l + 1 == md) store_vec<P>(ring.v[0], cb, rv[l]);
 }
 
 // convergence test, alpha recurrences and scalar scroll (ks_multicg_offset.c:358-463)
@@ -204,15 +253,24 @@ __global__ void k_ms_alpha(const double* __restrict__ partials, int nblocks, MSS
     else
       sc->alpha[j] = 0.0;
   }
+  // pm_j = zeta_ip1 r + alpha pm_j (:447-453) as window coefficients
+  const int m = sc->m;
+  for (int j = 0; j < now; j++) {
+    double a = sc->alpha[j];
+    sc->wa[j] *= a;
+    for (int l = 0; l < m - 1; l++) sc->wc[j][l] *= a;
+    sc->wc[j][m - 1] = sc->zeta_ip1[j];
+  }
   for (int j = 0; j < now; j++) {
     sc->beta_im1[j] = sc->beta_i[j];
     sc->zeta_im1[j] = sc->zeta_i[j];
     sc->zeta_i[j] = sc->zeta_ip1[j];
   }
-  sc->pending = 1;
+  // the loop of ks_multicg_offset.c ends when iteration reaches niter
+  if (1 + sc->iters >= sc->max_iter) sc->halt = 1;
 }
 
-// cg_p = r + alpha_low cg_p  (pm[j_low] with zeta_low = 1)
+// cg_p = r^(m) + alpha_low cg_p  (pm[j_low] with zeta_low = 1)
 template <int P>
 __global__ void k_ms_p(VecField p, VecField r, int Vh, const MSScalars* __restrict__ sc) {
   if (sc->halt) return;
@@ -289,33 +347,45 @@ static int multicg_run(const VecField& src64, std::vector<VecField>& x64, const 
     h.beta_im1[j] = -1.0;
     h.alpha[j] = 0.0;
   }
+  h.max_iter = (int)std::min<long>(mp.max_iters, 1L << 30);
   // MILC stores shift0 = -shifts[j_low] with shifts[j_low] = offset_low
   MSScalars* d_sc = nullptr;
   if ((rc = dev_alloc((void**)&d_sc, sizeof(MSScalars)))) return rc;
   MH_CHECK(hipMemcpyAsync(d_sc, &h, sizeof h, hipMemcpyHostToDevice, c.stream));
-  int nb = (c.Vh + 255) / 256;
-  long iteration = 1;  // the initialisation counts as one (ks_multicg_offset.c:246)
-  int batch = mp.batch > 0 ? mp.batch : 16;
-  long max_iter = mp.max_iters;
-  while (iteration < max_iter) {
-    int k = (int)std::min<long>(batch, max_iter - iteration);
-    for (int it = 0; it < k; it++) {
+  VecRing ring{};
+  ring.v[0] = r;
+  for (int l = 1; l < MS_RING; l++)
+    if ((rc = alloc_vec(ring.v[l], P, c.stride))) return rc;
+  const int nb = (c.Vh + 255) / 256;
+  // one window: reset, MS_KW iterations (ks_multicg_offset.c:291-460), flush
+  auto window = [&]() -> int {
+    hipLaunchKernelGGL(k_ms_reset, dim3(1), dim3(64), 0, c.stream, d_sc);
+    for (int m = 1; m <= MS_KW; m++) {
       hipLaunchKernelGGL(k_ms_dslash1<P>, dim3(nb), dim3(256), 0, c.stream, c.geom, L.fat, L.lng, p, tmp, pb ^ 1,
                          d_sc);
       hipLaunchKernelGGL(k_ms_dslash2<P>, dim3(nb), dim3(256), 0, c.stream, c.geom, L.fat, L.lng, tmp, p, ttt, pb,
                          d_sc, c.ws.partials);
       hipLaunchKernelGGL(k_ms_beta, dim3(1), dim3(1024), 0, c.stream, c.ws.partials, nb, d_sc);
-      hipLaunchKernelGGL(k_ms_update<P>, dim3(nb), dim3(256), 0, c.stream, psim, pm, p, r, ttt, c.Vh, d_sc,
+      hipLaunchKernelGGL(k_ms_r<P>, dim3(nb), dim3(256), 0, c.stream, ring.v[m - 1], ring.v[m], ttt, c.Vh, d_sc,
                          c.ws.partials);
       hipLaunchKernelGGL(k_ms_alpha, dim3(1), dim3(1024), 0, c.stream, c.ws.partials, nb, d_sc);
-      hipLaunchKernelGGL(k_ms_p<P>, dim3(nb), dim3(256), 0, c.stream, p, r, c.Vh, d_sc);
+      hipLaunchKernelGGL(k_ms_p<P>, dim3(nb), dim3(256), 0, c.stream, p, ring.v[m], c.Vh, d_sc);
     }
+    hipLaunchKernelGGL(k_ms_flush<P>, dim3(nb), dim3(256), 0, c.stream, ring, psim, pm, c.Vh, d_sc);
     MH_CHECK(hipGetLastError());
+    return 0;
+  };
+  HipGraph gr;
+  if ((rc = capture_graph(gr, window))) return rc;
+  long iteration = 1;  // the initialisation counts as one (ks_multicg_offset.c:246)
+  while (iteration < mp.max_iters) {
+    MH_CHECK(hipGraphLaunch(gr.e, c.stream));
     MH_CHECK(hipMemcpyAsync(&h, d_sc, sizeof h, hipMemcpyDeviceToHost, c.stream));
     MH_CHECK(hipStreamSynchronize(c.stream));
     iteration = 1 + h.iters;
     if (h.halt) break;
   }
+  for (int l = 1; l < MS_RING; l++) free_vec(ring.v[l]);
   for (int j = 0; j < n; j++)
     if ((rc = convert_vec(psim.v[j], P, x64[j], P_F64))) return rc;
   MH_CHECK(hipStreamSynchronize(c.stream));

@@ -48,7 +48,8 @@ EXPORTS = [
     "milc_hip_asqtad_force", "milc_hip_fermion_force_fn_multi",
     "milc_hip_invert_block", "milc_hip_multishift_invert_block", "milc_hip_dev_block_tune",
     "milc_hip_dev_block_dslash_bench", "milc_hip_dev_block_dslash", "milc_hip_dev_multishift_block",
-    "milc_hip_dev_multishift", "milc_hip_dev_hmc_step",
+    "milc_hip_dev_block_describe",
+    "milc_hip_dev_multishift", "milc_hip_dev_hmc_step", "milc_hip_hisq_force", "milc_hip_eo_fermion_force_multi",
 ]
 
 
@@ -95,10 +96,13 @@ def load_library(path=LIB_PATH):
                                                    ctypes.POINTER(P), PD, PI]
     L.milc_hip_dev_block_tune.argtypes = [I, I, PI, PD, ctypes.c_char_p, I]
     L.milc_hip_dev_block_dslash_bench.argtypes = [I, I, I, I, PD]
+    L.milc_hip_dev_block_describe.argtypes = [I, I, I, ctypes.c_char_p, I]
     L.milc_hip_dev_block_dslash.argtypes = [I, I, I, I, PI, I, PI]
     L.milc_hip_dev_multishift_block.argtypes = [I, PI, I, PD, D, D, I, I, I, I, PD]
     L.milc_hip_dev_multishift.argtypes = [I, I, I, PD, D, I, I, PD]
-    L.milc_hip_dev_hmc_step.argtypes = [I, I, PD, PD, D, I, I, PD]
+    L.milc_hip_dev_hmc_step.argtypes = [I, I, PD, PD, D, I, I, I, PD]
+    L.milc_hip_hisq_force.argtypes = [I, I, I, D, ctypes.POINTER(PD), ctypes.POINTER(P), PD, PD, P, P, P, P]
+    L.milc_hip_eo_fermion_force_multi.argtypes = [I, D, PD, ctypes.POINTER(P), I, P, P]
     _lib = L
     return L
 
@@ -357,6 +361,35 @@ class Lattice:
                "milc_hip_fermion_force_fn_multi")
         return mom
 
+    def eo_fermion_force_multi(self, eps, residues, multi_x, links, mom):
+        """HISQ eo_fermion_force_multi (fermion_force_hisq_multi.c): mom (V, 4, 10)
+        += TA(2 eps U F_U); links = thin links with phases in."""
+        self._activate()
+        n = len(multi_x)
+        xs = [np.ascontiguousarray(x, dtype=np.float64) for x in multi_x]
+        qptr = (ctypes.c_void_p * n)(*[x.ctypes.data for x in xs])
+        res = (ctypes.c_double * n)(*residues)
+        U = np.ascontiguousarray(links, dtype=np.float64)
+        _check(self.lib.milc_hip_eo_fermion_force_multi(2, eps, res, qptr, n, _p(U), _p(mom)),
+               "milc_hip_eo_fermion_force_multi")
+        return mom
+
+    def hisq_force(self, eps, coeff, multi_x, level2_coeff, fat7_coeff, W, V, U, mom, num_naik_terms=0):
+        """qudaHisqForce form: coeff[t] = (one-hop, three-hop) weights."""
+        self._activate()
+        n = len(multi_x)
+        xs = [np.ascontiguousarray(x, dtype=np.float64) for x in multi_x]
+        qptr = (ctypes.c_void_p * n)(*[x.ctypes.data for x in xs])
+        rows = [(ctypes.c_double * 2)(*c) for c in coeff]
+        cptr = (ctypes.POINTER(ctypes.c_double) * n)(*[ctypes.cast(r, ctypes.POINTER(ctypes.c_double))
+                                                       for r in rows])
+        l2 = (ctypes.c_double * 6)(*level2_coeff)
+        l1 = (ctypes.c_double * 6)(*fat7_coeff)
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (W, V, U)]
+        _check(self.lib.milc_hip_hisq_force(2, n, num_naik_terms, eps, cptr, qptr, l2, l1, _p(arrs[0]), _p(arrs[1]),
+                                            _p(arrs[2]), _p(mom)), "milc_hip_hisq_force")
+        return mom
+
     def asqtad_force(self, act_path_coeff, one_link_oprod, naik_oprod, links, mom):
         """qudaAsqtadForce (generic_ks/fermion_force_asqtad_gpu.c:133-136): outer
         products (V, 4, 3, 3, 2) already scaled by 2 eps and the term weights;
@@ -399,6 +432,12 @@ class Lattice:
                    "milc_hip_dev_block_tune")
         return n, v.value, us.value, buf.value.decode()
 
+    def dev_block_describe(self, prec, nrhs, variant):
+        self._activate()
+        buf = ctypes.create_string_buffer(256)
+        _check(self.lib.milc_hip_dev_block_describe(prec, nrhs, variant, buf, 256), "milc_hip_dev_block_describe")
+        return buf.value.decode()
+
     def dev_block_dslash_bench(self, prec, nrhs, variant, reps):
         self._activate()
         us = ctypes.c_double(0)
@@ -437,17 +476,17 @@ class Lattice:
                     "milc_hip_dev_multishift")
         return it, st[1], int(st[2])
 
-    def dev_hmc_step(self, slot_phi, offsets, residues, resid, mixed=1, compare_fp64=True):
+    def dev_hmc_step(self, slot_phi, offsets, residues, resid, mixed=1, compare_fp64=True, thin_force=False):
         self._activate()
         n = len(offsets)
         offs = (ctypes.c_double * n)(*offsets)
         res = (ctypes.c_double * n)(*residues)
-        st = (ctypes.c_double * 9)()
-        _check(self.lib.milc_hip_dev_hmc_step(slot_phi, n, offs, res, resid, mixed, int(bool(compare_fp64)), st),
-               "milc_hip_dev_hmc_step")
+        st = (ctypes.c_double * 11)()
+        _check(self.lib.milc_hip_dev_hmc_step(slot_phi, n, offs, res, resid, mixed, int(bool(compare_fp64)),
+                                              int(bool(thin_force)), st), "milc_hip_dev_hmc_step")
         return {"force_seconds": st[0], "force_iters": int(st[1]), "S": st[2], "action": st[3],
                 "action_seconds": st[4], "action_iters": int(st[5]), "rel_err_O1": st[6], "rel_err_O3": st[7],
-                "fp64_force_seconds": st[8]}
+                "fp64_force_seconds": st[8], "thin_force_seconds": st[9], "rel_err_mom": st[10]}
 
     def dev_read(self, slot, parity):
         self._activate()

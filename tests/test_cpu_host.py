@@ -74,3 +74,16 @@ def test_device_indexing_matches_milc_neighbours(dims, tmp_path):
             expect = [nbr[mu][i], nbr[8 + mu][i], nbr[7 - mu][i], nbr[15 - mu][i]]
             got = [(1 - pb) * Vh + int(v) for v in nb[mu]]
             assert got == list(expect), (dims, pb, cb, mu)
+
+
+@pytest.mark.parametrize("dims,block", [((24, 24, 24, 48), (8, 8, 8, 4)), ((8, 6, 4, 10), (4, 2, 2, 2)),
+                                        ((16, 16, 16, 32), (8, 4, 4, 4)), ((8, 8, 8, 8), (4, 4, 4, 4))])
+def test_blocked_site_order_is_a_bijection(tmp_path, dims, block):
+    """the batched stencil's 4-D blocked site order visits every checkerboard
+    site of each parity exactly once, with the coordinates site_coords gives"""
+    exe = tmp_path / "border_check"
+    subprocess.run(["/opt/rocm/bin/hipcc", "-O1", "-std=c++17", "-o", str(exe),
+                    os.path.join(ROOT, "tests", "border_check.cpp")], check=True)
+    out = subprocess.run([str(exe)] + [str(d) for d in dims] + [str(b) for b in block], capture_output=True,
+                         text=True, check=True).stdout
+    assert out.strip() == "OK", out

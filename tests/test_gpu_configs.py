@@ -7,7 +7,8 @@ oracle on a small lattice.
   FP64 true residual below 1e-9 and source 0 within 1e-6 of an FP64 solve.
 * 48^3 x 96, m = 0.01, same shifts, residues 0.1 (i+1), source seed 2400,
   gauge 1004: one RHMC force (mixed) against the FP64 force on the device,
-  relative error below 1e-7; the FP64 action positive and finite.
+  relative error below 1e-7 for the outer products and for the HISQ
+  thin-link momentum; the FP64 action positive and finite.
 * 8^4: milc_hip_dev_hmc_step's sum_t r_t <phi, X_t> and action against the oracle.
 """
 import math
@@ -47,9 +48,10 @@ def test_hmc_step_48x48x48x96_mixed_vs_fp64_force():
     with mq.Lattice((48, 48, 48, 96)) as G:
         G.dev_gauge_hisq(1004, 0.4)
         G.dev_source(2400, mq.EVEN, 0)
-        st = G.dev_hmc_step(0, offs, res, 1e-9, mixed=1, compare_fp64=True)
+        st = G.dev_hmc_step(0, offs, res, 1e-9, mixed=1, compare_fp64=True, thin_force=True)
         print(st)
         assert st["rel_err_O1"] < 1e-7 and st["rel_err_O3"] < 1e-7
+        assert 0 <= st["rel_err_mom"] < 1e-7
         assert math.isfinite(st["action"]) and st["action"] > 0
         assert st["S"] > 0
 
