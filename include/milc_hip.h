@@ -146,6 +146,15 @@ int milc_hip_hisq_force(int precision, int num_terms, int num_naik_terms, double
 int milc_hip_eo_fermion_force_multi(int precision, double eps, const double* residues, void** multi_x, int nterms,
                                     const void* thin_links, void* momentum);
 
+/* Evaluation form of the thin-link force adjoints (FN and HISQ): 0 = gather
+ * over the path table for every smearing; 1 (default) = reverse mode through
+ * the generalised staples of the fattening wherever that is exact for any
+ * links (no Lepage term, or the U(3)-projected HISQ links), path table
+ * otherwise; 2 = staple form for every smearing (exact for unitary links: the
+ * optimised Lepage term retraces U U^+).  Returns -2 for another value.  No
+ * counterpart in the reference (MILC's CPU code walks sorted paths). */
+int milc_hip_set_force_form(int form);
+
 /* ---- many right-hand sides ------------------------------------------------- */
 /* Block CG: replaces ks_congrad_block_parity_gpu (generic_ks/d_congrad5_fn_gpu.c:174-176):
  * nsrc sources on parity args.evenodd, solution[s] holds the initial guess on

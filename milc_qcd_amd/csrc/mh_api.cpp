@@ -1474,3 +1474,9 @@ extern "C" int milc_hip_eo_fermion_force_multi(int precision, double eps, const 
   if (rc) return rc;
   return add_momentum(precision, b.mom, momentum);
 }
+
+extern "C" int milc_hip_set_force_form(int form) {
+  if (form < 0 || form > 2) return -2;
+  set_force_form(form);
+  return 0;
+}

@@ -413,6 +413,289 @@ __global__ void __launch_bounds__(128) k_u3_adjoint(Geom g, LinkField Vf, LinkFi
     }
 }
 
+// ---------------------------------------------------------------------------
+// Staple form of the smearing adjoint: reverse mode through the optimised
+// fattening X = (c1 - 6 c_lep) U + sum_nu [c3 S1 + c_lep S_nu(S1) + sum_rho
+// (c5 S2 + sum_sig c7 S_sig(S2))], S1 = S_nu(U_mu), S2 = S_rho(S1),
+//   S_nu(A)(x) = U_nu(x) A(x+nu) U_nu(x+mu)^dag + U_nu(x-nu)^dag A(x-nu) U_nu(x-nu+mu),
+// plus the Naik product.  Every step is a gather over one lattice of 3x3
+// matrices (no atomics), 38 launches per direction mu instead of the path
+// walk per (path, position).  For unitary links the result differs from the
+// path-table gather only by components the traceless anti-Hermitian part (or
+// the U(3) projection derivative) removes: the retracing Lepage terms are
+// constant on the group.
+// ---------------------------------------------------------------------------
+// a matrix per site and parity: element e of site (pb, cb) at
+// data[(pb * pstride + off + e) * stride + cb]  (pstride 36 / off 9 mu for one
+// direction of a link field, pstride 9 / off 0 for a site field)
+struct MF {
+  double2* data;
+  int stride, pstride, off;
+};
+__host__ inline MF mf_link(const LinkField& f, int mu) { return MF{(double2*)f.data, f.stride, 36, 9 * mu}; }
+__host__ inline MF mf_site(double2* d, int stride) { return MF{d, stride, 9, 0}; }
+
+__device__ __forceinline__ void mfl(const MF& f, int pb, int cb, Mat3<double>& m) {
+  const double2* p = f.data + (size_t)(pb * f.pstride + f.off) * f.stride + cb;
+#pragma unroll
+  for (int e = 0; e < 9; e++) {
+    double2 a = p[(size_t)e * f.stride];
+    m.e[e / 3][e % 3].re = a.x;
+    m.e[e / 3][e % 3].im = a.y;
+  }
+}
+__device__ __forceinline__ void mfs(const MF& f, int pb, int cb, const Mat3<double>& m) {
+  double2* p = f.data + (size_t)(pb * f.pstride + f.off) * f.stride + cb;
+#pragma unroll
+  for (int e = 0; e < 9; e++) p[(size_t)e * f.stride] = make_double2(m.e[e / 3][e % 3].re, m.e[e / 3][e % 3].im);
+}
+// f += c m
+__device__ __forceinline__ void mfa(const MF& f, int pb, int cb, double c, const Mat3<double>& m) {
+  double2* p = f.data + (size_t)(pb * f.pstride + f.off) * f.stride + cb;
+#pragma unroll
+  for (int e = 0; e < 9; e++) {
+    double2 a = p[(size_t)e * f.stride];
+    a.x += c * m.e[e / 3][e % 3].re;
+    a.y += c * m.e[e / 3][e % 3].im;
+    p[(size_t)e * f.stride] = a;
+  }
+}
+__device__ __forceinline__ void mmul_na(const Mat3<double>& a, const Mat3<double>& b, Mat3<double>& c) {
+  Mat3<double> t = b;
+  madj(t);
+  mmul(a, t, c);
+}
+__device__ __forceinline__ void mmul_an(const Mat3<double>& a, const Mat3<double>& b, Mat3<double>& c) {
+  Mat3<double> t = a;
+  madj(t);
+  mmul(t, b, c);
+}
+__device__ __forceinline__ void madd(Mat3<double>& a, const Mat3<double>& b) {
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      a.e[i][j].re += b.e[i][j].re;
+      a.e[i][j].im += b.e[i][j].im;
+    }
+}
+__device__ __forceinline__ void madd_adj(Mat3<double>& a, const Mat3<double>& b) {
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      a.e[i][j].re += b.e[j][i].re;
+      a.e[i][j].im -= b.e[j][i].im;
+    }
+}
+
+// neighbour (cb, parity) of site (cb, pb) after hops (d1, s1) then (d2, s2)
+__device__ __forceinline__ int hop2(const Geom& g, int cb, int pb, int d1, int s1, int d2, int s2, int& opb) {
+  Coord c = site_coords(g, cb, pb);
+  int n1 = neighbour(g, c, cb, d1, s1);
+  Coord c1 = site_coords(g, n1, pb ^ 1);
+  opb = pb;
+  return neighbour(g, c1, n1, d2, s2);
+}
+
+// F(x) = eps(x)^with_eps F1_mu(x);  G_mu(x) += c1 F(x)
+__global__ void __launch_bounds__(128) k_sf_init(Geom g, MF F1, int with_eps, MF F, MF G, double c1) {
+  int cb = blockIdx.x * blockDim.x + threadIdx.x;
+  int pb = blockIdx.y;
+  if (cb >= g.Vh) return;
+  Mat3<double> m;
+  mfl(F1, pb, cb, m);
+  double e = (with_eps && pb) ? -1.0 : 1.0;
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      m.e[i][j].re *= e;
+      m.e[i][j].im *= e;
+    }
+  mfs(F, pb, cb, m);
+  if (c1 != 0.0) mfa(G, pb, cb, c1, m);
+}
+
+// out = c in
+__global__ void __launch_bounds__(128) k_sf_scale(Geom g, MF in, double c, MF out) {
+  int cb = blockIdx.x * blockDim.x + threadIdx.x;
+  int pb = blockIdx.y;
+  if (cb >= g.Vh) return;
+  Mat3<double> m;
+  mfl(in, pb, cb, m);
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      m.e[i][j].re *= c;
+      m.e[i][j].im *= c;
+    }
+  mfs(out, pb, cb, m);
+}
+
+// out = S_nu(A) for the mu link
+__global__ void __launch_bounds__(128) k_sf_fwd(Geom g, LinkField U, int mu, int nu, MF A, MF out) {
+  int cb = blockIdx.x * blockDim.x + threadIdx.x;
+  int pb = blockIdx.y;
+  if (cb >= g.Vh) return;
+  int po = pb ^ 1;
+  Coord c = site_coords(g, cb, pb);
+  int xn = neighbour(g, c, cb, nu, 1), xm = neighbour(g, c, cb, mu, 1), xd = neighbour(g, c, cb, nu, -1);
+  int dpb;
+  int xdm = hop2(g, cb, pb, nu, -1, mu, 1, dpb);  // x - nu + mu, parity pb
+  Mat3<double> a, b, t, up, lo;
+  mfl(A, po, xn, a);
+  mload(U, po, nu, xm, b);
+  mmul_na(a, b, t);
+  mload(U, pb, nu, cb, b);
+  mmul(b, t, up);
+  mload(U, po, nu, xd, a);
+  mfl(A, po, xd, b);
+  mmul_an(a, b, t);
+  mload(U, dpb, nu, xdm, b);
+  mmul(t, b, lo);
+  madd(up, lo);
+  mfs(out, pb, cb, up);
+}
+
+// reverse of O = S_nu(A) for the force Gs on O scaled by cg:
+//   FA(y)    += cg [U_nu(y-nu+mu)^dag G(y-nu) U_nu(y-nu) + U_nu(y+mu) G(y+nu) U_nu(y)^dag]
+//   FU_nu(y) += cg [A(y+nu) U_nu(y+mu)^dag G(y) + (G(y-mu) U_nu(y-mu) A(y-mu+nu))^dag
+//                   + (A(y) U_nu(y+mu) G(y+nu))^dag + G(y-mu+nu) U_nu(y-mu)^dag A(y-mu)]
+__global__ void __launch_bounds__(128) k_sf_adj(Geom g, LinkField U, int mu, int nu, MF A, MF Gs, double cg, MF FA,
+                                                int want_fa, MF FU) {
+  int cb = blockIdx.x * blockDim.x + threadIdx.x;
+  int pb = blockIdx.y;
+  if (cb >= g.Vh) return;
+  int po = pb ^ 1;
+  Coord c = site_coords(g, cb, pb);
+  int ypn = neighbour(g, c, cb, nu, 1), ymn = neighbour(g, c, cb, nu, -1);
+  int ypm = neighbour(g, c, cb, mu, 1), ymm = neighbour(g, c, cb, mu, -1);
+  int q;
+  int ymnpm = hop2(g, cb, pb, nu, -1, mu, 1, q);  // parity pb
+  int ymmpn = hop2(g, cb, pb, mu, -1, nu, 1, q);  // parity pb
+  Mat3<double> a, b, t, u, acc;
+  if (want_fa) {
+    mload(U, pb, nu, ymnpm, u);
+    mfl(Gs, po, ymn, b);
+    mmul_an(u, b, t);
+    mload(U, po, nu, ymn, u);
+    mmul(t, u, acc);
+    mload(U, po, nu, ypm, u);
+    mfl(Gs, po, ypn, b);
+    mmul(u, b, t);
+    mload(U, pb, nu, cb, u);
+    mmul_na(t, u, a);
+    madd(acc, a);
+    mfa(FA, pb, cb, cg, acc);
+  }
+  // A(y+nu) U_nu(y+mu)^dag G(y)
+  mfl(A, po, ypn, a);
+  mload(U, po, nu, ypm, u);
+  mmul_na(a, u, t);
+  mfl(Gs, pb, cb, b);
+  mmul(t, b, acc);
+  // (G(y-mu) U_nu(y-mu) A(y-mu+nu))^dag
+  mfl(Gs, po, ymm, b);
+  mload(U, po, nu, ymm, u);
+  mmul(b, u, t);
+  mfl(A, pb, ymmpn, a);
+  mmul(t, a, b);
+  madd_adj(acc, b);
+  // (A(y) U_nu(y+mu) G(y+nu))^dag
+  mfl(A, pb, cb, a);
+  mload(U, po, nu, ypm, u);
+  mmul(a, u, t);
+  mfl(Gs, po, ypn, b);
+  mmul(t, b, a);
+  madd_adj(acc, a);
+  // G(y-mu+nu) U_nu(y-mu)^dag A(y-mu)
+  mfl(Gs, pb, ymmpn, b);
+  mload(U, po, nu, ymm, u);
+  mmul_na(b, u, t);
+  mfl(A, po, ymm, a);
+  mmul(t, a, b);
+  madd(acc, b);
+  mfa(FU, pb, cb, cg, acc);
+}
+
+// Naik: G_mu(y) += cN [U(y+mu) U(y+2mu) F3(y) + U(y+mu) F3(y-mu) U(y-mu) + F3(y-2mu) U(y-2mu) U(y-mu)],
+// F3 multiplied by eps at its own site when with_eps
+__global__ void __launch_bounds__(128) k_sf_naik(Geom g, LinkField U, int mu, MF F3, int with_eps, double cN, MF G) {
+  int cb = blockIdx.x * blockDim.x + threadIdx.x;
+  int pb = blockIdx.y;
+  if (cb >= g.Vh) return;
+  int po = pb ^ 1;
+  Coord c = site_coords(g, cb, pb);
+  int p1 = neighbour(g, c, cb, mu, 1), m1 = neighbour(g, c, cb, mu, -1);
+  int q;
+  int p2 = hop2(g, cb, pb, mu, 1, mu, 1, q), m2 = hop2(g, cb, pb, mu, -1, mu, -1, q);
+  const double ey = (with_eps && pb) ? -1.0 : 1.0, eo = (with_eps && po) ? -1.0 : 1.0;
+  Mat3<double> u1, u2, f, t, acc, b;
+  mload(U, po, mu, p1, u1);
+  mload(U, pb, mu, p2, u2);
+  mmul(u1, u2, t);
+  mfl(F3, pb, cb, f);
+  mmul(t, f, acc);
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      acc.e[i][j].re *= ey;
+      acc.e[i][j].im *= ey;
+    }
+  mfl(F3, po, m1, f);
+  mmul(u1, f, t);
+  mload(U, po, mu, m1, u2);
+  mmul(t, u2, b);
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      acc.e[i][j].re += eo * b.e[i][j].re;
+      acc.e[i][j].im += eo * b.e[i][j].im;
+    }
+  mfl(F3, pb, m2, f);
+  mload(U, pb, mu, m2, u1);
+  mmul(f, u1, t);
+  mmul(t, u2, b);
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      acc.e[i][j].re += ey * b.e[i][j].re;
+      acc.e[i][j].im += ey * b.e[i][j].im;
+    }
+  mfa(G, pb, cb, cN, acc);
+}
+
+// mom += TA(scale U G), packed (as k_path_adjoint<1>)
+__global__ void __launch_bounds__(128) k_mom_ta(Geom g, LinkField U, LinkField G, double scale,
+                                                double* __restrict__ mom) {
+  int cb = blockIdx.x * blockDim.x + threadIdx.x;
+  int pm = blockIdx.y;
+  if (cb >= g.Vh) return;
+  int pb = pm >> 2, nu = pm & 3;
+  Mat3<double> Uy, Gy, X;
+  mload(U, pb, nu, cb, Uy);
+  mload(G, pb, nu, cb, Gy);
+  mmul(Uy, Gy, X);
+  double tr = (X.e[0][0].im + X.e[1][1].im + X.e[2][2].im) * (1.0 / 3.0);
+  double h = 0.5 * scale;
+  double* o = mom + ((size_t)(pb * g.Vh + cb) * 4 + nu) * 10;
+  o[0] = h * (X.e[0][1].re - X.e[1][0].re);
+  o[1] = h * (X.e[0][1].im + X.e[1][0].im);
+  o[2] = h * (X.e[0][2].re - X.e[2][0].re);
+  o[3] = h * (X.e[0][2].im + X.e[2][0].im);
+  o[4] = h * (X.e[1][2].re - X.e[2][1].re);
+  o[5] = h * (X.e[1][2].im + X.e[2][1].im);
+  o[6] = scale * (X.e[0][0].im - tr);
+  o[7] = scale * (X.e[1][1].im - tr);
+  o[8] = scale * (X.e[2][2].im - tr);
+  o[9] = 0.0;
+}
+
 // Path tables, built once per coefficient set and kept on the device (a few
 // sets: asqtad, HISQ level 2, HISQ level 1).
 struct FnTable {
@@ -496,17 +779,105 @@ static int path_adjoint_launch(int mode, const LinkField& Lk, const LinkField& F
 // the outer products O1/O3 (FP64), thin phased links U (FP64) and the path
 // coefficients; scale multiplies U G (2 eps in fermion_force_fn_multi, 1 when
 // the outer products already carry 2 eps, fermion_force_asqtad_gpu.c:53-56).
+static int smear_adjoint_dev(const LinkField& L, const LinkField& F1, const LinkField& F3, int with_eps,
+                             const double coeff[6], LinkField& G);
+// Which device form the thin-link force adjoints use.  The staple form walks
+// the optimised fattening, whose Lepage term includes retracing paths
+// U_nu U_nu^+ that equal 1 only for unitary links; the path table holds the
+// fattening's paths themselves.  0: path table everywhere; 1 (default): the
+// staple form where it is exact for any links (a smearing without a Lepage
+// term, or the U(3)-projected links W, unitary by construction), the path
+// table otherwise; 2: the staple form everywhere (exact for unitary links).
+static int g_force_form = 1;
+void set_force_form(int form) { g_force_form = form; }
+static bool use_staple_form(const double coeff[6], bool links_unitary) {
+  return g_force_form == 2 || (g_force_form == 1 && (links_unitary || coeff[5] == 0.0));
+}
+
 int fn_force_from_oprod(const LinkField& U, const LinkField& o1, const LinkField& o3, const double coeff[6],
                         double scale, double* mom_dev) {
   LinkField none{nullptr, 1.0f, 0};
-  return path_adjoint_launch(1, U, o1, o3, 1, coeff, none, U, scale, mom_dev);
+  if (!use_staple_form(coeff, false)) return path_adjoint_launch(1, U, o1, o3, 1, coeff, none, U, scale, mom_dev);
+  Context& c = ctx();
+  LinkField G{};
+  int rc;
+  if ((rc = alloc_links(G, P_F64, c.stride))) return rc;
+  rc = smear_adjoint_dev(U, o1, o3, 1, coeff, G);
+  if (!rc) {
+    hipLaunchKernelGGL(k_mom_ta, dim3((c.Vh + 127) / 128, 8), dim3(128), 0, c.stream, c.geom, U, G, scale, mom_dev);
+    rc = hipGetLastError() == hipSuccess ? 0 : -1;
+  }
+  MH_CHECK(hipStreamSynchronize(c.stream));
+  free_links(G);
+  return rc;
+}
+
+// Staple-form adjoint on the device: G (FP64 link field, zeroed here) = force on
+// the links L of the smearing with path coefficients coeff, from the force
+// F1 (non-Naik) / F3 (Naik) on its result (eps(x) applied when with_eps).
+static int smear_adjoint_dev(const LinkField& L, const LinkField& F1, const LinkField& F3, int with_eps,
+                             const double coeff[6], LinkField& G) {
+  Context& c = ctx();
+  const int st = c.stride;
+  double2 *F = nullptr, *S1 = nullptr, *S2 = nullptr, *G1 = nullptr, *G2 = nullptr;
+  int rc;
+  size_t bytes = (size_t)2 * 9 * st * sizeof(double2);
+  if ((rc = dev_alloc((void**)&F, bytes)) || (rc = dev_alloc((void**)&S1, bytes)) ||
+      (rc = dev_alloc((void**)&S2, bytes)) || (rc = dev_alloc((void**)&G1, bytes)) ||
+      (rc = dev_alloc((void**)&G2, bytes))) {
+    dev_free(F); dev_free(S1); dev_free(S2); dev_free(G1);
+    return rc;
+  }
+  MH_CHECK(hipMemsetAsync(G.data, 0, link_bytes(P_F64, st), c.stream));
+  const MF mF = mf_site(F, st), mS1 = mf_site(S1, st), mS2 = mf_site(S2, st), mG1 = mf_site(G1, st),
+           mG2 = mf_site(G2, st);
+  const double c1 = coeff[0] - 6.0 * coeff[5], cN = coeff[1], c3 = coeff[2], c5 = coeff[3], c7 = coeff[4],
+               cl = coeff[5];
+  const bool staples = c3 != 0.0 || cl != 0.0 || c5 != 0.0 || c7 != 0.0;
+  dim3 grid((c.Vh + 127) / 128, 2), blk(128);
+  for (int mu = 0; mu < 4; mu++) {
+    const MF Gmu = mf_link(G, mu);
+    hipLaunchKernelGGL(k_sf_init, grid, blk, 0, c.stream, c.geom, mf_link(F1, mu), with_eps, mF, Gmu, c1);
+    if (staples) {
+      for (int nu = 0; nu < 4; nu++) {
+        if (nu == mu) continue;
+        hipLaunchKernelGGL(k_sf_fwd, grid, blk, 0, c.stream, c.geom, L, mu, nu, mf_link(L, mu), mS1);
+        hipLaunchKernelGGL(k_sf_scale, grid, blk, 0, c.stream, c.geom, mF, c3, mG1);
+        if (cl != 0.0)
+          hipLaunchKernelGGL(k_sf_adj, grid, blk, 0, c.stream, c.geom, L, mu, nu, mS1, mF, cl, mG1, 1,
+                             mf_link(G, nu));
+        if (c5 != 0.0 || c7 != 0.0) {
+          for (int rho = 0; rho < 4; rho++) {
+            if (rho == mu || rho == nu) continue;
+            hipLaunchKernelGGL(k_sf_fwd, grid, blk, 0, c.stream, c.geom, L, mu, rho, mS1, mS2);
+            hipLaunchKernelGGL(k_sf_scale, grid, blk, 0, c.stream, c.geom, mF, c5, mG2);
+            if (c7 != 0.0)
+              for (int sig = 0; sig < 4; sig++) {
+                if (sig == mu || sig == nu || sig == rho) continue;
+                hipLaunchKernelGGL(k_sf_adj, grid, blk, 0, c.stream, c.geom, L, mu, sig, mS2, mF, c7, mG2, 1,
+                                   mf_link(G, sig));
+              }
+            hipLaunchKernelGGL(k_sf_adj, grid, blk, 0, c.stream, c.geom, L, mu, rho, mS1, mG2, 1.0, mG1, 1,
+                               mf_link(G, rho));
+          }
+        }
+        hipLaunchKernelGGL(k_sf_adj, grid, blk, 0, c.stream, c.geom, L, mu, nu, mf_link(L, mu), mG1, 1.0, Gmu, 1,
+                           mf_link(G, nu));
+      }
+    }
+    if (cN != 0.0)
+      hipLaunchKernelGGL(k_sf_naik, grid, blk, 0, c.stream, c.geom, L, mu, mf_link(F3, mu), with_eps, cN, Gmu);
+  }
+  MH_CHECK(hipGetLastError());
+  MH_CHECK(hipStreamSynchronize(c.stream));
+  dev_free(F); dev_free(S1); dev_free(S2); dev_free(G1); dev_free(G2);
+  return 0;
 }
 
 // HISQ thin-link force from the level-0 outer products (the chain of
 // fn_fermion_force_multi_hisq_wrapper_mx_cpu, fermion_force_hisq_multi.c:1182-1475):
 // F_W = level-2 adjoint on W (coeff2, eps(x) O1 / O3), F_V = U(3) projection
-// adjoint at V, then the fat7 adjoint on U (coeff1) straight into
-// mom += TA(scale U F_U).  Scratch: two FP64 link fields.
+// adjoint at V, then the fat7 adjoint on U (coeff1), mom += TA(scale U F_U).
 int hisq_force_from_oprod(const LinkField& U, const LinkField& V, const LinkField& W, const LinkField& o1,
                           const LinkField& o3, const double coeff2[6], const double coeff1[6], double scale,
                           double filter, double* mom_dev) {
@@ -518,12 +889,24 @@ int hisq_force_from_oprod(const LinkField& U, const LinkField& V, const LinkFiel
     return rc;
   }
   LinkField none{nullptr, 1.0f, 0};
-  rc = path_adjoint_launch(0, W, o1, o3, 1, coeff2, FW, none, 0.0, nullptr);
+  if (use_staple_form(coeff2, true)) rc = smear_adjoint_dev(W, o1, o3, 1, coeff2, FW);
+  else rc = path_adjoint_launch(0, W, o1, o3, 1, coeff2, FW, none, 0.0, nullptr);
   if (!rc) {
     hipLaunchKernelGGL(k_u3_adjoint, dim3((c.Vh + 127) / 128, 8), dim3(128), 0, c.stream, c.geom, V, FW, FV, filter);
     rc = hipGetLastError() == hipSuccess ? 0 : -1;
   }
-  if (!rc) rc = path_adjoint_launch(1, U, FV, FV, 0, coeff1, none, U, scale, mom_dev);
+  if (!rc) {
+    if (use_staple_form(coeff1, false)) {
+      rc = smear_adjoint_dev(U, FV, FV, 0, coeff1, FW);  // FW reused as F_U
+      if (!rc) {
+        hipLaunchKernelGGL(k_mom_ta, dim3((c.Vh + 127) / 128, 8), dim3(128), 0, c.stream, c.geom, U, FW, scale,
+                           mom_dev);
+        rc = hipGetLastError() == hipSuccess ? 0 : -1;
+      }
+    } else {
+      rc = path_adjoint_launch(1, U, FV, FV, 0, coeff1, none, U, scale, mom_dev);
+    }
+  }
   MH_CHECK(hipStreamSynchronize(c.stream));
   free_links(FW);
   free_links(FV);

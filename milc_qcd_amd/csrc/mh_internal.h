@@ -143,6 +143,8 @@ constexpr double kForceFilter = 5.0e-5;  // HISQ_FORCE_FILTER of ks_imp_rhmc/Mak
 int hisq_force_from_oprod(const LinkField& U, const LinkField& V, const LinkField& W, const LinkField& o1,
                           const LinkField& o3, const double coeff2[6], const double coeff1[6], double scale,
                           double filter, double* mom_dev);
+// thin-link force adjoint form: 1 staple form (default), 0 path-table gather
+void set_force_form(int form);
 // device path table of the FN force (freed with the context)
 void fn_table_release();
 // batched-solver caches (freed with the context)

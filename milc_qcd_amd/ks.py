@@ -50,6 +50,7 @@ EXPORTS = [
     "milc_hip_dev_block_dslash_bench", "milc_hip_dev_block_dslash", "milc_hip_dev_multishift_block",
     "milc_hip_dev_block_describe",
     "milc_hip_dev_multishift", "milc_hip_dev_hmc_step", "milc_hip_hisq_force", "milc_hip_eo_fermion_force_multi",
+    "milc_hip_set_force_form",
 ]
 
 
@@ -103,6 +104,7 @@ def load_library(path=LIB_PATH):
     L.milc_hip_dev_hmc_step.argtypes = [I, I, PD, PD, D, I, I, I, PD]
     L.milc_hip_hisq_force.argtypes = [I, I, I, D, ctypes.POINTER(PD), ctypes.POINTER(P), PD, PD, P, P, P, P]
     L.milc_hip_eo_fermion_force_multi.argtypes = [I, D, PD, ctypes.POINTER(P), I, P, P]
+    L.milc_hip_set_force_form.argtypes = [I]
     _lib = L
     return L
 
@@ -360,6 +362,12 @@ class Lattice:
         _check(self.lib.milc_hip_fermion_force_fn_multi(2, eps, res, qptr, n, c, _p(U), _p(mom)),
                "milc_hip_fermion_force_fn_multi")
         return mom
+
+    def set_force_form(self, form):
+        """0: path-table gather; 1 (default): staple form where exact for any
+        links, path table otherwise; 2: staple form everywhere (unitary links)"""
+        self._activate()
+        _check(self.lib.milc_hip_set_force_form(form), "milc_hip_set_force_form")
 
     def eo_fermion_force_multi(self, eps, residues, multi_x, links, mom):
         """HISQ eo_fermion_force_multi (fermion_force_hisq_multi.c): mom (V, 4, 10)

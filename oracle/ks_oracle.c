@@ -968,6 +968,145 @@ static void path_adjoint(const kso_geom *g, const double coeff[6], const double 
   }
 }
 
+/* The same adjoint in staple form: reverse mode through the optimised
+ * fattening of kso_fat_links (X = (c1 - 6 c_lep) U + sum_nu [c3 S1 + c_lep S_nu(S1)
+ * + sum_rho (c5 S2 + sum_sig c7 S_sig(S2))], S1 = S_nu(U_mu), S2 = S_rho(S1),
+ * S_nu(A)(x) = U_nu(x) A(x+nu) U_nu(x+mu)^dag + U_nu(x-nu)^dag A(x-nu) U_nu(x-nu+mu))
+ * plus the Naik product.  For unitary links it differs from path_adjoint only
+ * by components that the traceless anti-Hermitian projection (and the U(3)
+ * projection derivative) remove: the retracing Lepage terms are constant on
+ * the group. */
+static void staple_fwd(const kso_geom *g, int mu, int nu, const double *A, int strideA, const double *U,
+                       double *out) {
+#pragma omp parallel for
+  for (size_t x = 0; x < g->V; x++) {
+    double t1[18], up[18], lo[18];
+    size_t xn = NB(g, nu, x), xm = NB(g, mu, x), xd = NB(g, 7 - nu, x), xdm = NB(g, mu, xd);
+    m_na(A + (size_t)strideA * 18 * xn, U + (4 * xm + nu) * 18, t1);
+    m_nn(U + (4 * x + nu) * 18, t1, up);
+    m_an(U + (4 * xd + nu) * 18, A + (size_t)strideA * 18 * xd, t1);
+    m_nn(t1, U + (4 * xdm + nu) * 18, lo);
+    for (int k = 0; k < 18; k++) out[18 * x + k] = up[k] + lo[k];
+  }
+}
+
+/* reverse of O = S_nu(A) (mu-staple) for the force Gs on O, scaled by cg:
+ * FA += force on A (stride strideFA, may be NULL), FU[nu] += force on U_nu */
+static void staple_adj(const kso_geom *g, int mu, int nu, const double *A, int strideA, const double *Gs,
+                       double cg, const double *U, double *FA, int strideFA, double *FU) {
+#pragma omp parallel for
+  for (size_t y = 0; y < g->V; y++) {
+    double t1[18], t2[18], acc[18];
+    size_t ypn = NB(g, nu, y), ymn = NB(g, 7 - nu, y), ypm = NB(g, mu, y), ymm = NB(g, 7 - mu, y);
+    size_t ymnpm = NB(g, mu, ymn), ymmpn = NB(g, nu, ymm);
+    const double *Unu = U + nu * 18;
+#define UN(s) (Unu + 4 * 18 * (s))
+#define AA(s) (A + (size_t)strideA * 18 * (s))
+#define GG(s) (Gs + 18 * (s))
+    if (FA) {
+      /* FA(y) += U_nu(y-nu+mu)^dag G(y-nu) U_nu(y-nu) + U_nu(y+mu) G(y+nu) U_nu(y)^dag */
+      m_an(UN(ymnpm), GG(ymn), t1);
+      m_nn(t1, UN(ymn), acc);
+      m_nn(UN(ypm), GG(ypn), t1);
+      m_na(t1, UN(y), t2);
+      for (int k = 0; k < 18; k++) acc[k] += t2[k];
+      m_axpy(FA + (size_t)strideFA * 18 * y, cg, acc);
+    }
+    /* FU_nu(y) += A(y+nu) U_nu(y+mu)^dag G(y) + [G(y-mu) U_nu(y-mu) A(y-mu+nu)]^dag
+     *           + [A(y) U_nu(y+mu) G(y+nu)]^dag + G(y-mu+nu) U_nu(y-mu)^dag A(y-mu) */
+    m_na(AA(ypn), UN(ypm), t1);
+    m_nn(t1, GG(y), acc);
+    m_nn(GG(ymm), UN(ymm), t1);
+    m_nn(t1, AA(ymmpn), t2);
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) { RE(acc, i, j) += RE(t2, j, i); IM(acc, i, j) -= IM(t2, j, i); }
+    m_nn(AA(y), UN(ypm), t1);
+    m_nn(t1, GG(ypn), t2);
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) { RE(acc, i, j) += RE(t2, j, i); IM(acc, i, j) -= IM(t2, j, i); }
+    m_na(GG(ymmpn), UN(ymm), t1);
+    m_nn(t1, AA(ymm), t2);
+    for (int k = 0; k < 18; k++) acc[k] += t2[k];
+    m_axpy(FU + (4 * y + nu) * 18, cg, acc);
+#undef UN
+#undef AA
+#undef GG
+  }
+}
+
+static void smear_adjoint(const kso_geom *g, const double coeff[6], const double *L, const double *F1,
+                          const double *F3, int with_eps, double *G) {
+  const size_t V = g->V;
+  double *F = (double *)malloc(sizeof(double) * 18 * V);
+  double *S1 = (double *)malloc(sizeof(double) * 18 * V);
+  double *S2 = (double *)malloc(sizeof(double) * 18 * V);
+  double *G1 = (double *)malloc(sizeof(double) * 18 * V);
+  double *G2 = (double *)malloc(sizeof(double) * 18 * V);
+  const double c1 = coeff[0] - 6.0 * coeff[5], cN = coeff[1], c3 = coeff[2], c5 = coeff[3], c7 = coeff[4],
+               cl = coeff[5];
+  for (int mu = 0; mu < 4; mu++) {
+    for (size_t x = 0; x < V; x++) {
+      double e = (with_eps && x >= g->Vh) ? -1.0 : 1.0;
+      for (int k = 0; k < 18; k++) F[18 * x + k] = e * F1[(4 * x + mu) * 18 + k];
+      m_axpy(G + (4 * x + mu) * 18, c1, F + 18 * x);
+    }
+    if (c3 != 0.0 || cl != 0.0 || c5 != 0.0 || c7 != 0.0) {
+      for (int nu = 0; nu < 4; nu++) {
+        if (nu == mu) continue;
+        staple_fwd(g, mu, nu, L + 18 * mu, 4, L, S1);
+        for (size_t k = 0; k < 18 * V; k++) G1[k] = c3 * F[k];
+        if (cl != 0.0) staple_adj(g, mu, nu, S1, 1, F, cl, L, G1, 1, G);
+        if (c5 != 0.0 || c7 != 0.0) {
+          for (int rho = 0; rho < 4; rho++) {
+            if (rho == mu || rho == nu) continue;
+            staple_fwd(g, mu, rho, S1, 1, L, S2);
+            for (size_t k = 0; k < 18 * V; k++) G2[k] = c5 * F[k];
+            if (c7 != 0.0)
+              for (int sig = 0; sig < 4; sig++) {
+                if (sig == mu || sig == nu || sig == rho) continue;
+                staple_adj(g, mu, sig, S2, 1, F, c7, L, G2, 1, G);
+              }
+            staple_adj(g, mu, rho, S1, 1, G2, 1.0, L, G1, 1, G);
+          }
+        }
+        staple_adj(g, mu, nu, L + 18 * mu, 4, G1, 1.0, L, G + 18 * mu, 4, G);
+      }
+    }
+    if (cN != 0.0) {
+      /* L_mu(x) = U(x) U(x+mu) U(x+2mu): gather the three positions */
+      for (size_t x = 0; x < V; x++) {
+        double e = (with_eps && x >= g->Vh) ? -1.0 : 1.0;
+        for (int k = 0; k < 18; k++) F[18 * x + k] = e * F3[(4 * x + mu) * 18 + k];
+      }
+#pragma omp parallel for
+      for (size_t y = 0; y < V; y++) {
+        double t1[18], t2[18], acc[18];
+        size_t p1 = NB(g, mu, y), p2 = NB(g, mu, p1), m1 = NB(g, 7 - mu, y), m2 = NB(g, 7 - mu, m1);
+        const double *Um = L + mu * 18;
+        m_nn(Um + 72 * p1, Um + 72 * p2, t1);
+        m_nn(t1, F + 18 * y, acc);
+        m_nn(Um + 72 * p1, F + 18 * m1, t1);
+        m_nn(t1, Um + 72 * m1, t2);
+        for (int k = 0; k < 18; k++) acc[k] += t2[k];
+        m_nn(F + 18 * m2, Um + 72 * m2, t1);
+        m_nn(t1, Um + 72 * m1, t2);
+        for (int k = 0; k < 18; k++) acc[k] += t2[k];
+        m_axpy(G + (4 * y + mu) * 18, cN, acc);
+      }
+    }
+  }
+  free(F); free(S1); free(S2); free(G1); free(G2);
+}
+
+static int g_force_form = 0; /* 0: path table (gather), 1: staple form */
+void kso_set_force_form(int form) { g_force_form = form; }
+
+static void force_adjoint(const kso_geom *g, const double coeff[6], const double *L, const double *F1,
+                          const double *F3, int with_eps, double *G) {
+  if (g_force_form) smear_adjoint(g, coeff, L, F1, F3, with_eps, G);
+  else path_adjoint(g, coeff, L, F1, F3, with_eps, G);
+}
+
 /* mom = TA(scale U G) per link, full anti-Hermitian matrices (libraries/make_ahmat.c) */
 static void ta_force(const kso_geom *g, const double *U, const double *G, double scale, double *mom) {
 #pragma omp parallel for
@@ -993,7 +1132,7 @@ void kso_fn_force(const kso_geom *g, const double coeff[6], const double *U, con
   double *o3 = (double *)malloc(sizeof(double) * 18 * 4 * V);
   double *G = (double *)calloc(18 * 4 * V, sizeof(double));
   kso_force_oprod(g, multi_x, residues, nterms, o1, o3);
-  path_adjoint(g, coeff, U, o1, o3, 1, G);
+  force_adjoint(g, coeff, U, o1, o3, 1, G);
   ta_force(g, U, G, 2.0 * eps, mom);
   free(o1); free(o3); free(G);
 }
@@ -1142,10 +1281,10 @@ void kso_hisq_force(const kso_geom *g, const double *U, const double *const *mul
 #pragma omp parallel for
   for (size_t l = 0; l < 4 * g->V; l++) kso_u3_unitarize(Vl + 18 * l, W + 18 * l);
   kso_force_oprod(g, multi_x, residues, nterms, o1, o3);
-  path_adjoint(g, c2, W, o1, o3, 1, FW);
+  force_adjoint(g, c2, W, o1, o3, 1, FW);
 #pragma omp parallel for
   for (size_t l = 0; l < 4 * g->V; l++) kso_u3_adjoint(Vl + 18 * l, FW + 18 * l, FV + 18 * l);
-  path_adjoint(g, c1, U, FV, FV, 0, FU);
+  force_adjoint(g, c1, U, FV, FV, 0, FU);
   ta_force(g, U, FU, 2.0 * eps, mom);
   free(Vl); free(W); free(o1); free(o3); free(FW); free(FV); free(FU);
 }

@@ -101,6 +101,8 @@ void kso_fn_force(const kso_geom *g, const double coeff[6], const double *U, con
  * adjoint -> fat7 adjoint on U): mom = TA(2 eps U F_U), full matrices. */
 void kso_hisq_force(const kso_geom *g, const double *U, const double *const *multi_x,
                     const double *residues, int nterms, double eps, double *mom);
+/* 0: force adjoints through the path tables (default), 1: staple form */
+void kso_set_force_form(int form);
 /* force through W = V (V^dag V)^-1/2 at one link: F_V from F_W */
 void kso_u3_adjoint(const double *V, const double *FW, double *FV);
 /* the force's path table for direction mu (dirs[7*p+i], -1 padded) */

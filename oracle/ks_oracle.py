@@ -68,6 +68,7 @@ def lib():
         L.kso_fn_force.argtypes = [G, P, P, P, P, ctypes.c_int, ctypes.c_double, P]
         L.kso_hisq_force.argtypes = [G, P, P, P, ctypes.c_int, ctypes.c_double, P]
         L.kso_u3_adjoint.argtypes = [P, P, P]
+        L.kso_set_force_form.argtypes = [ctypes.c_int]
         L.kso_norm2.argtypes = [G, P, ctypes.c_int]
         L.kso_norm2.restype = ctypes.c_double
         L.kso_mdagm.argtypes = [G, P, P, ctypes.c_double, P, P, P, ctypes.c_int]
@@ -232,6 +233,11 @@ def u3_unitarize(V):
     W = np.zeros_like(V)
     svd = lib().kso_u3_unitarize(_p(V), _p(W))
     return W, svd
+
+
+def set_force_form(form):
+    """0: force adjoints through the path tables, 1: staple (reverse-mode) form"""
+    lib().kso_set_force_form(int(form))
 
 
 def u3_adjoint(V, FW):

@@ -426,3 +426,23 @@ def test_hisq_force_thin_link_finite_differences():
     U = L.rephase(L.gauge_random(78, 0.3))
     _thin_link_fd(L, U, lambda Ul: L.hisq_links(Ul)[:2],
                   lambda Ul, Z, r, e: L.hisq_force(Ul, Z, r, eps=e), seed=6)
+
+
+def test_staple_form_force_equals_path_table_form():
+    """The reverse-mode staple form of the smearing adjoint gives the same
+    thin-link momentum as the path-table form (asqtad FN and HISQ chains) on a
+    non-cubic lattice."""
+    L = ko.Lattice((4, 6, 4, 8))
+    U = L.rephase(L.gauge_random(12, 0.4))
+    xs = [L.gaussian_vector(40 + t, ko.EVENANDODD) for t in range(2)]
+    res = [0.5, -0.3]
+    try:
+        for f in (lambda: L.fn_force(ko.asqtad_coeffs(0.8), U, xs, res, eps=0.4),
+                  lambda: L.hisq_force(U, xs, res, eps=0.4)):
+            ko.set_force_form(0)
+            a = f()
+            ko.set_force_form(1)
+            b = f()
+            assert rel(b, a) < 1e-13
+    finally:
+        ko.set_force_form(0)

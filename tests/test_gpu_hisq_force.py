@@ -72,3 +72,51 @@ def test_hisq_force_quda_form_vs_oracle():
         mom = np.zeros((L.V, 4, 10))
         G.hisq_force(eps, coeff, xs, C2, C1, W, Vf, U, mom)
     assert rel(mom, ref) < 1e-12
+
+
+@pytest.mark.parametrize("form", [0, 1, 2])
+def test_every_device_form_vs_oracle(form):
+    """the path-table gather (0), the default choice (1: staple form where it is
+    exact for any links) and the staple form everywhere (2) each match the
+    oracle's path table, HISQ and asqtad FN, on unitary links"""
+    dims = (6, 4, 8, 10)
+    L = ko.Lattice(dims)
+    U = L.rephase(L.gauge_random(1023))
+    xs = [L.gaussian_vector(320 + t, ko.EVENANDODD) for t in range(2)]
+    res = [0.7, -0.4]
+    eps = 0.3
+    ref_h = pack_ah(L.hisq_force(U, xs, res, eps=eps))
+    c = ko.asqtad_coeffs(0.85)
+    ref_f = pack_ah(L.fn_force(c, U, xs, res, eps=eps))
+    with mq.Lattice(dims) as G:
+        G.set_force_form(form)
+        try:
+            mom = np.zeros((L.V, 4, 10))
+            G.eo_fermion_force_multi(eps, res, xs, U, mom)
+            assert rel(mom, ref_h) < 1e-12
+            mom = np.zeros((L.V, 4, 10))
+            G.fermion_force_fn_multi(eps, res, xs, c, U, mom)
+            assert rel(mom, ref_f) < 1e-12
+        finally:
+            G.set_force_form(1)
+
+
+@pytest.mark.parametrize("form", [0, 1])
+def test_non_unitary_links_exact_by_default(form):
+    """links 3% off U(3): the forms that use the staple form only where it is
+    exact (0, 1) still reproduce the path-table definition of the force"""
+    dims = (4, 4, 4, 6)
+    L = ko.Lattice(dims)
+    U = L.rephase(L.gauge_random(1024))
+    U = U * (1.0 + 0.03 * np.random.default_rng(5).standard_normal(U.shape))
+    xs = [L.gaussian_vector(330, ko.EVENANDODD)]
+    c = ko.asqtad_coeffs(0.4)
+    ref_f = pack_ah(L.fn_force(c, U, xs, [1.1], eps=0.5))
+    with mq.Lattice(dims) as G:
+        G.set_force_form(form)
+        try:
+            mom = np.zeros((L.V, 4, 10))
+            G.fermion_force_fn_multi(0.5, [1.1], xs, c, U, mom)
+            assert rel(mom, ref_f) < 1e-12
+        finally:
+            G.set_force_form(1)
