@@ -21,6 +21,9 @@ Usage: python bench.py [--gpus 1] [--steps 10] [--warmup 1] [--no-cpu]
            FP64 true residual 1e-9; prints its own JSON line)
        python bench.py --workload hmc   (one RHMC force + FP64 action on 48^3 x 96,
            m = 0.01, 12 shifts as above, residues 0.1 (i+1); prints its own JSON line)
+       python bench.py --workload force   (the HISQ fermion force alone on 48^3 x 96,
+           12 terms from resident vectors; prints its own JSON line)
+       --no-sequential (multishift: leave out the one-source comparison solves)
 """
 import argparse
 import json
@@ -201,12 +204,14 @@ def multishift_workload(args):
     errs = [G.dev_diff(16 + j, 40 + j) for j in range(n)]
     it64_target, _, _ = G.dev_multishift(0, offs, MS_RESID, 0)
     # one-source solves (same FP32 + refinement scheme), all 16 sources
-    t1 = time.perf_counter()
-    seq_iters = []
-    for j in range(16):
-        it, _, conv = G.dev_multishift(j, offs, MS_RESID, 1)
-        seq_iters.append(it)
-    t_seq = time.perf_counter() - t1
+    t_seq, seq_iters = None, None
+    if not args.no_sequential:
+        t1 = time.perf_counter()
+        seq_iters = []
+        for j in range(16):
+            it, _, conv = G.dev_multishift(j, offs, MS_RESID, 1)
+            seq_iters.append(it)
+        t_seq = time.perf_counter() - t1
     us_f32 = G.dev_dslash_bench(1, mq.EVEN, 20)
     cpu = None
     if not args.no_cpu:
@@ -239,8 +244,8 @@ def multishift_workload(args):
         "true_rel_resid_per_shift_max": st["true_rel"],
         "rel_err_vs_fp64_solve_src0": max(errs),
         "fp64_reference": {"iters": it64, "seconds": round(t64, 4)},
-        "sequential_16_one_source_solves_s": round(t_seq, 4),
-        "speedup_batched_vs_sequential": round(t_seq / t_batch, 3),
+        "sequential_16_one_source_solves_s": None if t_seq is None else round(t_seq, 4),
+        "speedup_batched_vs_sequential": None if t_seq is None else round(t_seq / t_batch, 3),
         "sequential_iters": seq_iters,
         "autotune": {"chosen": chosen, "desc": desc, "variants": variants},
         "batched_dslash": {"usec": round(us_best, 2), "alg_bytes_per_site": bytes_site,
@@ -312,7 +317,7 @@ def hmc_workload(args):
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 multi-shift + f64 recurrences, mixed refinement, f64 force and action",
+        "dtype": "f32 multi-shift + f64 recurrences, 16-bit/f64 mixed refinement per shift, f64 force and action",
         "data": "synthetic",
         "config": {"workload": "hisq_rhmc_force_action_48x48x48x96_m0.01_12shifts", "lattice": list(HMC_DIMS),
                    "mass": HMC_MASS, "sigmas": MS_SIGMAS, "residues": res, "gauge_seed": HMC_GAUGE_SEED,
@@ -335,6 +340,98 @@ def hmc_workload(args):
     print(json.dumps(out), flush=True)
 
 
+FORCE_TERMS = 12
+FORCE_SRC_SEEDS = [2600 + t for t in range(FORCE_TERMS)]
+FORCE_CPU_DIMS = (16, 16, 16, 32)
+
+
+def cpu_force_baseline(res, eps):
+    """MILC's own eo_fermion_force_multi (compiled reference, FP64, HISQ) on
+    the host cores: one call with the same number of terms on a 16^3 x 32
+    sample lattice (same gauge generator), scaled by the volume ratio."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import milc_ref as mr
+    if not mr.available():
+        return None
+    import ks_oracle as ko
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    L = ko.Lattice(FORCE_CPU_DIMS)
+    V = mr.init(FORCE_CPU_DIMS)
+    mr.set_thin_links(L.gauge_random(HMC_GAUGE_SEED))
+    mr.hisq_links(V)
+    xs = [L.gaussian_vector(sd, ko.EVENANDODD) for sd in FORCE_SRC_SEEDS]
+    t1 = time.perf_counter()
+    mr.hisq_force(xs, res, eps, V)
+    dt = time.perf_counter() - t1
+    scale = (HMC_DIMS[0] * HMC_DIMS[1] * HMC_DIMS[2] * HMC_DIMS[3]) / V
+    return {
+        "value": dt * scale,
+        "unit": "s/force",
+        "cores": threads,
+        "kind": "reference",
+        "sample": (f"MILC 7.8.1 eo_fermion_force_multi (HISQ, FP64, {len(res)} terms, OpenMP {threads} threads), "
+                   f"compiled from the reference sources, one call on a {'x'.join(map(str, FORCE_CPU_DIMS))} lattice "
+                   f"(thin links from gauge seed {HMC_GAUGE_SEED}, Gaussian Z_t): {dt:.2f} s, scaled by the volume "
+                   f"ratio {scale:.0f} to {'x'.join(map(str, HMC_DIMS))}"),
+    }
+
+
+def force_workload(args):
+    """The HISQ fermion force alone on 48^3 x 96: 12 terms, X_t resident on the
+    device (Gaussian even-site vectors; the force's cost does not depend on
+    them being solutions), D_oe X_t, outer products, level-2 smearing adjoint
+    on W, U(3) projection derivative, level-1 adjoint on U, TA into the
+    momentum.  One step = one force."""
+    import milc_qcd_amd as mq
+    res = [0.1 * (i + 1) for i in range(FORCE_TERMS)]
+    eps = 1.0
+    G = mq.Lattice(HMC_DIMS)
+    G.dev_gauge_hisq(HMC_GAUGE_SEED, 0.4)
+    for t, sd in enumerate(FORCE_SRC_SEEDS):
+        G.dev_source(sd, mq.EVEN, t)
+    steps = []
+    for i in range(args.warmup + args.steps):
+        st = G.dev_hisq_force(0, res, eps)
+        log(f"force {i}: {st}")
+        if i >= args.warmup:
+            steps.append(st)
+    G.close()
+    cpu = None
+    if not args.no_cpu:
+        try:
+            with stdout_to_stderr():
+                cpu = cpu_force_baseline(res, eps)
+        except Exception as e:
+            log(f"cpu baseline failed: {e!r}")
+    t_oprod = sum(s["oprod_seconds"] for s in steps) / len(steps)
+    t_thin = sum(s["thin_force_seconds"] for s in steps) / len(steps)
+    t = t_oprod + t_thin
+    out = {
+        "metric": "HISQ fermion force from resident solutions to the thin-link momentum (48^3x96, 12 terms)",
+        "value": round(t, 4),
+        "unit": "s/force",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t * 1e3, 2),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "hisq_fermion_force_48x48x48x96_12terms", "lattice": list(HMC_DIMS),
+                   "terms": FORCE_TERMS, "residues": res, "eps": eps, "gauge_seed": HMC_GAUGE_SEED,
+                   "source_seeds": FORCE_SRC_SEEDS},
+        "oprod_seconds": round(t_oprod, 4),
+        "thin_link_chain_seconds": round(t_thin, 4),
+        "aux_links_seconds": round(steps[-1]["aux_seconds"], 4),
+        "mom_norm2": steps[-1]["mom_norm2"],
+        "roofline": None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -343,7 +440,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--kernel-only", action="store_true",
                     help="only the Dslash and copy bursts (for rocprofv3 counter runs)")
-    ap.add_argument("--workload", choices=["cg", "multishift", "hmc"], default="cg")
+    ap.add_argument("--workload", choices=["cg", "multishift", "hmc", "force"], default="cg")
+    ap.add_argument("--no-sequential", action="store_true",
+                    help="multishift: leave out the 16 one-source comparison solves")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); "
@@ -354,6 +453,9 @@ def main():
         return
     if args.workload == "hmc":
         hmc_workload(args)
+        return
+    if args.workload == "force":
+        force_workload(args)
         return
 
     import milc_qcd_amd as mq

@@ -277,6 +277,19 @@ int milc_hip_dev_multishift(int slot_src, int parity, int num_offsets, const dou
  * rel. error of the thin-link momentum (-1 when not computed). */
 int milc_hip_dev_hmc_step(int slot_phi, int n, const double* offsets, const double* residues, double resid,
                           int mixed_precision, int compare_fp64, int thin_force, double* stats);
+
+/* HISQ thin-link force from device-resident solutions: the part of
+ * update_h_rhmc.c:77-107 after the solve, i.e. eo_fermion_force_multi
+ * (generic_ks/fermion_force_hisq_multi.c:1182-1475 through
+ * fermion_force_fn_multi_hisq) with multi_x on the device.  Slots slot0 ..
+ * slot0+n-1 hold X_t on even sites (FP64); the odd halves D_oe X_t are
+ * computed here, the outer products take coefficients residues[t], and
+ * TA(2 eps U F_U) goes through the HISQ chain on the links built by
+ * milc_hip_dev_gauge_hisq.  momentum (nullable: FP64 [V][4][10] packed
+ * anti_hermitmat, MILC site order) is ADDED to.  stats[0..3] = seconds for
+ * D_oe + outer products, seconds for the thin-link chain, |momentum
+ * increment|^2, seconds to rebuild the HISQ intermediate links V, W. */
+int milc_hip_dev_hisq_force(int slot0, int n, const double* residues, double eps, void* momentum, double* stats);
 /* |a - b| / |b| over the parity half held in two device slots. */
 int milc_hip_dev_diff(int slot_a, int slot_b, double* rel);
 

@@ -68,6 +68,9 @@ int upload_links(int ext_prec, const void* fat, const void* lng) {
   return 0;
 }
 
+}  // namespace
+
+namespace mh {
 // Sloppy copies of the FP64 links, built on demand.
 int ensure_links(int prec) {
   Context& c = ctx();
@@ -82,6 +85,9 @@ int ensure_links(int prec) {
   L.valid = true;
   return 0;
 }
+}  // namespace mh
+
+namespace {
 
 // Links refresh rule of the reference glue: re-upload when the caller signals
 // new links (*num_iters == -1) or passes a different link array.
@@ -663,6 +669,7 @@ int rhmc_action(const VecField& phi, int order, const double* roots, const doubl
                 double* action, int* iters);
 int rhmc_force_level0(const VecField& phi, int n, const double* offsets, const double* residues, double resid,
                       int mixed, double* S, LinkField& o1, LinkField& o3, int* iters);
+int force_level0_from_solutions(const std::vector<VecField>& x, const double* residues, LinkField& o1, LinkField& o3);
 }
 
 extern "C" int milc_hip_dev_rhmc_action(int slot_phi, int order, const double* roots, const double* residues,
@@ -1225,7 +1232,7 @@ __global__ void k_flat_diff(const double* __restrict__ a, const double* __restri
   }
 }
 
-int flat_rel_diff(const double* a, const double* b, size_t n, double* rel) {
+int flat_rel_diff(const double* a, const double* b, size_t n, double* rel, double* norm2_b = nullptr) {
   Context& c = ctx();
   const int nb = 1024;
   hipLaunchKernelGGL(k_flat_diff, dim3(nb), dim3(256), 0, c.stream, a, b, n, c.ws.partials);
@@ -1239,6 +1246,7 @@ int flat_rel_diff(const double* a, const double* b, size_t n, double* rel) {
     m += h[2 * i + 1];
   }
   *rel = m > 0 ? std::sqrt(d / m) : 0.0;
+  if (norm2_b) *norm2_b = m;
   return 0;
 }
 
@@ -1261,6 +1269,57 @@ int link_rel_diff(const LinkField& a, const LinkField& b, double* rel) {
   return 0;
 }
 }  // namespace
+
+extern "C" int milc_hip_dev_hisq_force(int slot0, int n, const double* residues, double eps, void* momentum,
+                                       double* stats) {
+  Context& c = ctx();
+  if (!c.initialised || !c.links[P_F64].valid) return -1;
+  if (!c.thin_valid) return -3;  // needs links built on the device
+  if (n <= 0 || n > 32 || !residues || !stats || slot0 < 0 || slot0 + n > kSlots) return -2;
+  for (int t = 0; t < n; t++)
+    if (!slot_ok(slot0 + t, true)) return -2;
+  LinkField o1{}, o3{}, V{}, W{};
+  double* mom = nullptr;
+  auto release = [&]() {
+    free_links(o1);
+    free_links(o3);
+    free_links(V);
+    free_links(W);
+    dev_free(mom);
+  };
+  const size_t nmom = (size_t)2 * c.Vh * 4 * 10;
+  int rc;
+  if ((rc = alloc_links(o1, P_F64, c.stride)) || (rc = alloc_links(o3, P_F64, c.stride)) ||
+      (rc = alloc_links(V, P_F64, c.stride)) || (rc = alloc_links(W, P_F64, c.stride)) ||
+      (rc = dev_alloc((void**)&mom, nmom * sizeof(double)))) {
+    release();
+    return rc;
+  }
+  MH_CHECK(hipMemsetAsync(mom, 0, nmom * sizeof(double), c.stream));
+  auto t0 = std::chrono::steady_clock::now();
+  rc = hisq_aux_dev(c.thin, V, W);
+  (void)hipStreamSynchronize(c.stream);
+  const double t_aux = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  std::vector<VecField> x(g_slots + slot0, g_slots + slot0 + n);
+  t0 = std::chrono::steady_clock::now();
+  if (!rc) rc = force_level0_from_solutions(x, residues, o1, o3);
+  (void)hipStreamSynchronize(c.stream);
+  const double t_oprod = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  t0 = std::chrono::steady_clock::now();
+  if (!rc) rc = hisq_force_from_oprod(c.thin, V, W, o1, o3, kHisqLevel2, kHisqLevel1, 2.0 * eps, kForceFilter, mom);
+  (void)hipStreamSynchronize(c.stream);
+  const double t_thin = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  double rel = 0, norm2 = 0;
+  if (!rc) rc = flat_rel_diff(mom, mom, nmom, &rel, &norm2);
+  if (!rc && momentum) rc = add_momentum(2, mom, momentum);
+  release();
+  if (rc) return rc;
+  stats[0] = t_oprod;
+  stats[1] = t_thin;
+  stats[2] = norm2;
+  stats[3] = t_aux;
+  return 0;
+}
 
 extern "C" int milc_hip_dev_hmc_step(int slot_phi, int n, const double* offsets, const double* residues,
                                      double resid, int mixed_precision, int compare_fp64, int thin_force,

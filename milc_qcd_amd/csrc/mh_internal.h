@@ -119,6 +119,8 @@ struct SolveStats {
   double seconds = 0;
 };
 int cg_solve(const VecField& src, VecField& x, double mass, const SolveParams& sp, SolveStats& st);
+// builds the sloppy copy of the FP64 device links in precision prec if absent
+int ensure_links(int prec);
 
 struct MultiParams {
   int parity;

@@ -97,27 +97,31 @@ static int rational_solve(const VecField& phi, int n, const double* offsets, dou
   MultiParams mp;
   mp.parity = parity;
   mp.max_iters = 20000;
-  mp.resid = mixed ? std::max(resid, 1e-6) : resid;
+  mp.resid = mixed ? std::max(resid, 1e-7) : resid;
   mp.prec = mixed ? P_F32 : P_F64;
   mp.batch = 16;
   int bulk = mp.prec;
   if (!c.links[bulk].valid) return -3;
   if ((rc = multicg_solve(phi, x, offs, mp, st))) return rc;
-  if (mixed) {
-    for (int t = 0; t < n; t++) {
-      SolveParams sp;
-      sp.parity = parity;
-      sp.max_iters = 20000;
-      sp.max_restarts = 1;
-      sp.resid = resid;
-      sp.relresid = 0;
-      sp.sloppy_prec = P_F32;
-      sp.delta = 0.1;
-      sp.batch = 16;
-      SolveStats s2;
-      if ((rc = cg_solve(phi, x[t], 0.5 * std::sqrt(offs[t]), sp, s2))) return rc;
-      st.iters += s2.iters;
-    }
+  if (!mixed) return 0;
+  // Refinement of every shift to the FP64 true-residual target (the per-shift
+  // CG of ks_multicg_hybrid_field, generic_ks/ks_multicg.c:170-236): mixed CG
+  // from the multi-shift solution, 16-bit sloppy operator, FP64 reliable updates.
+  if ((rc = ensure_links(P_H16))) return rc;
+  for (int t = 0; t < n; t++) {
+    SolveParams sp;
+    sp.parity = parity;
+    sp.max_iters = 20000;
+    sp.max_restarts = 1;
+    sp.resid = resid;
+    sp.relresid = 0;
+    sp.sloppy_prec = P_H16;
+    sp.delta = 0.1;
+    sp.batch = 16;
+    SolveStats s2;
+    if ((rc = cg_solve(phi, x[t], 0.5 * std::sqrt(offs[t]), sp, s2))) return rc;
+    st.iters += s2.iters;
+    st.converged &= s2.converged;
   }
   return 0;
 }
@@ -157,6 +161,8 @@ int rhmc_action(const VecField& phi, int order, const double* roots, const doubl
   return rc;
 }
 
+int force_level0_from_solutions(const std::vector<VecField>& x, const double* residues, LinkField& o1, LinkField& o3);
+
 // S = sum_t r_t Re<phi, X_t>, X_t = (M^dag M + sigma_t)^-1 phi on even sites,
 // and the level-0 force O1/O3 with coefficients r_t from Z_t = (X_t, D_oe X_t).
 int rhmc_force_level0(const VecField& phi, int n, const double* offsets, const double* residues, double resid,
@@ -173,17 +179,27 @@ int rhmc_force_level0(const VecField& phi, int n, const double* offsets, const d
     s += residues[t] * d;
   }
   *S = s;
+  rc = force_level0_from_solutions(x, residues, o1, o3);
+  for (auto& v : x) free_vec(v);
+  if (iters) *iters = st.iters;
+  return rc;
+}
+
+// Level-0 force O1/O3 with coefficients r_t from the even-site solutions X_t:
+// Z_t = (X_t, D_oe X_t) (update_h_rhmc.c:77-107), the odd halves in temporaries.
+int force_level0_from_solutions(const std::vector<VecField>& x, const double* residues, LinkField& o1, LinkField& o3) {
+  Context& c = ctx();
+  const int n = (int)x.size();
   std::vector<VecField> y(n);
   std::vector<double> r(residues, residues + n);
-  for (int t = 0; t < n; t++) {
-    if ((rc = alloc_vec(y[t], P_F64, c.stride))) return rc;
-    if ((rc = dslash(c.links[P_F64], x[t], y[t], 1, P_F64))) return rc;  // odd sites: D_oe X_t
+  int rc = 0;
+  for (int t = 0; t < n && !rc; t++) {
+    rc = alloc_vec(y[t], P_F64, c.stride);
+    if (!rc) rc = dslash(c.links[P_F64], x[t], y[t], 1, P_F64);  // odd sites: D_oe X_t
   }
-  rc = force_oprod_coeff(x, y, r, r, o1, o3);
-  MH_CHECK(hipStreamSynchronize(c.stream));
-  for (auto& v : x) free_vec(v);
+  if (!rc) rc = force_oprod_coeff(x, y, r, r, o1, o3);
+  (void)hipStreamSynchronize(c.stream);
   for (auto& v : y) free_vec(v);
-  if (iters) *iters = st.iters;
   return rc;
 }
 

@@ -50,7 +50,7 @@ EXPORTS = [
     "milc_hip_dev_block_dslash_bench", "milc_hip_dev_block_dslash", "milc_hip_dev_multishift_block",
     "milc_hip_dev_block_describe",
     "milc_hip_dev_multishift", "milc_hip_dev_hmc_step", "milc_hip_hisq_force", "milc_hip_eo_fermion_force_multi",
-    "milc_hip_set_force_form",
+    "milc_hip_set_force_form", "milc_hip_dev_hisq_force",
 ]
 
 
@@ -105,6 +105,7 @@ def load_library(path=LIB_PATH):
     L.milc_hip_hisq_force.argtypes = [I, I, I, D, ctypes.POINTER(PD), ctypes.POINTER(P), PD, PD, P, P, P, P]
     L.milc_hip_eo_fermion_force_multi.argtypes = [I, D, PD, ctypes.POINTER(P), I, P, P]
     L.milc_hip_set_force_form.argtypes = [I]
+    L.milc_hip_dev_hisq_force.argtypes = [I, I, P, ctypes.c_double, P, P]
     _lib = L
     return L
 
@@ -495,6 +496,20 @@ class Lattice:
         return {"force_seconds": st[0], "force_iters": int(st[1]), "S": st[2], "action": st[3],
                 "action_seconds": st[4], "action_iters": int(st[5]), "rel_err_O1": st[6], "rel_err_O3": st[7],
                 "fp64_force_seconds": st[8], "thin_force_seconds": st[9], "rel_err_mom": st[10]}
+
+    def dev_hisq_force(self, slot0, residues, eps, mom=None):
+        """HISQ thin-link force from the even-site solutions in slots slot0..;
+        adds into mom ((V, 4, 10) FP64, MILC order) when given"""
+        self._activate()
+        n = len(residues)
+        res = (ctypes.c_double * n)(*residues)
+        st = (ctypes.c_double * 4)()
+        mp = None
+        if mom is not None:
+            assert mom.dtype == np.float64 and mom.flags.c_contiguous and mom.shape == (self.V, 4, 10)
+            mp = _p(mom)
+        _check(self.lib.milc_hip_dev_hisq_force(slot0, n, res, eps, mp, st), "milc_hip_dev_hisq_force")
+        return {"oprod_seconds": st[0], "thin_force_seconds": st[1], "mom_norm2": st[2], "aux_seconds": st[3]}
 
     def dev_read(self, slot, parity):
         self._activate()

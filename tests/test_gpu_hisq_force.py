@@ -74,6 +74,35 @@ def test_hisq_force_quda_form_vs_oracle():
     assert rel(mom, ref) < 1e-12
 
 
+def test_dev_hisq_force_from_resident_solutions():
+    """milc_hip_dev_hisq_force: X_t on even sites in device slots, links built
+    on the device from the seed; the oracle gets the full Z_t = (X_t, D_oe X_t)
+    and the same thin links"""
+    dims = (6, 4, 8, 10)
+    L = ko.Lattice(dims)
+    U = L.rephase(L.gauge_random(1025))
+    fat, lng, _ = L.hisq_links(U)
+    res = [0.4, 1.2, -0.3]
+    eps = 0.45
+    E = L.parity_slice(ko.EVEN)
+    zs = []
+    for t in range(len(res)):
+        x = L.gaussian_vector(340 + t, ko.EVEN)
+        zs.append(x + L.dslash(fat, lng, x, ko.ODD))
+        assert np.array_equal(zs[-1][E], x[E])
+    ref = pack_ah(L.hisq_force(U, zs, res, eps=eps))
+    with mq.Lattice(dims) as G:
+        G.dev_gauge_hisq(1025)
+        for t in range(len(res)):
+            G.dev_source(340 + t, mq.EVEN, 5 + t)
+        mom = np.zeros((L.V, 4, 10))
+        st = G.dev_hisq_force(5, res, eps, mom)
+        print(st)
+        assert rel(mom, ref) < 1e-12
+        # |increment|^2 as reported; the packed form holds each off-diagonal once
+        assert st["mom_norm2"] > 0 and abs(st["mom_norm2"] - float(np.sum(mom ** 2))) < 1e-10 * st["mom_norm2"]
+
+
 @pytest.mark.parametrize("form", [0, 1, 2])
 def test_every_device_form_vs_oracle(form):
     """the path-table gather (0), the default choice (1: staple form where it is
