@@ -43,6 +43,8 @@ HBM_PEAK_GBS = 8000.0
 # algorithmic bytes per output site of one single-parity Dslash:
 # 16 link matrices + 1 source + 1 destination vector
 DSLASH_BYTES = {0: 16 * 144 + 48 + 48, 1: 16 * 72 + 24 + 24, 2: 16 * 36 + 16 + 16}
+# 16-bit stencil with the Naik links in the two-row form (rows 0, 1 and z: 28 B)
+DSLASH_BYTES_H16_TWO_ROW = 8 * 36 + 8 * 28 + 16 + 16
 DSLASH_FLOPS = 1146
 
 
@@ -282,10 +284,12 @@ def hmc_workload(args):
     G.dev_source(HMC_SRC_SEED, mq.EVEN, 0)
     Vh = G.Vh
     dsl = {}
+    two_row = G.link_compression(2) == 1
     for prec, name in ((0, "f64"), (1, "f32"), (2, "h16")):
         us = G.dev_dslash_bench(prec, mq.EVEN, 10)
-        dsl[name] = {"usec": round(us, 1), "GBps": round(DSLASH_BYTES[prec] * Vh / us / 1e3, 1),
-                     "frac_peak": round(DSLASH_BYTES[prec] * Vh / us / 1e3 / HBM_PEAK_GBS, 4)}
+        nb = DSLASH_BYTES_H16_TWO_ROW if (prec == 2 and two_row) else DSLASH_BYTES[prec]
+        dsl[name] = {"usec": round(us, 1), "alg_bytes_per_site": nb, "GBps": round(nb * Vh / us / 1e3, 1),
+                     "frac_peak": round(nb * Vh / us / 1e3 / HBM_PEAK_GBS, 4)}
     log(f"dslash {dsl}")
     steps = []
     for i in range(args.warmup + args.steps):
@@ -469,9 +473,11 @@ def main():
     XM, XD = 10, 11
     if args.kernel_only:
         res = {}
+        two_row = G.link_compression(2) == 1
         for prec, name in ((0, "f64"), (1, "f32"), (2, "h16")):
             us = G.dev_dslash_bench(prec, mq.EVEN, 20)
-            res[name] = {"usec": round(us, 2), "alg_bytes_per_launch": DSLASH_BYTES[prec] * G.Vh}
+            nb = DSLASH_BYTES_H16_TWO_ROW if (prec == 2 and two_row) else DSLASH_BYTES[prec]
+            res[name] = {"usec": round(us, 2), "alg_bytes_per_launch": nb * G.Vh}
         us = G.dev_copy_bench(1 << 30, 5)
         res["copy"] = {"usec": round(us, 2), "bytes_read": 1 << 30, "bytes_written": 1 << 30}
         G.close()
@@ -513,12 +519,26 @@ def main():
 
     # ---- Dslash roofline (HIP events on the library stream) ---------------
     Vh = G.Vh
+    two_row = G.link_compression(2) == 1
     dsl = {}
     for prec, name in ((0, "f64"), (1, "f32"), (2, "h16")):
         us = G.dev_dslash_bench(prec, mq.EVEN, 50)
-        dsl[name] = {"usec": round(us, 2), "GBps": round(DSLASH_BYTES[prec] * Vh / us / 1e3, 1),
-                     "frac_peak": round(DSLASH_BYTES[prec] * Vh / us / 1e3 / HBM_PEAK_GBS, 4),
+        nb = DSLASH_BYTES_H16_TWO_ROW if (prec == 2 and two_row) else DSLASH_BYTES[prec]
+        dsl[name] = {"usec": round(us, 2), "alg_bytes_per_site": nb, "GBps": round(nb * Vh / us / 1e3, 1),
+                     "frac_peak": round(nb * Vh / us / 1e3 / HBM_PEAK_GBS, 4),
                      "TFLOPs": round(DSLASH_FLOPS * Vh / us / 1e6, 2)}
+    if two_row:
+        # the same launch priced at the nine-element byte count, and the
+        # nine-element long links themselves (format switched for this probe)
+        dsl["h16"]["GBps_at_608_B_per_site"] = round(DSLASH_BYTES[2] * Vh / dsl["h16"]["usec"] / 1e3, 1)
+        G.set_link_compression(0)
+        us = G.dev_dslash_bench(2, mq.EVEN, 50)
+        dsl["h16_nine_element_long_links"] = {"usec": round(us, 2), "alg_bytes_per_site": DSLASH_BYTES[2],
+                                              "GBps": round(DSLASH_BYTES[2] * Vh / us / 1e3, 1)}
+        t1 = time.perf_counter()
+        G.dev_invert(0, XM, MASS, mq.EVEN, RESID, 2)
+        dsl["h16_nine_element_long_links"]["mixed_s_per_solve"] = round(time.perf_counter() - t1, 5)
+        G.set_link_compression(1)
     copy_bytes = 2 << 30
     us = G.dev_copy_bench(copy_bytes, 20)
     stream = 2 * copy_bytes / us / 1e3
@@ -571,7 +591,10 @@ def main():
         "stream_copy_GBps": round(stream, 1),
         "roofline": {"bound": "hbm", "achieved": h16["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": h16["frac_peak"], "traffic": traffic,
-                     "kernel": "k_dslash<P_H16> (16-bit links+vectors, 608 B/site alg.)"},
+                     "kernel": (f"k_dslash<P_H16> (16-bit fat links, "
+                                f"{'two-row' if two_row else 'nine-element'} 16-bit long links, 16-bit vectors, "
+                                f"{DSLASH_BYTES_H16_TWO_ROW if two_row else DSLASH_BYTES[2]} B/site alg.)")},
+        "long_links_two_row": two_row,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -146,6 +146,24 @@ int milc_hip_hisq_force(int precision, int num_terms, int num_naik_terms, double
 int milc_hip_eo_fermion_force_multi(int precision, double eps, const double* residues, void** multi_x, int nterms,
                                     const void* thin_links, void* momentum);
 
+/* dst slot (FP64) = D src slot on output parity parity_out, the stencil run in
+ * precision prec (0 FP64, 1 FP32, 2 16-bit: the source converted to and the
+ * result converted back from that precision) on the current device links. */
+int milc_hip_dev_dslash(int prec, int slot_src, int parity_out, int slot_dst);
+
+/* 16-bit sloppy long links in the two-row form (on = 1, the default): a long
+ * link that is a real multiple of a U(3) matrix, L = c U (HISQ and asqtad
+ * Naik links), is stored as rows 0 and 1 plus z = det(U)/c, and row 2 =
+ * z conj(row0 x row1) is rebuilt in the stencil: 28 instead of 36 bytes per
+ * link, 544 instead of 608 bytes per output site of the 16-bit Dslash.  The
+ * form is used only when every long link passes |L^dag L - c^2 1| <= 1e-5 c^2;
+ * other links stay in the nine-element form.  No counterpart in the
+ * reference (QUDA's reconstruct-13 plays this role under MILC's QUDA glue). */
+int milc_hip_set_link_compression(int on);
+/* Long-link format of the precision-prec device links (building them if
+ * needed): 1 two-row form, 0 nine elements, negative on error. */
+int milc_hip_link_compression(int prec);
+
 /* Evaluation form of the thin-link force adjoints (FN and HISQ): 0 = gather
  * over the path table for every smearing; 1 (default) = reverse mode through
  * the generalised staples of the fattening wherever that is exact for any

@@ -71,6 +71,12 @@ int upload_links(int ext_prec, const void* fat, const void* lng) {
 }  // namespace
 
 namespace mh {
+// 16-bit long links in the two-row form when they are scaled-unitary (on by default)
+static int g_long_recon = 1;
+// relative deviation |L^dag L - c^2 1| / c^2 accepted for the two-row form: below
+// the 16-bit quantisation step (2^-15), so the format adds no error of its own order
+static const double kLongReconTol = 1e-5;
+
 // Sloppy copies of the FP64 links, built on demand.
 int ensure_links(int prec) {
   Context& c = ctx();
@@ -80,7 +86,11 @@ int ensure_links(int prec) {
   int rc;
   if ((rc = alloc_links(L.fat, prec, c.stride)) || (rc = alloc_links(L.lng, prec, c.stride))) return rc;
   if ((rc = convert_links(c.links[P_F64].fat, P_F64, L.fat, prec))) return rc;
-  if ((rc = convert_links(c.links[P_F64].lng, P_F64, L.lng, prec))) return rc;
+  L.lng.recon = 0;
+  rc = 1;
+  if (prec == P_H16 && g_long_recon && (rc = convert_long_h16r(c.links[P_F64].lng, L.lng, kLongReconTol)) < 0)
+    return rc;
+  if (rc == 1 && (rc = convert_links(c.links[P_F64].lng, P_F64, L.lng, prec))) return rc;
   L.prec = prec;
   L.valid = true;
   return 0;
@@ -892,6 +902,29 @@ int ensure_slot(int s) {
 }
 }  // namespace
 
+extern "C" int milc_hip_dev_dslash(int prec, int slot_src, int parity_out, int slot_dst) {
+  Context& c = ctx();
+  if (!c.initialised || !c.links[P_F64].valid) return -1;
+  if (prec < P_F64 || prec > P_H16 || (parity_out != EVEN && parity_out != ODD)) return -2;
+  if (!slot_ok(slot_src, true) || !slot_ok(slot_dst, false) || slot_src == slot_dst) return -2;
+  int rc;
+  if ((rc = ensure_links(prec)) || (rc = ensure_slot(slot_dst))) return rc;
+  const int pb = pbit(parity_out);
+  if (prec == P_F64) return dslash(c.links[P_F64], g_slots[slot_src], g_slots[slot_dst], pb, P_F64);
+  VecField s{}, d{};
+  if ((rc = alloc_vec(s, prec, c.stride)) || (rc = alloc_vec(d, prec, c.stride))) {
+    free_vec(s);
+    return rc;
+  }
+  rc = convert_vec(g_slots[slot_src], P_F64, s, prec);
+  if (!rc) rc = dslash(c.links[prec], s, d, pb, prec);
+  if (!rc) rc = convert_vec(d, prec, g_slots[slot_dst], P_F64);
+  (void)hipStreamSynchronize(c.stream);
+  free_vec(s);
+  free_vec(d);
+  return rc;
+}
+
 extern "C" int milc_hip_dev_block_tune(int prec, int nrhs, int* variant, double* usec, char* desc, int ndesc) {
   Context& c = ctx();
   if (!c.initialised || !c.links[P_F64].valid) return -1;
@@ -1532,6 +1565,27 @@ extern "C" int milc_hip_eo_fermion_force_multi(int precision, double eps, const 
   release_fields(xo);
   if (rc) return rc;
   return add_momentum(precision, b.mom, momentum);
+}
+
+extern "C" int milc_hip_set_link_compression(int on) {
+  Context& c = ctx();
+  if (on != 0 && on != 1) return -2;
+  if (on != g_long_recon && c.initialised && c.links[P_H16].valid) {
+    // rebuilt in the other format on next use
+    free_links(c.links[P_H16].fat);
+    free_links(c.links[P_H16].lng);
+    c.links[P_H16].valid = false;
+  }
+  g_long_recon = on;
+  return 0;
+}
+
+extern "C" int milc_hip_link_compression(int prec) {
+  Context& c = ctx();
+  if (!c.initialised || prec < P_F64 || prec > P_H16) return -1;
+  int rc;
+  if ((rc = ensure_links(prec))) return rc;
+  return c.links[prec].lng.recon;
 }
 
 extern "C" int milc_hip_set_force_form(int form) {

@@ -206,10 +206,15 @@ struct VecField {
 };
 
 // A link field (fat or long) for both parities and all four directions.
+// H16 with recon = 1 (long links that are a real multiple of a U(3) matrix,
+// L = c U): rows 0 and 1 are stored, row 2 = z conj(row0 x row1) with the
+// per-link z = det(U) / c, i.e. 28 bytes per link instead of 36.
 struct LinkField {
   void* data;
-  float scale;  // H16: value = q * scale / 32767
+  float scale;         // H16: value = q * scale / 32767
   int stride;
+  float zscale = 1.0f; // H16 recon: z = q * zscale / 32767
+  int recon = 0;
 };
 
 typedef double nat_d2 __attribute__((ext_vector_type(2)));
@@ -222,6 +227,8 @@ __device__ __forceinline__ nat_f4 ldnt(const float4* p) { return __builtin_nonte
 __device__ __forceinline__ nat_f2 ldnt(const float2* p) { return __builtin_nontemporal_load((const nat_f2*)p); }
 __device__ __forceinline__ nat_i4 ldnt(const int4* p) { return __builtin_nontemporal_load((const nat_i4*)p); }
 __device__ __forceinline__ int ldnt(const int* p) { return __builtin_nontemporal_load(p); }
+typedef int nat_i2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ nat_i2 ldnt(const int2* p) { return __builtin_nontemporal_load((const nat_i2*)p); }
 
 __device__ __forceinline__ float q16lo(int w) { return (float)(short)(w & 0xffff); }
 __device__ __forceinline__ float q16hi(int w) { return (float)(w >> 16); }
@@ -313,8 +320,8 @@ __device__ __forceinline__ void load_link<P_F32>(const LinkField& f, int pb, int
 #pragma unroll
   for (int e = 0; e < 9; e++) { m.e[e / 3][e % 3].re = r[2 * e]; m.e[e / 3][e % 3].im = r[2 * e + 1]; }
 }
-template <>
-__device__ __forceinline__ void load_link<P_H16>(const LinkField& f, int pb, int mu, int cb, Mat3<float>& m) {
+// 16-bit link, all nine elements
+__device__ __forceinline__ void load_link_h16f(const LinkField& f, int pb, int mu, int cb, Mat3<float>& m) {
   const int4* p4 = (const int4*)f.data + (size_t)((pb * 4 + mu) * 2) * f.stride + cb;
   const int* p1 = (const int*)((const int4*)f.data + (size_t)16 * f.stride) + (size_t)(pb * 4 + mu) * f.stride + cb;
   float s = f.scale * (1.0f / 32767.0f);
@@ -327,6 +334,41 @@ __device__ __forceinline__ void load_link<P_H16>(const LinkField& f, int pb, int
   w[8] = ldnt(p1);
 #pragma unroll
   for (int e = 0; e < 9; e++) { m.e[e / 3][e % 3].re = q16lo(w[e]) * s; m.e[e / 3][e % 3].im = q16hi(w[e]) * s; }
+}
+
+// 16-bit long link in the two-row form: planes of int4 (row-0 elements and
+// element (1,0)), int2 (elements (1,1), (1,2)) and int (z), each plane
+// 8 * stride records; row 2 = z conj(row0 x row1).
+__device__ __forceinline__ void load_link_h16r(const LinkField& f, int pb, int mu, int cb, Mat3<float>& m) {
+  const size_t S = (size_t)f.stride, k = (size_t)(pb * 4 + mu) * S + cb;
+  const int4* p4 = (const int4*)f.data;
+  const int2* p2 = (const int2*)(p4 + 8 * S);
+  const int* p1 = (const int*)(p2 + 8 * S);
+  nat_i4 a = ldnt(p4 + k);
+  nat_i2 b = ldnt(p2 + k);
+  int zq = ldnt(p1 + k);
+  const float s = f.scale * (1.0f / 32767.0f), zs = f.zscale * (1.0f / 32767.0f);
+  const int w[6] = {a.x, a.y, a.z, a.w, b.x, b.y};
+#pragma unroll
+  for (int e = 0; e < 6; e++) { m.e[e / 3][e % 3].re = q16lo(w[e]) * s; m.e[e / 3][e % 3].im = q16hi(w[e]) * s; }
+  const float zr = q16lo(zq) * zs, zi = q16hi(zq) * zs;
+#pragma unroll
+  for (int k3 = 0; k3 < 3; k3++) {
+    const int i = (k3 + 1) % 3, j = (k3 + 2) % 3;
+    // cross_k = r0_i r1_j - r0_j r1_i
+    const Cplx<float>&a0 = m.e[0][i], &b1 = m.e[1][j], &a1 = m.e[0][j], &b0 = m.e[1][i];
+    float cr = a0.re * b1.re - a0.im * b1.im - (a1.re * b0.re - a1.im * b0.im);
+    float ci = a0.re * b1.im + a0.im * b1.re - (a1.re * b0.im + a1.im * b0.re);
+    // z conj(cross)
+    m.e[2][k3].re = zr * cr + zi * ci;
+    m.e[2][k3].im = zi * cr - zr * ci;
+  }
+}
+
+template <>
+__device__ __forceinline__ void load_link<P_H16>(const LinkField& f, int pb, int mu, int cb, Mat3<float>& m) {
+  if (f.recon) load_link_h16r(f, pb, mu, cb, m);
+  else load_link_h16f(f, pb, mu, cb, m);
 }
 
 // Store one link; for H16 `inv_scale` = 1/scale of the destination field.

@@ -63,6 +63,8 @@ void free_vec(VecField& v) {
 int alloc_links(LinkField& l, int prec, int stride) {
   l.stride = stride;
   l.scale = 1.0f;
+  l.zscale = 1.0f;
+  l.recon = 0;
   return dev_alloc(&l.data, link_bytes(prec, stride));
 }
 void free_links(LinkField& l) {
@@ -165,6 +167,103 @@ __global__ void k_convert_links(LinkField in, LinkField out, int Vh, float inv_s
     d.e[e / 3][e % 3].im = (double)m.e[e / 3][e % 3].im;
   }
   store_link<PO, double>(out, pb, mu, cb, d, inv_scale);
+}
+
+// Two-row form of scaled-unitary long links L = c U (c > 0 real, U in U(3)):
+// c^2 = |L|_F^2 / 3, z = det(U) / c = det(L) / c^4, row 2 = z conj(row0 x row1).
+__device__ __forceinline__ void long_link_z(const Mat3<double>& L, double& c2, double& zr, double& zi,
+                                            double& dev) {
+  double n2 = 0;
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) n2 += L.e[i][j].re * L.e[i][j].re + L.e[i][j].im * L.e[i][j].im;
+  c2 = n2 / 3.0;
+  // |L^dag L - c^2 1|_F / c^2
+  double d = 0;
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      double re = 0, im = 0;
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        re += L.e[k][i].re * L.e[k][j].re + L.e[k][i].im * L.e[k][j].im;
+        im += L.e[k][i].re * L.e[k][j].im - L.e[k][i].im * L.e[k][j].re;
+      }
+      if (i == j) re -= c2;
+      d += re * re + im * im;
+    }
+  dev = c2 > 0 ? sqrt(d) / c2 : 1e300;
+  // det(L) along row 0 with the cofactors of rows 1, 2
+  double dr = 0, di = 0;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const int i = (k + 1) % 3, j = (k + 2) % 3;
+    const Cplx<double>&a = L.e[1][i], &b = L.e[2][j], &e = L.e[1][j], &f = L.e[2][i];
+    double cr = a.re * b.re - a.im * b.im - (e.re * f.re - e.im * f.im);
+    double ci = a.re * b.im + a.im * b.re - (e.re * f.im + e.im * f.re);
+    dr += L.e[0][k].re * cr - L.e[0][k].im * ci;
+    di += L.e[0][k].re * ci + L.e[0][k].im * cr;
+  }
+  const double inv = c2 > 0 ? 1.0 / (c2 * c2) : 0.0;
+  zr = dr * inv;
+  zi = di * inv;
+}
+
+// per-block max of the deviation from a scaled unitary matrix and of |z| components
+__global__ void __launch_bounds__(256) k_long_check(LinkField in, int Vh, double* __restrict__ partials) {
+  double dmax = 0, zmax = 0;
+  const size_t n = (size_t)8 * Vh;
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+    const int pm = (int)(t / Vh), cb = (int)(t % Vh);
+    Mat3<double> L;
+    load_link<P_F64>(in, pm >> 2, pm & 3, cb, L);
+    double c2, zr, zi, dev;
+    long_link_z(L, c2, zr, zi, dev);
+    dmax = fmax(dmax, dev);
+    zmax = fmax(zmax, fmax(fabs(zr), fabs(zi)));
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    dmax = fmax(dmax, __shfl_xor(dmax, off, 64));
+    zmax = fmax(zmax, __shfl_xor(zmax, off, 64));
+  }
+  __shared__ double sm[2][4];
+  if ((threadIdx.x & 63) == 0) {
+    sm[0][threadIdx.x >> 6] = dmax;
+    sm[1][threadIdx.x >> 6] = zmax;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < 4; w++) {
+      dmax = fmax(dmax, sm[0][w]);
+      zmax = fmax(zmax, sm[1][w]);
+    }
+    partials[2 * blockIdx.x] = dmax;
+    partials[2 * blockIdx.x + 1] = zmax;
+  }
+}
+
+// 16-bit two-row store (the layout read by load_link_h16r)
+__global__ void k_store_h16r(LinkField in, LinkField out, int Vh, float inv_scale, float inv_zscale) {
+  int cb = blockIdx.x * blockDim.x + threadIdx.x;
+  int pm = blockIdx.y;
+  if (cb >= Vh) return;
+  Mat3<double> L;
+  load_link<P_F64>(in, pm >> 2, pm & 3, cb, L);
+  double c2, zr, zi, dev;
+  long_link_z(L, c2, zr, zi, dev);
+  int w[6];
+#pragma unroll
+  for (int e = 0; e < 6; e++)
+    w[e] = q16pack((float)(L.e[e / 3][e % 3].re * inv_scale), (float)(L.e[e / 3][e % 3].im * inv_scale));
+  const size_t S = (size_t)out.stride, k = (size_t)pm * S + cb;
+  int4* p4 = (int4*)out.data;
+  int2* p2 = (int2*)(p4 + 8 * S);
+  int* p1 = (int*)(p2 + 8 * S);
+  p4[k] = make_int4(w[0], w[1], w[2], w[3]);
+  p2[k] = make_int2(w[4], w[5]);
+  p1[k] = q16pack((float)(zr * inv_zscale), (float)(zi * inv_zscale));
 }
 
 template <int PI, int PO>
@@ -310,6 +409,35 @@ int convert_links(const LinkField& in, int pin, LinkField& out, int pout) {
     hipLaunchKernelGGL((k_convert_links<P_F64, P_H16>), grid, dim3(256), 0, c.stream, in, out, c.Vh,
                        (float)(1.0 / out.scale));
   }
+  MH_CHECK(hipGetLastError());
+  return 0;
+}
+
+// FP64 long links -> 16-bit two-row form when every link is a real multiple of
+// a U(3) matrix to tol (relative); returns 1 (out untouched) when they are not.
+int convert_long_h16r(const LinkField& in, LinkField& out, double tol) {
+  Context& c = ctx();
+  const int nb = 512;
+  hipLaunchKernelGGL(k_long_check, dim3(nb), dim3(256), 0, c.stream, in, c.Vh, c.ws.partials);
+  MH_CHECK(hipGetLastError());
+  std::vector<double> h(2 * nb);
+  MH_CHECK(hipMemcpyAsync(h.data(), c.ws.partials, h.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  MH_CHECK(hipStreamSynchronize(c.stream));
+  double dmax = 0, zmax = 0;
+  for (int b = 0; b < nb; b++) {
+    dmax = std::max(dmax, h[2 * b]);
+    zmax = std::max(zmax, h[2 * b + 1]);
+  }
+  if (!(dmax <= tol) || !(zmax > 0)) return 1;
+  double m;
+  int rc = link_absmax_f64(in, &m);
+  if (rc) return rc;
+  out.scale = (float)(m > 0 ? m : 1.0);
+  out.zscale = (float)zmax;
+  out.recon = 1;
+  dim3 grid(nblocks_for(c.Vh, 256), 8);
+  hipLaunchKernelGGL(k_store_h16r, grid, dim3(256), 0, c.stream, in, out, c.Vh, (float)(1.0 / out.scale),
+                     (float)(1.0 / out.zscale));
   MH_CHECK(hipGetLastError());
   return 0;
 }

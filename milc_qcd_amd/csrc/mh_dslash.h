@@ -16,10 +16,23 @@ namespace mh {
 template <int P> struct DslashUnroll { static constexpr int value = 4; };
 template <> struct DslashUnroll<P_H16> { static constexpr int value = 2; };
 
-template <int P>
-__device__ __forceinline__ void dslash_site(const Geom& g, const LinkField& fat, const LinkField& lng,
-                                            const VecField& src, int cb, int pb,
-                                            Vec3<typename PrecTraits<P>::Real>& acc) {
+// Link loads of the stencil: the 16-bit format is fixed at compile time (full
+// nine elements for fat links; LR = 1: long links in the two-row form).
+template <int P, int LR>
+__device__ __forceinline__ void load_stencil_link(const LinkField& f, int pb, int mu, int cb,
+                                                  Mat3<typename PrecTraits<P>::Real>& m) {
+  if constexpr (P == P_H16) {
+    if constexpr (LR) load_link_h16r(f, pb, mu, cb, m);
+    else load_link_h16f(f, pb, mu, cb, m);
+  } else {
+    load_link<P>(f, pb, mu, cb, m);
+  }
+}
+
+template <int P, int LR>
+__device__ __forceinline__ void dslash_site_impl(const Geom& g, const LinkField& fat, const LinkField& lng,
+                                                 const VecField& src, int cb, int pb,
+                                                 Vec3<typename PrecTraits<P>::Real>& acc) {
   typedef typename PrecTraits<P>::Real R;
   Coord c = site_coords(g, cb, pb);
   int po = pb ^ 1;
@@ -31,25 +44,39 @@ __device__ __forceinline__ void dslash_site(const Geom& g, const LinkField& fat,
     int n;
     // forward one hop
     n = neighbour(g, c, cb, mu, 1);
-    load_link<P>(fat, pb, mu, cb, m);
+    load_stencil_link<P, 0>(fat, pb, mu, cb, m);
     load_vec<P>(src, n, v);
     mv_acc(m, v, acc);
     // forward three hops
     n = neighbour(g, c, cb, mu, 3);
-    load_link<P>(lng, pb, mu, cb, m);
+    load_stencil_link<P, LR>(lng, pb, mu, cb, m);
     load_vec<P>(src, n, v);
     mv_acc(m, v, acc);
     // backward one hop: F_mu(x-mu) lives on the other parity
     n = neighbour(g, c, cb, mu, -1);
-    load_link<P>(fat, po, mu, n, m);
+    load_stencil_link<P, 0>(fat, po, mu, n, m);
     load_vec<P>(src, n, v);
     amv_sub(m, v, acc);
     // backward three hops
     n = neighbour(g, c, cb, mu, -3);
-    load_link<P>(lng, po, mu, n, m);
+    load_stencil_link<P, LR>(lng, po, mu, n, m);
     load_vec<P>(src, n, v);
     amv_sub(m, v, acc);
   }
+}
+
+// The 16-bit long-link format is uniform over the launch: one branch per thread.
+template <int P>
+__device__ __forceinline__ void dslash_site(const Geom& g, const LinkField& fat, const LinkField& lng,
+                                            const VecField& src, int cb, int pb,
+                                            Vec3<typename PrecTraits<P>::Real>& acc) {
+  if constexpr (P == P_H16) {
+    if (lng.recon) {
+      dslash_site_impl<P, 1>(g, fat, lng, src, cb, pb, acc);
+      return;
+    }
+  }
+  dslash_site_impl<P, 0>(g, fat, lng, src, cb, pb, acc);
 }
 
 }  // namespace mh

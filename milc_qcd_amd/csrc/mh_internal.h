@@ -96,6 +96,7 @@ int vec_to_milc_dev(const VecField& in, int prec, int pb, double* dev_aos);
 int links_from_milc_dev(const double* dev_aos, LinkField& out, int prec);
 int links_to_milc_dev(const LinkField& in, int prec, double* dev_aos);
 int convert_links(const LinkField& in, int pin, LinkField& out, int pout);
+int convert_long_h16r(const LinkField& in, LinkField& out, double tol);
 int convert_vec(const VecField& in, int pin, VecField& out, int pout);
 
 // ---- stencil ---------------------------------------------------------------

@@ -50,7 +50,7 @@ EXPORTS = [
     "milc_hip_dev_block_dslash_bench", "milc_hip_dev_block_dslash", "milc_hip_dev_multishift_block",
     "milc_hip_dev_block_describe",
     "milc_hip_dev_multishift", "milc_hip_dev_hmc_step", "milc_hip_hisq_force", "milc_hip_eo_fermion_force_multi",
-    "milc_hip_set_force_form", "milc_hip_dev_hisq_force",
+    "milc_hip_set_force_form", "milc_hip_dev_dslash", "milc_hip_set_link_compression", "milc_hip_link_compression", "milc_hip_dev_hisq_force",
 ]
 
 
@@ -105,6 +105,9 @@ def load_library(path=LIB_PATH):
     L.milc_hip_hisq_force.argtypes = [I, I, I, D, ctypes.POINTER(PD), ctypes.POINTER(P), PD, PD, P, P, P, P]
     L.milc_hip_eo_fermion_force_multi.argtypes = [I, D, PD, ctypes.POINTER(P), I, P, P]
     L.milc_hip_set_force_form.argtypes = [I]
+    L.milc_hip_dev_dslash.argtypes = [I, I, I, I]
+    L.milc_hip_set_link_compression.argtypes = [I]
+    L.milc_hip_link_compression.argtypes = [I]
     L.milc_hip_dev_hisq_force.argtypes = [I, I, P, ctypes.c_double, P, P]
     _lib = L
     return L
@@ -364,6 +367,16 @@ class Lattice:
                "milc_hip_fermion_force_fn_multi")
         return mom
 
+    def set_link_compression(self, on):
+        """1 (default): 16-bit long links in the two-row form when scaled-unitary"""
+        self._activate()
+        _check(self.lib.milc_hip_set_link_compression(int(on)), "milc_hip_set_link_compression")
+
+    def link_compression(self, prec):
+        """long-link format of the device links of precision prec (1: two-row form)"""
+        self._activate()
+        return _check(self.lib.milc_hip_link_compression(prec), "milc_hip_link_compression")
+
     def set_force_form(self, form):
         """0: path-table gather; 1 (default): staple form where exact for any
         links, path table otherwise; 2: staple form everywhere (unitary links)"""
@@ -528,6 +541,11 @@ class Lattice:
         lng = np.zeros((self.V, 4, 3, 3, 2))
         _check(self.lib.milc_hip_dev_read_links(_p(fat), _p(lng)), "milc_hip_dev_read_links")
         return fat, lng
+
+    def dev_dslash(self, prec, slot_src, parity_out, slot_dst):
+        """slot_dst = D slot_src on parity_out, stencil in precision prec (0/1/2)"""
+        self._activate()
+        _check(self.lib.milc_hip_dev_dslash(prec, slot_src, parity_out, slot_dst), "milc_hip_dev_dslash")
 
     def dev_dslash_bench(self, prec, parity, reps):
         self._activate()
