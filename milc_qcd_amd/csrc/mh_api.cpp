@@ -225,7 +225,7 @@ int milc_hip_invert(int external_precision, int internal_precision, double mass,
   sp.resid = target_resid;
   sp.relresid = 0;
   sp.sloppy_prec = ps;
-  sp.delta = 0.1;
+  sp.delta = kReliableDelta;
   sp.batch = 16;
   SolveStats st;
   rc = cg_solve(b, x, mass, sp, st);
@@ -308,7 +308,7 @@ int milc_hip_dev_invert(int slot_b, int slot_x, double mass, int parity, double 
   sp.resid = resid;
   sp.relresid = 0;
   sp.sloppy_prec = ps;
-  sp.delta = 0.1;
+  sp.delta = kReliableDelta;
   sp.batch = 16;
   SolveStats st;
   if ((rc = vec_zero(g_slots[slot_x], P_F64))) return rc;
@@ -506,7 +506,7 @@ extern "C" int milc_hip_multishift_invert(int external_precision, int internal_p
       sp.resid = target_resid[j] > 0 ? target_resid[j] : target_resid[0];
       sp.relresid = 0;
       sp.sloppy_prec = P_F32;
-      sp.delta = 0.1;
+      sp.delta = kReliableDelta;
       sp.batch = 16;
       SolveStats s2;
       if ((rc = cg_solve(b, xs[j], 0.5 * std::sqrt(offs[j]), sp, s2))) return rc;

@@ -871,7 +871,7 @@ static int cg_solve_mixed(const VecField& src, VecField& x, double mass, const S
   double msq_x4 = 4.0 * mass * mass;
   double rsqmin = sp.resid * sp.resid;
   long max_cg = (long)sp.max_restarts * sp.max_iters;
-  double delta = sp.delta > 0 ? sp.delta : 0.1;
+  double delta = sp.delta > 0 ? sp.delta : kReliableDelta;
   st = SolveStats();
   double source_norm;
   int rc = vec_norm2(src, P_F64, &source_norm);

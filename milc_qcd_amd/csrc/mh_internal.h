@@ -114,6 +114,14 @@ struct SolveParams {
   double delta;        // reliable-update threshold (mixed)
   int batch;           // iterations enqueued per host check
 };
+// Reliable-update threshold of the mixed CG: each sloppy phase runs until its
+// iterated residual has fallen by this factor, then x and r are rebuilt in
+// FP64.  A 16-bit sloppy residual keeps about 3e-5 relative resolution, so a
+// 1e3 drop leaves it accurate to a few percent; the stopping test is the FP64
+// true residual either way.  (32^3x64, m = 0.01, 1e-10: 1e-1 gives 10 updates
+// and 100 ms per solve, 1e-3 gives 4-5 updates and 91 ms.)
+constexpr double kReliableDelta = 1e-3;
+
 struct SolveStats {
   double final_rsq = 0, final_relrsq = 0, size_r = 0;
   int iters = 0, restarts = 0, converged = 0, reliable_updates = 0;

@@ -116,7 +116,7 @@ static int rational_solve(const VecField& phi, int n, const double* offsets, dou
     sp.resid = resid;
     sp.relresid = 0;
     sp.sloppy_prec = P_H16;
-    sp.delta = 0.1;
+    sp.delta = kReliableDelta;
     sp.batch = 16;
     SolveStats s2;
     if ((rc = cg_solve(phi, x[t], 0.5 * std::sqrt(offs[t]), sp, s2))) return rc;
