@@ -23,6 +23,8 @@ Usage: python bench.py [--gpus 1] [--steps 10] [--warmup 1] [--no-cpu]
            m = 0.01, 12 shifts as above, residues 0.1 (i+1); prints its own JSON line)
        python bench.py --workload force   (the HISQ fermion force alone on 48^3 x 96,
            12 terms from resident vectors; prints its own JSON line)
+       python bench.py --workload io   (lattice files at 32^3 x 64: v5 and SciDAC
+           write / read throughput, file -> device HISQ links)
        --no-sequential (multishift: leave out the one-source comparison solves)
 """
 import argparse
@@ -30,6 +32,8 @@ import json
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -436,6 +440,95 @@ def force_workload(args):
     print(json.dumps(out), flush=True)
 
 
+def cpu_io_baseline(path, nbytes):
+    """the FP32 SciDAC file read by the numpy oracle reader (LIME parse, CRC-32
+    checksums, byte swap), one pass, on this host"""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import ks_io as oio
+    t1 = time.perf_counter()
+    f = oio.read_lime_field(path)
+    dt = time.perf_counter() - t1
+    ok = f["suma"] == f["calca"] and f["sumb"] == f["calcb"]
+    return {"value": round(nbytes / dt / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
+            "sample": (f"numpy restatement of the SciDAC reader (oracle/ks_io.py) on the same "
+                       f"{nbytes / 1e6:.0f} MB file: {dt:.2f} s, checksum {'ok' if ok else 'MISMATCH'}")}
+
+
+def io_workload(args):
+    """Lattice files on the 32^3 x 64 configuration: the device-generated thin
+    links written as MILC v5 and as FP32 SciDAC, read back (parse, checksum,
+    byte swap, reorder to MILC's node_index order) and loaded onto the device,
+    where the HISQ links are rebuilt.  value = SciDAC read throughput (file
+    bytes / read time, file in the page cache)."""
+    import shutil
+    import tempfile
+    import milc_qcd_amd as mq
+    from milc_qcd_amd import lattice_io as lio
+    G = mq.Lattice(DIMS)
+    G.dev_gauge_hisq(GAUGE_SEED, 0.4)
+    fat0, lng0 = G.dev_read_links()
+    U = G.dev_read_thin_links()
+    d = tempfile.mkdtemp(prefix="milc_io_")
+    res = {}
+    try:
+        dims = list(DIMS)
+        for name, flag in (("scidac_f32", lio.SAVE_SERIAL_SCIDAC), ("v5", lio.SAVE_SERIAL)):
+            f = os.path.join(d, name)
+            tw, tr = [], []
+            for i in range(args.warmup + args.steps):
+                t1 = time.perf_counter()
+                lio.save_lattice(flag, f, U, dims, precision=1)
+                t2 = time.perf_counter()
+                U2, info = lio.reload_lattice(f, dims)
+                t3 = time.perf_counter()
+                if not info["checksum_ok"]:
+                    raise RuntimeError(f"{name}: checksum mismatch after round trip")
+                if i >= args.warmup:
+                    tw.append(t2 - t1)
+                    tr.append(t3 - t2)
+            nbytes = os.path.getsize(f)
+            res[name] = {"file_bytes": nbytes, "write_s": round(min(tw), 4), "read_s": round(min(tr), 4),
+                         "write_GBps": round(nbytes / min(tw) / 1e9, 3), "read_GBps": round(nbytes / min(tr) / 1e9, 3)}
+            log(f"{name}: {res[name]}")
+        # file -> device thin links -> HISQ links on the device
+        t1 = time.perf_counter()
+        G.dev_load_thin_links(U2)
+        t_load = time.perf_counter() - t1
+        fat, lng = G.dev_read_links()
+        err = float(np.linalg.norm(fat - fat0) / np.linalg.norm(fat0))
+        cpu = None
+        if not args.no_cpu:
+            try:
+                cpu = cpu_io_baseline(os.path.join(d, "scidac_f32"), res["scidac_f32"]["file_bytes"])
+            except Exception as e:
+                log(f"cpu baseline failed: {e!r}")
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    G.close()
+    sc = res["scidac_f32"]
+    out = {
+        "metric": "SciDAC gauge-file read (LIME parse, CRC-32 checksums, byte swap, reorder) throughput, 32^3x64",
+        "value": sc["read_GBps"],
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(sc["read_s"] * 1e3, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 file, f64 in memory",
+        "data": "synthetic",
+        "config": {"workload": "lattice_files_32x32x32x64", "lattice": list(DIMS), "gauge_seed": GAUGE_SEED},
+        "formats": res,
+        "file_to_device_hisq_links_s": round(t_load, 4),
+        "fat_links_rel_diff_after_fp32_file": err,
+        "roofline": None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -444,7 +537,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--kernel-only", action="store_true",
                     help="only the Dslash and copy bursts (for rocprofv3 counter runs)")
-    ap.add_argument("--workload", choices=["cg", "multishift", "hmc", "force"], default="cg")
+    ap.add_argument("--workload", choices=["cg", "multishift", "hmc", "force", "io"], default="cg")
     ap.add_argument("--no-sequential", action="store_true",
                     help="multishift: leave out the 16 one-source comparison solves")
     args = ap.parse_args()
@@ -460,6 +553,9 @@ def main():
         return
     if args.workload == "force":
         force_workload(args)
+        return
+    if args.workload == "io":
+        io_workload(args)
         return
 
     import milc_qcd_amd as mq

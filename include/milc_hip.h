@@ -214,6 +214,16 @@ int milc_hip_load_hisq_links(int precision, const void* thin, void* fatlink, voi
  * entries; counter-based generator shared with the oracle), KS phases in,
  * then the HISQ fat/Naik links, kept on the device as the current links. */
 int milc_hip_dev_gauge_hisq(uint64_t seed, double scale, int* n_svd);
+/* Thin links from the host (MILC layout, precision 1 float / 2 double), e.g.
+ * as read by milc_io_read_gauge: the KS phases are applied on the device
+ * unless phases_in (rephase.c:64-143), the HISQ links are built on the device
+ * (load_hisq_aux_links_cpu + load_hisq_fn_links,
+ * fermion_links_hisq_load_milc.c:546-702), and both become the current device
+ * links -- the device side of reload_lattice followed by load_ferm_links. */
+int milc_hip_dev_load_thin_links(int precision, const void* thin, int phases_in, int* n_svd);
+/* The current device thin links, MILC layout, FP64, with the KS phases
+ * (with_phases = 1) or without (0, as save_lattice writes them). */
+int milc_hip_dev_read_thin_links(double* thin, int with_phases);
 /* Gaussian even/odd source (<re^2> = 1/2, grsource_imp.c:169-195) into device
  * slot `slot` (0..63). */
 int milc_hip_dev_source(uint64_t seed, int parity, int slot);

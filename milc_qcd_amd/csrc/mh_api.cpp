@@ -406,6 +406,87 @@ extern "C" int milc_hip_dev_gauge_hisq(uint64_t seed, double scale, int* n_svd) 
   return 0;
 }
 
+// Thin links from the host (MILC layout, FP32 or FP64): KS phases applied on
+// the device unless phases_in, HISQ fat + Naik links built on the device; both
+// become the current device links (the dev_* solvers and the thin-link force
+// use them).
+extern "C" int milc_hip_dev_load_thin_links(int precision, const void* thin, int phases_in, int* n_svd) {
+  Context& c = ctx();
+  if (!c.initialised) return -1;
+  if ((precision != 1 && precision != 2) || !thin) return -2;
+  size_t n = (size_t)2 * c.Vh * 4 * 18;
+  double* stage = nullptr;
+  int rc;
+  if ((rc = dev_alloc((void**)&stage, n * sizeof(double)))) return rc;
+  if (precision == 2) {
+    rc = hipMemcpyAsync(stage, thin, n * sizeof(double), hipMemcpyHostToDevice, c.stream) == hipSuccess ? 0 : -1;
+  } else {
+    std::vector<double> conv(n);
+    for (size_t i = 0; i < n; i++) conv[i] = ((const float*)thin)[i];
+    rc = hipMemcpyAsync(stage, conv.data(), n * sizeof(double), hipMemcpyHostToDevice, c.stream) == hipSuccess ? 0 : -1;
+    if (!rc) rc = hipStreamSynchronize(c.stream) == hipSuccess ? 0 : -1;
+  }
+  for (int p = 0; p < 3; p++) {
+    if (c.links[p].valid) {
+      free_links(c.links[p].fat);
+      free_links(c.links[p].lng);
+      c.links[p].valid = false;
+    }
+  }
+  c.link_key = c.link_key2 = nullptr;
+  LinkField U{};
+  DevLinks& L = c.links[P_F64];
+  if (!rc) rc = alloc_links(U, P_F64, c.stride);
+  if (!rc) rc = links_from_milc_dev(stage, U, P_F64);
+  if (!rc && !phases_in) rc = rephase_dev(U);
+  if (!rc && !(rc = alloc_links(L.fat, P_F64, c.stride))) rc = alloc_links(L.lng, P_F64, c.stride);
+  if (!rc) rc = hisq_links_dev(U, L.fat, L.lng, nullptr, n_svd);
+  (void)hipStreamSynchronize(c.stream);
+  dev_free(stage);
+  if (rc) {
+    free_links(U);
+    free_links(L.fat);
+    free_links(L.lng);
+    return rc;
+  }
+  if (c.thin_valid) free_links(c.thin);
+  c.thin = U;
+  c.thin_valid = true;
+  L.prec = P_F64;
+  L.valid = true;
+  c.links_version++;
+  return 0;
+}
+
+// The current device thin links in MILC layout (FP64), with the KS phases
+// (with_phases) or without them (as lattice files hold them).
+extern "C" int milc_hip_dev_read_thin_links(double* thin, int with_phases) {
+  Context& c = ctx();
+  if (!c.initialised || !c.thin_valid) return -1;
+  if (!thin) return -2;
+  size_t n = (size_t)2 * c.Vh * 4 * 18;
+  double* stage = nullptr;
+  LinkField T{};
+  int rc;
+  if ((rc = dev_alloc((void**)&stage, n * sizeof(double)))) return rc;
+  const LinkField* src = &c.thin;
+  if (!with_phases) {
+    // the phases are signs: applying them again removes them
+    if (!(rc = alloc_links(T, P_F64, c.stride))) {
+      rc = hipMemcpyAsync(T.data, c.thin.data, link_bytes(P_F64, c.stride), hipMemcpyDeviceToDevice, c.stream) ==
+                   hipSuccess ? 0 : -1;
+      if (!rc) rc = rephase_dev(T);
+    }
+    src = &T;
+  }
+  if (!rc) rc = links_to_milc_dev(*src, P_F64, stage);
+  if (!rc) rc = hipMemcpyAsync(thin, stage, n * sizeof(double), hipMemcpyDeviceToHost, c.stream) == hipSuccess ? 0 : -1;
+  (void)hipStreamSynchronize(c.stream);
+  dev_free(stage);
+  free_links(T);
+  return rc;
+}
+
 extern "C" int milc_hip_load_hisq_links(int precision, const void* thin, void* fatlink, void* longlink, void* W_link,
                                         int* n_svd) {
   Context& c = ctx();

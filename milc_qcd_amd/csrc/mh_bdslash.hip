@@ -65,68 +65,65 @@ __global__ void __launch_bounds__(256) k_bds_valu(Geom g, LinkField fat, LinkFie
     Vec3<R> acc[RPT];
 #pragma unroll
     for (int i = 0; i < RPT; i++) zero(acc[i]);
+    // this thread's right-hand sides: RPT consecutive records r0 .. r0 + RPT - 1
+    const int r0 = RPT * q;
 #pragma unroll 1
     for (int mu = 0; mu < 4; mu++) {
       Mat3<R> m;
-      Vec3<R> v;
+      Vec3<R> v[RPT];
       int n = neighbour(g, c, cb, mu, 1);
       load_link<P>(fat, pb, mu, cb, m);
+      bld_blk<R, NR, RPT>(src, n, r0, v);
 #pragma unroll
-      for (int i = 0; i < RPT; i++) {
-        bld<R, NR>(src, n, q + TPS * i, v);
-        mv_acc(m, v, acc[i]);
-      }
+      for (int i = 0; i < RPT; i++) mv_acc(m, v[i], acc[i]);
       n = neighbour(g, c, cb, mu, 3);
       load_link<P>(lng, pb, mu, cb, m);
+      bld_blk<R, NR, RPT>(src, n, r0, v);
 #pragma unroll
-      for (int i = 0; i < RPT; i++) {
-        bld<R, NR>(src, n, q + TPS * i, v);
-        mv_acc(m, v, acc[i]);
-      }
+      for (int i = 0; i < RPT; i++) mv_acc(m, v[i], acc[i]);
       n = neighbour(g, c, cb, mu, -1);
       load_link<P>(fat, po, mu, n, m);
+      bld_blk<R, NR, RPT>(src, n, r0, v);
 #pragma unroll
-      for (int i = 0; i < RPT; i++) {
-        bld<R, NR>(src, n, q + TPS * i, v);
-        amv_sub(m, v, acc[i]);
-      }
+      for (int i = 0; i < RPT; i++) amv_sub(m, v[i], acc[i]);
       n = neighbour(g, c, cb, mu, -3);
       load_link<P>(lng, po, mu, n, m);
+      bld_blk<R, NR, RPT>(src, n, r0, v);
 #pragma unroll
-      for (int i = 0; i < RPT; i++) {
-        bld<R, NR>(src, n, q + TPS * i, v);
-        amv_sub(m, v, acc[i]);
-      }
+      for (int i = 0; i < RPT; i++) amv_sub(m, v[i], acc[i]);
     }
+    if (KIND == BK_STORE) {
+      if (valid) bst_blk<R, NR, RPT>(dst, cb, r0, acc);
+      continue;
+    }
+    Vec3<R> pp[RPT], t[RPT];
+    bld_blk<R, NR, RPT>(pv, cb, r0, pp);
 #pragma unroll
     for (int i = 0; i < RPT; i++) {
-      int r = q + TPS * i;
-      if (KIND == BK_STORE) {
-        if (valid) bst<R, NR>(dst, cb, r, acc[i]);
-        continue;
-      }
-      Vec3<R> pp, t;
-      bld<R, NR>(pv, cb, r, pp);
-      R a = coef.c[r];
+      R a = coef.c[r0 + i];
 #pragma unroll
       for (int k = 0; k < 3; k++) {
-        t.c[k].re = acc[i].c[k].re + a * pp.c[k].re;
-        t.c[k].im = acc[i].c[k].im + a * pp.c[k].im;
+        t[i].c[k].re = acc[i].c[k].re + a * pp[i].c[k].re;
+        t[i].c[k].im = acc[i].c[k].im + a * pp[i].c[k].im;
       }
-      if (!valid) continue;
-      bst<R, NR>(dst, cb, r, t);
+    }
+    if (!valid) continue;
+    bst_blk<R, NR, RPT>(dst, cb, r0, t);
+#pragma unroll
+    for (int i = 0; i < RPT; i++)
 #pragma unroll
       for (int k = 0; k < 3; k++)
-        d[0][i] += (double)pp.c[k].re * (double)t.c[k].re + (double)pp.c[k].im * (double)t.c[k].im;
-      if (KIND == BK_CG) {
-        Vec3<R> rr;
-        bld<R, NR>(rv, cb, r, rr);
+        d[0][i] += (double)pp[i].c[k].re * (double)t[i].c[k].re + (double)pp[i].c[k].im * (double)t[i].c[k].im;
+    if (KIND == BK_CG) {
+      Vec3<R> rr[RPT];
+      bld_blk<R, NR, RPT>(rv, cb, r0, rr);
+#pragma unroll
+      for (int i = 0; i < RPT; i++)
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-          d[1 % NV][i] += (double)rr.c[k].re * (double)t.c[k].re + (double)rr.c[k].im * (double)t.c[k].im;
-          d[2 % NV][i] += (double)t.c[k].re * (double)t.c[k].re + (double)t.c[k].im * (double)t.c[k].im;
+          d[1 % NV][i] += (double)rr[i].c[k].re * (double)t[i].c[k].re + (double)rr[i].c[k].im * (double)t[i].c[k].im;
+          d[2 % NV][i] += (double)t[i].c[k].re * (double)t[i].c[k].re + (double)t[i].c[k].im * (double)t[i].c[k].im;
         }
-      }
     }
   }
   if (KIND != BK_STORE) bsum_valu<NV, NR, TPS>(d, partials, pstride);

@@ -91,6 +91,76 @@ __device__ __forceinline__ void bst(R* __restrict__ f, int cb, int r, const Vec3
   a[3] = v.c[0].im; a[4] = v.c[1].im; a[5] = v.c[2].im;
 }
 
+// RPT consecutive records (rhs r0 .. r0+RPT-1) of site cb with 16-byte loads
+// (8-byte for a lone FP32 record): the records of a site are contiguous, so the
+// threads of a site read whole cache lines with few instructions.
+template <typename R> struct Chunk16 { typedef float4 T; static constexpr int n = 4; };
+template <> struct Chunk16<double> { typedef double2 T; static constexpr int n = 2; };
+
+template <typename R, int NR, int RPT>
+__device__ __forceinline__ void bld_blk(const R* __restrict__ f, int cb, int r0, Vec3<R> (&v)[RPT]) {
+  constexpr int NE = 6 * RPT;
+  R e[NE];
+  const R* a = f + ((size_t)cb * NR + r0) * 6;
+  if constexpr ((NE * sizeof(R)) % 16 == 0) {
+    typedef typename Chunk16<R>::T C;
+    constexpr int EPC = Chunk16<R>::n;
+    const C* p = (const C*)a;
+#pragma unroll
+    for (int k = 0; k < NE / EPC; k++) {
+      C w = p[k];
+      const R* wr = (const R*)&w;
+#pragma unroll
+      for (int j = 0; j < EPC; j++) e[k * EPC + j] = wr[j];
+    }
+  } else {
+    const float2* p = (const float2*)a;
+#pragma unroll
+    for (int k = 0; k < NE / 2; k++) {
+      float2 w = p[k];
+      e[2 * k] = w.x;
+      e[2 * k + 1] = w.y;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < RPT; i++)
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      v[i].c[k].re = e[6 * i + k];
+      v[i].c[k].im = e[6 * i + 3 + k];
+    }
+}
+template <typename R, int NR, int RPT>
+__device__ __forceinline__ void bst_blk(R* __restrict__ f, int cb, int r0, const Vec3<R> (&v)[RPT]) {
+  constexpr int NE = 6 * RPT;
+  R e[NE];
+#pragma unroll
+  for (int i = 0; i < RPT; i++)
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      e[6 * i + k] = v[i].c[k].re;
+      e[6 * i + 3 + k] = v[i].c[k].im;
+    }
+  R* a = f + ((size_t)cb * NR + r0) * 6;
+  if constexpr ((NE * sizeof(R)) % 16 == 0) {
+    typedef typename Chunk16<R>::T C;
+    constexpr int EPC = Chunk16<R>::n;
+    C* p = (C*)a;
+#pragma unroll
+    for (int k = 0; k < NE / EPC; k++) {
+      C w;
+      R* wr = (R*)&w;
+#pragma unroll
+      for (int j = 0; j < EPC; j++) wr[j] = e[k * EPC + j];
+      p[k] = w;
+    }
+  } else {
+    float2* p = (float2*)a;
+#pragma unroll
+    for (int k = 0; k < NE / 2; k++) p[k] = make_float2(e[2 * k], e[2 * k + 1]);
+  }
+}
+
 // Per-rhs block sums when rhs = lane % NR: written to LDS, then one value per
 // (dot, rhs) channel to partials[ch * pstride + blockIdx.x].
 template <int NV, int NR>
@@ -113,7 +183,7 @@ __device__ __forceinline__ void bsum_lanes(double (&d)[NV], double* __restrict__
   }
 }
 
-// Same for a thread owning rhs q + TPS i (q = lane % TPS).
+// Same for a thread owning rhs RPT q + i (q = lane % TPS, RPT = NR / TPS).
 template <int NV, int NR, int TPS>
 __device__ __forceinline__ void bsum_valu(double (&d)[NV][NR / TPS], double* __restrict__ partials, int pstride) {
   constexpr int RPT = NR / TPS;
@@ -128,7 +198,7 @@ __device__ __forceinline__ void bsum_valu(double (&d)[NV][NR / TPS], double* __r
 #pragma unroll
     for (int v = 0; v < NV; v++)
 #pragma unroll
-      for (int i = 0; i < RPT; i++) sm[v][lane + TPS * i][w] = d[v][i];
+      for (int i = 0; i < RPT; i++) sm[v][RPT * lane + i][w] = d[v][i];
   __syncthreads();
   int t = threadIdx.x;
   if (t < NV * NR) {

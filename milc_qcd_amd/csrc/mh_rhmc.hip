@@ -167,7 +167,6 @@ int force_level0_from_solutions(const std::vector<VecField>& x, const double* re
 // and the level-0 force O1/O3 with coefficients r_t from Z_t = (X_t, D_oe X_t).
 int rhmc_force_level0(const VecField& phi, int n, const double* offsets, const double* residues, double resid,
                       int mixed, double* S, LinkField& o1, LinkField& o3, int* iters) {
-  Context& c = ctx();
   std::vector<VecField> x;
   SolveStats st;
   int rc = rational_solve(phi, n, offsets, resid, mixed, EVEN, x, st);

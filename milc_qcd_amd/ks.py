@@ -50,7 +50,7 @@ EXPORTS = [
     "milc_hip_dev_block_dslash_bench", "milc_hip_dev_block_dslash", "milc_hip_dev_multishift_block",
     "milc_hip_dev_block_describe",
     "milc_hip_dev_multishift", "milc_hip_dev_hmc_step", "milc_hip_hisq_force", "milc_hip_eo_fermion_force_multi",
-    "milc_hip_set_force_form", "milc_hip_dev_dslash", "milc_hip_set_link_compression", "milc_hip_link_compression", "milc_hip_dev_hisq_force",
+    "milc_hip_set_force_form", "milc_hip_dev_load_thin_links", "milc_hip_dev_read_thin_links", "milc_hip_dev_dslash", "milc_hip_set_link_compression", "milc_hip_link_compression", "milc_hip_dev_hisq_force",
 ]
 
 
@@ -105,6 +105,8 @@ def load_library(path=LIB_PATH):
     L.milc_hip_hisq_force.argtypes = [I, I, I, D, ctypes.POINTER(PD), ctypes.POINTER(P), PD, PD, P, P, P, P]
     L.milc_hip_eo_fermion_force_multi.argtypes = [I, D, PD, ctypes.POINTER(P), I, P, P]
     L.milc_hip_set_force_form.argtypes = [I]
+    L.milc_hip_dev_load_thin_links.argtypes = [I, P, I, ctypes.POINTER(I)]
+    L.milc_hip_dev_read_thin_links.argtypes = [P, I]
     L.milc_hip_dev_dslash.argtypes = [I, I, I, I]
     L.milc_hip_set_link_compression.argtypes = [I]
     L.milc_hip_link_compression.argtypes = [I]
@@ -534,6 +536,22 @@ class Lattice:
         self._activate()
         v = np.ascontiguousarray(v, dtype=np.float64)
         _check(self.lib.milc_hip_dev_write(slot, parity, _p(v)), "milc_hip_dev_write")
+
+    def dev_load_thin_links(self, thin, phases_in=False):
+        """thin links (V, 4, 3, 3, 2) in MILC layout -> device, HISQ links built
+        there; returns the number of links that took the SVD fallback"""
+        self._activate()
+        U = np.ascontiguousarray(thin, dtype=np.float64)
+        nsvd = ctypes.c_int(0)
+        _check(self.lib.milc_hip_dev_load_thin_links(2, _p(U), int(bool(phases_in)), ctypes.byref(nsvd)),
+               "milc_hip_dev_load_thin_links")
+        return nsvd.value
+
+    def dev_read_thin_links(self, with_phases=False):
+        self._activate()
+        U = np.zeros((self.V, 4, 3, 3, 2))
+        _check(self.lib.milc_hip_dev_read_thin_links(_p(U), int(bool(with_phases))), "milc_hip_dev_read_thin_links")
+        return U
 
     def dev_read_links(self):
         self._activate()

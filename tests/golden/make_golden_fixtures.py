@@ -9,6 +9,9 @@ site order (x fastest), as .npy files next to this script.  The files are data
 LIME record layout: 144-byte header (u32 magic 0x456789ab, u16 version,
 u16 flags, u64 payload length, 128-byte type string), payload padded to 8 bytes.
 
+The lattice files used by the file-format tests are copied unchanged into
+tests/golden/io/.
+
 Usage: python tests/golden/make_golden_fixtures.py [/root/reference]
 """
 import os
@@ -65,12 +68,28 @@ FILES = {
 }
 
 
+# lattice files of MILC's own tests, kept byte for byte for the file-format
+# readers: v5 in both byte orders, SciDAC, ILDG, a foreign ILDG file without
+# checksum, a v5 file whose stored checksum does not match its data, and a
+# SciDAC colour-vector file
+IO_FILES = [
+    "lat.sample.l4444", "lat.sample.l4444.scidac", "lat.sample.l4444.ildg",
+    "lat.sample.l4448", "lat.sample.l4448.scidac", "lat.sample.l4448.foreign",
+    "lat.sample.5p7", "lat.sample.l6666", "ksvector.src.sample.l8",
+]
+
+
 def main(ref="/root/reference"):
     for src, dst in FILES.items():
         dims, arr = scidac_field(os.path.join(ref, "binary_samples", src))
         assert dims == [8, 8, 8, 8]
         np.save(os.path.join(HERE, dst), arr)
         print(dst, arr.shape, arr.dtype)
+    os.makedirs(os.path.join(HERE, "io"), exist_ok=True)
+    for name in IO_FILES:
+        data = open(os.path.join(ref, "binary_samples", name), "rb").read()
+        open(os.path.join(HERE, "io", name), "wb").write(data)
+        print("io/" + name, len(data))
 
 
 if __name__ == "__main__":

@@ -17,9 +17,15 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def header_symbols():
-    txt = open(os.path.join(ROOT, "include", "milc_hip.h")).read()
-    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    return sorted(set(re.findall(r"\b(milc_hip_[a-z_0-9]+)\s*\(", txt)))
+    """every function declared in include/*.h (milc_hip_* and milc_io_*)"""
+    syms = set()
+    for h in sorted(os.listdir(os.path.join(ROOT, "include"))):
+        if not h.endswith(".h"):
+            continue
+        txt = open(os.path.join(ROOT, "include", h)).read()
+        txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+        syms |= set(re.findall(r"\b(milc_(?:hip|io)_[a-z_0-9]+)\s*\(", txt))
+    return sorted(syms)
 
 
 def test_library_exports_every_header_symbol():
