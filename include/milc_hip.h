@@ -151,6 +151,30 @@ int milc_hip_eo_fermion_force_multi(int precision, double eps, const double* res
  * result converted back from that precision) on the current device links. */
 int milc_hip_dev_dslash(int prec, int slot_src, int parity_out, int slot_dst);
 
+/* ---- gauge action and molecular-dynamics link update ---------------------
+ * Links: su3_matrix[4 V] in MILC layout and precision (1 float, 2 double),
+ * phases_in = the KS phases are in (they are removed for the gauge loops).
+ * Momenta: anti_hermitmat[4 V] (m01, m02, m12, m00im, m11im, m22im, space).
+ * loop_coeff: the three loop coefficients of the Symanzik 1x1 + 1x2 + 1x1x1
+ * action (num_loop_types must be 3; generic/imp_actions/symanzik_1loop_*). */
+
+/* momentum += -eb3 TA(U G), G = sum over the action's loops through each link
+ * of the loop coefficient times the rest of the loop.  Replaces
+ * qudaGaugeForcePhased (generic/gauge_force_imp_gpu.c:31-34, eb3 = eps beta/3)
+ * and imp_gauge_force_cpu (generic/gauge_force_imp.c:37-219). */
+int milc_hip_gauge_force(int precision, int num_loop_types, const double* loop_coeff, double eb3, const void* links,
+                         int phases_in, void* momentum);
+/* *action = sum over loops and sites of coeff (3 - Re Tr P): imp_gauge_action
+ * (generic/gauge_stuff.c:248-322); MILC's gauge action is beta/3 times it
+ * (ks_imp_rhmc/d_action_rhmc.c:27). */
+int milc_hip_gauge_action(int precision, int num_loop_types, const double* loop_coeff, const void* links,
+                          int phases_in, double* action);
+/* links <- exp(eps P) links to eighth order (Horner), in place: replaces
+ * qudaUpdateUPhased / update_u (ks_imp_rhmc/update_u.c:14-34, :66-111). */
+int milc_hip_update_u(int precision, double eps, void* links, const void* momentum, int phases_in);
+/* *action = sum over links of |P|^2/2 - 4: hmom_action (ks_imp_rhmc/d_action_rhmc.c:86-101). */
+int milc_hip_mom_action(int precision, const void* momentum, double* action);
+
 /* 16-bit sloppy long links in the two-row form (on = 1, the default): a long
  * link that is a real multiple of a U(3) matrix, L = c U (HISQ and asqtad
  * Naik links), is stored as rows 0 and 1 plus z = det(U)/c, and row 2 =

@@ -182,6 +182,7 @@ int milc_hip_finalize(void) {
   }
   for (int s = 0; s < kSlots; s++) free_vec(g_slots[s]);
   fn_table_release();
+  gauge_table_release();
   batch_release();
   if (c.thin_valid) free_links(c.thin);
   c.thin_valid = false;
@@ -1646,6 +1647,112 @@ extern "C" int milc_hip_eo_fermion_force_multi(int precision, double eps, const 
   release_fields(xo);
   if (rc) return rc;
   return add_momentum(precision, b.mom, momentum);
+}
+
+// ---------------------------------------------------------------------------
+// gauge action, gauge force, link update, momentum action (host arrays)
+// ---------------------------------------------------------------------------
+namespace {
+struct GaugeBuffers {
+  LinkField U{};
+  double* stage = nullptr;
+  double* mom = nullptr;
+  ~GaugeBuffers() {
+    free_links(U);
+    dev_free(stage);
+    dev_free(mom);
+  }
+  int alloc(bool with_mom) {
+    Context& c = ctx();
+    int rc;
+    if ((rc = alloc_links(U, P_F64, c.stride)) ||
+        (rc = dev_alloc((void**)&stage, (size_t)2 * c.Vh * 4 * 18 * sizeof(double))))
+      return rc;
+    if (with_mom && (rc = dev_alloc((void**)&mom, (size_t)2 * c.Vh * 4 * 10 * sizeof(double)))) return rc;
+    return 0;
+  }
+};
+
+int loop_coeffs_ok(int num_loop_types, const double* loop_coeff) {
+  return num_loop_types == 3 && loop_coeff != nullptr;  // 1x1, 1x2, 1x1x1
+}
+
+// host momenta (anti_hermitmat[4 V], Real precision) -> FP64 device copy
+int mom_in(int ext_prec, const void* h, double* dev) {
+  Context& c = ctx();
+  size_t n = (size_t)2 * c.Vh * 4 * 10;
+  std::vector<double> conv;
+  const double* src = (const double*)h;
+  if (ext_prec == 1) {
+    conv.resize(n);
+    for (size_t i = 0; i < n; i++) conv[i] = ((const float*)h)[i];
+    src = conv.data();
+  }
+  MH_CHECK(hipMemcpyAsync(dev, src, n * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  MH_CHECK(hipStreamSynchronize(c.stream));
+  return 0;
+}
+}  // namespace
+
+extern "C" int milc_hip_gauge_force(int precision, int num_loop_types, const double* loop_coeff, double eb3,
+                                    const void* links, int phases_in, void* momentum) {
+  Context& c = ctx();
+  if (!c.initialised) return -1;
+  if ((precision != 1 && precision != 2) || !loop_coeffs_ok(num_loop_types, loop_coeff) || !links || !momentum)
+    return -2;
+  GaugeBuffers b;
+  int rc;
+  if ((rc = b.alloc(true)) || (rc = host_links_in(precision, links, b.U, b.stage))) return rc;
+  if (phases_in && (rc = rephase_dev(b.U))) return rc;  // the phases are signs: applied again they cancel
+  MH_CHECK(hipMemsetAsync(b.mom, 0, (size_t)2 * c.Vh * 4 * 10 * sizeof(double), c.stream));
+  if ((rc = gauge_force_dev(b.U, loop_coeff, eb3, b.mom))) return rc;
+  return add_momentum(precision, b.mom, momentum);
+}
+
+extern "C" int milc_hip_gauge_action(int precision, int num_loop_types, const double* loop_coeff, const void* links,
+                                     int phases_in, double* action) {
+  Context& c = ctx();
+  if (!c.initialised) return -1;
+  if ((precision != 1 && precision != 2) || !loop_coeffs_ok(num_loop_types, loop_coeff) || !links || !action)
+    return -2;
+  GaugeBuffers b;
+  int rc;
+  if ((rc = b.alloc(false)) || (rc = host_links_in(precision, links, b.U, b.stage))) return rc;
+  if (phases_in && (rc = rephase_dev(b.U))) return rc;
+  return gauge_action_dev(b.U, loop_coeff, action);
+}
+
+extern "C" int milc_hip_update_u(int precision, double eps, void* links, const void* momentum, int phases_in) {
+  Context& c = ctx();
+  if (!c.initialised) return -1;
+  if ((precision != 1 && precision != 2) || !links || !momentum) return -2;
+  (void)phases_in;  // exp(eps P) multiplies from the left: the sign of a phased link rides along
+  GaugeBuffers b;
+  int rc;
+  if ((rc = b.alloc(true)) || (rc = host_links_in(precision, links, b.U, b.stage)) ||
+      (rc = mom_in(precision, momentum, b.mom)) || (rc = update_u_dev(b.U, b.mom, eps)) ||
+      (rc = links_to_milc_dev(b.U, P_F64, b.stage)))
+    return rc;
+  size_t n = (size_t)2 * c.Vh * 4 * 18;
+  std::vector<double> h(n);
+  MH_CHECK(hipMemcpyAsync(h.data(), b.stage, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  MH_CHECK(hipStreamSynchronize(c.stream));
+  if (precision == 2) memcpy(links, h.data(), n * sizeof(double));
+  else
+    for (size_t i = 0; i < n; i++) ((float*)links)[i] = (float)h[i];
+  return 0;
+}
+
+extern "C" int milc_hip_mom_action(int precision, const void* momentum, double* action) {
+  Context& c = ctx();
+  if (!c.initialised) return -1;
+  if ((precision != 1 && precision != 2) || !momentum || !action) return -2;
+  double* mom = nullptr;
+  int rc;
+  if ((rc = dev_alloc((void**)&mom, (size_t)2 * c.Vh * 4 * 10 * sizeof(double)))) return rc;
+  if (!(rc = mom_in(precision, momentum, mom))) rc = mom_action_dev(mom, action);
+  dev_free(mom);
+  return rc;
 }
 
 extern "C" int milc_hip_set_link_compression(int on) {

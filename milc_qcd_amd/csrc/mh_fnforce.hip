@@ -14,6 +14,8 @@
 // path start x = y - offset; no atomics, deterministic order.  The path table
 // (344 paths, ~508 entries per target direction) is read uniformly by a
 // wavefront and stays in the scalar/L1 caches.
+#include <algorithm>
+#include <array>
 #include <vector>
 
 #include "mh_internal.h"
@@ -216,10 +218,15 @@ __global__ void __launch_bounds__(128) k_path_adjoint(Geom g, LinkField Lk, Link
       }
     }
     Mat3<double> Ob, A, M;
-    mload(P->naik ? F3 : F1, xpb, E->mu, xcb, Ob);
     double w = ((with_eps && xpb) ? -1.0 : 1.0) * P->c;
-    mmul(suf, Ob, A);
-    mmul(A, pre, M);
+    if (F1.data) {
+      mload(P->naik ? F3 : F1, xpb, E->mu, xcb, Ob);
+      mmul(suf, Ob, A);
+      mmul(A, pre, M);
+    } else {
+      // closed loops (gauge force): the identity at the loop's base
+      mmul(suf, pre, M);
+    }
     if (E->adj) madj(M);
 #pragma unroll
     for (int i = 0; i < 3; i++)
@@ -672,7 +679,7 @@ __global__ void __launch_bounds__(128) k_sf_naik(Geom g, LinkField U, int mu, MF
 
 // mom += TA(scale U G), packed (as k_path_adjoint<1>)
 __global__ void __launch_bounds__(128) k_mom_ta(Geom g, LinkField U, LinkField G, double scale,
-                                                double* __restrict__ mom) {
+                                                double* __restrict__ mom, int accumulate) {
   int cb = blockIdx.x * blockDim.x + threadIdx.x;
   int pm = blockIdx.y;
   if (cb >= g.Vh) return;
@@ -684,15 +691,15 @@ __global__ void __launch_bounds__(128) k_mom_ta(Geom g, LinkField U, LinkField G
   double tr = (X.e[0][0].im + X.e[1][1].im + X.e[2][2].im) * (1.0 / 3.0);
   double h = 0.5 * scale;
   double* o = mom + ((size_t)(pb * g.Vh + cb) * 4 + nu) * 10;
-  o[0] = h * (X.e[0][1].re - X.e[1][0].re);
-  o[1] = h * (X.e[0][1].im + X.e[1][0].im);
-  o[2] = h * (X.e[0][2].re - X.e[2][0].re);
-  o[3] = h * (X.e[0][2].im + X.e[2][0].im);
-  o[4] = h * (X.e[1][2].re - X.e[2][1].re);
-  o[5] = h * (X.e[1][2].im + X.e[2][1].im);
-  o[6] = scale * (X.e[0][0].im - tr);
-  o[7] = scale * (X.e[1][1].im - tr);
-  o[8] = scale * (X.e[2][2].im - tr);
+  o[0] = (accumulate ? o[0] : 0.0) + h * (X.e[0][1].re - X.e[1][0].re);
+  o[1] = (accumulate ? o[1] : 0.0) + h * (X.e[0][1].im + X.e[1][0].im);
+  o[2] = (accumulate ? o[2] : 0.0) + h * (X.e[0][2].re - X.e[2][0].re);
+  o[3] = (accumulate ? o[3] : 0.0) + h * (X.e[0][2].im + X.e[2][0].im);
+  o[4] = (accumulate ? o[4] : 0.0) + h * (X.e[1][2].re - X.e[2][1].re);
+  o[5] = (accumulate ? o[5] : 0.0) + h * (X.e[1][2].im + X.e[2][1].im);
+  o[6] = (accumulate ? o[6] : 0.0) + scale * (X.e[0][0].im - tr);
+  o[7] = (accumulate ? o[7] : 0.0) + scale * (X.e[1][1].im - tr);
+  o[8] = (accumulate ? o[8] : 0.0) + scale * (X.e[2][2].im - tr);
   o[9] = 0.0;
 }
 
@@ -804,11 +811,278 @@ int fn_force_from_oprod(const LinkField& U, const LinkField& o1, const LinkField
   if ((rc = alloc_links(G, P_F64, c.stride))) return rc;
   rc = smear_adjoint_dev(U, o1, o3, 1, coeff, G);
   if (!rc) {
-    hipLaunchKernelGGL(k_mom_ta, dim3((c.Vh + 127) / 128, 8), dim3(128), 0, c.stream, c.geom, U, G, scale, mom_dev);
+    hipLaunchKernelGGL(k_mom_ta, dim3((c.Vh + 127) / 128, 8), dim3(128), 0, c.stream, c.geom, U, G, scale, mom_dev, 0);
     rc = hipGetLastError() == hipSuccess ? 0 : -1;
   }
   MH_CHECK(hipStreamSynchronize(c.stream));
   free_links(G);
+  return rc;
+}
+
+// Gauge loops of the Symanzik 1x1 + 1x2 + 1x1x1 action: every rotation and
+// reflection of the plaquette, the 1x2 rectangle and the 1x1x1 parallelogram,
+// one loop per class of cyclic shifts and reversal (the loop set of
+// make_loop_table, generic/gauge_stuff.c:68-181: 6 + 12 + 16 loops), as
+// closed paths for the gather kernel.
+static void build_gauge_paths(const double coeff[3], std::vector<FnPath>& paths, std::vector<int>& path_mu) {
+  static const int shapes[3][6] = {{0, 1, 7, 6, 0, 0}, {0, 0, 1, 7, 7, 6}, {0, 1, 2, 7, 6, 5}};
+  static const int slen[3] = {4, 6, 6};
+  std::vector<std::vector<int>> keys;
+  auto canon = [](const std::vector<int>& d) {
+    std::vector<int> best;
+    const int n = (int)d.size();
+    std::vector<int> rv(n);
+    for (int j = 0; j < n; j++) rv[j] = 7 - d[n - 1 - j];
+    const std::vector<int>* both[2] = {&d, &rv};
+    for (const std::vector<int>* v : both)
+      for (int r = 0; r < n; r++) {
+        std::vector<int> t(n);
+        for (int j = 0; j < n; j++) t[j] = (*v)[(r + j) % n];
+        if (best.empty() || t < best) best = t;
+      }
+    return best;
+  };
+  int perm[4] = {0, 1, 2, 3};
+  std::vector<std::array<int, 4>> perms;
+  do perms.push_back({perm[0], perm[1], perm[2], perm[3]});
+  while (std::next_permutation(perm, perm + 4));
+  for (int s = 0; s < 3; s++)
+    for (const auto& pm : perms)
+      for (int refl = 0; refl < 16; refl++) {
+        int map[8];
+        for (int a = 0; a < 4; a++) {
+          map[a] = ((refl >> a) & 1) ? 7 - pm[a] : pm[a];
+          map[7 - a] = 7 - map[a];
+        }
+        std::vector<int> d(slen[s]);
+        for (int j = 0; j < slen[s]; j++) d[j] = map[shapes[s][j]];
+        std::vector<int> k = canon(d);
+        k.push_back(s);
+        if (std::find(keys.begin(), keys.end(), k) != keys.end()) continue;
+        keys.push_back(k);
+        FnPath p{};
+        p.n = slen[s];
+        for (int j = 0; j < p.n; j++) p.d[j] = d[j];
+        p.c = coeff[s];
+        p.naik = 0;
+        paths.push_back(p);
+        path_mu.push_back(0);  // unused: the loop closes on its base, no force field is read
+      }
+}
+
+// The gauge-loop table (one, rebuilt when the coefficients change)
+struct GaugeTable {
+  double coeff[3] = {0, 0, 0};
+  FnPath* paths = nullptr;
+  FnEntry* ent = nullptr;
+  int start[5] = {0, 0, 0, 0, 0};
+  int npaths = 0;
+  bool valid = false;
+};
+static GaugeTable g_gauge_table;
+
+static int gauge_table(const double coeff[3], GaugeTable*& out) {
+  Context& c = ctx();
+  GaugeTable& t = g_gauge_table;
+  if (t.valid && t.coeff[0] == coeff[0] && t.coeff[1] == coeff[1] && t.coeff[2] == coeff[2]) {
+    out = &t;
+    return 0;
+  }
+  dev_free(t.paths);
+  dev_free(t.ent);
+  t = GaugeTable{};
+  std::vector<FnPath> paths;
+  std::vector<int> path_mu;
+  std::vector<FnEntry> ent;
+  build_gauge_paths(coeff, paths, path_mu);
+  build_entries(paths, path_mu, ent, t.start);
+  int rc;
+  if ((rc = dev_alloc((void**)&t.paths, paths.size() * sizeof(FnPath))) ||
+      (rc = dev_alloc((void**)&t.ent, ent.size() * sizeof(FnEntry))))
+    return rc;
+  MH_CHECK(hipMemcpyAsync(t.paths, paths.data(), paths.size() * sizeof(FnPath), hipMemcpyHostToDevice, c.stream));
+  MH_CHECK(hipMemcpyAsync(t.ent, ent.data(), ent.size() * sizeof(FnEntry), hipMemcpyHostToDevice, c.stream));
+  MH_CHECK(hipStreamSynchronize(c.stream));
+  for (int k = 0; k < 3; k++) t.coeff[k] = coeff[k];
+  t.npaths = (int)paths.size();
+  t.valid = true;
+  out = &t;
+  return 0;
+}
+
+void gauge_table_release() {
+  dev_free(g_gauge_table.paths);
+  dev_free(g_gauge_table.ent);
+  g_gauge_table = GaugeTable{};
+}
+
+// mom_dev (packed, node_index order) += -eb3 TA(U G), G = sum over the loops
+// through each link of coeff (rest of the loop from the link's end, adjoint for
+// a link run backwards) -- imp_gauge_force_cpu, generic/gauge_force_imp.c:37-219.
+// U: links without KS phases.
+int gauge_force_dev(const LinkField& U, const double coeff[3], double eb3, double* mom_dev) {
+  Context& c = ctx();
+  GaugeTable* t;
+  int rc;
+  if ((rc = gauge_table(coeff, t))) return rc;
+  LinkField G{};
+  if ((rc = alloc_links(G, P_F64, c.stride))) return rc;
+  LinkField none{nullptr, 1.0f, 0};
+  int4 lo = make_int4(t->start[0], t->start[1], t->start[2], t->start[3]);
+  int4 hi = make_int4(t->start[1], t->start[2], t->start[3], t->start[4]);
+  dim3 grid((c.Vh + 127) / 128, 8);
+  hipLaunchKernelGGL(k_path_adjoint<0>, grid, dim3(128), 0, c.stream, c.geom, U, none, none, 0, t->paths, t->ent,
+                     lo, hi, G, U, 0.0, nullptr);
+  hipLaunchKernelGGL(k_mom_ta, grid, dim3(128), 0, c.stream, c.geom, U, G, -eb3, mom_dev, 1);
+  rc = hipGetLastError() == hipSuccess ? 0 : -1;
+  MH_CHECK(hipStreamSynchronize(c.stream));
+  free_links(G);
+  return rc;
+}
+
+// Per-block partial sums of coeff (3 - Re Tr P) over the loops based at each
+// site (imp_gauge_action, generic/gauge_stuff.c:248-322)
+__global__ void __launch_bounds__(128) k_gauge_action(Geom g, LinkField U, const FnPath* __restrict__ paths,
+                                                      int npaths, double* __restrict__ partials) {
+  int cb = blockIdx.x * blockDim.x + threadIdx.x;
+  int pb = blockIdx.y;
+  double s = 0.0;
+  if (cb < g.Vh) {
+    Coord c = site_coords(g, cb, pb);
+    for (int q = 0; q < npaths; q++) {
+      const FnPath* P = paths + q;
+      int pos[4] = {c.x, c.y, c.z, c.t};
+      Mat3<double> acc, L, T;
+      munit(acc);
+      for (int k = 0; k < P->n; k++) {
+        int d = P->d[k];
+        bool back = d >= 4;
+        int ld = back ? 7 - d : d;
+        int lpb, lcb;
+        if (!back) lcb = cb_of(g, pos, lpb);
+        pos[ld] = wrap(pos[ld] + (back ? -1 : 1), g.X[ld]);
+        if (back) lcb = cb_of(g, pos, lpb);
+        mload(U, lpb, ld, lcb, L);
+        if (back) madj(L);
+        mmul(acc, L, T);
+        acc = T;
+      }
+      s += P->c * (3.0 - (acc.e[0][0].re + acc.e[1][1].re + acc.e[2][2].re));
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  __shared__ double sm[2];
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) partials[blockIdx.y * gridDim.x + blockIdx.x] = sm[0] + sm[1];
+}
+
+int gauge_action_dev(const LinkField& U, const double coeff[3], double* action) {
+  Context& c = ctx();
+  GaugeTable* t;
+  int rc;
+  if ((rc = gauge_table(coeff, t))) return rc;
+  const int nb = (c.Vh + 127) / 128;
+  double* part = nullptr;
+  if ((rc = dev_alloc((void**)&part, (size_t)2 * nb * sizeof(double)))) return rc;
+  hipLaunchKernelGGL(k_gauge_action, dim3(nb, 2), dim3(128), 0, c.stream, c.geom, U, t->paths, t->npaths, part);
+  std::vector<double> h((size_t)2 * nb);
+  rc = hipGetLastError() == hipSuccess ? 0 : -1;
+  if (!rc) rc = hipMemcpyAsync(h.data(), part, h.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream) ==
+                        hipSuccess ? 0 : -1;
+  (void)hipStreamSynchronize(c.stream);
+  dev_free(part);
+  double sum = 0.0;
+  for (double v : h) sum += v;
+  *action = sum;
+  return rc;
+}
+
+// packed anti_hermitmat (m01, m02, m12, m00im, m11im, m22im) -> matrix
+__device__ __forceinline__ void unpack_ah(const double* o, Mat3<double>& P) {
+  P.e[0][1].re = o[0];  P.e[0][1].im = o[1];
+  P.e[0][2].re = o[2];  P.e[0][2].im = o[3];
+  P.e[1][2].re = o[4];  P.e[1][2].im = o[5];
+  P.e[1][0].re = -o[0]; P.e[1][0].im = o[1];
+  P.e[2][0].re = -o[2]; P.e[2][0].im = o[3];
+  P.e[2][1].re = -o[4]; P.e[2][1].im = o[5];
+  P.e[0][0].re = 0.0;   P.e[0][0].im = o[6];
+  P.e[1][1].re = 0.0;   P.e[1][1].im = o[7];
+  P.e[2][2].re = 0.0;   P.e[2][2].im = o[8];
+}
+
+// U <- sum_{k=0}^{8} (eps P)^k / k! U, Horner form (update_u, ks_imp_rhmc/update_u.c:66-111)
+__global__ void __launch_bounds__(128) k_update_u(Geom g, LinkField U, const double* __restrict__ mom, double eps) {
+  int cb = blockIdx.x * blockDim.x + threadIdx.x;
+  int pm = blockIdx.y;
+  if (cb >= g.Vh) return;
+  int pb = pm >> 2, mu = pm & 3;
+  Mat3<double> P, L, t1, t2;
+  unpack_ah(mom + ((size_t)(pb * g.Vh + cb) * 4 + mu) * 10, P);
+  mload(U, pb, mu, cb, L);
+  mmul(P, L, t1);
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      t2.e[i][j].re = L.e[i][j].re + (eps / 8.0) * t1.e[i][j].re;
+      t2.e[i][j].im = L.e[i][j].im + (eps / 8.0) * t1.e[i][j].im;
+    }
+  for (int m = 7; m >= 1; m--) {
+    mmul(P, t2, t1);
+    const double f = eps / m;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        t2.e[i][j].re = L.e[i][j].re + f * t1.e[i][j].re;
+        t2.e[i][j].im = L.e[i][j].im + f * t1.e[i][j].im;
+      }
+  }
+  double2* out = (double2*)U.data + (size_t)((pb * 4 + mu) * 9) * U.stride + cb;
+#pragma unroll
+  for (int e = 0; e < 9; e++) out[(size_t)e * U.stride] = make_double2(t2.e[e / 3][e % 3].re, t2.e[e / 3][e % 3].im);
+}
+
+int update_u_dev(LinkField& U, const double* mom_dev, double eps) {
+  Context& c = ctx();
+  hipLaunchKernelGGL(k_update_u, dim3((c.Vh + 127) / 128, 8), dim3(128), 0, c.stream, c.geom, U, mom_dev, eps);
+  MH_CHECK(hipGetLastError());
+  return 0;
+}
+
+// per-block partial sums of ahmat_mag_sq - 4 (hmom_action, ks_imp_rhmc/d_action_rhmc.c:86-124)
+__global__ void __launch_bounds__(256) k_mom_action(const double* __restrict__ mom, size_t nlinks,
+                                                    double* __restrict__ partials) {
+  double s = 0.0;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nlinks; i += (size_t)gridDim.x * blockDim.x) {
+    const double* o = mom + 10 * i;
+    s += 0.5 * (o[6] * o[6] + o[7] * o[7] + o[8] * o[8]) + o[0] * o[0] + o[1] * o[1] + o[2] * o[2] + o[3] * o[3] +
+         o[4] * o[4] + o[5] * o[5] - 4.0;
+  }
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  __shared__ double sm[4];
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) partials[blockIdx.x] = sm[0] + sm[1] + sm[2] + sm[3];
+}
+
+int mom_action_dev(const double* mom_dev, double* action) {
+  Context& c = ctx();
+  const int nb = 1024;
+  double* part = nullptr;
+  int rc;
+  if ((rc = dev_alloc((void**)&part, nb * sizeof(double)))) return rc;
+  hipLaunchKernelGGL(k_mom_action, dim3(nb), dim3(256), 0, c.stream, mom_dev, (size_t)2 * c.Vh * 4, part);
+  std::vector<double> h(nb);
+  rc = hipGetLastError() == hipSuccess ? 0 : -1;
+  if (!rc) rc = hipMemcpyAsync(h.data(), part, nb * sizeof(double), hipMemcpyDeviceToHost, c.stream) == hipSuccess
+                    ? 0 : -1;
+  (void)hipStreamSynchronize(c.stream);
+  dev_free(part);
+  double sum = 0.0;
+  for (double v : h) sum += v;
+  *action = sum;
   return rc;
 }
 
@@ -900,7 +1174,7 @@ int hisq_force_from_oprod(const LinkField& U, const LinkField& V, const LinkFiel
       rc = smear_adjoint_dev(U, FV, FV, 0, coeff1, FW);  // FW reused as F_U
       if (!rc) {
         hipLaunchKernelGGL(k_mom_ta, dim3((c.Vh + 127) / 128, 8), dim3(128), 0, c.stream, c.geom, U, FW, scale,
-                           mom_dev);
+                           mom_dev, 0);
         rc = hipGetLastError() == hipSuccess ? 0 : -1;
       }
     } else {

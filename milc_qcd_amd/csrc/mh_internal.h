@@ -156,6 +156,14 @@ int hisq_force_from_oprod(const LinkField& U, const LinkField& V, const LinkFiel
                           double filter, double* mom_dev);
 // thin-link force adjoint form: 1 staple form (default), 0 path-table gather
 void set_force_form(int form);
+// Symanzik-improved gauge action on links without KS phases (coeff: plaquette,
+// rectangle, parallelogram), its force into packed momenta, the link update and
+// the momentum action
+int gauge_force_dev(const LinkField& U, const double coeff[3], double eb3, double* mom_dev);
+int gauge_action_dev(const LinkField& U, const double coeff[3], double* action);
+int update_u_dev(LinkField& U, const double* mom_dev, double eps);
+int mom_action_dev(const double* mom_dev, double* action);
+void gauge_table_release();
 // device path table of the FN force (freed with the context)
 void fn_table_release();
 // batched-solver caches (freed with the context)

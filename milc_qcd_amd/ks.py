@@ -50,7 +50,7 @@ EXPORTS = [
     "milc_hip_dev_block_dslash_bench", "milc_hip_dev_block_dslash", "milc_hip_dev_multishift_block",
     "milc_hip_dev_block_describe",
     "milc_hip_dev_multishift", "milc_hip_dev_hmc_step", "milc_hip_hisq_force", "milc_hip_eo_fermion_force_multi",
-    "milc_hip_set_force_form", "milc_hip_dev_load_thin_links", "milc_hip_dev_read_thin_links", "milc_hip_dev_dslash", "milc_hip_set_link_compression", "milc_hip_link_compression", "milc_hip_dev_hisq_force",
+    "milc_hip_set_force_form", "milc_hip_gauge_force", "milc_hip_gauge_action", "milc_hip_update_u", "milc_hip_mom_action", "milc_hip_dev_load_thin_links", "milc_hip_dev_read_thin_links", "milc_hip_dev_dslash", "milc_hip_set_link_compression", "milc_hip_link_compression", "milc_hip_dev_hisq_force",
 ]
 
 
@@ -105,6 +105,10 @@ def load_library(path=LIB_PATH):
     L.milc_hip_hisq_force.argtypes = [I, I, I, D, ctypes.POINTER(PD), ctypes.POINTER(P), PD, PD, P, P, P, P]
     L.milc_hip_eo_fermion_force_multi.argtypes = [I, D, PD, ctypes.POINTER(P), I, P, P]
     L.milc_hip_set_force_form.argtypes = [I]
+    L.milc_hip_gauge_force.argtypes = [I, I, ctypes.POINTER(D), D, P, I, P]
+    L.milc_hip_gauge_action.argtypes = [I, I, ctypes.POINTER(D), P, I, ctypes.POINTER(D)]
+    L.milc_hip_update_u.argtypes = [I, D, P, P, I]
+    L.milc_hip_mom_action.argtypes = [I, P, ctypes.POINTER(D)]
     L.milc_hip_dev_load_thin_links.argtypes = [I, P, I, ctypes.POINTER(I)]
     L.milc_hip_dev_read_thin_links.argtypes = [P, I]
     L.milc_hip_dev_dslash.argtypes = [I, I, I, I]
@@ -378,6 +382,39 @@ class Lattice:
         """long-link format of the device links of precision prec (1: two-row form)"""
         self._activate()
         return _check(self.lib.milc_hip_link_compression(prec), "milc_hip_link_compression")
+
+    def gauge_force(self, loop_coeff, eb3, links, mom, phases_in=False):
+        """mom (V, 4, 10) packed += -eb3 TA(U G) (qudaGaugeForcePhased / imp_gauge_force)"""
+        self._activate()
+        c = (ctypes.c_double * 3)(*loop_coeff)
+        U = np.ascontiguousarray(links, dtype=np.float64)
+        assert mom.dtype == np.float64 and mom.flags.c_contiguous
+        _check(self.lib.milc_hip_gauge_force(2, 3, c, eb3, _p(U), int(bool(phases_in)), _p(mom)),
+               "milc_hip_gauge_force")
+        return mom
+
+    def gauge_action(self, loop_coeff, links, phases_in=False):
+        self._activate()
+        c = (ctypes.c_double * 3)(*loop_coeff)
+        a = ctypes.c_double(0)
+        _check(self.lib.milc_hip_gauge_action(2, 3, c, _p(np.ascontiguousarray(links, dtype=np.float64)),
+                                              int(bool(phases_in)), ctypes.byref(a)), "milc_hip_gauge_action")
+        return a.value
+
+    def update_u(self, eps, links, mom, phases_in=False):
+        """links (V, 4, 3, 3, 2) FP64 updated in place"""
+        self._activate()
+        assert links.dtype == np.float64 and links.flags.c_contiguous
+        _check(self.lib.milc_hip_update_u(2, eps, _p(links), _p(np.ascontiguousarray(mom, dtype=np.float64)),
+                                          int(bool(phases_in))), "milc_hip_update_u")
+        return links
+
+    def mom_action(self, mom):
+        self._activate()
+        a = ctypes.c_double(0)
+        _check(self.lib.milc_hip_mom_action(2, _p(np.ascontiguousarray(mom, dtype=np.float64)), ctypes.byref(a)),
+               "milc_hip_mom_action")
+        return a.value
 
     def set_force_form(self, form):
         """0: path-table gather; 1 (default): staple form where exact for any

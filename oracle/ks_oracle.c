@@ -1288,3 +1288,192 @@ void kso_hisq_force(const kso_geom *g, const double *U, const double *const *mul
   ta_force(g, U, FU, 2.0 * eps, mom);
   free(Vl); free(W); free(o1); free(o3); free(FW); free(FV); free(FU);
 }
+
+/* ------------------------------------------------------------------------ */
+/* Symanzik-improved gauge action (1x1 + 1x2 + 1x1x1), its force on the      */
+/* momenta, the momentum action and the link update.                         */
+/* ------------------------------------------------------------------------ */
+
+/* Loop coefficients of generic/imp_actions/symanzik_1loop_hisq_action.h
+ * (hisq = 1, HISQ quark loops, nf = total dynamical flavours) and
+ * symanzik_1loop_action.h (hisq = 0). */
+void kso_gauge_coeffs(double u0, int nf, int hisq, double coeff[3]) {
+  double l = log(u0);
+  coeff[0] = 1.0;
+  if (hisq) {
+    coeff[1] = -1.0 / (20.0 * u0 * u0) * (1.0 - (0.6264 - 1.1746 * nf) * l);
+    coeff[2] = 1.0 / (u0 * u0) * (0.0433 - 0.0156 * nf) * l;
+  } else {
+    coeff[1] = -1.0 / (20.0 * u0 * u0) * (1.0 - 0.6264 * l);
+    coeff[2] = 1.0 / (u0 * u0) * 0.04335 * l;
+  }
+}
+
+typedef struct { int n, shape; int d[6]; } kso_loop;
+
+/* smallest rotation of d[0..n-1] as a base-8 number */
+static long loop_key1(const int *d, int n) {
+  long best = -1;
+  for (int r = 0; r < n; r++) {
+    long k = 0;
+    for (int j = 0; j < n; j++) k = 8 * k + d[(r + j) % n];
+    if (best < 0 || k < best) best = k;
+  }
+  return best;
+}
+
+/* Every rotation and reflection of the three loop shapes, one loop per class
+ * of cyclic shifts and reversal (generic/gauge_stuff.c:68-181): 6 plaquettes,
+ * 12 rectangles, 16 parallelograms.  Directions in MILC's codes. */
+int kso_gauge_loops(int *dirs, int *len, int *shape) {
+  static const int shapes[3][6] = {{0, 1, 7, 6, 0, 0}, {0, 0, 1, 7, 7, 6}, {0, 1, 2, 7, 6, 5}};
+  static const int slen[3] = {4, 6, 6};
+  kso_loop out[64];
+  long keys[64];
+  int count = 0;
+  int perm[4];
+  for (perm[0] = 0; perm[0] < 4; perm[0]++)
+    for (perm[1] = 0; perm[1] < 4; perm[1]++)
+      for (perm[2] = 0; perm[2] < 4; perm[2]++)
+        for (perm[3] = 0; perm[3] < 4; perm[3]++) {
+          if (perm[0] == perm[1] || perm[0] == perm[2] || perm[0] == perm[3] || perm[1] == perm[2] ||
+              perm[1] == perm[3] || perm[2] == perm[3])
+            continue;
+          for (int refl = 0; refl < 16; refl++) {
+            int map[8];
+            for (int a = 0; a < 4; a++) {
+              map[a] = ((refl >> a) & 1) ? 7 - perm[a] : perm[a];
+              map[7 - a] = 7 - map[a];
+            }
+            for (int s = 0; s < 3; s++) {
+              int d[6], rv[6], n = slen[s];
+              for (int j = 0; j < n; j++) d[j] = map[shapes[s][j]];
+              for (int j = 0; j < n; j++) rv[j] = 7 - d[n - 1 - j];
+              long k1 = loop_key1(d, n), k2 = loop_key1(rv, n), key = (k1 < k2 ? k1 : k2) * 4 + s;
+              int seen = 0;
+              for (int q = 0; q < count; q++) seen |= keys[q] == key;
+              if (seen) continue;
+              keys[count] = key;
+              out[count].n = n;
+              out[count].shape = s;
+              memcpy(out[count].d, d, sizeof d);
+              count++;
+            }
+          }
+        }
+  for (int q = 0; q < count; q++) {
+    if (len) len[q] = out[q].n;
+    if (shape) shape[q] = out[q].shape;
+    if (dirs)
+      for (int j = 0; j < 6; j++) dirs[6 * q + j] = j < out[q].n ? out[q].d[j] : -1;
+  }
+  return count;
+}
+
+/* sum over loops and base sites of coeff[shape] (3 - Re Tr P) (imp_gauge_action,
+ * generic/gauge_stuff.c:248-322); the gauge action is beta/3 times this
+ * (ks_imp_rhmc/d_action_rhmc.c:27) */
+double kso_gauge_action(const kso_geom *g, const double coeff[3], const double *U) {
+  int dirs[64 * 6], len[64], shp[64];
+  int nl = kso_gauge_loops(dirs, len, shp);
+  double total = 0.0;
+  for (int q = 0; q < nl; q++) {
+    double sum = 0.0;
+#pragma omp parallel for reduction(+ : sum)
+    for (size_t x = 0; x < g->V; x++) {
+      double P[18], T[18], L[18];
+      memset(P, 0, sizeof P);
+      RE(P, 0, 0) = RE(P, 1, 1) = RE(P, 2, 2) = 1.0;
+      size_t y = x;
+      for (int j = 0; j < len[q]; j++) {
+        int d = dirs[6 * q + j];
+        size_t yn = NB(g, d, y);
+        if (d < 4) memcpy(L, U + (4 * y + d) * 18, sizeof L);
+        else m_adj(U + (4 * yn + (7 - d)) * 18, L);
+        m_nn(P, L, T);
+        memcpy(P, T, sizeof P);
+        y = yn;
+      }
+      sum += coeff[shp[q]] * (3.0 - (RE(P, 0, 0) + RE(P, 1, 1) + RE(P, 2, 2)));
+    }
+    total += sum;
+  }
+  return total;
+}
+
+/* mom[4V][18] (full anti-Hermitian matrices) += -eb3 TA(U G), G = sum over loop
+ * occurrences of the link of coeff (rest of the loop, in the loop's order from
+ * the link's end), adjoint for a link run backwards -- the update of
+ * imp_gauge_force_cpu (generic/gauge_force_imp.c:37-219), eb3 = eps beta / 3 */
+void kso_gauge_force(const kso_geom *g, const double coeff[3], double eb3, const double *U, double *mom) {
+  int dirs[64 * 6], len[64], shp[64];
+  int nl = kso_gauge_loops(dirs, len, shp);
+  const size_t V = g->V;
+  double *G = (double *)calloc(18 * 4 * V, sizeof(double));
+  double *F = (double *)malloc(sizeof(double) * 18 * 4 * V);
+  for (int q = 0; q < nl; q++) {
+    const int n = len[q];
+    for (size_t x = 0; x < V; x++) {
+      size_t y[7];
+      double Lk[6][18], pre[7][18], suf[6][18];
+      y[0] = x;
+      for (int j = 0; j < n; j++) {
+        int d = dirs[6 * q + j];
+        y[j + 1] = NB(g, d, y[j]);
+        if (d < 4) memcpy(Lk[j], U + (4 * y[j] + d) * 18, sizeof Lk[j]);
+        else m_adj(U + (4 * y[j + 1] + (7 - d)) * 18, Lk[j]);
+      }
+      memset(pre[0], 0, sizeof pre[0]);
+      RE(pre[0], 0, 0) = RE(pre[0], 1, 1) = RE(pre[0], 2, 2) = 1.0;
+      for (int j = 0; j < n; j++) m_nn(pre[j], Lk[j], pre[j + 1]);
+      memset(suf[n - 1], 0, sizeof suf[0]);
+      RE(suf[n - 1], 0, 0) = RE(suf[n - 1], 1, 1) = RE(suf[n - 1], 2, 2) = 1.0;
+      for (int j = n - 2; j >= 0; j--) m_nn(Lk[j + 1], suf[j + 1], suf[j]);
+      for (int j = 0; j < n; j++) {
+        double M[18];
+        m_nn(suf[j], pre[j], M);
+        int d = dirs[6 * q + j];
+        if (d < 4) m_axpy(G + (4 * y[j] + d) * 18, coeff[shp[q]], M);
+        else {
+          double Md[18];
+          m_adj(M, Md);
+          m_axpy(G + (4 * y[j + 1] + (7 - d)) * 18, coeff[shp[q]], Md);
+        }
+      }
+    }
+  }
+  ta_force(g, U, G, -eb3, F);
+  for (size_t i = 0; i < 18 * 4 * V; i++) mom[i] += F[i];
+  free(G);
+  free(F);
+}
+
+/* sum over links of |P|^2 / 2 - 4 (hmom_action, ks_imp_rhmc/d_action_rhmc.c:86-101),
+ * P as full anti-Hermitian matrices */
+double kso_mom_action(const kso_geom *g, const double *mom) {
+  double sum = 0.0;
+#pragma omp parallel for reduction(+ : sum)
+  for (size_t i = 0; i < 4 * g->V; i++) {
+    double s = 0.0;
+    for (int k = 0; k < 18; k++) s += mom[18 * i + k] * mom[18 * i + k];
+    sum += 0.5 * s - 4.0;
+  }
+  return sum;
+}
+
+/* U <- sum_{k=0}^{8} (eps P)^k / k! U, Horner form (update_u,
+ * ks_imp_rhmc/update_u.c:66-111) */
+void kso_update_u(const kso_geom *g, double eps, double *U, const double *mom) {
+#pragma omp parallel for
+  for (size_t i = 0; i < 4 * g->V; i++) {
+    const double *P = mom + 18 * i;
+    double *L = U + 18 * i, t1[18], t2[18];
+    m_nn(P, L, t1);
+    for (int k = 0; k < 18; k++) t2[k] = L[k] + (eps / 8.0) * t1[k];
+    for (int m = 7; m >= 1; m--) {
+      m_nn(P, t2, t1);
+      for (int k = 0; k < 18; k++) t2[k] = L[k] + (eps / m) * t1[k];
+    }
+    memcpy(L, t2, sizeof t2);
+  }
+}

@@ -99,6 +99,13 @@ void kso_fn_force(const kso_geom *g, const double coeff[6], const double *U, con
                   const double *residues, int nterms, double eps, double *mom);
 /* HISQ thin-link force (level 0 -> level-2 adjoint on W -> U(3) projection
  * adjoint -> fat7 adjoint on U): mom = TA(2 eps U F_U), full matrices. */
+/* Symanzik-improved gauge action, its momentum force, momentum action, link update */
+void kso_gauge_coeffs(double u0, int nf, int hisq, double coeff[3]);
+int kso_gauge_loops(int *dirs, int *len, int *shape);
+double kso_gauge_action(const kso_geom *g, const double coeff[3], const double *U);
+void kso_gauge_force(const kso_geom *g, const double coeff[3], double eb3, const double *U, double *mom);
+double kso_mom_action(const kso_geom *g, const double *mom);
+void kso_update_u(const kso_geom *g, double eps, double *U, const double *mom);
 void kso_hisq_force(const kso_geom *g, const double *U, const double *const *multi_x,
                     const double *residues, int nterms, double eps, double *mom);
 /* 0: force adjoints through the path tables (default), 1: staple form */

@@ -69,6 +69,14 @@ def lib():
         L.kso_hisq_force.argtypes = [G, P, P, P, ctypes.c_int, ctypes.c_double, P]
         L.kso_u3_adjoint.argtypes = [P, P, P]
         L.kso_set_force_form.argtypes = [ctypes.c_int]
+        L.kso_gauge_coeffs.argtypes = [ctypes.c_double, ctypes.c_int, ctypes.c_int, P]
+        L.kso_gauge_loops.argtypes = [P, P, P]
+        L.kso_gauge_action.argtypes = [G, P, P]
+        L.kso_gauge_action.restype = ctypes.c_double
+        L.kso_gauge_force.argtypes = [G, P, ctypes.c_double, P, P]
+        L.kso_mom_action.argtypes = [G, P]
+        L.kso_mom_action.restype = ctypes.c_double
+        L.kso_update_u.argtypes = [G, ctypes.c_double, P, P]
         L.kso_norm2.argtypes = [G, P, ctypes.c_int]
         L.kso_norm2.restype = ctypes.c_double
         L.kso_mdagm.argtypes = [G, P, P, ctypes.c_double, P, P, P, ctypes.c_int]
@@ -209,6 +217,27 @@ class Lattice:
         lib().kso_fn_force(self.g, _p(c), _p(np.ascontiguousarray(U_phased)), ptrs, _p(res), n, eps, _p(mom))
         return mom
 
+    def gauge_action(self, coeff, U):
+        """sum over loops and sites of coeff (3 - Re Tr P) (bare links)"""
+        return lib().kso_gauge_action(self.g, _p(np.ascontiguousarray(coeff, dtype=np.float64)),
+                                      _p(np.ascontiguousarray(U)))
+
+    def gauge_force(self, coeff, eb3, U, mom):
+        """mom (V, 4, 3, 3, 2) full anti-Hermitian += -eb3 TA(U G) (in place)"""
+        assert mom.dtype == np.float64 and mom.flags.c_contiguous
+        lib().kso_gauge_force(self.g, _p(np.ascontiguousarray(coeff, dtype=np.float64)), eb3,
+                              _p(np.ascontiguousarray(U)), _p(mom))
+        return mom
+
+    def mom_action(self, mom):
+        return lib().kso_mom_action(self.g, _p(np.ascontiguousarray(mom)))
+
+    def update_u(self, eps, U, mom):
+        """U <- exp_8(eps P) U in place"""
+        assert U.dtype == np.float64 and U.flags.c_contiguous
+        lib().kso_update_u(self.g, eps, _p(U), _p(np.ascontiguousarray(mom)))
+        return U
+
     def hisq_force(self, U_phased, multi_x, residues, eps=1.0):
         """HISQ thin-link force TA(2 eps U F_U), full 3x3 matrices (V, 4, 3, 3, 2)."""
         n = len(multi_x)
@@ -233,6 +262,24 @@ def u3_unitarize(V):
     W = np.zeros_like(V)
     svd = lib().kso_u3_unitarize(_p(V), _p(W))
     return W, svd
+
+
+def gauge_coeffs(u0, nf, hisq=True):
+    """loop coefficients (plaquette, rectangle, parallelogram) of the
+    Symanzik 1-loop action with HISQ (or asqtad) quark loops"""
+    c = np.zeros(3)
+    lib().kso_gauge_coeffs(u0, int(nf), int(bool(hisq)), _p(c))
+    return c
+
+
+def gauge_loops():
+    """(dirs (n, 6) MILC codes, -1 padded; lengths; shapes) of the action's loops"""
+    d = np.zeros(64 * 6, dtype=np.int32)
+    ln = np.zeros(64, dtype=np.int32)
+    sh = np.zeros(64, dtype=np.int32)
+    n = lib().kso_gauge_loops(d.ctypes.data_as(ctypes.c_void_p), ln.ctypes.data_as(ctypes.c_void_p),
+                              sh.ctypes.data_as(ctypes.c_void_p))
+    return d.reshape(64, 6)[:n], ln[:n], sh[:n]
 
 
 def set_force_form(form):

@@ -1,0 +1,156 @@
+"""Oracle of the gauge part of the molecular dynamics (oracle/ks_oracle.c:
+kso_gauge_*, kso_mom_action, kso_update_u) against MILC's numbers and the
+mathematics:
+
+* loop set and coefficients: the multiplicities (6, 12, 16) and coefficients
+  MILC printed for u0 = 0.89 with 2+1 flavours (ks_imp_rhmc/test/
+  su3_rhmc_hisq.1.sample-out);
+* the gauge action of lat.sample.l6666.hisq: beta/3 times it equals the
+  g_action MILC printed at the start of that run (beta = 7.4);
+* the force against central differences of the action along random
+  anti-Hermitian directions;
+* the link update against the exact exponential (scipy) and against the
+  explicit eighth-order series; the momentum action against |P|_F^2 / 2;
+* a pure-gauge leapfrog trajectory: the energy violation falls as eps^2
+  (which fixes the relative sign and normalisation of force, action, momentum
+  action and update) and the trajectory is reversible.
+"""
+import os
+
+import numpy as np
+import pytest
+from scipy.linalg import expm
+
+import ks_io as oio
+import ks_oracle as ko
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def to_c(M):
+    return M[..., 0] + 1j * M[..., 1]
+
+
+def to_r(Mc):
+    return np.ascontiguousarray(np.stack([Mc.real, Mc.imag], -1))
+
+
+def rand_ah(rng, shape, scale=1.0):
+    A = rng.standard_normal(shape + (3, 3)) + 1j * rng.standard_normal(shape + (3, 3))
+    A = 0.5 * (A - np.conj(np.swapaxes(A, -1, -2)))
+    A -= np.eye(3) * np.trace(A, axis1=-2, axis2=-1)[..., None, None] / 3
+    return scale * A
+
+
+def test_loop_set_and_coefficients_match_milc():
+    d, ln, sh = ko.gauge_loops()
+    assert list(np.bincount(sh)) == [6, 12, 16]
+    assert list(ln[sh == 0]) == [4] * 6 and set(ln[sh > 0]) == {6}
+    # every loop closes
+    for row, n in zip(d, ln):
+        disp = np.zeros(4, dtype=int)
+        for k in row[:n]:
+            if k < 4:
+                disp[k] += 1
+            else:
+                disp[7 - k] -= 1
+        assert not disp.any()
+    c = ko.gauge_coeffs(0.89, 3, hisq=True)
+    # MILC's run kept the coefficients in single precision and printed 7 digits
+    np.testing.assert_allclose(np.float32(c).astype(np.float64), [1.0, -4.181005e-02, 5.149204e-04], rtol=1.01e-6 / 2)
+
+
+def test_gauge_action_matches_milc_run():
+    r = oio.read_v5(os.path.join(HERE, "golden", "io", "lat.sample.l6666.hisq"))
+    assert (r["sum29"], r["sum31"]) == (r["calc29"], r["calc31"]) == (0xC9687FE2, 0xB52BD146)
+    L = ko.Lattice(tuple(r["dims"]))
+    U = np.ascontiguousarray(r["links"].astype(np.float64)[L.lex_of_site()])
+    c = ko.gauge_coeffs(0.89, 3, hisq=True)
+    g_action = 7.4 / 3.0 * L.gauge_action(c, U)
+    # MILC ran in single precision: agreement to its rounding
+    assert abs(g_action / 1.91689577889074e+04 - 1) < 2e-7
+
+
+@pytest.fixture(scope="module")
+def small():
+    L = ko.Lattice((4, 4, 4, 6))
+    U = np.ascontiguousarray(L.gauge_random(1100))
+    return L, U
+
+
+def test_gauge_force_central_differences(small):
+    L, U = small
+    rng = np.random.default_rng(4)
+    c = ko.gauge_coeffs(0.86, 3, hisq=True)
+    A = rand_ah(rng, (L.V, 4))
+    Uc = to_c(U)
+
+    def S_at(t):
+        E = np.array([[expm(t * A[i, m]) for m in range(4)] for i in range(L.V)])
+        return L.gauge_action(c, to_r(E @ Uc))
+
+    h = 1e-4
+    fd = (S_at(h) - S_at(-h)) / (2 * h)
+    mom = np.zeros((L.V, 4, 3, 3, 2))
+    L.gauge_force(c, 1.0, U, mom)  # eb3 = 1: mom = -TA(U G)
+    pairing = float(np.sum(np.trace(A @ to_c(mom), axis1=-2, axis2=-1).real))
+    assert abs(pairing / fd - 1) < 1e-7
+    # a flipped sign or a dropped loop shape fails
+    mom2 = np.zeros_like(mom)
+    L.gauge_force([c[0], 0.0, c[2]], 1.0, U, mom2)
+    assert abs(float(np.sum(np.trace(A @ to_c(mom2), axis1=-2, axis2=-1).real)) / fd - 1) > 1e-3
+
+
+def test_update_u_and_mom_action(small):
+    L, U = small
+    rng = np.random.default_rng(5)
+    P = rand_ah(rng, (L.V, 4), 0.7)
+    Pr = to_r(P)
+    eps = 0.02  # truncation (eps |P|)^9 / 9! below 1e-16
+    U1 = U.copy()
+    L.update_u(eps, U1, Pr)
+    Uc = to_c(U)
+    series = np.zeros_like(Uc)
+    term = Uc.copy()
+    for k in range(9):
+        series += term
+        term = (eps * P) @ term / (k + 1)
+    np.testing.assert_allclose(to_c(U1), series, atol=1e-14)
+    exact = np.array([[expm(eps * P[i, m]) for m in range(4)] for i in range(L.V)]) @ Uc
+    assert np.abs(to_c(U1) - exact).max() < 1e-14
+    S = L.mom_action(Pr)
+    assert abs(S - float(np.sum(0.5 * np.abs(P) ** 2) - 4 * 4 * L.V)) < 1e-9 * abs(S)
+
+
+def leapfrog(L, c, beta, U, P, eps, nsteps):
+    """pure-gauge leapfrog: P(eps/2) U(eps) ... P(eps/2), MILC's update order"""
+    U = U.copy()
+    P = P.copy()
+    eb3 = beta / 3.0
+    L.gauge_force(c, 0.5 * eps * eb3, U, P)
+    for s in range(nsteps):
+        L.update_u(eps, U, P)
+        L.gauge_force(c, (eps if s < nsteps - 1 else 0.5 * eps) * eb3, U, P)
+    return U, P
+
+
+def test_pure_gauge_trajectory_energy_and_reversibility(small):
+    L, U = small
+    rng = np.random.default_rng(6)
+    c = ko.gauge_coeffs(0.86, 3, hisq=True)
+    beta = 6.0
+    P0 = to_r(rand_ah(rng, (L.V, 4)))
+
+    def H(U, P):
+        return beta / 3.0 * L.gauge_action(c, U) + L.mom_action(P)
+
+    H0 = H(U, P0)
+    dH = []
+    for eps, n in ((0.04, 5), (0.02, 10)):
+        U1, P1 = leapfrog(L, c, beta, U, P0, eps, n)
+        dH.append(abs(H(U1, P1) - H0))
+    assert 3.0 < dH[0] / dH[1] < 5.0, dH
+    # reversibility: flip the momenta and integrate back
+    U1, P1 = leapfrog(L, c, beta, U, P0, 0.02, 10)
+    U2, P2 = leapfrog(L, c, beta, U1, -P1, 0.02, 10)
+    assert np.abs(U2 - U).max() < 1e-11 and np.abs(P2 + P0).max() < 1e-11
