@@ -175,6 +175,44 @@ int milc_hip_update_u(int precision, double eps, void* links, const void* moment
 /* *action = sum over links of |P|^2/2 - 4: hmom_action (ks_imp_rhmc/d_action_rhmc.c:86-101). */
 int milc_hip_mom_action(int precision, const void* momentum, double* action);
 
+/* ---- one RHMC trajectory on the device -------------------------------------
+ * Rational functions in MILC's layout (params_ratfunc, ks_imp_rhmc/params_rhmc.h):
+ * R(x) = res[0] + sum_{k=1..order} res[k] / (x + pole[k]), x = M^dag M. */
+typedef struct {
+  int order;
+  double res[33];
+  double pole[33];
+} milc_hip_rational;
+
+typedef struct {
+  int steps;             /* MD steps (force evaluations); even for Omelyan */
+  int integrator;        /* 0 INT_LEAPFROG, 1 INT_OMELYAN (lambda = 0.8) */
+  double epsilon;        /* step size */
+  double beta;           /* gauge coupling */
+  double loop_coeff[3];  /* Symanzik loop coefficients (1x1, 1x2, 1x1x1) */
+  double mass;           /* HISQ quark mass of the pseudofermion */
+  milc_hip_rational md, gr, fa;  /* force, heatbath, action */
+  double resid_md, resid_gr, resid_fa;  /* solver true-residual targets */
+  int mixed_md;          /* MD solves: FP32 multi-shift + 16-bit mixed refinement */
+  int action_from_md;    /* 1: fermion action phi^dag R_MD phi (the MD force's own
+                            action, for integrator tests); 0: |R_FA phi|^2 */
+  int accept;            /* 0 keep the new links, 1 Metropolis, 2 restore the old */
+  unsigned long long seed;  /* momenta, pseudofermion noise, accept draw */
+} milc_hip_rhmc_params;
+
+/* One trajectory of update() (ks_imp_rhmc/update_rhmc.c:157-345) on the
+ * current device links (milc_hip_dev_gauge_hisq or milc_hip_dev_load_thin_links):
+ * Gaussian momenta (ranmom), phi = R_GR chi (grsource_imp_rhmc), the actions
+ * (d_action_rhmc: beta/3 gauge loops, momenta, fermion), the integrator over
+ * update_u and update_h_rhmc (gauge force + HISQ fermion force, fat and Naik
+ * links rebuilt after every link update), the actions again, accept/reject.
+ * The links are not reunitarised between steps (the eighth-order exponential
+ * keeps U(3) to its truncation).  stats[0..15] = g0, h0, f0, g1, h1, f1, dH,
+ * accepted, seconds, MD solver iterations, fermion forces, seconds in fermion
+ * forces, in gauge forces, in link updates and rebuilds, in actions, heatbath
+ * solver iterations.  Random streams are this library's own. */
+int milc_hip_dev_rhmc_trajectory(const milc_hip_rhmc_params* p, double* stats);
+
 /* 16-bit sloppy long links in the two-row form (on = 1, the default): a long
  * link that is a real multiple of a U(3) matrix, L = c U (HISQ and asqtad
  * Naik links), is stored as rows 0 and 1 plus z = det(U)/c, and row 2 =

@@ -1755,6 +1755,39 @@ extern "C" int milc_hip_mom_action(int precision, const void* momentum, double* 
   return rc;
 }
 
+extern "C" int milc_hip_dev_rhmc_trajectory(const milc_hip_rhmc_params* p, double* stats) {
+  Context& c = ctx();
+  if (!c.initialised) return -1;
+  if (!p || !stats) return -2;
+  RhmcParams q{};
+  q.steps = p->steps;
+  q.integrator = p->integrator;
+  q.epsilon = p->epsilon;
+  q.beta = p->beta;
+  for (int k = 0; k < 3; k++) q.loop_coeff[k] = p->loop_coeff[k];
+  q.mass = p->mass;
+  const milc_hip_rational* src[3] = {&p->md, &p->gr, &p->fa};
+  RhmcRat* dst[3] = {&q.md, &q.gr, &q.fa};
+  for (int r = 0; r < 3; r++) {
+    dst[r]->order = src[r]->order;
+    memcpy(dst[r]->res, src[r]->res, sizeof dst[r]->res);
+    memcpy(dst[r]->pole, src[r]->pole, sizeof dst[r]->pole);
+  }
+  q.resid_md = p->resid_md;
+  q.resid_gr = p->resid_gr;
+  q.resid_fa = p->resid_fa;
+  q.mixed_md = p->mixed_md;
+  q.action_from_md = p->action_from_md;
+  q.accept = p->accept;
+  q.seed = p->seed;
+  if (q.integrator != 0 && q.integrator != 1) return -2;
+  if (q.mixed_md) {
+    int rc;
+    if ((rc = ensure_links(P_F32))) return rc;
+  }
+  return rhmc_trajectory(q, stats);
+}
+
 extern "C" int milc_hip_set_link_compression(int on) {
   Context& c = ctx();
   if (on != 0 && on != 1) return -2;

@@ -156,6 +156,22 @@ int hisq_force_from_oprod(const LinkField& U, const LinkField& V, const LinkFiel
                           double filter, double* mom_dev);
 // thin-link force adjoint form: 1 staple form (default), 0 path-table gather
 void set_force_form(int form);
+// One RHMC trajectory on the device links (one HISQ pseudofermion): rational
+// functions in MILC's layout, res[0] = constant, res[1..order], pole[1..order]
+struct RhmcRat {
+  int order;
+  double res[33], pole[33];
+};
+struct RhmcParams {
+  int steps, integrator;  // 0 leapfrog, 1 Omelyan (lambda = 0.8)
+  double epsilon, beta, loop_coeff[3], mass;
+  RhmcRat md, gr, fa;
+  double resid_md, resid_gr, resid_fa;
+  int mixed_md, action_from_md, accept;
+  uint64_t seed;
+};
+int rhmc_trajectory(const RhmcParams& p, double* stats);
+
 // Symanzik-improved gauge action on links without KS phases (coeff: plaquette,
 // rectangle, parallelogram), its force into packed momenta, the link update and
 // the momentum action
@@ -176,6 +192,8 @@ int asqtad_links_dev(const LinkField& U, double u0, LinkField& fat, LinkField& l
 int hisq_aux_dev(const LinkField& U, LinkField& V, LinkField& W);
 int rephase_dev(LinkField& U);
 int gaussian_vec_dev(uint64_t seed, int pb, VecField& v);
+// Gaussian anti-Hermitian momenta (packed, node_index order), weight exp(-|P|^2/2)
+int ranmom_dev(uint64_t seed, double* mom);
 
 // ---- BLAS --------------------------------------------------------------------
 int vec_norm2(const VecField& v, int prec, double* out);  // synchronous

@@ -386,6 +386,44 @@ int gauge_random_dev(uint64_t seed, double scale, LinkField& U) {
   return 0;
 }
 
+// Gaussian anti-Hermitian momenta, packed (m01, m02, m12, m00im, m11im,
+// m22im, space), weight exp(-|P|^2/2): every real component of the
+// off-diagonal entries and r3, r8 with <x^2> = 1/2, diagonal r3 + r8/sqrt(3),
+// -r3 + r8/sqrt(3), -2 r8/sqrt(3) (random_anti_hermitian,
+// libraries/rand_ahmat.c:17-33).  The random stream is this library's own
+// counter-based generator, keyed by seed, lexicographic site and direction.
+__global__ void k_ranmom(Geom g, uint64_t key, double* __restrict__ mom) {
+  int cb = blockIdx.x * blockDim.x + threadIdx.x;
+  int pm = blockIdx.y;
+  if (cb >= g.Vh) return;
+  int pb = pm >> 2, mu = pm & 3;
+  Coord c = site_coords(g, cb, pb);
+  uint64_t s = lex_of(g, c);
+  const double h = 0.70710678118654752440, st = 0.57735026918962576451;
+  double r[8];
+  for (int k = 0; k < 4; k++) gauss_pair(key, (4 * s + mu) * 4 + k, r[2 * k], r[2 * k + 1]);
+  for (int k = 0; k < 8; k++) r[k] *= h;
+  double* o = mom + ((size_t)(pb * g.Vh + cb) * 4 + mu) * 10;
+  o[0] = r[2];
+  o[1] = r[3];
+  o[2] = r[4];
+  o[3] = r[5];
+  o[4] = r[6];
+  o[5] = r[7];
+  o[6] = r[0] + st * r[1];
+  o[7] = -r[0] + st * r[1];
+  o[8] = -2.0 * st * r[1];
+  o[9] = 0.0;
+}
+
+int ranmom_dev(uint64_t seed, double* mom) {
+  Context& c = ctx();
+  uint64_t key = sm_mix(seed ^ 0x9E3779B97F4A7C15ULL);
+  hipLaunchKernelGGL(k_ranmom, dim3(nb_for(c.Vh), 8), dim3(256), 0, c.stream, c.geom, key, mom);
+  MH_CHECK(hipGetLastError());
+  return 0;
+}
+
 int gaussian_vec_dev(uint64_t seed, int pb, VecField& v) {
   Context& c = ctx();
   uint64_t key = sm_mix(seed ^ 0x2545F4914F6CDD1DULL);

@@ -50,8 +50,31 @@ EXPORTS = [
     "milc_hip_dev_block_dslash_bench", "milc_hip_dev_block_dslash", "milc_hip_dev_multishift_block",
     "milc_hip_dev_block_describe",
     "milc_hip_dev_multishift", "milc_hip_dev_hmc_step", "milc_hip_hisq_force", "milc_hip_eo_fermion_force_multi",
-    "milc_hip_set_force_form", "milc_hip_gauge_force", "milc_hip_gauge_action", "milc_hip_update_u", "milc_hip_mom_action", "milc_hip_dev_load_thin_links", "milc_hip_dev_read_thin_links", "milc_hip_dev_dslash", "milc_hip_set_link_compression", "milc_hip_link_compression", "milc_hip_dev_hisq_force",
+    "milc_hip_set_force_form", "milc_hip_dev_rhmc_trajectory", "milc_hip_gauge_force", "milc_hip_gauge_action", "milc_hip_update_u", "milc_hip_mom_action", "milc_hip_dev_load_thin_links", "milc_hip_dev_read_thin_links", "milc_hip_dev_dslash", "milc_hip_set_link_compression", "milc_hip_link_compression", "milc_hip_dev_hisq_force",
 ]
+
+
+class Rational(ctypes.Structure):
+    """MILC's params_ratfunc: R(x) = res[0] + sum_k res[k] / (x + pole[k])"""
+    _fields_ = [("order", ctypes.c_int), ("res", ctypes.c_double * 33), ("pole", ctypes.c_double * 33)]
+
+    @classmethod
+    def make(cls, norm, residues, poles):
+        r = cls()
+        r.order = len(residues)
+        r.res[0] = norm
+        for k, (a, b) in enumerate(zip(residues, poles)):
+            r.res[k + 1] = a
+            r.pole[k + 1] = b
+        return r
+
+
+class RhmcParams(ctypes.Structure):
+    _fields_ = [("steps", ctypes.c_int), ("integrator", ctypes.c_int), ("epsilon", ctypes.c_double),
+                ("beta", ctypes.c_double), ("loop_coeff", ctypes.c_double * 3), ("mass", ctypes.c_double),
+                ("md", Rational), ("gr", Rational), ("fa", Rational), ("resid_md", ctypes.c_double),
+                ("resid_gr", ctypes.c_double), ("resid_fa", ctypes.c_double), ("mixed_md", ctypes.c_int),
+                ("action_from_md", ctypes.c_int), ("accept", ctypes.c_int), ("seed", ctypes.c_ulonglong)]
 
 
 def load_library(path=LIB_PATH):
@@ -105,6 +128,7 @@ def load_library(path=LIB_PATH):
     L.milc_hip_hisq_force.argtypes = [I, I, I, D, ctypes.POINTER(PD), ctypes.POINTER(P), PD, PD, P, P, P, P]
     L.milc_hip_eo_fermion_force_multi.argtypes = [I, D, PD, ctypes.POINTER(P), I, P, P]
     L.milc_hip_set_force_form.argtypes = [I]
+    L.milc_hip_dev_rhmc_trajectory.argtypes = [ctypes.POINTER(RhmcParams), ctypes.POINTER(D)]
     L.milc_hip_gauge_force.argtypes = [I, I, ctypes.POINTER(D), D, P, I, P]
     L.milc_hip_gauge_action.argtypes = [I, I, ctypes.POINTER(D), P, I, ctypes.POINTER(D)]
     L.milc_hip_update_u.argtypes = [I, D, P, P, I]
@@ -415,6 +439,22 @@ class Lattice:
         _check(self.lib.milc_hip_mom_action(2, _p(np.ascontiguousarray(mom, dtype=np.float64)), ctypes.byref(a)),
                "milc_hip_mom_action")
         return a.value
+
+    def dev_rhmc_trajectory(self, steps, epsilon, beta, loop_coeff, mass, md, gr, fa, integrator=1,
+                            resid_md=1e-9, resid_gr=1e-10, resid_fa=1e-10, mixed_md=1, action_from_md=False,
+                            accept=1, seed=1):
+        """one RHMC trajectory on the device links (update(), ks_imp_rhmc/update_rhmc.c)"""
+        self._activate()
+        p = RhmcParams(steps=steps, integrator=integrator, epsilon=epsilon, beta=beta, mass=mass, md=md, gr=gr,
+                       fa=fa, resid_md=resid_md, resid_gr=resid_gr, resid_fa=resid_fa, mixed_md=int(mixed_md),
+                       action_from_md=int(bool(action_from_md)), accept=accept, seed=seed)
+        for k in range(3):
+            p.loop_coeff[k] = loop_coeff[k]
+        st = (ctypes.c_double * 16)()
+        _check(self.lib.milc_hip_dev_rhmc_trajectory(ctypes.byref(p), st), "milc_hip_dev_rhmc_trajectory")
+        keys = ["g0", "h0", "f0", "g1", "h1", "f1", "dH", "accepted", "seconds", "md_iters", "fermion_forces",
+                "fermion_force_seconds", "gauge_force_seconds", "link_seconds", "action_seconds", "gr_iters"]
+        return dict(zip(keys, list(st)))
 
     def set_force_form(self, form):
         """0: path-table gather; 1 (default): staple form where exact for any
