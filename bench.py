@@ -25,6 +25,8 @@ Usage: python bench.py [--gpus 1] [--steps 10] [--warmup 1] [--no-cpu]
            12 terms from resident vectors; prints its own JSON line)
        python bench.py --workload io   (lattice files at 32^3 x 64: v5 and SciDAC
            write / read throughput, file -> device HISQ links)
+       python bench.py --workload trajectory   (one RHMC trajectory at 48^3 x 96 with the
+           rational functions of MILC's HISQ RHMC test; FP64 dH)
        --no-sequential (multishift: leave out the one-source comparison solves)
 """
 import argparse
@@ -440,6 +442,68 @@ def force_workload(args):
     print(json.dumps(out), flush=True)
 
 
+TRAJ_DIMS = (48, 48, 48, 96)
+TRAJ_STEPS = 4         # Omelyan: 2 double steps, 4 gauge + fermion force evaluations
+TRAJ_EPS = 0.02
+TRAJ_BETA, TRAJ_U0, TRAJ_NF = 7.4, 0.89, 3   # the couplings of MILC's su3_rhmc_hisq test
+
+
+def trajectory_workload(args):
+    """One RHMC trajectory on the device at 48^3 x 96 with the first
+    pseudofermion's rational functions of MILC's HISQ RHMC test (MD order 9,
+    GR and FA order 11; masses 0.0018 / 0.018 / 0.2 folded into the poles):
+    Gaussian momenta, phi = R_GR chi, FP64 actions, Omelyan integration with
+    the gauge force and the HISQ fermion force (mixed-precision MD solves),
+    links rebuilt after every update, FP64 dH.  Each timed trajectory starts
+    from the same links (the evolved links are restored)."""
+    import milc_qcd_amd as mq
+    rats = mq.load_rhmc_params(os.path.join(ROOT, "tests", "golden", "rationals.sample.su3_rhmc_hisq"))[0]
+    coeff = mq.gauge_loop_coeffs(TRAJ_U0, TRAJ_NF, True)
+    G = mq.Lattice(TRAJ_DIMS)
+    G.dev_gauge_hisq(HMC_GAUGE_SEED, 0.4)
+    runs = []
+    for i in range(args.warmup + args.steps):
+        st = G.dev_rhmc_trajectory(TRAJ_STEPS, TRAJ_EPS, TRAJ_BETA, coeff, 0.0, rats["MD"], rats["GR"], rats["FA"],
+                                   integrator=1, resid_md=1e-8, resid_gr=1e-10, resid_fa=1e-10, mixed_md=1,
+                                   accept=2, seed=100 + i)
+        log(f"trajectory {i}: {st}")
+        if i >= args.warmup:
+            runs.append(st)
+    G.close()
+    t = sum(r["seconds"] for r in runs) / len(runs)
+    last = runs[-1]
+    out = {
+        "metric": "RHMC trajectory on one GPU (48^3x96 HISQ, Omelyan, 4 force evaluations, FP64 dH)",
+        "value": round(t, 3),
+        "unit": "s/trajectory",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t * 1e3, 1),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 actions, forces and link updates; f32/16-bit mixed MD solves",
+        "data": "synthetic",
+        "config": {"workload": "rhmc_trajectory_48x48x48x96_omelyan4", "lattice": list(TRAJ_DIMS),
+                   "md_steps": TRAJ_STEPS, "epsilon": TRAJ_EPS, "beta": TRAJ_BETA, "u0": TRAJ_U0,
+                   "rationals": "tests/golden/rationals.sample.su3_rhmc_hisq, pseudofermion 0",
+                   "gauge_seed": HMC_GAUGE_SEED},
+        "dH": [r["dH"] for r in runs],
+        "actions_start": {"gauge": last["g0"], "momentum": last["h0"], "fermion": last["f0"]},
+        "per_force_evaluation_s": round(sum(r["fermion_force_seconds"] + r["gauge_force_seconds"] for r in runs)
+                                        / sum(r["fermion_forces"] for r in runs), 4),
+        "fermion_force_s": round(sum(r["fermion_force_seconds"] for r in runs) / len(runs), 3),
+        "gauge_force_s": round(sum(r["gauge_force_seconds"] for r in runs) / len(runs), 3),
+        "link_update_rebuild_s": round(sum(r["link_seconds"] for r in runs) / len(runs), 3),
+        "action_s": round(sum(r["action_seconds"] for r in runs) / len(runs), 3),
+        "md_solver_iters": last["md_iters"],
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def cpu_io_baseline(path, nbytes):
     """the FP32 SciDAC file read by the numpy oracle reader (LIME parse, CRC-32
     checksums, byte swap), one pass, on this host"""
@@ -537,7 +601,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--kernel-only", action="store_true",
                     help="only the Dslash and copy bursts (for rocprofv3 counter runs)")
-    ap.add_argument("--workload", choices=["cg", "multishift", "hmc", "force", "io"], default="cg")
+    ap.add_argument("--workload", choices=["cg", "multishift", "hmc", "force", "io", "trajectory"], default="cg")
     ap.add_argument("--no-sequential", action="store_true",
                     help="multishift: leave out the 16 one-source comparison solves")
     args = ap.parse_args()
@@ -556,6 +620,9 @@ def main():
         return
     if args.workload == "io":
         io_workload(args)
+        return
+    if args.workload == "trajectory":
+        trajectory_workload(args)
         return
 
     import milc_qcd_amd as mq

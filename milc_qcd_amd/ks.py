@@ -69,6 +69,45 @@ class Rational(ctypes.Structure):
         return r
 
 
+def load_rhmc_params(path):
+    """MILC's rational-function parameter file (ks_imp_rhmc/load_rhmc_params.c):
+    one dict per pseudofermion, {"MD"|"GR"|"FA": Rational}, res[0] = constant"""
+    out, cur, raw = [], None, {}
+    for line in open(path):
+        t = line.split()
+        if not t or t[0].startswith("#"):
+            continue
+        for kind in ("MD", "GR", "FA"):
+            if t[0] == "order_" + kind:
+                if kind == "MD":
+                    raw = {}
+                    out.append(raw)
+                raw[kind] = {"order": int(t[1]), "res": [], "pole": []}
+            elif t[0] == "res_" + kind:
+                raw[kind]["res"].append(float(t[1]))
+            elif t[0] == "pole_" + kind:
+                raw[kind]["pole"].append(float(t[1]))
+    for raw in out:
+        for kind, v in raw.items():
+            if len(v["res"]) != v["order"] + 1 or len(v["pole"]) != v["order"] + 1:
+                raise ValueError(f"{path}: {kind} has order {v['order']} but {len(v['res'])} residues")
+            raw[kind] = Rational.make(v["res"][0], v["res"][1:], v["pole"][1:])
+    return out
+
+
+def gauge_loop_coeffs(u0, nf, hisq=True):
+    """(plaquette, 1x2 rectangle, 1x1x1 parallelogram) coefficients of the
+    Symanzik 1-loop gauge action with HISQ (nf dynamical flavours) or asqtad
+    quark loops (generic/imp_actions/symanzik_1loop_hisq_action.h,
+    symanzik_1loop_action.h)"""
+    import math
+    lg = math.log(u0)
+    if hisq:
+        return [1.0, -1.0 / (20.0 * u0 * u0) * (1.0 - (0.6264 - 1.1746 * nf) * lg),
+                1.0 / (u0 * u0) * (0.0433 - 0.0156 * nf) * lg]
+    return [1.0, -1.0 / (20.0 * u0 * u0) * (1.0 - 0.6264 * lg), 1.0 / (u0 * u0) * 0.04335 * lg]
+
+
 class RhmcParams(ctypes.Structure):
     _fields_ = [("steps", ctypes.c_int), ("integrator", ctypes.c_int), ("epsilon", ctypes.c_double),
                 ("beta", ctypes.c_double), ("loop_coeff", ctypes.c_double * 3), ("mass", ctypes.c_double),

@@ -154,3 +154,15 @@ def test_pure_gauge_trajectory_energy_and_reversibility(small):
     U1, P1 = leapfrog(L, c, beta, U, P0, 0.02, 10)
     U2, P2 = leapfrog(L, c, beta, U1, -P1, 0.02, 10)
     assert np.abs(U2 - U).max() < 1e-11 and np.abs(P2 + P0).max() < 1e-11
+
+
+def test_package_mirrors_match_oracle():
+    """the package's host-side coefficient formula and rational-file reader"""
+    import milc_qcd_amd as mq
+    for u0, nf, hisq in ((0.89, 3, True), (0.86, 4, True), (0.87, 0, False)):
+        np.testing.assert_allclose(mq.gauge_loop_coeffs(u0, nf, hisq), ko.gauge_coeffs(u0, nf, hisq), rtol=1e-15)
+    rats = mq.load_rhmc_params(os.path.join(HERE, "golden", "rationals.sample.su3_rhmc_hisq"))
+    assert len(rats) == 4
+    md = rats[0]["MD"]
+    assert md.order == 9 and md.res[0] == 1.0000007752683888e+00 and md.pole[9] == 9.8983752589959229e-02
+    assert rats[0]["GR"].order == 11 and rats[0]["FA"].order == 11
