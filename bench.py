@@ -446,6 +446,7 @@ TRAJ_DIMS = (48, 48, 48, 96)
 TRAJ_STEPS = 4         # Omelyan: 2 double steps, 4 gauge + fermion force evaluations
 TRAJ_EPS = 0.02
 TRAJ_BETA, TRAJ_U0, TRAJ_NF = 7.4, 0.89, 3   # the couplings of MILC's su3_rhmc_hisq test
+TRAJ_GAUGE_SCALE = 0.1  # U = exp(0.1 i H): a smooth start, forces of a physical size
 
 
 def trajectory_workload(args):
@@ -460,7 +461,7 @@ def trajectory_workload(args):
     rats = mq.load_rhmc_params(os.path.join(ROOT, "tests", "golden", "rationals.sample.su3_rhmc_hisq"))[0]
     coeff = mq.gauge_loop_coeffs(TRAJ_U0, TRAJ_NF, True)
     G = mq.Lattice(TRAJ_DIMS)
-    G.dev_gauge_hisq(HMC_GAUGE_SEED, 0.4)
+    G.dev_gauge_hisq(HMC_GAUGE_SEED, TRAJ_GAUGE_SCALE)
     runs = []
     for i in range(args.warmup + args.steps):
         st = G.dev_rhmc_trajectory(TRAJ_STEPS, TRAJ_EPS, TRAJ_BETA, coeff, 0.0, rats["MD"], rats["GR"], rats["FA"],
@@ -488,7 +489,7 @@ def trajectory_workload(args):
         "config": {"workload": "rhmc_trajectory_48x48x48x96_omelyan4", "lattice": list(TRAJ_DIMS),
                    "md_steps": TRAJ_STEPS, "epsilon": TRAJ_EPS, "beta": TRAJ_BETA, "u0": TRAJ_U0,
                    "rationals": "tests/golden/rationals.sample.su3_rhmc_hisq, pseudofermion 0",
-                   "gauge_seed": HMC_GAUGE_SEED},
+                   "gauge_seed": HMC_GAUGE_SEED, "gauge_scale": TRAJ_GAUGE_SCALE},
         "dH": [r["dH"] for r in runs],
         "actions_start": {"gauge": last["g0"], "momentum": last["h0"], "fermion": last["f0"]},
         "per_force_evaluation_s": round(sum(r["fermion_force_seconds"] + r["gauge_force_seconds"] for r in runs)

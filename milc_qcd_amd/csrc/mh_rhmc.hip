@@ -297,6 +297,8 @@ struct Traj {
     if (rc) return rc;
     double S;
     int it = 0;
+    // the sloppy links of the mixed solve were dropped by the last link update
+    if (p.mixed_md && (rc = ensure_links(P_F32))) return rc;
     rc = rhmc_force_level0(phi, p.md.order, offs_md + 1, p.md.res + 1, p.resid_md, p.mixed_md, &S, o1, o3, &it);
     if (!rc) rc = hisq_aux_dev(c.thin, V, W);
     if (!rc) rc = hisq_force_from_oprod(c.thin, V, W, o1, o3, kHisqLevel2, kHisqLevel1, 2.0 * e, kForceFilter, mtmp);

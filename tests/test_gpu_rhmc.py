@@ -90,3 +90,16 @@ def test_energy_conservation_and_restore():
         Uc = U1[..., 0] + 1j * U1[..., 1]
         dev = np.abs(np.conj(np.swapaxes(Uc, -1, -2)) @ Uc - np.eye(3)).max()
         assert dev < 1e-12
+
+
+def test_mixed_md_solves_track_fp64():
+    """mixed-precision MD solves (FP32 multi-shift + 16-bit refinement, links
+    rebuilt each step) give the FP64-solve trajectory's dH to the solver
+    tolerance"""
+    with mq.Lattice(DIMS) as G:
+        G.dev_gauge_hisq(1200, 0.4)
+        a = traj(G, 4, 0.05)
+        b = G.dev_rhmc_trajectory(4, 0.05, BETA, COEFF, MASS, rat(MD), rat(GR), rat(MD), integrator=0,
+                                  resid_md=1e-10, resid_gr=1e-12, resid_fa=1e-12, mixed_md=1, action_from_md=True,
+                                  accept=2, seed=7)
+    assert abs(a["dH"] - b["dH"]) < 1e-5 * max(1.0, abs(a["dH"]))
