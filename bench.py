@@ -443,30 +443,34 @@ def force_workload(args):
 
 
 TRAJ_DIMS = (48, 48, 48, 96)
-TRAJ_STEPS = 4         # Omelyan: 2 double steps, 4 gauge + fermion force evaluations
+TRAJ_STEPS = 4         # INT_3G1F: 2 step pairs, 4 fermion and 12 gauge kicks
 TRAJ_EPS = 0.02
 TRAJ_BETA, TRAJ_U0, TRAJ_NF = 7.4, 0.89, 3   # the couplings of MILC's su3_rhmc_hisq test
 TRAJ_GAUGE_SCALE = 0.1  # U = exp(0.1 i H): a smooth start, forces of a physical size
+# cgresid_md_fa_gr of MILC's su3_rhmc_hisq.1 input, per pseudofermion (MD, GR, FA)
+TRAJ_RESID = [(5e-4, 1e-4, 1e-4)] + [(5e-6, 1e-6, 1e-6)] * 3
 
 
 def trajectory_workload(args):
-    """One RHMC trajectory on the device at 48^3 x 96 with the first
-    pseudofermion's rational functions of MILC's HISQ RHMC test (MD order 9,
-    GR and FA order 11; masses 0.0018 / 0.018 / 0.2 folded into the poles):
-    Gaussian momenta, phi = R_GR chi, FP64 actions, Omelyan integration with
-    the gauge force and the HISQ fermion force (mixed-precision MD solves),
-    links rebuilt after every update, FP64 dH.  Each timed trajectory starts
-    from the same links (the evolved links are restored)."""
+    """One RHMC trajectory on the device at 48^3 x 96 as MILC's HISQ build runs
+    it (su3_rhmc_hisq: INT_3G1F, four pseudofermions with the rational
+    functions of its test, masses 0.0018 / 0.018 folded into the poles, its
+    solver residuals): Gaussian momenta, phi_i = R_GR chi_i, actions, the
+    3G1F integrator (gauge force 3x per fermion force, HISQ fermion force
+    from all 30 MD terms, mixed-precision MD solves), fat and Naik links
+    rebuilt when used, reunitarisation per step pair, FP64 dH.  Each timed
+    trajectory starts from the same links (the evolved links are restored)."""
     import milc_qcd_amd as mq
-    rats = mq.load_rhmc_params(os.path.join(ROOT, "tests", "golden", "rationals.sample.su3_rhmc_hisq"))[0]
+    pfs = mq.pseudofermions_from_file(os.path.join(ROOT, "tests", "golden", "rationals.sample.su3_rhmc_hisq"),
+                                      [r[0] for r in TRAJ_RESID], [r[1] for r in TRAJ_RESID],
+                                      [r[2] for r in TRAJ_RESID])
     coeff = mq.gauge_loop_coeffs(TRAJ_U0, TRAJ_NF, True)
     G = mq.Lattice(TRAJ_DIMS)
     G.dev_gauge_hisq(HMC_GAUGE_SEED, TRAJ_GAUGE_SCALE)
     runs = []
     for i in range(args.warmup + args.steps):
-        st = G.dev_rhmc_trajectory(TRAJ_STEPS, TRAJ_EPS, TRAJ_BETA, coeff, 0.0, rats["MD"], rats["GR"], rats["FA"],
-                                   integrator=1, resid_md=1e-8, resid_gr=1e-10, resid_fa=1e-10, mixed_md=1,
-                                   accept=2, seed=100 + i)
+        st = G.dev_rhmc_trajectory(TRAJ_STEPS, TRAJ_EPS, TRAJ_BETA, coeff, pfs, integrator=mq.INT_3G1F,
+                                   mixed_md=1, accept=2, seed=100 + i)
         log(f"trajectory {i}: {st}")
         if i >= args.warmup:
             runs.append(st)
@@ -474,7 +478,7 @@ def trajectory_workload(args):
     t = sum(r["seconds"] for r in runs) / len(runs)
     last = runs[-1]
     out = {
-        "metric": "RHMC trajectory on one GPU (48^3x96 HISQ, Omelyan, 4 force evaluations, FP64 dH)",
+        "metric": "RHMC trajectory on one GPU (48^3x96 HISQ, 3G1F, 4 pseudofermions, FP64 dH)",
         "value": round(t, 3),
         "unit": "s/trajectory",
         "n_gpus": 1,
@@ -486,14 +490,17 @@ def trajectory_workload(args):
         "vs_baseline": None,
         "dtype": "f64 actions, forces and link updates; f32/16-bit mixed MD solves",
         "data": "synthetic",
-        "config": {"workload": "rhmc_trajectory_48x48x48x96_omelyan4", "lattice": list(TRAJ_DIMS),
-                   "md_steps": TRAJ_STEPS, "epsilon": TRAJ_EPS, "beta": TRAJ_BETA, "u0": TRAJ_U0,
-                   "rationals": "tests/golden/rationals.sample.su3_rhmc_hisq, pseudofermion 0",
-                   "gauge_seed": HMC_GAUGE_SEED, "gauge_scale": TRAJ_GAUGE_SCALE},
+        "config": {"workload": "rhmc_trajectory_48x48x48x96_3g1f4", "lattice": list(TRAJ_DIMS),
+                   "integrator": "INT_3G1F", "md_steps": TRAJ_STEPS, "epsilon": TRAJ_EPS, "beta": TRAJ_BETA,
+                   "u0": TRAJ_U0, "rationals": "tests/golden/rationals.sample.su3_rhmc_hisq, 4 pseudofermions",
+                   "resid_md_gr_fa": TRAJ_RESID, "gauge_seed": HMC_GAUGE_SEED, "gauge_scale": TRAJ_GAUGE_SCALE},
         "dH": [r["dH"] for r in runs],
         "actions_start": {"gauge": last["g0"], "momentum": last["h0"], "fermion": last["f0"]},
-        "per_force_evaluation_s": round(sum(r["fermion_force_seconds"] + r["gauge_force_seconds"] for r in runs)
-                                        / sum(r["fermion_forces"] for r in runs), 4),
+        "per_fermion_force_s": round(sum(r["fermion_force_seconds"] for r in runs)
+                                     / sum(r["fermion_forces"] for r in runs), 4),
+        "per_gauge_force_s": round(sum(r["gauge_force_seconds"] for r in runs)
+                                   / sum(r["gauge_forces"] for r in runs), 4),
+        "link_rebuilds": last["link_rebuilds"],
         "fermion_force_s": round(sum(r["fermion_force_seconds"] for r in runs) / len(runs), 3),
         "gauge_force_s": round(sum(r["gauge_force_seconds"] for r in runs) / len(runs), 3),
         "link_update_rebuild_s": round(sum(r["link_seconds"] for r in runs) / len(runs), 3),

@@ -172,6 +172,15 @@ int milc_hip_gauge_action(int precision, int num_loop_types, const double* loop_
 /* links <- exp(eps P) links to eighth order (Horner), in place: replaces
  * qudaUpdateUPhased / update_u (ks_imp_rhmc/update_u.c:14-34, :66-111). */
 int milc_hip_update_u(int precision, double eps, void* links, const void* momentum, int phases_in);
+/* reunitarize_ks (generic_ks/reunitarize_ks.c:9-29): every link made SU(3) by
+ * reunit_su3 (generic/reunitarize2.c:133-205; rows 0 and 1 by Gram-Schmidt,
+ * row 2 = conj(row0 x row1)), KS phases taken out and put back when
+ * phases_in; replaces qudaUnitarizeSU3Phased (reunitarize2.c:210-227) and
+ * reunitarize_cpu (:231).  *errors = the number of deviations at or above
+ * tolerance (MILC's TOLERANCE is 1e-4; it stops above 100), *max_deviation the
+ * largest. */
+int milc_hip_reunitarize(int precision, double tolerance, void* links, int phases_in, double* max_deviation,
+                         int* errors);
 /* *action = sum over links of |P|^2/2 - 4: hmom_action (ks_imp_rhmc/d_action_rhmc.c:86-101). */
 int milc_hip_mom_action(int precision, const void* momentum, double* action);
 
@@ -184,34 +193,71 @@ typedef struct {
   double pole[33];
 } milc_hip_rational;
 
+#define MILC_HIP_MAX_PSEUDO 8
+/* one pseudofermion (rparam[i] and the cgresid_md_fa_gr line of MILC's input) */
 typedef struct {
-  int steps;             /* MD steps (force evaluations); even for Omelyan */
-  int integrator;        /* 0 INT_LEAPFROG, 1 INT_OMELYAN (lambda = 0.8) */
-  double epsilon;        /* step size */
+  double mass;                          /* 4 mass^2 is added to every pole; 0 when the
+                                           poles already hold it, as in MILC's files */
+  milc_hip_rational md, gr, fa;         /* force, heatbath, action */
+  double resid_md, resid_gr, resid_fa;  /* solver targets |r|/|b| (MILC: sqrt(rsqmin)) */
+} milc_hip_pseudofermion;
+
+typedef struct {
+  int steps;             /* steps_per_trajectory (fermion-force steps); even except for
+                            the leapfrog */
+  int integrator;        /* MILC's int_alg_t (ks_imp_rhmc/ks_imp_includes.h:58-60):
+                            0 INT_LEAPFROG, 1 INT_OMELYAN (lambda = 0.8), 4 INT_2G1F,
+                            5 INT_3G1F (the HISQ build's), 6 INT_5G1F, 7 INT_6G1F
+                            (alpha = beta = 0.1) */
+  double epsilon;        /* microcanonical_time_step */
   double beta;           /* gauge coupling */
   double loop_coeff[3];  /* Symanzik loop coefficients (1x1, 1x2, 1x1x1) */
-  double mass;           /* HISQ quark mass of the pseudofermion */
-  milc_hip_rational md, gr, fa;  /* force, heatbath, action */
-  double resid_md, resid_gr, resid_fa;  /* solver true-residual targets */
+  int n_pseudo;          /* 1..MILC_HIP_MAX_PSEUDO */
+  milc_hip_pseudofermion pf[MILC_HIP_MAX_PSEUDO];
   int mixed_md;          /* MD solves: FP32 multi-shift + 16-bit mixed refinement */
   int action_from_md;    /* 1: fermion action phi^dag R_MD phi (the MD force's own
                             action, for integrator tests); 0: |R_FA phi|^2 */
   int accept;            /* 0 keep the new links, 1 Metropolis, 2 restore the old */
-  unsigned long long seed;  /* momenta, pseudofermion noise, accept draw */
+  int site_rng;          /* 1: MILC's per-site generators (milc_hip_dev_seed_rng), which
+                            continue from call to call, and MILC's node generator for
+                            the Metropolis draw; 0: this library's streams from seed */
+  unsigned long long seed;  /* site_rng = 0: momenta, pseudofermion noise, accept draw */
 } milc_hip_rhmc_params;
 
-/* One trajectory of update() (ks_imp_rhmc/update_rhmc.c:157-345) on the
+/* One trajectory of update() (ks_imp_rhmc/update_rhmc.c:157-745) on the
  * current device links (milc_hip_dev_gauge_hisq or milc_hip_dev_load_thin_links):
- * Gaussian momenta (ranmom), phi = R_GR chi (grsource_imp_rhmc), the actions
- * (d_action_rhmc: beta/3 gauge loops, momenta, fermion), the integrator over
- * update_u and update_h_rhmc (gauge force + HISQ fermion force, fat and Naik
- * links rebuilt after every link update), the actions again, accept/reject.
- * The links are not reunitarised between steps (the eighth-order exponential
- * keeps U(3) to its truncation).  stats[0..15] = g0, h0, f0, g1, h1, f1, dH,
- * accepted, seconds, MD solver iterations, fermion forces, seconds in fermion
- * forces, in gauge forces, in link updates and rebuilds, in actions, heatbath
- * solver iterations.  Random streams are this library's own. */
+ * Gaussian momenta (ranmom), phi_i = R_GR,i chi_i for each pseudofermion
+ * (grsource_imp_rhmc), the actions (d_action_rhmc: beta/3 gauge loops, momenta,
+ * sum over pseudofermions of the fermion action), the integrator's link
+ * updates (update_u) and kicks (update_h_gauge: gauge force; update_h_fermion:
+ * every pseudofermion's MD solve, then one HISQ force from all terms), with the
+ * fat and Naik links rebuilt when a fermion kick or action needs them and the
+ * links reunitarised (reunitarize_ks, tolerance 1e-4) after each leapfrog step
+ * or step pair, the actions again, accept/reject.
+ * stats: 32 doubles = g0, h0, f0, g1, h1, f1, dH, accepted, seconds, MD solver
+ * iterations, fermion kicks, seconds in fermion kicks, in gauge kicks, in link
+ * updates/rebuilds/reunitarisation, in actions, heatbath solver iterations,
+ * gauge kicks, largest reunitarisation deviation, reunitarisation errors, link
+ * rebuilds, [20..27] |chi_i|^2 of the heatbath noise (MILC's "GRSOURCE: sum";
+ * with site_rng and single-precision generators summed as MILC's
+ * single-precision build does), [28] action solver iterations, [29..31] 0.
+ * Returns -6 when a reunitarisation finds more than 100 deviations (MILC
+ * terminates there). */
 int milc_hip_dev_rhmc_trajectory(const milc_hip_rhmc_params* p, double* stats);
+
+/* MILC's per-site random number generators on the device: seeded as
+ * initialize_site_prn_from_seed(iseed) (generic/ranstuff.c:117-131, index =
+ * lexicographic site index) and the node generator as initialize_prn(&node_prn,
+ * iseed, volume) (ks_imp_rhmc/setup.c:140).  precision 1 rounds the Gaussian
+ * draws as a MILC_PRECISION=1 build, 2 as a double-precision build. */
+int milc_hip_dev_seed_rng(int iseed, int precision);
+/* ranmom() (generic/ranmom.c:10-41) from the site generators: momentum =
+ * anti_hermitmat[4 V] as 10 doubles each, node_index order. */
+int milc_hip_dev_ranmom(double* momentum);
+/* three complex Gaussians per site of `parity` from the site generators
+ * (grsource_imp_rhmc, ks_imp_rhmc/grsource_rhmc.c:30-37) into device slot
+ * `slot`; *sum = their |.|^2 summed over the sites ("GRSOURCE: sum"). */
+int milc_hip_dev_gaussian_source(int parity, int slot, double* sum);
 
 /* 16-bit sloppy long links in the two-row form (on = 1, the default): a long
  * link that is a real multiple of a U(3) matrix, L = c U (HISQ and asqtad

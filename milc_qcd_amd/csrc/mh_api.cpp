@@ -184,6 +184,7 @@ int milc_hip_finalize(void) {
   fn_table_release();
   gauge_table_release();
   batch_release();
+  prn_release();
   if (c.thin_valid) free_links(c.thin);
   c.thin_valid = false;
   dev_free(c.ws.partials);
@@ -1743,6 +1744,62 @@ extern "C" int milc_hip_update_u(int precision, double eps, void* links, const v
   return 0;
 }
 
+extern "C" int milc_hip_reunitarize(int precision, double tolerance, void* links, int phases_in,
+                                    double* max_deviation, int* errors) {
+  Context& c = ctx();
+  if (!c.initialised) return -1;
+  if ((precision != 1 && precision != 2) || !links || !(tolerance > 0)) return -2;
+  GaugeBuffers b;
+  int rc, nerr = 0;
+  double md = 0;
+  if ((rc = b.alloc(false)) || (rc = host_links_in(precision, links, b.U, b.stage)) ||
+      (rc = reunitarize_dev(b.U, phases_in ? 1 : 0, tolerance, &nerr, &md)) ||
+      (rc = links_to_milc_dev(b.U, P_F64, b.stage)))
+    return rc;
+  size_t n = (size_t)2 * c.Vh * 4 * 18;
+  std::vector<double> h(n);
+  MH_CHECK(hipMemcpyAsync(h.data(), b.stage, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  MH_CHECK(hipStreamSynchronize(c.stream));
+  if (precision == 2) memcpy(links, h.data(), n * sizeof(double));
+  else
+    for (size_t i = 0; i < n; i++) ((float*)links)[i] = (float)h[i];
+  if (max_deviation) *max_deviation = md;
+  if (errors) *errors = nerr;
+  return 0;
+}
+
+extern "C" int milc_hip_dev_seed_rng(int iseed, int precision) {
+  Context& c = ctx();
+  if (!c.initialised) return -1;
+  if (precision != 1 && precision != 2) return -2;
+  return prn_seed_dev(iseed, precision == 1);
+}
+
+extern "C" int milc_hip_dev_ranmom(double* momentum) {
+  Context& c = ctx();
+  if (!c.initialised) return -1;
+  if (!momentum) return -2;
+  size_t n = (size_t)2 * c.Vh * 4 * 10;
+  double* mom = nullptr;
+  int rc;
+  if ((rc = dev_alloc((void**)&mom, n * sizeof(double)))) return rc;
+  if (!(rc = ranmom_prn_dev(mom))) {
+    rc = hipMemcpyAsync(momentum, mom, n * sizeof(double), hipMemcpyDeviceToHost, c.stream) == hipSuccess ? 0 : -1;
+    (void)hipStreamSynchronize(c.stream);
+  }
+  dev_free(mom);
+  return rc;
+}
+
+extern "C" int milc_hip_dev_gaussian_source(int parity, int slot, double* sum) {
+  Context& c = ctx();
+  if (!c.initialised || slot < 0 || slot >= kSlots) return -1;
+  if (parity != EVEN && parity != ODD) return -2;
+  int rc;
+  if (!g_slots[slot].data && (rc = alloc_vec(g_slots[slot], P_F64, c.stride))) return rc;
+  return gaussian_prn_dev(pbit(parity), g_slots[slot], sum);
+}
+
 extern "C" int milc_hip_mom_action(int precision, const void* momentum, double* action) {
   Context& c = ctx();
   if (!c.initialised) return -1;
@@ -1759,32 +1816,36 @@ extern "C" int milc_hip_dev_rhmc_trajectory(const milc_hip_rhmc_params* p, doubl
   Context& c = ctx();
   if (!c.initialised) return -1;
   if (!p || !stats) return -2;
+  if (p->n_pseudo <= 0 || p->n_pseudo > MILC_HIP_MAX_PSEUDO) return -2;
+  static_assert(MILC_HIP_MAX_PSEUDO <= RHMC_MAX_PSEUDO, "pseudofermion limits");
   RhmcParams q{};
   q.steps = p->steps;
   q.integrator = p->integrator;
   q.epsilon = p->epsilon;
   q.beta = p->beta;
   for (int k = 0; k < 3; k++) q.loop_coeff[k] = p->loop_coeff[k];
-  q.mass = p->mass;
-  const milc_hip_rational* src[3] = {&p->md, &p->gr, &p->fa};
-  RhmcRat* dst[3] = {&q.md, &q.gr, &q.fa};
-  for (int r = 0; r < 3; r++) {
-    dst[r]->order = src[r]->order;
-    memcpy(dst[r]->res, src[r]->res, sizeof dst[r]->res);
-    memcpy(dst[r]->pole, src[r]->pole, sizeof dst[r]->pole);
+  q.n_pseudo = p->n_pseudo;
+  for (int i = 0; i < p->n_pseudo; i++) {
+    const milc_hip_pseudofermion& a = p->pf[i];
+    RhmcPseudo& b = q.pf[i];
+    b.mass = a.mass;
+    const milc_hip_rational* src[3] = {&a.md, &a.gr, &a.fa};
+    RhmcRat* dst[3] = {&b.md, &b.gr, &b.fa};
+    for (int r = 0; r < 3; r++) {
+      dst[r]->order = src[r]->order;
+      memcpy(dst[r]->res, src[r]->res, sizeof dst[r]->res);
+      memcpy(dst[r]->pole, src[r]->pole, sizeof dst[r]->pole);
+    }
+    b.resid_md = a.resid_md;
+    b.resid_gr = a.resid_gr;
+    b.resid_fa = a.resid_fa;
   }
-  q.resid_md = p->resid_md;
-  q.resid_gr = p->resid_gr;
-  q.resid_fa = p->resid_fa;
   q.mixed_md = p->mixed_md;
   q.action_from_md = p->action_from_md;
   q.accept = p->accept;
+  q.site_rng = p->site_rng;
   q.seed = p->seed;
-  if (q.integrator != 0 && q.integrator != 1) return -2;
-  if (q.mixed_md) {
-    int rc;
-    if ((rc = ensure_links(P_F32))) return rc;
-  }
+  if (q.accept < 0 || q.accept > 2) return -2;
   return rhmc_trajectory(q, stats);
 }
 

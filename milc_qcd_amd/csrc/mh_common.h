@@ -117,6 +117,19 @@ __host__ __device__ __forceinline__ int neighbour(const Geom& g, const Coord& c,
   return cb + d * stride;
 }
 
+// MILC's KS phase of link (c, mu) with antiperiodic time boundary
+// (generic_ks/rephase.c:64-118): eta_x = (-1)^t, eta_y = (-1)^(t+x),
+// eta_z = (-1)^(t+x+y), eta_t = 1, times -1 on the last time slice.
+__device__ __forceinline__ double ks_phase(const Geom& g, const Coord& c, int mu) {
+  double ph[4];
+  ph[3] = 1.0;
+  ph[0] = (c.t % 2 == 1) ? -1.0 : 1.0;
+  ph[1] = (c.x % 2 == 1) ? -ph[0] : ph[0];
+  ph[2] = (c.y % 2 == 1) ? -ph[1] : ph[1];
+  if (c.t == g.X[3] - 1) ph[3] = -ph[3];
+  return ph[mu];
+}
+
 // ---------------------------------------------------------------------------
 // complex arithmetic in registers
 // ---------------------------------------------------------------------------

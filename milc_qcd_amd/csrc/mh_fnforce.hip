@@ -16,6 +16,7 @@
 // wavefront and stays in the scalar/L1 caches.
 #include <algorithm>
 #include <array>
+#include <cstring>
 #include <vector>
 
 #include "mh_internal.h"
@@ -1049,6 +1050,120 @@ int update_u_dev(LinkField& U, const double* mom_dev, double eps) {
   hipLaunchKernelGGL(k_update_u, dim3((c.Vh + 127) / 128, 8), dim3(128), 0, c.stream, c.geom, U, mom_dev, eps);
   MH_CHECK(hipGetLastError());
   return 0;
+}
+
+// reunit_su3 (generic/reunitarize2.c:133-205) on every link, with the KS phase
+// taken out first when the links carry it (reunitarize_ks,
+// generic_ks/reunitarize_ks.c:9-29): row 0 normalised, row 0's component taken
+// out of row 1, row 1 normalised, row 2 = conj(row0 x row1).  The four
+// deviations MILC checks (|row0|^2 - 1, |<row0,row1>|^2, |row1'|^2 - 1,
+// |row2 change|^2) are counted against tol and their maximum kept.
+__global__ void __launch_bounds__(128) k_reunitarize(Geom g, LinkField U, int phases_in, double tol,
+                                                     unsigned int* __restrict__ errors,
+                                                     unsigned long long* __restrict__ maxdev_bits) {
+  int cb = blockIdx.x * blockDim.x + threadIdx.x;
+  int pm = blockIdx.y;
+  if (cb >= g.Vh) return;
+  int pb = pm >> 2, mu = pm & 3;
+  Mat3<double> M;
+  mload(U, pb, mu, cb, M);
+  double ph = phases_in ? ks_phase(g, site_coords(g, cb, pb), mu) : 1.0;
+  double n0 = 0, n1 = 0, dev[4];
+#pragma unroll
+  for (int k = 0; k < 3; k++) n0 += ph * ph * (M.e[0][k].re * M.e[0][k].re + M.e[0][k].im * M.e[0][k].im);
+  dev[0] = fabs(n0 - 1.0);
+  const double s0 = ph / sqrt(n0);
+  Cplx<double> r0[3], r1[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    r0[k].re = s0 * M.e[0][k].re;
+    r0[k].im = s0 * M.e[0][k].im;
+    r1[k].re = ph * M.e[1][k].re;
+    r1[k].im = ph * M.e[1][k].im;
+  }
+  double ore = 0, oim = 0;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    ore += r0[k].re * r1[k].re + r0[k].im * r1[k].im;
+    oim += r0[k].re * r1[k].im - r0[k].im * r1[k].re;
+  }
+  dev[1] = ore * ore + oim * oim;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    r1[k].re -= ore * r0[k].re - oim * r0[k].im;
+    r1[k].im -= ore * r0[k].im + oim * r0[k].re;
+    n1 += r1[k].re * r1[k].re + r1[k].im * r1[k].im;
+  }
+  dev[2] = fabs(n1 - 1.0);
+  const double s1 = 1.0 / sqrt(n1);
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    r1[k].re *= s1;
+    r1[k].im *= s1;
+  }
+  Cplx<double> r2[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const int a = (k + 1) % 3, b = (k + 2) % 3;
+    // conj(r0[a] r1[b] - r0[b] r1[a])
+    double re = r0[a].re * r1[b].re - r0[a].im * r1[b].im - (r0[b].re * r1[a].re - r0[b].im * r1[a].im);
+    double im = r0[a].re * r1[b].im + r0[a].im * r1[b].re - (r0[b].re * r1[a].im + r0[b].im * r1[a].re);
+    r2[k].re = re;
+    r2[k].im = -im;
+  }
+  dev[3] = 0;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    double dr = ph * M.e[2][k].re - r2[k].re, di = ph * M.e[2][k].im - r2[k].im;
+    dev[3] += dr * dr + di * di;
+  }
+  unsigned int nerr = 0;
+  double dmax = 0;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    nerr += dev[k] >= tol;
+    dmax = fmax(dmax, dev[k]);
+  }
+  if (nerr) atomicAdd(errors, nerr);
+  // non-negative doubles order as their bit patterns
+  atomicMax(maxdev_bits, (unsigned long long)__double_as_longlong(dmax));
+  double2* out = (double2*)U.data + (size_t)((pb * 4 + mu) * 9) * U.stride + cb;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    out[(size_t)k * U.stride] = make_double2(ph * r0[k].re, ph * r0[k].im);
+    out[(size_t)(3 + k) * U.stride] = make_double2(ph * r1[k].re, ph * r1[k].im);
+    out[(size_t)(6 + k) * U.stride] = make_double2(ph * r2[k].re, ph * r2[k].im);
+  }
+}
+
+// the two flag words: error count and the bits of the largest deviation
+__global__ void k_reunit_clear(unsigned long long* flags) {
+  flags[0] = 0;
+  flags[1] = 0;
+}
+
+int reunitarize_dev(LinkField& U, int phases_in, double tol, int* errors, double* maxdev) {
+  Context& c = ctx();
+  unsigned long long* flags = nullptr;
+  int rc;
+  if ((rc = dev_alloc((void**)&flags, 2 * sizeof(unsigned long long)))) return rc;
+  unsigned int* err = (unsigned int*)flags;
+  hipLaunchKernelGGL(k_reunit_clear, dim3(1), dim3(1), 0, c.stream, flags);
+  hipLaunchKernelGGL(k_reunitarize, dim3((c.Vh + 127) / 128, 8), dim3(128), 0, c.stream, c.geom, U, phases_in, tol,
+                     err, flags + 1);
+  unsigned long long h[2] = {0, 0};
+  rc = hipGetLastError() == hipSuccess ? 0 : -1;
+  if (!rc)
+    rc = hipMemcpyAsync(h, flags, sizeof h, hipMemcpyDeviceToHost, c.stream) == hipSuccess ? 0 : -1;
+  (void)hipStreamSynchronize(c.stream);
+  dev_free(flags);
+  if (errors) *errors = (int)(unsigned int)(h[0] & 0xffffffffu);
+  if (maxdev) {
+    double d;
+    memcpy(&d, &h[1], sizeof d);
+    *maxdev = d;
+  }
+  return rc;
 }
 
 // per-block partial sums of ahmat_mag_sq - 4 (hmom_action, ks_imp_rhmc/d_action_rhmc.c:86-124)

@@ -162,15 +162,33 @@ struct RhmcRat {
   int order;
   double res[33], pole[33];
 };
-struct RhmcParams {
-  int steps, integrator;  // 0 leapfrog, 1 Omelyan (lambda = 0.8)
-  double epsilon, beta, loop_coeff[3], mass;
+struct RhmcPseudo {
+  double mass;  // 4 mass^2 is added to every pole
   RhmcRat md, gr, fa;
   double resid_md, resid_gr, resid_fa;
-  int mixed_md, action_from_md, accept;
+};
+#define RHMC_MAX_PSEUDO 8
+struct RhmcParams {
+  int steps;
+  int integrator;  // MILC's int_alg_t: 0 leapfrog, 1 Omelyan, 4 2G1F, 5 3G1F, 6 5G1F, 7 6G1F
+  double epsilon, beta, loop_coeff[3];
+  int n_pseudo;
+  RhmcPseudo pf[RHMC_MAX_PSEUDO];
+  int mixed_md, action_from_md, accept, site_rng;
   uint64_t seed;
 };
+// stats: 32 doubles (see milc_hip_dev_rhmc_trajectory)
 int rhmc_trajectory(const RhmcParams& p, double* stats);
+
+// MILC's per-site generators (SITERAND) on the device
+int prn_seed_dev(int iseed, int single);
+bool prn_valid();
+int ranmom_prn_dev(double* mom_dev);
+int gaussian_prn_dev(int pb, VecField& v, double* sum);
+double prn_node_uniform();
+void prn_release();
+// reunit_su3 on every link (KS phases taken out and put back when phases_in)
+int reunitarize_dev(LinkField& U, int phases_in, double tol, int* errors, double* maxdev);
 
 // Symanzik-improved gauge action on links without KS phases (coeff: plaquette,
 // rectangle, parallelogram), its force into packed momenta, the link update and

@@ -170,16 +170,6 @@ __global__ void k_gaussian_vec(Geom g, uint64_t key, int pb, VecField v) {
   store_vec<P_F64>(v, cb, a);
 }
 
-__device__ __forceinline__ double ks_phase(const Geom& g, const Coord& c, int mu) {
-  double ph[4];
-  ph[3] = 1.0;
-  ph[0] = (c.t % 2 == 1) ? -1.0 : 1.0;
-  ph[1] = (c.x % 2 == 1) ? -ph[0] : ph[0];
-  ph[2] = (c.y % 2 == 1) ? -ph[1] : ph[1];
-  if (c.t == g.X[3] - 1) ph[3] = -ph[3];
-  return ph[mu];
-}
-
 __global__ void k_rephase(Geom g, LinkField U) {
   int cb = blockIdx.x * blockDim.x + threadIdx.x;
   int pm = blockIdx.y;

@@ -11,6 +11,7 @@
 //                            grsource_imp_rhmc, d_action_rhmc, the leapfrog
 //                            and Omelyan integrators over update_u /
 //                            update_h_rhmc, d_action_rhmc, Metropolis
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -240,25 +241,88 @@ __global__ void k_axpy_flat(double* __restrict__ y, const double* __restrict__ x
 
 namespace {
 
+constexpr double kReunitTol = 1e-4;   // TOLERANCE, generic/reunitarize2.c:16
+constexpr int kReunitMaxErrors = 100;  // MAXERRCOUNT, generic/reunitarize2.c:17
+
+// One cycle of an integrator (update_rhmc.c:309-630): momentum kicks at times
+// t (units of epsilon from the cycle start), the links carried from kick to
+// kick by update_u, then to the cycle end, then reunitarize_ks.
+enum KickKind { KICK_GAUGE = 0, KICK_FERMION = 1, KICK_BOTH = 2 };
+struct Kick {
+  double t;
+  int kind;
+  double size;  // the kick is update_h_*(size * epsilon)
+};
+struct Schedule {
+  double length = 0;
+  int steps = 0;  // MD steps (fermion forces) per cycle
+  std::vector<Kick> kicks;
+};
+
+// MILC's int_alg_t values (ks_imp_rhmc/ks_imp_includes.h:58-60)
+int make_schedule(int alg, Schedule& s) {
+  s.kicks.clear();
+  if (alg == 0) {  // INT_LEAPFROG: U(e/2) H(e) U(e/2)
+    s.length = 1;
+    s.steps = 1;
+    s.kicks.push_back({0.5, KICK_BOTH, 1.0});
+    return 0;
+  }
+  if (alg == 1) {  // INT_OMELYAN, lambda = 0.8: U(le/2) H(e) U((2-l)e) H(e) U(le/2)
+    const double lam = 0.8;
+    s.length = 2;
+    s.steps = 2;
+    s.kicks.push_back({0.5 * lam, KICK_BOTH, 1.0});
+    s.kicks.push_back({0.5 * lam + (2.0 - lam), KICK_BOTH, 1.0});
+    return 0;
+  }
+  int n = alg == 4 ? 2 : alg == 5 ? 3 : alg == 6 ? 5 : alg == 7 ? 6 : 0;
+  if (!n) return -2;
+  // INT_nG1F (alpha = beta = 0.1): n gauge kicks of epsilon/n per fermion
+  // step, at ((2k+1)/(2n) -+ alpha/n) epsilon, fermion kicks of epsilon at
+  // (1/2 - beta) epsilon and (3/2 + beta) epsilon
+  const double alpha = 0.1, beta = 0.1;
+  s.length = 2;
+  s.steps = 2;
+  for (int k = 0; k < 2 * n; k++)
+    s.kicks.push_back({(2.0 * k + 1.0) / (2.0 * n) + ((k & 1) ? alpha : -alpha) / n, KICK_GAUGE, 1.0 / n});
+  s.kicks.push_back({0.5 - beta, KICK_FERMION, 1.0});
+  s.kicks.push_back({1.5 + beta, KICK_FERMION, 1.0});
+  // in time order; at equal times the gauge kick first (INT_6G1F: 5/12 - alpha/6 = 1/2 - beta)
+  std::stable_sort(s.kicks.begin(), s.kicks.end(), [](const Kick& a, const Kick& b) {
+    if (std::fabs(a.t - b.t) > 1e-12) return a.t < b.t;
+    return a.kind < b.kind;
+  });
+  return 0;
+}
+
+struct Pseudo {
+  VecField phi{};
+  double offs_md[33], offs_gr[33], offs_fa[33];
+};
+
 struct Traj {
   const RhmcParams& p;
   LinkField Ub{}, V{}, W{}, o1{}, o3{}, Uold{};
   double *mom = nullptr, *mtmp = nullptr;
-  VecField phi{}, chi{};
-  double offs_md[33], offs_gr[33], offs_fa[33];
+  std::vector<Pseudo> pf;
+  bool dirty = false;  // thin links changed since the fat and Naik links were built
   double t_ferm = 0, t_gauge = 0, t_links = 0;
-  long md_iters = 0;
-  int n_ferm = 0;
+  long md_iters = 0, fa_iters = 0;
+  int n_ferm = 0, n_gauge = 0, n_rebuild = 0, reunit_errors = 0;
+  double reunit_maxdev = 0;
   explicit Traj(const RhmcParams& prm) : p(prm) {}
   ~Traj() {
     free_links(Ub); free_links(V); free_links(W); free_links(o1); free_links(o3); free_links(Uold);
     dev_free(mom); dev_free(mtmp);
-    free_vec(phi); free_vec(chi);
+    for (auto& q : pf) free_vec(q.phi);
   }
   size_t nmom() const { return (size_t)2 * ctx().Vh * 4 * 10; }
 
-  // the thin links changed: rebuild the FP64 fat + Naik links, drop sloppy copies
-  int links_changed() {
+  // fat and Naik links of the current thin links, rebuilt only when used
+  // (MILC invalidates them in update_u and rebuilds on demand)
+  int fresh() {
+    if (!dirty) return 0;
     Context& c = ctx();
     auto t0 = std::chrono::steady_clock::now();
     DevLinks& L = c.links[P_F64];
@@ -273,11 +337,30 @@ struct Traj {
     c.links_version++;
     (void)hipStreamSynchronize(c.stream);
     t_links += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    n_rebuild++;
+    dirty = false;
     return rc;
   }
   int update_u(double e) {
+    auto t0 = std::chrono::steady_clock::now();
     int rc = update_u_dev(ctx().thin, mom, e);
-    return rc ? rc : links_changed();
+    (void)hipStreamSynchronize(ctx().stream);
+    t_links += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    dirty = true;
+    return rc;
+  }
+  // reunitarize_ks (generic_ks/reunitarize_ks.c): phases out, reunit_su3, phases in
+  int reunitarize() {
+    auto t0 = std::chrono::steady_clock::now();
+    int err = 0;
+    double md = 0;
+    int rc = reunitarize_dev(ctx().thin, 1, kReunitTol, &err, &md);
+    t_links += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    dirty = true;
+    reunit_errors += err;
+    reunit_maxdev = std::max(reunit_maxdev, md);
+    if (!rc && err > kReunitMaxErrors) rc = -6;  // MILC terminates: "Unitarity error count exceeded"
+    return rc;
   }
   // links without the KS phases, for the gauge loops
   int bare() {
@@ -285,21 +368,39 @@ struct Traj {
     MH_CHECK(hipMemcpyAsync(Ub.data, c.thin.data, link_bytes(P_F64, c.stride), hipMemcpyDeviceToDevice, c.stream));
     return rephase_dev(Ub);
   }
-  // update_h_rhmc: gauge force, then the fermion force of the MD rational
-  int force(double e) {
-    Context& c = ctx();
+  // update_h_gauge: imp_gauge_force_ks(e)
+  int gauge_kick(double e) {
     auto t0 = std::chrono::steady_clock::now();
     int rc = bare();
     if (!rc) rc = gauge_force_dev(Ub, p.loop_coeff, e * p.beta / 3.0, mom);
-    (void)hipStreamSynchronize(c.stream);
-    auto t1 = std::chrono::steady_clock::now();
-    t_gauge += std::chrono::duration<double>(t1 - t0).count();
-    if (rc) return rc;
-    double S;
-    int it = 0;
+    (void)hipStreamSynchronize(ctx().stream);
+    t_gauge += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    n_gauge++;
+    return rc;
+  }
+  // update_h_fermion (update_h_rhmc.c:38-111): every pseudofermion's MD
+  // rational solve on the current links, then one force from all the terms
+  int fermion_kick(double e) {
+    Context& c = ctx();
+    int rc;
+    if ((rc = fresh())) return rc;
+    auto t0 = std::chrono::steady_clock::now();
     // the sloppy links of the mixed solve were dropped by the last link update
     if (p.mixed_md && (rc = ensure_links(P_F32))) return rc;
-    rc = rhmc_force_level0(phi, p.md.order, offs_md + 1, p.md.res + 1, p.resid_md, p.mixed_md, &S, o1, o3, &it);
+    std::vector<VecField> all;
+    std::vector<double> res;
+    for (int i = 0; i < p.n_pseudo && !rc; i++) {
+      std::vector<VecField> x;
+      SolveStats st;
+      rc = rational_solve(pf[i].phi, p.pf[i].md.order, pf[i].offs_md + 1, p.pf[i].resid_md, p.mixed_md, EVEN, x, st);
+      md_iters += st.iters;
+      for (int t = 0; t < (int)x.size(); t++) {
+        all.push_back(x[t]);
+        res.push_back(p.pf[i].md.res[t + 1]);
+      }
+    }
+    if (!rc) rc = force_level0_from_solutions(all, res.data(), o1, o3);
+    for (auto& v : all) free_vec(v);
     if (!rc) rc = hisq_aux_dev(c.thin, V, W);
     if (!rc) rc = hisq_force_from_oprod(c.thin, V, W, o1, o3, kHisqLevel2, kHisqLevel1, 2.0 * e, kForceFilter, mtmp);
     if (!rc) {
@@ -307,37 +408,53 @@ struct Traj {
       rc = hipGetLastError() == hipSuccess ? 0 : -1;
     }
     (void)hipStreamSynchronize(c.stream);
-    t_ferm += std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count();
-    md_iters += it;
+    t_ferm += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     n_ferm++;
     return rc;
   }
-  // gauge (beta/3 imp_gauge_action), momentum (hmom_action) and fermion actions
+  int kick(const Kick& k) {
+    const double e = k.size * p.epsilon;
+    if (k.kind == KICK_GAUGE) return gauge_kick(e);
+    if (k.kind == KICK_FERMION) return fermion_kick(e);
+    int rc = gauge_kick(e);  // update_h_rhmc: gauge force, then the fermion force
+    return rc ? rc : fermion_kick(e);
+  }
+  // d_action_rhmc: beta/3 imp_gauge_action, hmom_action, fermion_action
   int actions(double* g, double* h, double* f) {
     int rc = bare();
     double ga = 0;
     if (!rc) rc = gauge_action_dev(Ub, p.loop_coeff, &ga);
     *g = p.beta / 3.0 * ga;
     if (!rc) rc = mom_action_dev(mom, h);
-    if (rc) return rc;
-    int it;
-    if (p.action_from_md) {
-      // phi^dag R_MD phi: the action whose derivative the MD force is
-      std::vector<VecField> x;
-      SolveStats st;
-      rc = rational_solve(phi, p.md.order, offs_md + 1, p.resid_fa, 0, EVEN, x, st);
-      double s = 0, n2 = 0;
-      for (int t = 0; t < p.md.order && !rc; t++) {
-        double d;
-        rc = redot(phi, x[t], &d);
-        s += p.md.res[t + 1] * d;
+    if (!rc) rc = fresh();
+    double fs = 0;
+    for (int i = 0; i < p.n_pseudo && !rc; i++) {
+      const RhmcPseudo& q = p.pf[i];
+      if (p.action_from_md) {
+        // phi^dag R_MD phi: the action whose derivative the MD force is
+        std::vector<VecField> x;
+        SolveStats st;
+        rc = rational_solve(pf[i].phi, q.md.order, pf[i].offs_md + 1, q.resid_fa, 0, EVEN, x, st);
+        fa_iters += st.iters;
+        double s = 0, n2 = 0;
+        for (int t = 0; t < q.md.order && !rc; t++) {
+          double d;
+          rc = redot(pf[i].phi, x[t], &d);
+          s += q.md.res[t + 1] * d;
+        }
+        if (!rc) rc = vec_norm2(pf[i].phi, P_F64, &n2);
+        for (auto& v : x) free_vec(v);
+        fs += q.md.res[0] * n2 + s;
+      } else {
+        double a = 0;
+        int it = 0;
+        rc = rhmc_action(pf[i].phi, q.fa.order, pf[i].offs_fa, q.fa.res, q.resid_fa, 0, &a, &it);
+        fa_iters += it;
+        fs += a;
       }
-      if (!rc) rc = vec_norm2(phi, P_F64, &n2);
-      for (auto& v : x) free_vec(v);
-      *f = p.md.res[0] * n2 + s;
-      return rc;
     }
-    return rhmc_action(phi, p.fa.order, offs_fa, p.fa.res, p.resid_fa, 0, f, &it);
+    *f = fs;
+    return rc;
   }
 };
 
@@ -348,60 +465,83 @@ uint64_t host_mix(uint64_t z) {
   return z ^ (z >> 31);
 }
 
+bool rat_ok(const RhmcRat& r) { return r.order > 0 && r.order <= 32; }
+
 }  // namespace
 
 int rhmc_trajectory(const RhmcParams& p, double* stats) {
   Context& c = ctx();
   if (!c.thin_valid || !c.links[P_F64].valid) return -3;
-  if (p.steps <= 0 || (p.integrator == 1 && p.steps % 2) || p.md.order <= 0 || p.md.order > 32 ||
-      p.gr.order <= 0 || p.gr.order > 32 || p.fa.order <= 0 || p.fa.order > 32)
-    return -2;
+  Schedule sch;
+  if (make_schedule(p.integrator, sch) || p.steps <= 0 || p.steps % sch.steps) return -2;
+  if (p.n_pseudo <= 0 || p.n_pseudo > RHMC_MAX_PSEUDO) return -2;
+  if (p.site_rng && !prn_valid()) return -3;
   auto t_start = std::chrono::steady_clock::now();
   Traj T(p);
-  const double m4 = 4.0 * p.mass * p.mass;
-  T.offs_md[0] = T.offs_gr[0] = T.offs_fa[0] = 0.0;
-  for (int k = 1; k <= 32; k++) {
-    T.offs_md[k] = k <= p.md.order ? m4 + p.md.pole[k] : 0.0;
-    T.offs_gr[k] = k <= p.gr.order ? m4 + p.gr.pole[k] : 0.0;
-    T.offs_fa[k] = k <= p.fa.order ? m4 + p.fa.pole[k] : 0.0;
+  T.pf.resize(p.n_pseudo);
+  for (int i = 0; i < p.n_pseudo; i++) {
+    const RhmcPseudo& q = p.pf[i];
+    if (!rat_ok(q.md) || !rat_ok(q.gr) || !rat_ok(q.fa)) return -2;
+    const double m4 = 4.0 * q.mass * q.mass;
+    Pseudo& P = T.pf[i];
+    P.offs_md[0] = P.offs_gr[0] = P.offs_fa[0] = 0.0;
+    for (int k = 1; k <= 32; k++) {
+      P.offs_md[k] = k <= q.md.order ? m4 + q.md.pole[k] : 0.0;
+      P.offs_gr[k] = k <= q.gr.order ? m4 + q.gr.pole[k] : 0.0;
+      P.offs_fa[k] = k <= q.fa.order ? m4 + q.fa.pole[k] : 0.0;
+    }
+    for (int k = 1; k <= q.md.order; k++)
+      if (!(P.offs_md[k] > 0)) return -2;
+    for (int k = 1; k <= q.gr.order; k++)
+      if (!(P.offs_gr[k] > 0)) return -2;
+    for (int k = 1; k <= q.fa.order; k++)
+      if (!(P.offs_fa[k] > 0)) return -2;
   }
-  for (int k = 1; k <= p.md.order; k++)
-    if (!(T.offs_md[k] > 0)) return -2;
-  for (int k = 1; k <= p.gr.order; k++)
-    if (!(T.offs_gr[k] > 0)) return -2;
-  for (int k = 1; k <= p.fa.order; k++)
-    if (!(T.offs_fa[k] > 0)) return -2;
   int rc;
   if ((rc = alloc_links(T.Ub, P_F64, c.stride)) || (rc = alloc_links(T.V, P_F64, c.stride)) ||
       (rc = alloc_links(T.W, P_F64, c.stride)) || (rc = alloc_links(T.o1, P_F64, c.stride)) ||
       (rc = alloc_links(T.o3, P_F64, c.stride)) || (rc = alloc_links(T.Uold, P_F64, c.stride)) ||
       (rc = dev_alloc((void**)&T.mom, T.nmom() * sizeof(double))) ||
-      (rc = dev_alloc((void**)&T.mtmp, T.nmom() * sizeof(double))) || (rc = alloc_vec(T.phi, P_F64, c.stride)) ||
-      (rc = alloc_vec(T.chi, P_F64, c.stride)))
+      (rc = dev_alloc((void**)&T.mtmp, T.nmom() * sizeof(double))))
     return rc;
-  // momenta and pseudofermion: phi = R_GR(M^dag M) chi (grsource_imp_rhmc)
-  if ((rc = ranmom_dev(host_mix(p.seed), T.mom)) || (rc = gaussian_vec_dev(host_mix(p.seed + 1), 0, T.chi)))
-    return rc;
+  for (auto& P : T.pf)
+    if ((rc = alloc_vec(P.phi, P_F64, c.stride))) return rc;
+  VecField chi{};
+  if ((rc = alloc_vec(chi, P_F64, c.stride))) return rc;
+  // ranmom, then grsource_imp_rhmc for each pseudofermion: phi = R_GR(M^dag M) chi
+  double grsum[RHMC_MAX_PSEUDO] = {0};
   int it_gr = 0;
-  if ((rc = rational_apply(T.chi, p.gr.order, T.offs_gr + 1, p.gr.res, p.resid_gr, T.phi, &it_gr))) return rc;
+  rc = p.site_rng ? ranmom_prn_dev(T.mom) : ranmom_dev(host_mix(p.seed), T.mom);
+  for (int i = 0; i < p.n_pseudo && !rc; i++) {
+    if (p.site_rng) {
+      rc = gaussian_prn_dev(0, chi, &grsum[i]);
+    } else {
+      rc = gaussian_vec_dev(host_mix(p.seed + 1) + 0x632BE59BD9B4E019ULL * (uint64_t)i, 0, chi);
+      if (!rc) rc = vec_norm2(chi, P_F64, &grsum[i]);
+    }
+    int it = 0;
+    if (!rc)
+      rc = rational_apply(chi, p.pf[i].gr.order, T.pf[i].offs_gr + 1, p.pf[i].gr.res, p.pf[i].resid_gr, T.pf[i].phi,
+                          &it);
+    it_gr += it;
+  }
+  free_vec(chi);
+  if (rc) return rc;
   double g0, h0, f0, g1, h1, f1;
   auto t_a0 = std::chrono::steady_clock::now();
   if ((rc = T.actions(&g0, &h0, &f0))) return rc;
   double t_act = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_a0).count();
   MH_CHECK(hipMemcpyAsync(T.Uold.data, c.thin.data, link_bytes(P_F64, c.stride), hipMemcpyDeviceToDevice, c.stream));
   const double e = p.epsilon;
-  if (p.integrator == 0) {
-    for (int s = 1; s <= p.steps && !rc; s++) {
-      if (!(rc = T.update_u(0.5 * e)) && !(rc = T.force(e))) rc = T.update_u(0.5 * e);
+  for (int cyc = 0; cyc < p.steps / sch.steps && !rc; cyc++) {
+    double now = 0;
+    for (const Kick& k : sch.kicks) {
+      if (k.t - now > 1e-12 && (rc = T.update_u(e * (k.t - now)))) break;
+      now = k.t;
+      if ((rc = T.kick(k))) break;
     }
-  } else {
-    const double lam = 0.8;
-    for (int s = 2; s <= p.steps && !rc; s += 2) {
-      if ((rc = T.update_u(0.5 * e * lam)) || (rc = T.force(e)) || (rc = T.update_u(e * (2.0 - lam))) ||
-          (rc = T.force(e)))
-        break;
-      rc = T.update_u(0.5 * e * lam);
-    }
+    if (!rc && sch.length - now > 1e-12) rc = T.update_u(e * (sch.length - now));
+    if (!rc) rc = T.reunitarize();
   }
   if (rc) return rc;
   auto t_a1 = std::chrono::steady_clock::now();
@@ -410,19 +550,26 @@ int rhmc_trajectory(const RhmcParams& p, double* stats) {
   const double dH = (g1 + h1 + f1) - (g0 + h0 + f0);
   int accepted = 1;
   if (p.accept == 1) {
-    double r = (double)(host_mix(p.seed + 2) >> 11) * (1.0 / 9007199254740992.0);
-    accepted = std::exp(-dH) >= r;
+    // update(): reject when exp(-dS) < a uniform draw of the node generator
+    double r = p.site_rng ? prn_node_uniform()
+                          : (double)(host_mix(p.seed + 2) >> 11) * (1.0 / 9007199254740992.0);
+    accepted = !(std::exp(-dH) < r);
   } else if (p.accept == 2) {
     accepted = 0;
   }
   if (!accepted) {
     MH_CHECK(hipMemcpyAsync(c.thin.data, T.Uold.data, link_bytes(P_F64, c.stride), hipMemcpyDeviceToDevice, c.stream));
-    if ((rc = T.links_changed())) return rc;
+    T.dirty = true;
   }
+  // leave the fat and Naik links of the final thin links in place
+  if ((rc = T.fresh())) return rc;
   (void)hipStreamSynchronize(c.stream);
   double tot = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
-  double out[16] = {g0, h0, f0, g1, h1, f1, dH, (double)accepted, tot, (double)T.md_iters, (double)T.n_ferm,
-                    T.t_ferm, T.t_gauge, T.t_links, t_act, (double)it_gr};
+  double out[32] = {g0, h0, f0, g1, h1, f1, dH, (double)accepted, tot, (double)T.md_iters, (double)T.n_ferm,
+                    T.t_ferm, T.t_gauge, T.t_links, t_act, (double)it_gr, (double)T.n_gauge, T.reunit_maxdev,
+                    (double)T.reunit_errors, (double)T.n_rebuild};
+  for (int i = 0; i < p.n_pseudo && i < 8; i++) out[20 + i] = grsum[i];
+  out[28] = (double)T.fa_iters;
   memcpy(stats, out, sizeof out);
   return 0;
 }

@@ -50,7 +50,10 @@ EXPORTS = [
     "milc_hip_dev_block_dslash_bench", "milc_hip_dev_block_dslash", "milc_hip_dev_multishift_block",
     "milc_hip_dev_block_describe",
     "milc_hip_dev_multishift", "milc_hip_dev_hmc_step", "milc_hip_hisq_force", "milc_hip_eo_fermion_force_multi",
-    "milc_hip_set_force_form", "milc_hip_dev_rhmc_trajectory", "milc_hip_gauge_force", "milc_hip_gauge_action", "milc_hip_update_u", "milc_hip_mom_action", "milc_hip_dev_load_thin_links", "milc_hip_dev_read_thin_links", "milc_hip_dev_dslash", "milc_hip_set_link_compression", "milc_hip_link_compression", "milc_hip_dev_hisq_force",
+    "milc_hip_set_force_form", "milc_hip_dev_rhmc_trajectory", "milc_hip_gauge_force", "milc_hip_gauge_action",
+    "milc_hip_update_u", "milc_hip_mom_action", "milc_hip_dev_load_thin_links", "milc_hip_dev_read_thin_links",
+    "milc_hip_dev_dslash", "milc_hip_set_link_compression", "milc_hip_link_compression", "milc_hip_dev_hisq_force",
+    "milc_hip_reunitarize", "milc_hip_dev_seed_rng", "milc_hip_dev_ranmom", "milc_hip_dev_gaussian_source",
 ]
 
 
@@ -108,12 +111,42 @@ def gauge_loop_coeffs(u0, nf, hisq=True):
     return [1.0, -1.0 / (20.0 * u0 * u0) * (1.0 - 0.6264 * lg), 1.0 / (u0 * u0) * 0.04335 * lg]
 
 
+MAX_PSEUDO = 8
+
+
+class Pseudofermion(ctypes.Structure):
+    """one pseudofermion: mass (4 m^2 added to the poles; 0 for MILC's files,
+    whose poles hold it), MD / GR / FA rationals, solver targets"""
+    _fields_ = [("mass", ctypes.c_double), ("md", Rational), ("gr", Rational), ("fa", Rational),
+                ("resid_md", ctypes.c_double), ("resid_gr", ctypes.c_double), ("resid_fa", ctypes.c_double)]
+
+
 class RhmcParams(ctypes.Structure):
     _fields_ = [("steps", ctypes.c_int), ("integrator", ctypes.c_int), ("epsilon", ctypes.c_double),
-                ("beta", ctypes.c_double), ("loop_coeff", ctypes.c_double * 3), ("mass", ctypes.c_double),
-                ("md", Rational), ("gr", Rational), ("fa", Rational), ("resid_md", ctypes.c_double),
-                ("resid_gr", ctypes.c_double), ("resid_fa", ctypes.c_double), ("mixed_md", ctypes.c_int),
-                ("action_from_md", ctypes.c_int), ("accept", ctypes.c_int), ("seed", ctypes.c_ulonglong)]
+                ("beta", ctypes.c_double), ("loop_coeff", ctypes.c_double * 3), ("n_pseudo", ctypes.c_int),
+                ("pf", Pseudofermion * MAX_PSEUDO), ("mixed_md", ctypes.c_int), ("action_from_md", ctypes.c_int),
+                ("accept", ctypes.c_int), ("site_rng", ctypes.c_int), ("seed", ctypes.c_ulonglong)]
+
+
+# MILC's int_alg_t (ks_imp_rhmc/ks_imp_includes.h:58-60)
+INT_LEAPFROG, INT_OMELYAN, INT_2G1F, INT_3G1F, INT_5G1F, INT_6G1F = 0, 1, 4, 5, 6, 7
+
+TRAJECTORY_STATS = ["g0", "h0", "f0", "g1", "h1", "f1", "dH", "accepted", "seconds", "md_iters", "fermion_forces",
+                    "fermion_force_seconds", "gauge_force_seconds", "link_seconds", "action_seconds", "gr_iters",
+                    "gauge_forces", "reunit_max_deviation", "reunit_errors", "link_rebuilds"]
+
+
+def pseudofermions_from_file(path, resid_md, resid_gr, resid_fa, mass=0.0):
+    """Pseudofermion list from MILC's rational file; resid_* are per-pseudofermion
+    lists (or one value for all), MILC's cgresid_md_fa_gr lines"""
+    rats = load_rhmc_params(path)
+    n = len(rats)
+
+    def per(v):
+        return list(v) if hasattr(v, "__len__") else [v] * n
+    rm, rg, rf, ms = per(resid_md), per(resid_gr), per(resid_fa), per(mass)
+    return [Pseudofermion(mass=ms[i], md=r["MD"], gr=r["GR"], fa=r["FA"], resid_md=rm[i], resid_gr=rg[i],
+                          resid_fa=rf[i]) for i, r in enumerate(rats)]
 
 
 def load_library(path=LIB_PATH):
@@ -168,6 +201,10 @@ def load_library(path=LIB_PATH):
     L.milc_hip_eo_fermion_force_multi.argtypes = [I, D, PD, ctypes.POINTER(P), I, P, P]
     L.milc_hip_set_force_form.argtypes = [I]
     L.milc_hip_dev_rhmc_trajectory.argtypes = [ctypes.POINTER(RhmcParams), ctypes.POINTER(D)]
+    L.milc_hip_dev_seed_rng.argtypes = [I, I]
+    L.milc_hip_dev_ranmom.argtypes = [P]
+    L.milc_hip_dev_gaussian_source.argtypes = [I, I, ctypes.POINTER(D)]
+    L.milc_hip_reunitarize.argtypes = [I, D, P, I, ctypes.POINTER(D), ctypes.POINTER(I)]
     L.milc_hip_gauge_force.argtypes = [I, I, ctypes.POINTER(D), D, P, I, P]
     L.milc_hip_gauge_action.argtypes = [I, I, ctypes.POINTER(D), P, I, ctypes.POINTER(D)]
     L.milc_hip_update_u.argtypes = [I, D, P, P, I]
@@ -479,21 +516,55 @@ class Lattice:
                "milc_hip_mom_action")
         return a.value
 
-    def dev_rhmc_trajectory(self, steps, epsilon, beta, loop_coeff, mass, md, gr, fa, integrator=1,
-                            resid_md=1e-9, resid_gr=1e-10, resid_fa=1e-10, mixed_md=1, action_from_md=False,
-                            accept=1, seed=1):
-        """one RHMC trajectory on the device links (update(), ks_imp_rhmc/update_rhmc.c)"""
+    def dev_rhmc_trajectory(self, steps, epsilon, beta, loop_coeff, pseudofermions, integrator=INT_OMELYAN,
+                            mixed_md=1, action_from_md=False, accept=1, seed=1, site_rng=False):
+        """one RHMC trajectory on the device links (update(), ks_imp_rhmc/update_rhmc.c);
+        pseudofermions: list of Pseudofermion"""
         self._activate()
-        p = RhmcParams(steps=steps, integrator=integrator, epsilon=epsilon, beta=beta, mass=mass, md=md, gr=gr,
-                       fa=fa, resid_md=resid_md, resid_gr=resid_gr, resid_fa=resid_fa, mixed_md=int(mixed_md),
-                       action_from_md=int(bool(action_from_md)), accept=accept, seed=seed)
+        if not 1 <= len(pseudofermions) <= MAX_PSEUDO:
+            raise MilcHipError(f"1..{MAX_PSEUDO} pseudofermions")
+        p = RhmcParams(steps=steps, integrator=integrator, epsilon=epsilon, beta=beta, n_pseudo=len(pseudofermions),
+                       mixed_md=int(mixed_md), action_from_md=int(bool(action_from_md)), accept=accept,
+                       site_rng=int(bool(site_rng)), seed=seed)
         for k in range(3):
             p.loop_coeff[k] = loop_coeff[k]
-        st = (ctypes.c_double * 16)()
+        for i, q in enumerate(pseudofermions):
+            p.pf[i] = q
+        st = (ctypes.c_double * 32)()
         _check(self.lib.milc_hip_dev_rhmc_trajectory(ctypes.byref(p), st), "milc_hip_dev_rhmc_trajectory")
-        keys = ["g0", "h0", "f0", "g1", "h1", "f1", "dH", "accepted", "seconds", "md_iters", "fermion_forces",
-                "fermion_force_seconds", "gauge_force_seconds", "link_seconds", "action_seconds", "gr_iters"]
-        return dict(zip(keys, list(st)))
+        out = dict(zip(TRAJECTORY_STATS, list(st)))
+        out["grsource"] = list(st)[20:20 + len(pseudofermions)]
+        out["action_iters"] = st[28]
+        return out
+
+    def dev_seed_rng(self, iseed, precision=1):
+        """MILC's per-site generators (initialize_site_prn_from_seed) and node
+        generator; precision 1 rounds the draws as MILC's single-precision build"""
+        self._activate()
+        _check(self.lib.milc_hip_dev_seed_rng(iseed, precision), "milc_hip_dev_seed_rng")
+
+    def dev_ranmom(self):
+        """ranmom() from the site generators: packed momenta (V, 4, 10)"""
+        self._activate()
+        mom = np.zeros((self.V, 4, 10))
+        _check(self.lib.milc_hip_dev_ranmom(_p(mom)), "milc_hip_dev_ranmom")
+        return mom
+
+    def dev_gaussian_source(self, parity, slot):
+        """three complex Gaussians per site of parity from the site generators
+        into device slot; returns MILC's GRSOURCE sum"""
+        self._activate()
+        s = ctypes.c_double(0)
+        _check(self.lib.milc_hip_dev_gaussian_source(parity, slot, ctypes.byref(s)), "milc_hip_dev_gaussian_source")
+        return s.value
+
+    def reunitarize(self, links, tolerance=1e-4, phases_in=True):
+        """reunitarize_ks on host links (V, 4, 3, 3, 2) in place: (errors, max deviation)"""
+        self._activate()
+        md, ne = ctypes.c_double(0), ctypes.c_int(0)
+        _check(self.lib.milc_hip_reunitarize(2, tolerance, _p(links), int(bool(phases_in)), ctypes.byref(md),
+                                             ctypes.byref(ne)), "milc_hip_reunitarize")
+        return ne.value, md.value
 
     def set_force_form(self, form):
         """0: path-table gather; 1 (default): staple form where exact for any

@@ -1477,3 +1477,171 @@ void kso_update_u(const kso_geom *g, double eps, double *U, const double *mom) {
     memcpy(L, t2, sizeof t2);
   }
 }
+
+/* ------------------------------------------------------------------------- */
+/* MILC's per-site random numbers (SITERAND builds), so that a trajectory can
+ * be reproduced draw for draw.
+ *
+ * Generator: generic/ranstuff.c:66-115 (the default variant, 64-bit
+ * congruential part).  State per site, 9 words: r[0..6] (24-bit values of a
+ * feedback shift register), mult, ic.  Seeding from (seed, index): a 32-bit
+ * linear congruential sequence s <- (69607 + 8 index) s + 12345 (int
+ * arithmetic, wrapping) gives r[k] = bits 8..31 of the k-th value, and the 8th
+ * value, sign-extended to 64 bits, starts ic; mult = 100000005 + 8 index.
+ * Draw: t = ((r5 >> 7 | r6 << 17) ^ (r4 >> 1 | r5 << 23)) & 0xffffff, the
+ * register shifts (r0 <- t), ic <- ic mult + 12345 (mod 2^64), and the value
+ * is (t ^ bits 40..63 of ic) / 2^24.  index = the site's lexicographic index
+ * (generic/make_lattice.c:49,55).
+ * Gaussian: polar Box-Muller with weight exp(-x^2) (<x^2> = 1/2), the complex
+ * draw being (v2 fac, v1 fac) (libraries/gaussrand.c, default variant);
+ * momenta: random_anti_hermitian (libraries/rand_ahmat.c, default variant).
+ * single = 1 rounds as a MILC_PRECISION=1 build (Real = float) does: the
+ * rejection radius, fac and the products in float, fac from a double sqrt/log.
+ * Pinned by the GRSOURCE sums and the starting momentum action MILC printed in
+ * ks_imp_rhmc/test/su3_rhmc_hisq.1.sample-out (tests/test_cpu_rng.py). */
+static void prn_init(uint64_t *st, int seed, int index) {
+  uint32_t a = 69607u + 8u * (uint32_t)index, s = (uint32_t)seed;
+  for (int k = 0; k < 7; k++) {
+    s = a * s + 12345u;
+    st[k] = (s >> 8) & 0xffffffu;
+  }
+  s = a * s + 12345u;
+  st[7] = (uint64_t)(100000005u + 8u * (uint32_t)index);
+  st[8] = (uint64_t)(int64_t)(int32_t)s;
+}
+
+static uint32_t prn_next(uint64_t *st) {
+  uint64_t t = (((st[5] >> 7) | (st[6] << 17)) ^ ((st[4] >> 1) | (st[5] << 23))) & 0xffffffu;
+  for (int k = 6; k > 0; k--) st[k] = st[k - 1];
+  st[0] = t;
+  st[8] = st[8] * st[7] + 12345u;
+  return (uint32_t)(t ^ ((st[8] >> 40) & 0xffffffu));
+}
+
+static void prn_cgauss(uint64_t *st, int single, double *re, double *im) {
+  if (single) {
+    volatile float v1, v2, a, b, r;
+    do {
+      v1 = (float)(2.0 * (prn_next(st) * (1.0 / 16777216.0)) - 1.0);
+      v2 = (float)(2.0 * (prn_next(st) * (1.0 / 16777216.0)) - 1.0);
+      a = v1 * v1;
+      b = v2 * v2;
+      r = a + b;
+    } while (r >= 1.0f);
+    volatile float fac = (float)sqrt(-log((double)r) / (double)r);
+    volatile float x = v2 * fac, y = v1 * fac;
+    *re = x;
+    *im = y;
+  } else {
+    double v1, v2, r;
+    do {
+      v1 = 2.0 * (prn_next(st) * (1.0 / 16777216.0)) - 1.0;
+      v2 = 2.0 * (prn_next(st) * (1.0 / 16777216.0)) - 1.0;
+      r = v1 * v1 + v2 * v2;
+    } while (r >= 1.0);
+    double fac = sqrt(-log(r) / r);
+    *re = v2 * fac;
+    *im = v1 * fac;
+  }
+}
+
+void kso_prn_seed(const kso_geom *g, int iseed, uint64_t *st) {
+  for (size_t i = 0; i < g->V; i++) {
+    const int *c = g->coords + 4 * i;
+    int index = c[0] + g->dims[0] * (c[1] + g->dims[1] * (c[2] + g->dims[2] * c[3]));
+    prn_init(st + 9 * i, iseed, index);
+  }
+}
+
+void kso_prn_seed_one(int seed, int index, uint64_t *st) { prn_init(st, seed, index); }
+uint32_t kso_prn_next(uint64_t *st) { return prn_next(st); }
+
+/* ranmom(): every site, directions XUP..TUP in turn; packed anti_hermitmat
+ * [V][4][10] = m01, m02, m12 (re, im), m00im, m11im, m22im, 0 */
+void kso_ranmom_prn(const kso_geom *g, uint64_t *st, int single, double *mom) {
+  for (size_t i = 0; i < g->V; i++)
+    for (int dir = 0; dir < 4; dir++) {
+      double *m = mom + 10 * (4 * i + dir), a, b;
+      prn_cgauss(st + 9 * i, single, &a, &b);
+      if (single) {
+        volatile float s3 = (float)sqrt(1.0 / 3.0), fa = (float)a, fb = (float)b;
+        volatile float p = s3 * fb, d0 = fa + p, d1 = -fa + p;
+        m[6] = d0;
+        m[7] = d1;
+        m[8] = (float)(-2.0 * (double)s3 * (double)fb);
+      } else {
+        double s3 = sqrt(1.0 / 3.0);
+        m[6] = a + s3 * b;
+        m[7] = -a + s3 * b;
+        m[8] = -2.0 * s3 * b;
+      }
+      for (int k = 0; k < 3; k++) prn_cgauss(st + 9 * i, single, &m[2 * k], &m[2 * k + 1]);
+      m[9] = 0.0;
+    }
+}
+
+/* three complex Gaussians on each site of `parity` (grsource_imp_rhmc,
+ * ks_imp_rhmc/grsource_rhmc.c:30-37); other sites zeroed; returns the sum of
+ * |v|^2 over the drawn sites (MILC prints it as GRSOURCE: sum) */
+double kso_gaussian_prn(const kso_geom *g, uint64_t *st, int single, int parity, double *v) {
+  double sum = 0.0;
+  memset(v, 0, 6 * g->V * sizeof(double));
+  size_t lo = parity == KSO_ODD ? g->Vh : 0, hi = parity == KSO_EVEN ? g->Vh : g->V;
+  for (size_t i = lo; i < hi; i++)
+    for (int j = 0; j < 3; j++) {
+      prn_cgauss(st + 9 * i, single, &v[6 * i + 2 * j], &v[6 * i + 2 * j + 1]);
+      sum += v[6 * i + 2 * j] * v[6 * i + 2 * j] + v[6 * i + 2 * j + 1] * v[6 * i + 2 * j + 1];
+    }
+  return sum;
+}
+
+/* reunit_su3 (generic/reunitarize2.c:133-205) on one matrix: normalise row 0,
+ * take row 0's component out of row 1 and normalise it, row 2 = conj(row0 x
+ * row1).  Returns the number of the four deviations (|row0|^2 - 1, |<row0,
+ * row1>|^2, |row1'|^2 - 1, |row2 change|^2) at or above tol; *maxdev the
+ * largest.  Pinned by tests/test_cpu_gauge.py: numpy's QR of U^T gives the
+ * same rows 0 and 1, det = 1, unitarity, SU(3) inputs unchanged. */
+int kso_reunit_su3(double *M, double tol, double *maxdev) {
+  double complex r0[3], r1[3], r2[3], old[3];
+  for (int k = 0; k < 3; k++) {
+    r0[k] = M[2 * k] + I * M[2 * k + 1];
+    r1[k] = M[6 + 2 * k] + I * M[6 + 2 * k + 1];
+    old[k] = M[12 + 2 * k] + I * M[12 + 2 * k + 1];
+  }
+  double dev[4], n0 = 0, n1 = 0;
+  for (int k = 0; k < 3; k++) n0 += creal(r0[k] * conj(r0[k]));
+  dev[0] = fabs(n0 - 1.0);
+  for (int k = 0; k < 3; k++) r0[k] /= sqrt(n0);
+  double complex ov = 0;
+  for (int k = 0; k < 3; k++) ov += conj(r0[k]) * r1[k];
+  dev[1] = creal(ov * conj(ov));
+  for (int k = 0; k < 3; k++) r1[k] -= ov * r0[k];
+  for (int k = 0; k < 3; k++) n1 += creal(r1[k] * conj(r1[k]));
+  dev[2] = fabs(n1 - 1.0);
+  for (int k = 0; k < 3; k++) r1[k] /= sqrt(n1);
+  r2[0] = conj(r0[1] * r1[2] - r0[2] * r1[1]);
+  r2[1] = conj(r0[2] * r1[0] - r0[0] * r1[2]);
+  r2[2] = conj(r0[0] * r1[1] - r0[1] * r1[0]);
+  dev[3] = 0;
+  for (int k = 0; k < 3; k++) dev[3] += creal((old[k] - r2[k]) * conj(old[k] - r2[k]));
+  int errors = 0;
+  for (int k = 0; k < 4; k++) {
+    if (dev[k] > *maxdev) *maxdev = dev[k];
+    errors += dev[k] >= tol;
+  }
+  double complex *rows[3] = {r0, r1, r2};
+  for (int r = 0; r < 3; r++)
+    for (int k = 0; k < 3; k++) {
+      M[6 * r + 2 * k] = creal(rows[r][k]);
+      M[6 * r + 2 * k + 1] = cimag(rows[r][k]);
+    }
+  return errors;
+}
+
+/* reunitarize every link (links without KS phases); returns the error count */
+int kso_reunitarize(const kso_geom *g, double tol, double *U, double *maxdev) {
+  int errors = 0;
+  *maxdev = 0.0;
+  for (size_t i = 0; i < 4 * g->V; i++) errors += kso_reunit_su3(U + 18 * i, tol, maxdev);
+  return errors;
+}

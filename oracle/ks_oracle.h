@@ -115,6 +115,17 @@ void kso_u3_adjoint(const double *V, const double *FW, double *FV);
 /* the force's path table for direction mu (dirs[7*p+i], -1 padded) */
 int kso_fn_paths(int mu, const double coeff[6], int *dirs, double *c, int *len, int *naik);
 
+/* MILC's per-site generator (9 words per site, node_index order), its
+ * momentum and Gaussian-vector draws (single = 1: MILC_PRECISION=1 rounding),
+ * and reunit_su3 */
+void kso_prn_seed(const kso_geom *g, int iseed, uint64_t *st);
+void kso_prn_seed_one(int seed, int index, uint64_t *st);
+uint32_t kso_prn_next(uint64_t *st);
+void kso_ranmom_prn(const kso_geom *g, uint64_t *st, int single, double *mom);
+double kso_gaussian_prn(const kso_geom *g, uint64_t *st, int single, int parity, double *v);
+int kso_reunit_su3(double *M, double tol, double *maxdev);
+int kso_reunitarize(const kso_geom *g, double tol, double *U, double *maxdev);
+
 /* --- reductions and helpers used by tests ---------------------------------- */
 double kso_norm2(const kso_geom *g, const double *v, int parity);
 void kso_mdagm(const kso_geom *g, const double *fat, const double *lng, double offset,

@@ -80,6 +80,15 @@ def lib():
         L.kso_norm2.argtypes = [G, P, ctypes.c_int]
         L.kso_norm2.restype = ctypes.c_double
         L.kso_mdagm.argtypes = [G, P, P, ctypes.c_double, P, P, P, ctypes.c_int]
+        L.kso_prn_seed.argtypes = [G, ctypes.c_int, P]
+        L.kso_prn_seed_one.argtypes = [ctypes.c_int, ctypes.c_int, P]
+        L.kso_prn_next.argtypes = [P]
+        L.kso_prn_next.restype = ctypes.c_uint32
+        L.kso_ranmom_prn.argtypes = [G, P, ctypes.c_int, P]
+        L.kso_gaussian_prn.argtypes = [G, P, ctypes.c_int, ctypes.c_int, P]
+        L.kso_gaussian_prn.restype = ctypes.c_double
+        L.kso_reunit_su3.argtypes = [P, ctypes.c_double, ctypes.POINTER(ctypes.c_double)]
+        L.kso_reunitarize.argtypes = [G, ctypes.c_double, P, ctypes.POINTER(ctypes.c_double)]
         _lib = L
     return _lib
 
@@ -238,6 +247,35 @@ class Lattice:
         lib().kso_update_u(self.g, eps, _p(U), _p(np.ascontiguousarray(mom)))
         return U
 
+    def prn_seed(self, iseed):
+        """MILC's per-site generators seeded as initialize_site_prn_from_seed:
+        uint64 state (V, 9) in node_index order"""
+        st = np.zeros((self.V, 9), dtype=np.uint64)
+        lib().kso_prn_seed(self.g, iseed, st.ctypes.data_as(ctypes.c_void_p))
+        return st
+
+    def ranmom_prn(self, st, single=True):
+        """ranmom() from the site generators st (advanced in place): packed
+        anti_hermitmat momenta (V, 4, 10)"""
+        assert st.dtype == np.uint64 and st.flags.c_contiguous
+        mom = np.zeros((self.V, 4, 10))
+        lib().kso_ranmom_prn(self.g, st.ctypes.data_as(ctypes.c_void_p), int(single), _p(mom))
+        return mom
+
+    def gaussian_prn(self, st, parity=EVEN, single=True):
+        """three complex Gaussians per site of parity from st: (vector, sum |v|^2)"""
+        assert st.dtype == np.uint64 and st.flags.c_contiguous
+        v = self.new_vector()
+        s = lib().kso_gaussian_prn(self.g, st.ctypes.data_as(ctypes.c_void_p), int(single), parity, _p(v))
+        return v, s
+
+    def reunitarize(self, U, tol=1e-4):
+        """reunit_su3 on every (bare) link in place: (error count, max deviation)"""
+        assert U.dtype == np.float64 and U.flags.c_contiguous
+        md = ctypes.c_double(0)
+        n = lib().kso_reunitarize(self.g, tol, _p(U), ctypes.byref(md))
+        return n, md.value
+
     def hisq_force(self, U_phased, multi_x, residues, eps=1.0):
         """HISQ thin-link force TA(2 eps U F_U), full 3x3 matrices (V, 4, 3, 3, 2)."""
         n = len(multi_x)
@@ -255,6 +293,26 @@ class Lattice:
         o1, o3 = self.new_links(), self.new_links()
         lib().kso_force_oprod(self.g, ptrs, _p(res), n, _p(o1), _p(o3))
         return o1, o3
+
+
+def prn_stream(seed, index, n):
+    """n raw 24-bit draws of one MILC generator seeded with (seed, index)"""
+    st = np.zeros(9, dtype=np.uint64)
+    lib().kso_prn_seed_one(seed, index, st.ctypes.data_as(ctypes.c_void_p))
+    return np.array([lib().kso_prn_next(st.ctypes.data_as(ctypes.c_void_p)) for _ in range(n)], dtype=np.uint32)
+
+
+def unpack_ah(p):
+    """packed anti_hermitmat [..., 10] -> full matrices [..., 3, 3, 2]"""
+    F = np.zeros(p.shape[:-1] + (3, 3, 2))
+    F[..., 0, 1, :] = p[..., 0:2]
+    F[..., 0, 2, :] = p[..., 2:4]
+    F[..., 1, 2, :] = p[..., 4:6]
+    F[..., 1, 0, 0], F[..., 1, 0, 1] = -p[..., 0], p[..., 1]
+    F[..., 2, 0, 0], F[..., 2, 0, 1] = -p[..., 2], p[..., 3]
+    F[..., 2, 1, 0], F[..., 2, 1, 1] = -p[..., 4], p[..., 5]
+    F[..., 0, 0, 1], F[..., 1, 1, 1], F[..., 2, 2, 1] = p[..., 6], p[..., 7], p[..., 8]
+    return F
 
 
 def u3_unitarize(V):

@@ -1,28 +1,44 @@
 """One RHMC trajectory on the device (milc_hip_dev_rhmc_trajectory, the
 update() of ks_imp_rhmc/update_rhmc.c): gauge force + HISQ fermion force,
-link updates with the fat and Naik links rebuilt, actions, accept/reject.
+link updates with the fat and Naik links rebuilt, reunitarisation, actions,
+accept/reject.
 
+* MILC's own regression trajectory, ks_imp_rhmc/test/su3_rhmc_hisq.1
+  (lat.sample.l6666.hisq, iseed 86658, four pseudofermions of
+  rationals.sample.su3_rhmc_hisq, INT_3G1F, 10 steps of 0.02, beta 7.4,
+  u0 0.89): with MILC's per-site generators the heatbath noise reproduces the
+  printed GRSOURCE sums to every digit and the start actions the printed
+  ACTION line; dS and the final plaquette agree with the printed ones within
+  the tolerances of MILC's own check (su3_rhmc_hisq.1.errtol: delta S 0.02,
+  PLAQ 4e-5);
+* the device generators against the oracle's restatement of MILC's: momenta
+  identical, sources identical;
 * the starting actions against the oracle: beta/3 times the gauge-loop sum of
   the device links, and phi^dag R_MD phi with phi = R_GR chi, chi the
   device's Gaussian source of the trajectory's seed (FP64 solves, 1e-9);
 * energy conservation: with the fermion action the MD force's own
   (action_from_md), |dH| falls by about 4 when eps halves at fixed
-  trajectory length (leapfrog), and Omelyan's dH is smaller than leapfrog's;
+  trajectory length (leapfrog); Omelyan and the nG1F integrators do better;
 * accept = 2 restores the starting links exactly; accept = 0 keeps links
-  that stay unitary.
+  that are SU(3) after the reunitarisation.
 
-Workload: 4x4x4x8 lattice, gauge seed 1200 (scale 0.4), m = 0.1, beta = 5,
-loop coefficients for u0 = 0.86 with 2+1+1 flavours, a two-pole MD and a
-one-pole GR rational (positive poles and residues).
+Workload of the small tests: 4x4x4x8 lattice, gauge seed 1200 (scale 0.4),
+m = 0.1, beta = 5, loop coefficients for u0 = 0.86 with 2+1+1 flavours, a
+two-pole MD and a one-pole GR rational (positive poles and residues).
 """
+import os
+
 import numpy as np
 import pytest
 
+import ks_io as oio
 import ks_oracle as ko
 import milc_qcd_amd as mq
+from milc_qcd_amd import lattice_io as lio
 
 pytestmark = pytest.mark.gpu
 
+HERE = os.path.dirname(os.path.abspath(__file__))
 DIMS = (4, 4, 4, 8)
 MASS, BETA = 0.1, 5.0
 MD = ([0.5, 0.3], [0.01, 0.3], 0.1)   # residues, poles, constant
@@ -42,10 +58,101 @@ def splitmix(z):
     return z ^ (z >> 31)
 
 
-def traj(G, steps, eps, integrator=0, accept=2, seed=7):
-    return G.dev_rhmc_trajectory(steps, eps, BETA, COEFF, MASS, rat(MD), rat(GR), rat(MD), integrator=integrator,
-                                 resid_md=1e-12, resid_gr=1e-12, resid_fa=1e-12, mixed_md=0, action_from_md=True,
-                                 accept=accept, seed=seed)
+def pf(resid_md=1e-12):
+    return mq.Pseudofermion(mass=MASS, md=rat(MD), gr=rat(GR), fa=rat(MD), resid_md=resid_md, resid_gr=1e-12,
+                            resid_fa=1e-12)
+
+
+def traj(G, steps, eps, integrator=mq.INT_LEAPFROG, accept=2, seed=7, mixed_md=0, resid_md=1e-12):
+    return G.dev_rhmc_trajectory(steps, eps, BETA, COEFF, [pf(resid_md)], integrator=integrator, mixed_md=mixed_md,
+                                 action_from_md=True, accept=accept, seed=seed)
+
+
+# ---- MILC's su3_rhmc_hisq.1 regression run (sample-out lines cited) --------
+MILC_ISEED = 86658
+MILC_GRSOURCE = [1.9636750663e+03, 1.9027332576e+03, 1.9328576741e+03, 1.9363357361e+03]  # 526-532
+MILC_ACTION0 = (1.91689577889074e+04, -8.03776340931654e+01, 7.73560159669816e+03)       # 539
+MILC_ACTION1 = (1.89814624700873e+04, 1.10639809951186e+02, 7.73207863940299e+03)        # 608
+MILC_DELTA_S = -8.320708e-04                                                               # 610
+MILC_PLAQ = (1.859026, 1.864015)                                                           # 611
+# cgresid_md_fa_gr of su3_rhmc_hisq.1.sample-in, per pseudofermion
+MILC_RESID = [(5e-4, 1e-4, 1e-4)] + [(5e-6, 1e-6, 1e-6)] * 3
+
+
+@pytest.mark.parametrize("mixed_md", [0, 1])
+def test_milc_su3_rhmc_hisq_trajectory(mixed_md):
+    links, info = lio.reload_lattice(os.path.join(HERE, "golden", "io", "lat.sample.l6666.hisq"))
+    assert info["checksum_ok"]
+    dims = (6, 6, 6, 6)
+    pfs = mq.pseudofermions_from_file(os.path.join(HERE, "golden", "rationals.sample.su3_rhmc_hisq"),
+                                      [r[0] for r in MILC_RESID], [r[2] for r in MILC_RESID],
+                                      [r[1] for r in MILC_RESID])
+    assert len(pfs) == 4
+    coeff = mq.gauge_loop_coeffs(0.89, 3)
+    with mq.Lattice(dims) as G:
+        G.dev_load_thin_links(links, phases_in=False)
+        G.dev_seed_rng(MILC_ISEED, precision=1)
+        st = G.dev_rhmc_trajectory(10, 0.02, 7.4, coeff, pfs, integrator=mq.INT_3G1F, mixed_md=mixed_md,
+                                   accept=1, site_rng=True)
+        U1 = G.dev_read_thin_links(with_phases=False)
+    print({k: v for k, v in st.items()})
+    np.testing.assert_allclose(st["grsource"], MILC_GRSOURCE, rtol=3e-11)
+    assert st["g0"] == pytest.approx(MILC_ACTION0[0], rel=2e-7)
+    assert st["h0"] == pytest.approx(MILC_ACTION0[1], abs=1e-4)   # MILC sums the float momenta in float
+    assert st["f0"] == pytest.approx(MILC_ACTION0[2], abs=0.05)
+    assert st["fermion_forces"] == 10 and st["gauge_forces"] == 30 and st["link_rebuilds"] == 11
+    assert st["reunit_errors"] == 0
+    assert st["accepted"] == 1
+    assert abs(st["dH"] - MILC_DELTA_S) < 0.02                     # su3_rhmc_hisq.1.errtol
+    ss, st_ = oio.plaquette(U1[oio.node_index_of_lex(list(dims))], list(dims))
+    assert abs(ss - MILC_PLAQ[0]) < 4e-5 and abs(st_ - MILC_PLAQ[1]) < 4e-5
+    for k, ref in zip(("g1", "h1", "f1"), MILC_ACTION1):
+        assert abs(st[k] - ref) < 0.05, (k, st[k], ref)
+
+
+def test_device_generators_match_oracle():
+    """milc_hip_dev_ranmom / dev_gaussian_source draw MILC's numbers: the
+    oracle's restatement (pinned to MILC's printed sums) gives the same values"""
+    dims = (6, 4, 8, 10)
+    L = ko.Lattice(dims)
+    E = L.parity_slice(ko.EVEN)
+    O = L.parity_slice(ko.ODD)
+    for single in (True, False):
+        st = L.prn_seed(4321)
+        mom_ref = L.ranmom_prn(st, single=single)
+        v_ref, s_ref = L.gaussian_prn(st, ko.EVEN, single=single)
+        w_ref, t_ref = L.gaussian_prn(st, ko.ODD, single=single)
+        with mq.Lattice(dims) as G:
+            G.dev_seed_rng(4321, precision=1 if single else 2)
+            mom = G.dev_ranmom()
+            s = G.dev_gaussian_source(mq.EVEN, 3)
+            t = G.dev_gaussian_source(mq.ODD, 4)
+            v = G.dev_read(3, mq.EVEN)
+            w = G.dev_read(4, mq.ODD)
+        assert np.abs(mom - mom_ref).max() < 1e-15
+        assert np.abs(v[E] - v_ref[E]).max() < 1e-15 and np.abs(w[O] - w_ref[O]).max() < 1e-15
+        # single: the sum is MILC's float accumulation, the oracle's is FP64
+        assert s == pytest.approx(s_ref, rel=1e-7 if single else 1e-13)
+        assert t == pytest.approx(t_ref, rel=1e-7 if single else 1e-13)
+
+
+def test_reunitarize_vs_oracle():
+    """milc_hip_reunitarize with the KS phases in = the oracle's reunit_su3 of
+    the bare links, phases put back"""
+    L = ko.Lattice(DIMS)
+    rng = np.random.default_rng(11)
+    U = L.gauge_random(1201, 0.4)
+    P = np.ascontiguousarray(U + 2e-3 * rng.standard_normal(U.shape))
+    ref = P.copy()
+    n_ref, md_ref = L.reunitarize(ref, tol=1e-4)
+    with mq.Lattice(DIMS) as G:
+        X = L.rephase(P.copy())
+        n, md = G.reunitarize(X, tolerance=1e-4, phases_in=True)
+        Y = P.copy()
+        n2, md2 = G.reunitarize(Y, tolerance=1e-4, phases_in=False)
+    assert np.abs(L.rephase(X) - ref).max() < 1e-14
+    assert np.abs(Y - ref).max() < 1e-14
+    assert n == n_ref == n2 and md == pytest.approx(md_ref, rel=1e-10) and md2 == pytest.approx(md_ref, rel=1e-10)
 
 
 def test_start_actions_vs_oracle():
@@ -68,6 +175,7 @@ def test_start_actions_vs_oracle():
         print(st)
     assert abs(st["g0"] - g_ref) < 1e-12 * abs(g_ref)
     assert abs(st["f0"] - f_ref) < 1e-9 * abs(f_ref)
+    assert st["grsource"][0] == pytest.approx(float(np.sum(chi[E] ** 2)), rel=1e-12)
 
 
 def test_energy_conservation_and_restore():
@@ -77,19 +185,39 @@ def test_energy_conservation_and_restore():
         a = traj(G, 4, 0.05)
         assert np.array_equal(G.dev_read_thin_links(), U0)  # accept = 2 restored the links
         b = traj(G, 8, 0.025)
-        o = traj(G, 4, 0.05, integrator=1)
+        o = traj(G, 4, 0.05, integrator=mq.INT_OMELYAN)
         print("leapfrog dH", a["dH"], b["dH"], "omelyan", o["dH"])
         assert 3.0 < abs(a["dH"]) / abs(b["dH"]) < 5.5
         assert abs(o["dH"]) < abs(a["dH"])
         assert a["fermion_forces"] == 4 and o["fermion_forces"] == 4
-        # accept = 0 keeps the evolved links, which stay unitary
+        # the links are rebuilt for each fermion kick, the final actions and,
+        # with accept = 2, once more for the restored links
+        assert a["link_rebuilds"] == 6 and o["link_rebuilds"] == 6
+        # accept = 0 keeps the evolved links, which are SU(3) after the reunitarisation
         k = traj(G, 4, 0.05, accept=0)
-        assert k["accepted"] == 1
+        assert k["accepted"] == 1 and k["reunit_errors"] == 0 and 0 < k["reunit_max_deviation"] < 1e-12
         U1 = G.dev_read_thin_links()
         assert not np.array_equal(U1, U0)
         Uc = U1[..., 0] + 1j * U1[..., 1]
         dev = np.abs(np.conj(np.swapaxes(Uc, -1, -2)) @ Uc - np.eye(3)).max()
-        assert dev < 1e-12
+        assert dev < 1e-14
+        assert np.abs(np.linalg.det(Uc) - 1).max() < 1e-14
+
+
+@pytest.mark.parametrize("alg,ngauge", [(mq.INT_2G1F, 2), (mq.INT_3G1F, 3), (mq.INT_5G1F, 5), (mq.INT_6G1F, 6)])
+def test_multi_timescale_integrators(alg, ngauge):
+    """INT_nG1F: n gauge kicks per fermion kick; dH falls about 4x when eps
+    halves, and stays below the leapfrog's at the same step"""
+    with mq.Lattice(DIMS) as G:
+        G.dev_gauge_hisq(1200, 0.4)
+        a = traj(G, 4, 0.05, integrator=alg)
+        b = traj(G, 8, 0.025, integrator=alg)
+        lf = traj(G, 4, 0.05)
+    print(alg, a["dH"], b["dH"], lf["dH"])
+    assert a["fermion_forces"] == 4 and a["gauge_forces"] == 4 * ngauge
+    assert a["link_rebuilds"] == 6
+    assert 2.5 < abs(a["dH"]) / abs(b["dH"]) < 6.0
+    assert abs(a["dH"]) < abs(lf["dH"])
 
 
 def test_mixed_md_solves_track_fp64():
@@ -99,7 +227,5 @@ def test_mixed_md_solves_track_fp64():
     with mq.Lattice(DIMS) as G:
         G.dev_gauge_hisq(1200, 0.4)
         a = traj(G, 4, 0.05)
-        b = G.dev_rhmc_trajectory(4, 0.05, BETA, COEFF, MASS, rat(MD), rat(GR), rat(MD), integrator=0,
-                                  resid_md=1e-10, resid_gr=1e-12, resid_fa=1e-12, mixed_md=1, action_from_md=True,
-                                  accept=2, seed=7)
+        b = traj(G, 4, 0.05, mixed_md=1, resid_md=1e-10)
     assert abs(a["dH"] - b["dH"]) < 1e-5 * max(1.0, abs(a["dH"]))
