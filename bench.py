@@ -457,7 +457,8 @@ def trajectory_workload(args):
     functions of its test, masses 0.0018 / 0.018 folded into the poles, its
     solver residuals): Gaussian momenta, phi_i = R_GR chi_i, actions, the
     3G1F integrator (gauge force 3x per fermion force, HISQ fermion force
-    from all 30 MD terms, mixed-precision MD solves), fat and Naik links
+    from all 30 MD terms), mixed-precision solves (FP32 multi-shift, every
+    shift refined to its FP64 true residual), fat and Naik links
     rebuilt when used, reunitarisation per step pair, FP64 dH.  Each timed
     trajectory starts from the same links (the evolved links are restored)."""
     import milc_qcd_amd as mq
@@ -470,7 +471,7 @@ def trajectory_workload(args):
     runs = []
     for i in range(args.warmup + args.steps):
         st = G.dev_rhmc_trajectory(TRAJ_STEPS, TRAJ_EPS, TRAJ_BETA, coeff, pfs, integrator=mq.INT_3G1F,
-                                   mixed_md=1, accept=2, seed=100 + i)
+                                   mixed_md=1, mixed_action=1, accept=2, seed=100 + i)
         log(f"trajectory {i}: {st}")
         if i >= args.warmup:
             runs.append(st)
@@ -488,7 +489,7 @@ def trajectory_workload(args):
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 actions, forces and link updates; f32/16-bit mixed MD solves",
+        "dtype": "f64 actions, forces and link updates; f32/16-bit mixed solves to f64 true residuals",
         "data": "synthetic",
         "config": {"workload": "rhmc_trajectory_48x48x48x96_3g1f4", "lattice": list(TRAJ_DIMS),
                    "integrator": "INT_3G1F", "md_steps": TRAJ_STEPS, "epsilon": TRAJ_EPS, "beta": TRAJ_BETA,

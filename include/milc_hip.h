@@ -215,6 +215,8 @@ typedef struct {
   int n_pseudo;          /* 1..MILC_HIP_MAX_PSEUDO */
   milc_hip_pseudofermion pf[MILC_HIP_MAX_PSEUDO];
   int mixed_md;          /* MD solves: FP32 multi-shift + 16-bit mixed refinement */
+  int mixed_action;      /* heatbath and action solves the same way (each shift still
+                            refined to its FP64 true residual) */
   int action_from_md;    /* 1: fermion action phi^dag R_MD phi (the MD force's own
                             action, for integrator tests); 0: |R_FA phi|^2 */
   int accept;            /* 0 keep the new links, 1 Metropolis, 2 restore the old */
@@ -240,7 +242,8 @@ typedef struct {
  * gauge kicks, largest reunitarisation deviation, reunitarisation errors, link
  * rebuilds, [20..27] |chi_i|^2 of the heatbath noise (MILC's "GRSOURCE: sum";
  * with site_rng and single-precision generators summed as MILC's
- * single-precision build does), [28] action solver iterations, [29..31] 0.
+ * single-precision build does), [28] action solver iterations, seconds in
+ * [29] set-up, [30] heatbath, [31] MD solves (part of the fermion kicks).
  * Returns -6 when a reunitarisation finds more than 100 deviations (MILC
  * terminates there). */
 int milc_hip_dev_rhmc_trajectory(const milc_hip_rhmc_params* p, double* stats);

@@ -194,6 +194,7 @@ int milc_hip_finalize(void) {
   c.staging = nullptr;
   c.staging_elems = 0;
   c.ws = Workspace();
+  dev_pool_release();
   (void)hipStreamDestroy(c.stream);
   c.stream = nullptr;
   c.link_key = c.link_key2 = nullptr;
@@ -1841,6 +1842,7 @@ extern "C" int milc_hip_dev_rhmc_trajectory(const milc_hip_rhmc_params* p, doubl
     b.resid_fa = a.resid_fa;
   }
   q.mixed_md = p->mixed_md;
+  q.mixed_action = p->mixed_action;
   q.action_from_md = p->action_from_md;
   q.accept = p->accept;
   q.site_rng = p->site_rng;

@@ -7,6 +7,9 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <iterator>
+#include <map>
+#include <unordered_map>
 #include <vector>
 
 #include "mh_dslash.h"
@@ -38,15 +41,77 @@ int capture_graph(HipGraph& gr, const std::function<int()>& enqueue) {
   return 0;
 }
 
+// Device memory: freed blocks are kept by size and handed out again, so the
+// solvers' per-call fields cost neither hipMalloc nor hipFree's implicit device
+// synchronisation.  All work is ordered on the context stream, so a block
+// reused by a later call is touched only after the kernels of the earlier one.
+// The pool keeps at most a quarter of the device memory and gives everything
+// back when an allocation fails or the context ends.
+namespace {
+struct Pool {
+  std::unordered_map<void*, size_t> live;
+  std::multimap<size_t, void*> spare;
+  size_t spare_bytes = 0, cap = 0;
+};
+Pool g_pool;
+
+void pool_release_spare(size_t keep) {
+  if (g_pool.spare.empty()) return;
+  if (ctx().stream) (void)hipStreamSynchronize(ctx().stream);
+  // largest blocks first
+  while (!g_pool.spare.empty() && g_pool.spare_bytes > keep) {
+    auto it = std::prev(g_pool.spare.end());
+    (void)hipFree(it->second);
+    g_pool.spare_bytes -= it->first;
+    g_pool.spare.erase(it);
+  }
+}
+}  // namespace
+
 int dev_alloc(void** p, size_t bytes) {
   *p = nullptr;
   if (bytes == 0) return 0;
-  MH_CHECK(hipMalloc(p, bytes));
+  bytes = (bytes + 255) & ~(size_t)255;
+  auto it = g_pool.spare.find(bytes);
+  if (it != g_pool.spare.end()) {
+    *p = it->second;
+    g_pool.spare.erase(it);
+    g_pool.spare_bytes -= bytes;
+    g_pool.live[*p] = bytes;
+    return 0;
+  }
+  hipError_t e = hipMalloc(p, bytes);
+  if (e == hipErrorOutOfMemory) {
+    (void)hipGetLastError();
+    pool_release_spare(0);
+    e = hipMalloc(p, bytes);
+  }
+  if (e != hipSuccess) {
+    *p = nullptr;
+    MH_CHECK(e);
+  }
+  g_pool.live[*p] = bytes;
   return 0;
 }
+
 void dev_free(void* p) {
-  if (p) (void)hipFree(p);
+  if (!p) return;
+  auto it = g_pool.live.find(p);
+  if (it == g_pool.live.end()) {
+    (void)hipFree(p);
+    return;
+  }
+  if (!g_pool.cap) {
+    size_t fr = 0, tot = 0;
+    g_pool.cap = hipMemGetInfo(&fr, &tot) == hipSuccess ? tot / 4 : ((size_t)16 << 30);
+  }
+  g_pool.spare.emplace(it->second, p);
+  g_pool.spare_bytes += it->second;
+  g_pool.live.erase(it);
+  if (g_pool.spare_bytes > g_pool.cap) pool_release_spare(g_pool.cap / 2);
 }
+
+void dev_pool_release() { pool_release_spare(0); }
 
 int alloc_vec(VecField& v, int prec, int stride) {
   v.stride = stride;

@@ -81,6 +81,8 @@ int capture_graph(HipGraph& gr, const std::function<int()>& enqueue);
 // device memory helpers
 int dev_alloc(void** p, size_t bytes);
 void dev_free(void* p);
+// hand the pooled free blocks back to the device (end of context)
+void dev_pool_release();
 int alloc_vec(VecField& v, int prec, int stride);
 void free_vec(VecField& v);
 int alloc_links(LinkField& l, int prec, int stride);
@@ -174,7 +176,7 @@ struct RhmcParams {
   double epsilon, beta, loop_coeff[3];
   int n_pseudo;
   RhmcPseudo pf[RHMC_MAX_PSEUDO];
-  int mixed_md, action_from_md, accept, site_rng;
+  int mixed_md, mixed_action, action_from_md, accept, site_rng;
   uint64_t seed;
 };
 // stats: 32 doubles (see milc_hip_dev_rhmc_trajectory)

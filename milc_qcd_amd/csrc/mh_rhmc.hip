@@ -134,11 +134,11 @@ static int rational_solve(const VecField& phi, int n, const double* offsets, dou
 
 // dst = res[0] src + sum_k res[k] (M^dag M + off[k])^-1 src on even sites (ks_rateval)
 static int rational_apply(const VecField& src, int order, const double* offs, const double* res, double resid,
-                          VecField& dst, int* iters) {
+                          int mixed, VecField& dst, int* iters) {
   Context& c = ctx();
   std::vector<VecField> x;
   SolveStats st;
-  int rc = rational_solve(src, order, offs, resid, 0, EVEN, x, st);
+  int rc = rational_solve(src, order, offs, resid, mixed, EVEN, x, st);
   if (!rc) {
     RatSet rs{};
     for (int t = 0; t < order; t++) {
@@ -307,7 +307,7 @@ struct Traj {
   double *mom = nullptr, *mtmp = nullptr;
   std::vector<Pseudo> pf;
   bool dirty = false;  // thin links changed since the fat and Naik links were built
-  double t_ferm = 0, t_gauge = 0, t_links = 0;
+  double t_ferm = 0, t_gauge = 0, t_links = 0, t_md_solve = 0;
   long md_iters = 0, fa_iters = 0;
   int n_ferm = 0, n_gauge = 0, n_rebuild = 0, reunit_errors = 0;
   double reunit_maxdev = 0;
@@ -399,6 +399,8 @@ struct Traj {
         res.push_back(p.pf[i].md.res[t + 1]);
       }
     }
+    (void)hipStreamSynchronize(c.stream);
+    t_md_solve += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (!rc) rc = force_level0_from_solutions(all, res.data(), o1, o3);
     for (auto& v : all) free_vec(v);
     if (!rc) rc = hisq_aux_dev(c.thin, V, W);
@@ -427,6 +429,7 @@ struct Traj {
     *g = p.beta / 3.0 * ga;
     if (!rc) rc = mom_action_dev(mom, h);
     if (!rc) rc = fresh();
+    if (!rc && p.mixed_action) rc = ensure_links(P_F32);
     double fs = 0;
     for (int i = 0; i < p.n_pseudo && !rc; i++) {
       const RhmcPseudo& q = p.pf[i];
@@ -434,7 +437,7 @@ struct Traj {
         // phi^dag R_MD phi: the action whose derivative the MD force is
         std::vector<VecField> x;
         SolveStats st;
-        rc = rational_solve(pf[i].phi, q.md.order, pf[i].offs_md + 1, q.resid_fa, 0, EVEN, x, st);
+        rc = rational_solve(pf[i].phi, q.md.order, pf[i].offs_md + 1, q.resid_fa, p.mixed_action, EVEN, x, st);
         fa_iters += st.iters;
         double s = 0, n2 = 0;
         for (int t = 0; t < q.md.order && !rc; t++) {
@@ -448,7 +451,7 @@ struct Traj {
       } else {
         double a = 0;
         int it = 0;
-        rc = rhmc_action(pf[i].phi, q.fa.order, pf[i].offs_fa, q.fa.res, q.resid_fa, 0, &a, &it);
+        rc = rhmc_action(pf[i].phi, q.fa.order, pf[i].offs_fa, q.fa.res, q.resid_fa, p.mixed_action, &a, &it);
         fa_iters += it;
         fs += a;
       }
@@ -508,9 +511,12 @@ int rhmc_trajectory(const RhmcParams& p, double* stats) {
     if ((rc = alloc_vec(P.phi, P_F64, c.stride))) return rc;
   VecField chi{};
   if ((rc = alloc_vec(chi, P_F64, c.stride))) return rc;
+  (void)hipStreamSynchronize(c.stream);
+  const double t_setup = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
   // ranmom, then grsource_imp_rhmc for each pseudofermion: phi = R_GR(M^dag M) chi
   double grsum[RHMC_MAX_PSEUDO] = {0};
   int it_gr = 0;
+  if (p.mixed_action && (rc = ensure_links(P_F32))) return rc;
   rc = p.site_rng ? ranmom_prn_dev(T.mom) : ranmom_dev(host_mix(p.seed), T.mom);
   for (int i = 0; i < p.n_pseudo && !rc; i++) {
     if (p.site_rng) {
@@ -521,12 +527,15 @@ int rhmc_trajectory(const RhmcParams& p, double* stats) {
     }
     int it = 0;
     if (!rc)
-      rc = rational_apply(chi, p.pf[i].gr.order, T.pf[i].offs_gr + 1, p.pf[i].gr.res, p.pf[i].resid_gr, T.pf[i].phi,
-                          &it);
+      rc = rational_apply(chi, p.pf[i].gr.order, T.pf[i].offs_gr + 1, p.pf[i].gr.res, p.pf[i].resid_gr,
+                          p.mixed_action, T.pf[i].phi, &it);
     it_gr += it;
   }
   free_vec(chi);
   if (rc) return rc;
+  (void)hipStreamSynchronize(c.stream);
+  const double t_heatbath =
+      std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count() - t_setup;
   double g0, h0, f0, g1, h1, f1;
   auto t_a0 = std::chrono::steady_clock::now();
   if ((rc = T.actions(&g0, &h0, &f0))) return rc;
@@ -570,6 +579,9 @@ int rhmc_trajectory(const RhmcParams& p, double* stats) {
                     (double)T.reunit_errors, (double)T.n_rebuild};
   for (int i = 0; i < p.n_pseudo && i < 8; i++) out[20 + i] = grsum[i];
   out[28] = (double)T.fa_iters;
+  out[29] = t_setup;
+  out[30] = t_heatbath;
+  out[31] = T.t_md_solve;
   memcpy(stats, out, sizeof out);
   return 0;
 }

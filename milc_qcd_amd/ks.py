@@ -124,7 +124,8 @@ class Pseudofermion(ctypes.Structure):
 class RhmcParams(ctypes.Structure):
     _fields_ = [("steps", ctypes.c_int), ("integrator", ctypes.c_int), ("epsilon", ctypes.c_double),
                 ("beta", ctypes.c_double), ("loop_coeff", ctypes.c_double * 3), ("n_pseudo", ctypes.c_int),
-                ("pf", Pseudofermion * MAX_PSEUDO), ("mixed_md", ctypes.c_int), ("action_from_md", ctypes.c_int),
+                ("pf", Pseudofermion * MAX_PSEUDO), ("mixed_md", ctypes.c_int), ("mixed_action", ctypes.c_int),
+                ("action_from_md", ctypes.c_int),
                 ("accept", ctypes.c_int), ("site_rng", ctypes.c_int), ("seed", ctypes.c_ulonglong)]
 
 
@@ -517,14 +518,15 @@ class Lattice:
         return a.value
 
     def dev_rhmc_trajectory(self, steps, epsilon, beta, loop_coeff, pseudofermions, integrator=INT_OMELYAN,
-                            mixed_md=1, action_from_md=False, accept=1, seed=1, site_rng=False):
+                            mixed_md=1, action_from_md=False, accept=1, seed=1, site_rng=False, mixed_action=0):
         """one RHMC trajectory on the device links (update(), ks_imp_rhmc/update_rhmc.c);
         pseudofermions: list of Pseudofermion"""
         self._activate()
         if not 1 <= len(pseudofermions) <= MAX_PSEUDO:
             raise MilcHipError(f"1..{MAX_PSEUDO} pseudofermions")
         p = RhmcParams(steps=steps, integrator=integrator, epsilon=epsilon, beta=beta, n_pseudo=len(pseudofermions),
-                       mixed_md=int(mixed_md), action_from_md=int(bool(action_from_md)), accept=accept,
+                       mixed_md=int(mixed_md), mixed_action=int(mixed_action),
+                       action_from_md=int(bool(action_from_md)), accept=accept,
                        site_rng=int(bool(site_rng)), seed=seed)
         for k in range(3):
             p.loop_coeff[k] = loop_coeff[k]
@@ -535,6 +537,7 @@ class Lattice:
         out = dict(zip(TRAJECTORY_STATS, list(st)))
         out["grsource"] = list(st)[20:20 + len(pseudofermions)]
         out["action_iters"] = st[28]
+        out["setup_seconds"], out["heatbath_seconds"], out["md_solve_seconds"] = st[29], st[30], st[31]
         return out
 
     def dev_seed_rng(self, iseed, precision=1):

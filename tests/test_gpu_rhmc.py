@@ -79,8 +79,8 @@ MILC_PLAQ = (1.859026, 1.864015)                                                
 MILC_RESID = [(5e-4, 1e-4, 1e-4)] + [(5e-6, 1e-6, 1e-6)] * 3
 
 
-@pytest.mark.parametrize("mixed_md", [0, 1])
-def test_milc_su3_rhmc_hisq_trajectory(mixed_md):
+@pytest.mark.parametrize("mixed", [0, 1])
+def test_milc_su3_rhmc_hisq_trajectory(mixed):
     links, info = lio.reload_lattice(os.path.join(HERE, "golden", "io", "lat.sample.l6666.hisq"))
     assert info["checksum_ok"]
     dims = (6, 6, 6, 6)
@@ -92,8 +92,8 @@ def test_milc_su3_rhmc_hisq_trajectory(mixed_md):
     with mq.Lattice(dims) as G:
         G.dev_load_thin_links(links, phases_in=False)
         G.dev_seed_rng(MILC_ISEED, precision=1)
-        st = G.dev_rhmc_trajectory(10, 0.02, 7.4, coeff, pfs, integrator=mq.INT_3G1F, mixed_md=mixed_md,
-                                   accept=1, site_rng=True)
+        st = G.dev_rhmc_trajectory(10, 0.02, 7.4, coeff, pfs, integrator=mq.INT_3G1F, mixed_md=mixed,
+                                   mixed_action=mixed, accept=1, site_rng=True)
         U1 = G.dev_read_thin_links(with_phases=False)
     print({k: v for k, v in st.items()})
     np.testing.assert_allclose(st["grsource"], MILC_GRSOURCE, rtol=3e-11)
