@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <array>
 #include <cstring>
+#include <initializer_list>
 #include <vector>
 
 #include "mh_internal.h"
@@ -917,28 +918,143 @@ void gauge_table_release() {
   g_gauge_table = GaugeTable{};
 }
 
+// The 48 staples of a link in direction mu, as paths from x+mu back to x:
+// signed directions (d < 4 forward, 7 - d backward) packed 3 bits per step,
+// path length, and the loop class (0 plaquette, 1 rectangle, 2 parallelogram).
+struct GaugeStaples {
+  int code[4][48];
+  int len[4][48];
+  int cls[4][48];
+};
+
+static GaugeStaples make_gauge_staples() {
+  GaugeStaples t{};
+  for (int mu = 0; mu < 4; mu++) {
+    const int f = mu, b = 7 - mu;
+    int q = 0;
+    auto add = [&](std::initializer_list<int> d, int cls) {
+      int code = 0, k = 0;
+      for (int x : d) code |= x << (3 * k++);
+      t.code[mu][q] = code;
+      t.len[mu][q] = k;
+      t.cls[mu][q] = cls;
+      q++;
+    };
+    for (int nu = 0; nu < 4; nu++) {
+      if (nu == mu) continue;
+      for (int s = 0; s < 2; s++) {
+        const int dn = s ? 7 - nu : nu, dnb = 7 - dn;
+        add({dn, b, dnb}, 0);
+        add({f, dn, b, b, dnb}, 1);
+        add({dn, b, b, dnb, f}, 1);
+        add({dn, dn, b, dnb, dnb}, 1);
+        for (int rho = 0; rho < 4; rho++) {
+          if (rho == mu || rho == nu) continue;
+          for (int r = 0; r < 2; r++) {
+            const int dr = r ? 7 - rho : rho;
+            add({dn, dr, b, dnb, 7 - dr}, 2);
+          }
+        }
+      }
+    }
+  }
+  return t;
+}
+
+// Gauge force of the Symanzik 1x1 + 1x2 + 1x1x1 action, one thread per link
+// (x, mu): G = sum of coeff x staple over the 48 staples S (U_mu(x) S a closed
+// loop) -- per nu != mu and sign s the plaquette [s nu, -mu, -s nu], the
+// rectangles [mu, s nu, -mu, -mu, -s nu], [s nu, -mu, -mu, -s nu, mu],
+// [s nu, s nu, -mu, -s nu, -s nu], per ordered pair nu != rho and signs the
+// parallelogram [s nu, r rho, -mu, -s nu, -r rho]: every loop of
+// imp_gauge_force_cpu (generic/gauge_force_imp.c:37-219) through the link,
+// once -- then mom += -eb3 TA(U_mu(x) G).  U: links without KS phases.
+//
+// Block order: workgroups are handed to the 8 XCDs in turn, so block b runs on
+// XCD b % 8; each XCD takes one contiguous eighth of the lattice (cbb_per
+// blocks of 128 sites) and, for each block of sites, all eight (parity,
+// direction) pairs in a row, so the links the staples share stay in its L2.
+__global__ void __launch_bounds__(128) k_gauge_staples(Geom g, LinkField U, GaugeStaples st, double c0, double c1,
+                                                       double c2, double eb3, double* __restrict__ mom,
+                                                       int cbb_per) {
+  const int xcd = blockIdx.x & 7, j = blockIdx.x >> 3;
+  const int pm = j & 7, cbb = xcd * cbb_per + (j >> 3);
+  int cb = cbb * 128 + threadIdx.x;
+  if (cb >= g.Vh) return;
+  const int pb = pm >> 2, mu = pm & 3;
+  Coord c = site_coords(g, cb, pb);
+  int x1[4] = {c.x, c.y, c.z, c.t};
+  x1[mu] = wrap(x1[mu] + 1, g.X[mu]);
+  Mat3<double> G, acc, L, T;
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) G.e[i][j].re = G.e[i][j].im = 0.0;
+#pragma unroll 1
+  for (int q = 0; q < 48; q++) {
+    const int code = st.code[mu][q], n = st.len[mu][q], cls = st.cls[mu][q];
+    const double w = cls == 0 ? c0 : (cls == 1 ? c1 : c2);
+    int pos[4] = {x1[0], x1[1], x1[2], x1[3]};
+#pragma unroll 1
+    for (int k = 0; k < n; k++) {
+      const int d = (code >> (3 * k)) & 7;
+      const bool back = d >= 4;
+      const int ld = back ? 7 - d : d;
+      int lpb, lcb;
+      if (!back) lcb = cb_of(g, pos, lpb);
+#pragma unroll
+      for (int a = 0; a < 4; a++)
+        if (a == ld) pos[a] = wrap(pos[a] + (back ? -1 : 1), g.X[a]);
+      if (back) lcb = cb_of(g, pos, lpb);
+      mload(U, lpb, ld, lcb, L);
+      if (back) madj(L);
+      if (k == 0) {
+        acc = L;
+      } else {
+        mmul(acc, L, T);
+        acc = T;
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        G.e[i][j].re += w * acc.e[i][j].re;
+        G.e[i][j].im += w * acc.e[i][j].im;
+      }
+  }
+  Mat3<double> X;
+  mload(U, pb, mu, cb, L);
+  mmul(L, G, X);
+  // mom += -eb3 TA(X) (make_anti_hermitian, libraries/make_ahmat.c)
+  double tr = (X.e[0][0].im + X.e[1][1].im + X.e[2][2].im) * (1.0 / 3.0);
+  const double sc = -eb3, h = 0.5 * sc;
+  double* o = mom + ((size_t)(pb * g.Vh + cb) * 4 + mu) * 10;
+  o[0] += h * (X.e[0][1].re - X.e[1][0].re);
+  o[1] += h * (X.e[0][1].im + X.e[1][0].im);
+  o[2] += h * (X.e[0][2].re - X.e[2][0].re);
+  o[3] += h * (X.e[0][2].im + X.e[2][0].im);
+  o[4] += h * (X.e[1][2].re - X.e[2][1].re);
+  o[5] += h * (X.e[1][2].im + X.e[2][1].im);
+  o[6] += sc * (X.e[0][0].im - tr);
+  o[7] += sc * (X.e[1][1].im - tr);
+  o[8] += sc * (X.e[2][2].im - tr);
+  o[9] = 0.0;
+}
+
 // mom_dev (packed, node_index order) += -eb3 TA(U G), G = sum over the loops
-// through each link of coeff (rest of the loop from the link's end, adjoint for
-// a link run backwards) -- imp_gauge_force_cpu, generic/gauge_force_imp.c:37-219.
-// U: links without KS phases.
+// through each link of coeff (rest of the loop from the link's end) --
+// imp_gauge_force_cpu, generic/gauge_force_imp.c:37-219.  U: links without KS
+// phases.
 int gauge_force_dev(const LinkField& U, const double coeff[3], double eb3, double* mom_dev) {
   Context& c = ctx();
-  GaugeTable* t;
-  int rc;
-  if ((rc = gauge_table(coeff, t))) return rc;
-  LinkField G{};
-  if ((rc = alloc_links(G, P_F64, c.stride))) return rc;
-  LinkField none{nullptr, 1.0f, 0};
-  int4 lo = make_int4(t->start[0], t->start[1], t->start[2], t->start[3]);
-  int4 hi = make_int4(t->start[1], t->start[2], t->start[3], t->start[4]);
-  dim3 grid((c.Vh + 127) / 128, 8);
-  hipLaunchKernelGGL(k_path_adjoint<0>, grid, dim3(128), 0, c.stream, c.geom, U, none, none, 0, t->paths, t->ent,
-                     lo, hi, G, U, 0.0, nullptr);
-  hipLaunchKernelGGL(k_mom_ta, grid, dim3(128), 0, c.stream, c.geom, U, G, -eb3, mom_dev, 1);
-  rc = hipGetLastError() == hipSuccess ? 0 : -1;
+  static const GaugeStaples st = make_gauge_staples();
+  const int nbx = (c.Vh + 127) / 128, cbb_per = (nbx + 7) / 8;
+  hipLaunchKernelGGL(k_gauge_staples, dim3(64 * cbb_per), dim3(128), 0, c.stream, c.geom, U, st, coeff[0], coeff[1],
+                     coeff[2], eb3, mom_dev, cbb_per);
+  MH_CHECK(hipGetLastError());
   MH_CHECK(hipStreamSynchronize(c.stream));
-  free_links(G);
-  return rc;
+  return 0;
 }
 
 // Per-block partial sums of coeff (3 - Re Tr P) over the loops based at each

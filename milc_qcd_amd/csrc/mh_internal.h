@@ -207,7 +207,9 @@ void batch_release();
 
 // ---- links ------------------------------------------------------------------
 int gauge_random_dev(uint64_t seed, double scale, LinkField& U);
-int hisq_links_dev(const LinkField& U, LinkField& fat, LinkField& lng, LinkField* W_out, int* nsvd);
+// V_out / W_out (nullable) receive the fat7 links and their U(3) projection
+int hisq_links_dev(const LinkField& U, LinkField& fat, LinkField& lng, LinkField* W_out, int* nsvd,
+                   LinkField* V_out = nullptr);
 int asqtad_links_dev(const LinkField& U, double u0, LinkField& fat, LinkField& lng);
 int hisq_aux_dev(const LinkField& U, LinkField& V, LinkField& W);
 int rephase_dev(LinkField& U);

@@ -471,7 +471,8 @@ static int fat_links_dev(const LinkField& U, const double coeff[6], LinkField& f
   return 0;
 }
 
-int hisq_links_dev(const LinkField& U, LinkField& fat, LinkField& lng, LinkField* W_out, int* nsvd) {
+int hisq_links_dev(const LinkField& U, LinkField& fat, LinkField& lng, LinkField* W_out, int* nsvd,
+                   LinkField* V_out) {
   Context& c = ctx();
   const double c1[6] = {1.0 / 8.0, 0.0, -1.0 / 8.0 * 0.5, 1.0 / 8.0 * 0.25 * 0.5, -1.0 / 8.0 * 0.125 * (1.0 / 6.0),
                         0.0};
@@ -479,7 +480,8 @@ int hisq_links_dev(const LinkField& U, LinkField& fat, LinkField& lng, LinkField
                         1.0 / 8.0 * 0.25 * 0.5, -1.0 / 8.0 * 0.125 * (1.0 / 6.0), -2.0 / 16.0};
   LinkField V, W;
   int rc;
-  if ((rc = alloc_links(V, P_F64, c.stride))) return rc;
+  if (V_out) V = *V_out;
+  else if ((rc = alloc_links(V, P_F64, c.stride))) return rc;
   if (W_out) W = *W_out;
   else if ((rc = alloc_links(W, P_F64, c.stride))) return rc;
   if ((rc = fat_links_dev(U, c1, V))) return rc;
@@ -488,7 +490,7 @@ int hisq_links_dev(const LinkField& U, LinkField& fat, LinkField& lng, LinkField
   MH_CHECK(hipMemsetAsync(d_nsvd, 0, sizeof(int), c.stream));
   hipLaunchKernelGGL(k_u3_unitarize, dim3(nb_for(c.Vh), 8), dim3(256), 0, c.stream, c.geom, V, W, d_nsvd);
   MH_CHECK(hipGetLastError());
-  free_links(V);
+  if (!V_out) free_links(V);
   if ((rc = fat_links_dev(W, c2, fat))) return rc;
   hipLaunchKernelGGL(k_long_links, dim3(nb_for(c.Vh), 8), dim3(256), 0, c.stream, c.geom, W, lng, c2[1]);
   MH_CHECK(hipGetLastError());

@@ -307,6 +307,7 @@ struct Traj {
   double *mom = nullptr, *mtmp = nullptr;
   std::vector<Pseudo> pf;
   bool dirty = false;  // thin links changed since the fat and Naik links were built
+  bool aux = false;    // V, W hold the fat7 links of the current thin links and their U(3) projection
   double t_ferm = 0, t_gauge = 0, t_links = 0, t_md_solve = 0;
   long md_iters = 0, fa_iters = 0;
   int n_ferm = 0, n_gauge = 0, n_rebuild = 0, reunit_errors = 0;
@@ -327,7 +328,8 @@ struct Traj {
     auto t0 = std::chrono::steady_clock::now();
     DevLinks& L = c.links[P_F64];
     int nsvd = 0;
-    int rc = hisq_links_dev(c.thin, L.fat, L.lng, nullptr, &nsvd);
+    int rc = hisq_links_dev(c.thin, L.fat, L.lng, &W, &nsvd, &V);
+    aux = rc == 0;
     for (int q = P_F32; q <= P_H16; q++)
       if (c.links[q].valid) {
         free_links(c.links[q].fat);
@@ -347,6 +349,7 @@ struct Traj {
     (void)hipStreamSynchronize(ctx().stream);
     t_links += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     dirty = true;
+    aux = false;
     return rc;
   }
   // reunitarize_ks (generic_ks/reunitarize_ks.c): phases out, reunit_su3, phases in
@@ -357,6 +360,7 @@ struct Traj {
     int rc = reunitarize_dev(ctx().thin, 1, kReunitTol, &err, &md);
     t_links += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     dirty = true;
+    aux = false;
     reunit_errors += err;
     reunit_maxdev = std::max(reunit_maxdev, md);
     if (!rc && err > kReunitMaxErrors) rc = -6;  // MILC terminates: "Unitarity error count exceeded"
@@ -403,7 +407,8 @@ struct Traj {
     t_md_solve += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (!rc) rc = force_level0_from_solutions(all, res.data(), o1, o3);
     for (auto& v : all) free_vec(v);
-    if (!rc) rc = hisq_aux_dev(c.thin, V, W);
+    if (!rc && !aux) rc = hisq_aux_dev(c.thin, V, W);
+    aux = rc == 0;
     if (!rc) rc = hisq_force_from_oprod(c.thin, V, W, o1, o3, kHisqLevel2, kHisqLevel1, 2.0 * e, kForceFilter, mtmp);
     if (!rc) {
       hipLaunchKernelGGL(k_axpy_flat, dim3(1024), dim3(256), 0, c.stream, mom, mtmp, nmom());
@@ -569,6 +574,7 @@ int rhmc_trajectory(const RhmcParams& p, double* stats) {
   if (!accepted) {
     MH_CHECK(hipMemcpyAsync(c.thin.data, T.Uold.data, link_bytes(P_F64, c.stride), hipMemcpyDeviceToDevice, c.stream));
     T.dirty = true;
+    T.aux = false;
   }
   // leave the fat and Naik links of the final thin links in place
   if ((rc = T.fresh())) return rc;
