@@ -117,6 +117,20 @@ __host__ __device__ __forceinline__ int neighbour(const Geom& g, const Coord& c,
   return cb + d * stride;
 }
 
+// Block order for kernels that read neighbours in every direction.  Workgroups
+// are dealt to the 8 XCDs in turn (block b runs on XCD b % 8), so a 1-D grid of
+// xcd_grid(nbx, ny) blocks gives XCD k the site blocks [k per, (k+1) per) and,
+// for each, all ny values of the second index in a row: the neighbour rows a
+// site block reads are then fetched through the same L2.
+__host__ __device__ __forceinline__ int xcd_per(int nbx) { return (nbx + 7) >> 3; }
+inline dim3 xcd_grid(int nbx, int ny) { return dim3(8 * xcd_per(nbx) * ny); }
+__device__ __forceinline__ bool xcd_block(int nbx, int ny, int& bx, int& by) {
+  const int j = blockIdx.x >> 3;
+  by = j % ny;
+  bx = (blockIdx.x & 7) * xcd_per(nbx) + j / ny;
+  return bx < nbx;
+}
+
 // MILC's KS phase of link (c, mu) with antiperiodic time boundary
 // (generic_ks/rephase.c:64-118): eta_x = (-1)^t, eta_y = (-1)^(t+x),
 // eta_z = (-1)^(t+x+y), eta_t = 1, times -1 on the last time slice.

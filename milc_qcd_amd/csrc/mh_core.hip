@@ -130,7 +130,11 @@ int alloc_links(LinkField& l, int prec, int stride) {
   l.scale = 1.0f;
   l.zscale = 1.0f;
   l.recon = 0;
-  return dev_alloc(&l.data, link_bytes(prec, stride));
+  int rc = dev_alloc(&l.data, link_bytes(prec, stride));
+  if (rc) return rc;
+  // pooled blocks keep their old contents: zero the field, padding included
+  MH_CHECK(hipMemsetAsync(l.data, 0, link_bytes(prec, stride), ctx().stream));
+  return 0;
 }
 void free_links(LinkField& l) {
   dev_free(l.data);
@@ -201,9 +205,12 @@ __global__ void k_links_to_milc(LinkField in, double* __restrict__ aos, int Vh) 
 }
 
 // max |element| of an FP64 device link field, for the 16-bit scale
-__global__ void k_link_absmax(const double2* __restrict__ p, size_t n, double* out_partial) {
+// largest |Re|, |Im| over the valid sites of an FP64 field of n / stride planes
+// (the padding of each plane, past Vh, holds whatever the memory held before)
+__global__ void k_link_absmax(const double2* __restrict__ p, size_t n, int stride, int Vh, double* out_partial) {
   double m = 0;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    if ((int)(i % stride) >= Vh) continue;
     double2 a = p[i];
     m = fmax(m, fmax(fabs(a.x), fabs(a.y)));
   }
@@ -412,7 +419,8 @@ static int link_absmax_f64(const LinkField& f, double* out) {
   Context& c = ctx();
   const int nb = 512;
   size_t n = 2ull * 4 * 9 * f.stride;
-  hipLaunchKernelGGL(k_link_absmax, dim3(nb), dim3(256), 0, c.stream, (const double2*)f.data, n, c.ws.partials);
+  hipLaunchKernelGGL(k_link_absmax, dim3(nb), dim3(256), 0, c.stream, (const double2*)f.data, n, f.stride, c.Vh,
+                     c.ws.partials);
   std::vector<double> h(nb);
   MH_CHECK(hipMemcpyAsync(h.data(), c.ws.partials, nb * sizeof(double), hipMemcpyDeviceToHost, c.stream));
   MH_CHECK(hipStreamSynchronize(c.stream));

@@ -172,8 +172,9 @@ __global__ void __launch_bounds__(128) k_path_adjoint(Geom g, LinkField Lk, Link
                                                       const FnEntry* __restrict__ ent, int4 e_lo, int4 e_hi,
                                                       LinkField Gout, LinkField U, double scale,
                                                       double* __restrict__ mom) {
-  int cb = blockIdx.x * blockDim.x + threadIdx.x;
-  int pm = blockIdx.y;
+  int bx, pm;
+  if (!xcd_block((g.Vh + blockDim.x - 1) / blockDim.x, 8, bx, pm)) return;
+  int cb = bx * blockDim.x + threadIdx.x;
   if (cb >= g.Vh) return;
   int pb = pm >> 2, nu = pm & 3;
   Coord c = site_coords(g, cb, pb);
@@ -545,8 +546,9 @@ __global__ void __launch_bounds__(128) k_sf_scale(Geom g, MF in, double c, MF ou
 
 // out = S_nu(A) for the mu link
 __global__ void __launch_bounds__(128) k_sf_fwd(Geom g, LinkField U, int mu, int nu, MF A, MF out) {
-  int cb = blockIdx.x * blockDim.x + threadIdx.x;
-  int pb = blockIdx.y;
+  int bx, pb;
+  if (!xcd_block((g.Vh + blockDim.x - 1) / blockDim.x, 2, bx, pb)) return;
+  int cb = bx * blockDim.x + threadIdx.x;
   if (cb >= g.Vh) return;
   int po = pb ^ 1;
   Coord c = site_coords(g, cb, pb);
@@ -574,8 +576,9 @@ __global__ void __launch_bounds__(128) k_sf_fwd(Geom g, LinkField U, int mu, int
 //                   + (A(y) U_nu(y+mu) G(y+nu))^dag + G(y-mu+nu) U_nu(y-mu)^dag A(y-mu)]
 __global__ void __launch_bounds__(128) k_sf_adj(Geom g, LinkField U, int mu, int nu, MF A, MF Gs, double cg, MF FA,
                                                 int want_fa, MF FU) {
-  int cb = blockIdx.x * blockDim.x + threadIdx.x;
-  int pb = blockIdx.y;
+  int bx, pb;
+  if (!xcd_block((g.Vh + blockDim.x - 1) / blockDim.x, 2, bx, pb)) return;
+  int cb = bx * blockDim.x + threadIdx.x;
   if (cb >= g.Vh) return;
   int po = pb ^ 1;
   Coord c = site_coords(g, cb, pb);
@@ -632,8 +635,9 @@ __global__ void __launch_bounds__(128) k_sf_adj(Geom g, LinkField U, int mu, int
 // Naik: G_mu(y) += cN [U(y+mu) U(y+2mu) F3(y) + U(y+mu) F3(y-mu) U(y-mu) + F3(y-2mu) U(y-2mu) U(y-mu)],
 // F3 multiplied by eps at its own site when with_eps
 __global__ void __launch_bounds__(128) k_sf_naik(Geom g, LinkField U, int mu, MF F3, int with_eps, double cN, MF G) {
-  int cb = blockIdx.x * blockDim.x + threadIdx.x;
-  int pb = blockIdx.y;
+  int bx, pb;
+  if (!xcd_block((g.Vh + blockDim.x - 1) / blockDim.x, 2, bx, pb)) return;
+  int cb = bx * blockDim.x + threadIdx.x;
   if (cb >= g.Vh) return;
   int po = pb ^ 1;
   Coord c = site_coords(g, cb, pb);
@@ -773,7 +777,7 @@ static int path_adjoint_launch(int mode, const LinkField& Lk, const LinkField& F
   if (rc) return rc;
   int4 lo = make_int4(t->start[0], t->start[1], t->start[2], t->start[3]);
   int4 hi = make_int4(t->start[1], t->start[2], t->start[3], t->start[4]);
-  dim3 grid((c.Vh + 127) / 128, 8);
+  dim3 grid = xcd_grid((c.Vh + 127) / 128, 8);
   if (mode == 0)
     hipLaunchKernelGGL(k_path_adjoint<0>, grid, dim3(128), 0, c.stream, c.geom, Lk, F1, F3, with_eps, t->paths,
                        t->ent, lo, hi, Gout, U, scale, mom);
@@ -1061,8 +1065,10 @@ int gauge_force_dev(const LinkField& U, const double coeff[3], double eb3, doubl
 // site (imp_gauge_action, generic/gauge_stuff.c:248-322)
 __global__ void __launch_bounds__(128) k_gauge_action(Geom g, LinkField U, const FnPath* __restrict__ paths,
                                                       int npaths, double* __restrict__ partials) {
-  int cb = blockIdx.x * blockDim.x + threadIdx.x;
-  int pb = blockIdx.y;
+  const int nbx = (g.Vh + blockDim.x - 1) / blockDim.x;
+  int bx, pb;
+  if (!xcd_block(nbx, 2, bx, pb)) return;
+  int cb = bx * blockDim.x + threadIdx.x;
   double s = 0.0;
   if (cb < g.Vh) {
     Coord c = site_coords(g, cb, pb);
@@ -1091,7 +1097,7 @@ __global__ void __launch_bounds__(128) k_gauge_action(Geom g, LinkField U, const
   __shared__ double sm[2];
   if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = s;
   __syncthreads();
-  if (threadIdx.x == 0) partials[blockIdx.y * gridDim.x + blockIdx.x] = sm[0] + sm[1];
+  if (threadIdx.x == 0) partials[pb * nbx + bx] = sm[0] + sm[1];
 }
 
 int gauge_action_dev(const LinkField& U, const double coeff[3], double* action) {
@@ -1102,7 +1108,7 @@ int gauge_action_dev(const LinkField& U, const double coeff[3], double* action) 
   const int nb = (c.Vh + 127) / 128;
   double* part = nullptr;
   if ((rc = dev_alloc((void**)&part, (size_t)2 * nb * sizeof(double)))) return rc;
-  hipLaunchKernelGGL(k_gauge_action, dim3(nb, 2), dim3(128), 0, c.stream, c.geom, U, t->paths, t->npaths, part);
+  hipLaunchKernelGGL(k_gauge_action, xcd_grid(nb, 2), dim3(128), 0, c.stream, c.geom, U, t->paths, t->npaths, part);
   std::vector<double> h((size_t)2 * nb);
   rc = hipGetLastError() == hipSuccess ? 0 : -1;
   if (!rc) rc = hipMemcpyAsync(h.data(), part, h.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream) ==
@@ -1339,39 +1345,40 @@ static int smear_adjoint_dev(const LinkField& L, const LinkField& F1, const Link
   const double c1 = coeff[0] - 6.0 * coeff[5], cN = coeff[1], c3 = coeff[2], c5 = coeff[3], c7 = coeff[4],
                cl = coeff[5];
   const bool staples = c3 != 0.0 || cl != 0.0 || c5 != 0.0 || c7 != 0.0;
-  dim3 grid((c.Vh + 127) / 128, 2), blk(128);
+  // init and scale: plain grid; the neighbour-reading kernels: XCD block order
+  dim3 grid((c.Vh + 127) / 128, 2), xgrid = xcd_grid((c.Vh + 127) / 128, 2), blk(128);
   for (int mu = 0; mu < 4; mu++) {
     const MF Gmu = mf_link(G, mu);
     hipLaunchKernelGGL(k_sf_init, grid, blk, 0, c.stream, c.geom, mf_link(F1, mu), with_eps, mF, Gmu, c1);
     if (staples) {
       for (int nu = 0; nu < 4; nu++) {
         if (nu == mu) continue;
-        hipLaunchKernelGGL(k_sf_fwd, grid, blk, 0, c.stream, c.geom, L, mu, nu, mf_link(L, mu), mS1);
+        hipLaunchKernelGGL(k_sf_fwd, xgrid, blk, 0, c.stream, c.geom, L, mu, nu, mf_link(L, mu), mS1);
         hipLaunchKernelGGL(k_sf_scale, grid, blk, 0, c.stream, c.geom, mF, c3, mG1);
         if (cl != 0.0)
-          hipLaunchKernelGGL(k_sf_adj, grid, blk, 0, c.stream, c.geom, L, mu, nu, mS1, mF, cl, mG1, 1,
+          hipLaunchKernelGGL(k_sf_adj, xgrid, blk, 0, c.stream, c.geom, L, mu, nu, mS1, mF, cl, mG1, 1,
                              mf_link(G, nu));
         if (c5 != 0.0 || c7 != 0.0) {
           for (int rho = 0; rho < 4; rho++) {
             if (rho == mu || rho == nu) continue;
-            hipLaunchKernelGGL(k_sf_fwd, grid, blk, 0, c.stream, c.geom, L, mu, rho, mS1, mS2);
+            hipLaunchKernelGGL(k_sf_fwd, xgrid, blk, 0, c.stream, c.geom, L, mu, rho, mS1, mS2);
             hipLaunchKernelGGL(k_sf_scale, grid, blk, 0, c.stream, c.geom, mF, c5, mG2);
             if (c7 != 0.0)
               for (int sig = 0; sig < 4; sig++) {
                 if (sig == mu || sig == nu || sig == rho) continue;
-                hipLaunchKernelGGL(k_sf_adj, grid, blk, 0, c.stream, c.geom, L, mu, sig, mS2, mF, c7, mG2, 1,
+                hipLaunchKernelGGL(k_sf_adj, xgrid, blk, 0, c.stream, c.geom, L, mu, sig, mS2, mF, c7, mG2, 1,
                                    mf_link(G, sig));
               }
-            hipLaunchKernelGGL(k_sf_adj, grid, blk, 0, c.stream, c.geom, L, mu, rho, mS1, mG2, 1.0, mG1, 1,
+            hipLaunchKernelGGL(k_sf_adj, xgrid, blk, 0, c.stream, c.geom, L, mu, rho, mS1, mG2, 1.0, mG1, 1,
                                mf_link(G, rho));
           }
         }
-        hipLaunchKernelGGL(k_sf_adj, grid, blk, 0, c.stream, c.geom, L, mu, nu, mf_link(L, mu), mG1, 1.0, Gmu, 1,
+        hipLaunchKernelGGL(k_sf_adj, xgrid, blk, 0, c.stream, c.geom, L, mu, nu, mf_link(L, mu), mG1, 1.0, Gmu, 1,
                            mf_link(G, nu));
       }
     }
     if (cN != 0.0)
-      hipLaunchKernelGGL(k_sf_naik, grid, blk, 0, c.stream, c.geom, L, mu, mf_link(F3, mu), with_eps, cN, Gmu);
+      hipLaunchKernelGGL(k_sf_naik, xgrid, blk, 0, c.stream, c.geom, L, mu, mf_link(F3, mu), with_eps, cN, Gmu);
   }
   MH_CHECK(hipGetLastError());
   MH_CHECK(hipStreamSynchronize(c.stream));

@@ -22,8 +22,9 @@ struct TermSet {
 };
 
 __global__ void __launch_bounds__(256) k_oprod(Geom g, TermSet ts, int nterms, LinkField o1, LinkField o3) {
-  int cb = blockIdx.x * blockDim.x + threadIdx.x;
-  int pm = blockIdx.y;
+  int bx, pm;
+  if (!xcd_block((g.Vh + blockDim.x - 1) / blockDim.x, 8, bx, pm)) return;
+  int cb = bx * blockDim.x + threadIdx.x;
   if (cb >= g.Vh) return;
   int pb = pm >> 2, mu = pm & 3;
   Coord c = site_coords(g, cb, pb);
@@ -78,7 +79,7 @@ int force_oprod_coeff(const std::vector<VecField>& xe, const std::vector<VecFiel
     ts.c1[t] = c1[t];
     ts.c3[t] = c3[t];
   }
-  hipLaunchKernelGGL(k_oprod, dim3((c.Vh + 255) / 256, 8), dim3(256), 0, c.stream, c.geom, ts, n, o1, o3);
+  hipLaunchKernelGGL(k_oprod, xcd_grid((c.Vh + 255) / 256, 8), dim3(256), 0, c.stream, c.geom, ts, n, o1, o3);
   MH_CHECK(hipGetLastError());
   return 0;
 }

@@ -205,8 +205,9 @@ __global__ void k_fat_init(Geom g, LinkField U, LinkField fat, int mu, double on
 // else fat_mu += coef*upper, then += coef*lower.
 __global__ void k_gen_staple(Geom g, LinkField U, LinkField Lf, int lmu, MatField Lm, MatField staple,
                              int save, LinkField fat, int mu, int nu, double coef) {
-  int cb = blockIdx.x * blockDim.x + threadIdx.x;
-  int pb = blockIdx.y;
+  int bx, pb;
+  if (!xcd_block((g.Vh + blockDim.x - 1) / blockDim.x, 2, bx, pb)) return;
+  int cb = bx * blockDim.x + threadIdx.x;
   if (cb >= g.Vh) return;
   int po = pb ^ 1;
   Coord c = site_coords(g, cb, pb);
@@ -441,24 +442,29 @@ static int fat_links_dev(const LinkField& U, const double coeff[6], LinkField& f
   MatField staple, temp, none{nullptr, 0};
   int rc;
   if ((rc = alloc_mat(staple, c.stride)) || (rc = alloc_mat(temp, c.stride))) return rc;
-  dim3 grid(nb_for(c.Vh), 2);
+  // k_fat_init streams: plain grid; k_gen_staple reads neighbours: XCD block order
+  dim3 grid(nb_for(c.Vh), 2), xgrid = xcd_grid(nb_for(c.Vh), 2);
   LinkField nol{nullptr, 1.0f, 0};
   for (int dir = 0; dir < 4; dir++) {
     hipLaunchKernelGGL(k_fat_init, grid, dim3(256), 0, c.stream, c.geom, U, fat, dir, one_link);
-    if (coeff[2] == 0.0 && coeff[5] == 0.0 && coeff[3] == 0.0) continue;
+    if (coeff[2] == 0.0 && coeff[5] == 0.0 && coeff[3] == 0.0 && coeff[4] == 0.0) continue;
     for (int nu = 0; nu < 4; nu++) {
       if (nu == dir) continue;
-      hipLaunchKernelGGL(k_gen_staple, grid, dim3(256), 0, c.stream, c.geom, U, U, dir, none, staple, 1, fat, dir,
+      hipLaunchKernelGGL(k_gen_staple, xgrid, dim3(256), 0, c.stream, c.geom, U, U, dir, none, staple, 1, fat, dir,
                          nu, coeff[2]);
-      hipLaunchKernelGGL(k_gen_staple, grid, dim3(256), 0, c.stream, c.geom, U, nol, -1, staple, none, 0, fat, dir,
-                         nu, coeff[5]);
+      // a term with coefficient 0 adds nothing: skipped
+      if (coeff[5] != 0.0)
+        hipLaunchKernelGGL(k_gen_staple, xgrid, dim3(256), 0, c.stream, c.geom, U, nol, -1, staple, none, 0, fat,
+                           dir, nu, coeff[5]);
+      if (coeff[3] == 0.0 && coeff[4] == 0.0) continue;
       for (int rho = 0; rho < 4; rho++) {
         if (rho == dir || rho == nu) continue;
-        hipLaunchKernelGGL(k_gen_staple, grid, dim3(256), 0, c.stream, c.geom, U, nol, -1, staple, temp, 1, fat,
+        hipLaunchKernelGGL(k_gen_staple, xgrid, dim3(256), 0, c.stream, c.geom, U, nol, -1, staple, temp, 1, fat,
                            dir, rho, coeff[3]);
+        if (coeff[4] == 0.0) continue;
         for (int sig = 0; sig < 4; sig++) {
           if (sig == dir || sig == nu || sig == rho) continue;
-          hipLaunchKernelGGL(k_gen_staple, grid, dim3(256), 0, c.stream, c.geom, U, nol, -1, temp, none, 0, fat,
+          hipLaunchKernelGGL(k_gen_staple, xgrid, dim3(256), 0, c.stream, c.geom, U, nol, -1, temp, none, 0, fat,
                              dir, sig, coeff[4]);
         }
       }
