@@ -110,7 +110,7 @@ __device__ __forceinline__ uint64_t lex_of(const Geom& g, const Coord& c) {
 }
 
 // U = exp(i*scale*H): 4 halvings, degree-14 Horner Taylor, 4 squarings.
-__global__ void k_gauge_random(Geom g, uint64_t key, double scale, LinkField U) {
+__global__ void __launch_bounds__(256) k_gauge_random(Geom g, uint64_t key, double scale, LinkField U) {
   int cb = blockIdx.x * blockDim.x + threadIdx.x;
   int pm = blockIdx.y;
   if (cb >= g.Vh) return;
@@ -203,7 +203,7 @@ __global__ void k_fat_init(Geom g, LinkField U, LinkField fat, int mu, double on
 //   lower(x) = [U_nu(x-nu)^dag L(x-nu)] U_nu(x-nu+mu)
 // staple != null: staple = upper + lower, fat_mu += coef*staple;
 // else fat_mu += coef*upper, then += coef*lower.
-__global__ void k_gen_staple(Geom g, LinkField U, LinkField Lf, int lmu, MatField Lm, MatField staple,
+__global__ void __launch_bounds__(256) k_gen_staple(Geom g, LinkField U, LinkField Lf, int lmu, MatField Lm, MatField staple,
                              int save, LinkField fat, int mu, int nu, double coef) {
   int bx, pb;
   if (!xcd_block((g.Vh + blockDim.x - 1) / blockDim.x, 2, bx, pb)) return;
@@ -259,8 +259,64 @@ __global__ void k_gen_staple(Geom g, LinkField U, LinkField Lf, int lmu, MatFiel
   st(f, fat.stride, cb, F);
 }
 
+// Up to three generalised staples of stored matrix fields for the mu link in
+// one pass: for job k, S = S_nu(in)(x) = U_nu(x) in(x+nu) U_nu(x+mu)^dag +
+// U_nu(x-nu)^dag in(x-nu) U_nu(x-nu+mu); fat_mu += coef S; out = S when out
+// is given.  The fat link is read and written once for all jobs.
+struct StapleJob {
+  MatField in;
+  int nu;
+  double coef;
+  MatField out;  // data == nullptr: not stored
+};
+struct StapleJobs {
+  StapleJob j[3];
+  int n;
+};
+
+__global__ void __launch_bounds__(256) k_gen_staples(Geom g, LinkField U, StapleJobs jobs, LinkField fat, int mu) {
+  int bx, pb;
+  if (!xcd_block((g.Vh + blockDim.x - 1) / blockDim.x, 2, bx, pb)) return;
+  int cb = bx * blockDim.x + threadIdx.x;
+  if (cb >= g.Vh) return;
+  int po = pb ^ 1;
+  Coord c = site_coords(g, cb, pb);
+  int xmu = neighbour(g, c, cb, mu, 1);  // x+mu, other parity
+  double2* f = lptr_w(fat, pb, mu);
+  Mat3<double> F;
+  ld(f, fat.stride, cb, F);
+  for (int k = 0; k < jobs.n; k++) {
+    const StapleJob& J = jobs.j[k];
+    const int nu = J.nu;
+    int xnu = neighbour(g, c, cb, nu, 1);   // x+nu, other parity
+    int xmn = neighbour(g, c, cb, nu, -1);  // x-nu, other parity
+    Coord cm = site_coords(g, xmn, po);
+    int xmnpm = neighbour(g, cm, xmn, mu, 1);  // x-nu+mu, parity of x
+    Mat3<double> A, B, C, t1, up, lo;
+    ld(mptr(J.in, po), J.in.stride, xnu, A);
+    ld(lptr(U, po, nu), U.stride, xmu, B);
+    mm_na(A, B, t1);
+    ld(lptr(U, pb, nu), U.stride, cb, C);
+    mm_nn(C, t1, up);
+    ld(lptr(U, po, nu), U.stride, xmn, A);
+    ld(mptr(J.in, po), J.in.stride, xmn, B);
+    mm_an(A, B, t1);
+    ld(lptr(U, pb, nu), U.stride, xmnpm, C);
+    mm_nn(t1, C, lo);
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) {
+        up.e[i][j].re += lo.e[i][j].re;
+        up.e[i][j].im += lo.e[i][j].im;
+        F.e[i][j].re += J.coef * up.e[i][j].re;
+        F.e[i][j].im += J.coef * up.e[i][j].im;
+      }
+    if (J.out.data) st(J.out.data + (size_t)pb * 9 * J.out.stride, J.out.stride, cb, up);
+  }
+  st(f, fat.stride, cb, F);
+}
+
 // lng_mu(x) = naik * U_mu(x) [U_mu(x+mu) U_mu(x+2mu)]
-__global__ void k_long_links(Geom g, LinkField U, LinkField lng, double naik) {
+__global__ void __launch_bounds__(256) k_long_links(Geom g, LinkField U, LinkField lng, double naik) {
   int cb = blockIdx.x * blockDim.x + threadIdx.x;
   int pm = blockIdx.y;
   if (cb >= g.Vh) return;
@@ -285,7 +341,7 @@ __global__ void k_long_links(Geom g, LinkField U, LinkField lng, double naik) {
 
 // W = V (V^dag V)^-1/2 with the KS phase taken out of V and put back into W;
 // counts links that meet MILC's SVD-fallback condition.
-__global__ void k_u3_unitarize(Geom g, LinkField Vf, LinkField Wf, int* nsvd) {
+__global__ void __launch_bounds__(256) k_u3_unitarize(Geom g, LinkField Vf, LinkField Wf, int* nsvd) {
   int cb = blockIdx.x * blockDim.x + threadIdx.x;
   int pm = blockIdx.y;
   if (cb >= g.Vh) return;
@@ -436,37 +492,42 @@ static int alloc_mat(MatField& m, int stride) {
 }
 
 // Optimised fattening: coeff = {one_link, naik, 3-staple, 5-staple, 7-staple, Lepage}
+// Per (mu, nu): the 3-staple of U (stored), then one pass for the Lepage term
+// and the two 5-staples of the 3-staple (stored), then one pass for the two
+// 7-staples of the 5-staples; terms with coefficient 0 are skipped.
 static int fat_links_dev(const LinkField& U, const double coeff[6], LinkField& fat) {
   Context& c = ctx();
   double one_link = coeff[0] - 6.0 * coeff[5];
-  MatField staple, temp, none{nullptr, 0};
+  MatField staple, temp, temp2, none{nullptr, 0};
   int rc;
-  if ((rc = alloc_mat(staple, c.stride)) || (rc = alloc_mat(temp, c.stride))) return rc;
-  // k_fat_init streams: plain grid; k_gen_staple reads neighbours: XCD block order
+  if ((rc = alloc_mat(staple, c.stride)) || (rc = alloc_mat(temp, c.stride)) || (rc = alloc_mat(temp2, c.stride)))
+    return rc;
+  // k_fat_init streams: plain grid; the staple kernels read neighbours: XCD block order
   dim3 grid(nb_for(c.Vh), 2), xgrid = xcd_grid(nb_for(c.Vh), 2);
-  LinkField nol{nullptr, 1.0f, 0};
+  const bool five = coeff[3] != 0.0 || coeff[4] != 0.0, seven = coeff[4] != 0.0;
   for (int dir = 0; dir < 4; dir++) {
     hipLaunchKernelGGL(k_fat_init, grid, dim3(256), 0, c.stream, c.geom, U, fat, dir, one_link);
-    if (coeff[2] == 0.0 && coeff[5] == 0.0 && coeff[3] == 0.0 && coeff[4] == 0.0) continue;
+    if (coeff[2] == 0.0 && coeff[5] == 0.0 && !five) continue;
     for (int nu = 0; nu < 4; nu++) {
       if (nu == dir) continue;
       hipLaunchKernelGGL(k_gen_staple, xgrid, dim3(256), 0, c.stream, c.geom, U, U, dir, none, staple, 1, fat, dir,
                          nu, coeff[2]);
-      // a term with coefficient 0 adds nothing: skipped
-      if (coeff[5] != 0.0)
-        hipLaunchKernelGGL(k_gen_staple, xgrid, dim3(256), 0, c.stream, c.geom, U, nol, -1, staple, none, 0, fat,
-                           dir, nu, coeff[5]);
-      if (coeff[3] == 0.0 && coeff[4] == 0.0) continue;
-      for (int rho = 0; rho < 4; rho++) {
-        if (rho == dir || rho == nu) continue;
-        hipLaunchKernelGGL(k_gen_staple, xgrid, dim3(256), 0, c.stream, c.geom, U, nol, -1, staple, temp, 1, fat,
-                           dir, rho, coeff[3]);
-        if (coeff[4] == 0.0) continue;
-        for (int sig = 0; sig < 4; sig++) {
-          if (sig == dir || sig == nu || sig == rho) continue;
-          hipLaunchKernelGGL(k_gen_staple, xgrid, dim3(256), 0, c.stream, c.geom, U, nol, -1, temp, none, 0, fat,
-                             dir, sig, coeff[4]);
-        }
+      int rho[2], sig[2], nr = 0;
+      for (int r = 0; r < 4; r++)
+        if (r != dir && r != nu) rho[nr++] = r;
+      for (int k = 0; k < 2; k++) sig[k] = 6 - dir - nu - rho[k];  // the fourth direction
+      StapleJobs a{};
+      if (coeff[5] != 0.0) a.j[a.n++] = StapleJob{staple, nu, coeff[5], none};
+      if (five) {
+        a.j[a.n++] = StapleJob{staple, rho[0], coeff[3], seven ? temp : none};
+        a.j[a.n++] = StapleJob{staple, rho[1], coeff[3], seven ? temp2 : none};
+      }
+      if (a.n) hipLaunchKernelGGL(k_gen_staples, xgrid, dim3(256), 0, c.stream, c.geom, U, a, fat, dir);
+      if (seven) {
+        StapleJobs b{};
+        b.j[b.n++] = StapleJob{temp, sig[0], coeff[4], none};
+        b.j[b.n++] = StapleJob{temp2, sig[1], coeff[4], none};
+        hipLaunchKernelGGL(k_gen_staples, xgrid, dim3(256), 0, c.stream, c.geom, U, b, fat, dir);
       }
     }
   }
@@ -474,6 +535,7 @@ static int fat_links_dev(const LinkField& U, const double coeff[6], LinkField& f
   MH_CHECK(hipStreamSynchronize(c.stream));
   dev_free(staple.data);
   dev_free(temp.data);
+  dev_free(temp2.data);
   return 0;
 }
 
