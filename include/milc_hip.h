@@ -245,7 +245,8 @@ typedef struct {
  * single-precision build does), [28] action solver iterations, seconds in
  * [29] set-up, [30] heatbath, [31] MD solves (part of the fermion kicks).
  * Returns -6 when a reunitarisation finds more than 100 deviations (MILC
- * terminates there). */
+ * terminates there); after any error the device links are the ones the call
+ * started with. */
 int milc_hip_dev_rhmc_trajectory(const milc_hip_rhmc_params* p, double* stats);
 
 /* MILC's per-site random number generators on the device: seeded as
