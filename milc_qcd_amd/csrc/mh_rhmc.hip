@@ -557,16 +557,17 @@ int rhmc_trajectory(const RhmcParams& p, double* stats) {
     if (!rc && sch.length - now > 1e-12) rc = T.update_u(e * (sch.length - now));
     if (!rc) rc = T.reunitarize();
   }
-  if (rc) {
-    // a failed trajectory leaves the starting links (and their fat and Naik links) in place
+  // a failed trajectory leaves the starting links (and their fat and Naik links) in place
+  auto fail = [&](int code) {
     (void)hipMemcpyAsync(c.thin.data, T.Uold.data, link_bytes(P_F64, c.stride), hipMemcpyDeviceToDevice, c.stream);
     T.dirty = true;
     T.aux = false;
     (void)T.fresh();
-    return rc;
-  }
+    return code;
+  };
+  if (rc) return fail(rc);
   auto t_a1 = std::chrono::steady_clock::now();
-  if ((rc = T.actions(&g1, &h1, &f1))) return rc;
+  if ((rc = T.actions(&g1, &h1, &f1))) return fail(rc);
   t_act += std::chrono::duration<double>(std::chrono::steady_clock::now() - t_a1).count();
   const double dH = (g1 + h1 + f1) - (g0 + h0 + f0);
   int accepted = 1;
