@@ -82,12 +82,18 @@ struct BMSStats {
 // x[j] (batched FP64, nr rhs) = (-D_eo D_oe + offsets[j])^-1 b for every rhs.
 int bmultishift(const void* b64, int nr, const std::vector<double>& offsets, const BMSParams& prm,
                 std::vector<void*>& x64, BMSStats& st);
+// The same with a shift set per rhs: x[j] rhs r = (-D_eo D_oe + offs[j nr + r])^-1 b_r,
+// the FP32 stage of rhs r to stage_resid[r], then every shift refined to
+// refine_resid[r] (nullptr: no refinement).  prm.resid / refine_resid unused.
+int bmultishift_sets(const void* b64, int nr, int n, const double* offs, const double* stage_resid,
+                     const double* refine_resid, const BMSParams& prm, std::vector<void*>& x64, BMSStats& st);
 
 struct BCGParams {
   int parity;
   int max_iters;
   double resid;       // true |r|/|b| target (all rhs)
   double delta;       // reliable-update threshold
+  const double* resid_r = nullptr;  // per-rhs targets (host array of nr), overriding resid
 };
 struct BCGStats {
   int iters = 0;               // max over rhs

@@ -549,7 +549,9 @@ int btime_variant(int prec, int nr, int variant, int reps, double* usec) {
 int btune(int prec, int nr, int* chosen, double* usec_best) {
   int pi = prec_index(prec), ni = nr_index(nr);
   if (pi < 0 || ni < 0) return -2;
-  if (g_bw.tuned[pi][ni] >= 0 && g_bw.W_version == ctx().links_version) {
+  // the timings depend on the lattice shape, not on the link values: a choice
+  // holds until the context ends
+  if (g_bw.tuned[pi][ni] >= 0) {
     *chosen = g_bw.tuned[pi][ni];
     if (usec_best) *usec_best = g_bw.tuned_us[pi][ni];
     return 0;

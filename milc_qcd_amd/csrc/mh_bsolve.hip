@@ -45,12 +45,13 @@ struct BMSR {  // one right-hand side of the multi-shift solve
   double zeta_i[BMS_MAX], zeta_im1[BMS_MAX], zeta_ip1[BMS_MAX];
   double beta_i[BMS_MAX], beta_im1[BMS_MAX], alpha[BMS_MAX];
   double wa[BMS_MAX], wx[BMS_MAX], wc[BMS_MAX][BKW], wb[BMS_MAX][BKW];
+  // this rhs's shifts: shifts[j_low] = its smallest offset, the others relative to it
+  double shifts[BMS_MAX];
   double rsq, oldrsq, source_norm, rsqstop;
-  int num_now, now_ws, m, halt, iters, converged, pad[2];
+  int num_now, now_ws, m, halt, iters, converged, j_low, pad;
 };
 struct BMSG {  // shared by all right-hand sides
-  double shifts[BMS_MAX];
-  int n, j_low, nr, halt_all, iters_max, pad[3];
+  int n, nr, halt_all, iters_max, pad[4];
 };
 struct BCGR {
   double rsq, oldrsq, a, b, rsq_stop, pad;
@@ -86,7 +87,7 @@ __global__ void k_bms_beta(const double* __restrict__ sums, BMSR* __restrict__ s
   if (g->halt_all) return;
   if (r < g->nr && !s[r].halt) {
     BMSR& S = s[r];
-    const int jl = g->j_low;
+    const int jl = S.j_low;
     double pkp = sums[r];
     S.iters += 1;
     int m = S.m + 1;
@@ -98,7 +99,7 @@ __global__ void k_bms_beta(const double* __restrict__ sums, BMSR* __restrict__ s
       if (j == jl) continue;
       S.zeta_ip1[j] = S.zeta_i[j] * S.zeta_im1[j] * S.beta_im1[jl];
       double c1 = S.beta_i[jl] * S.alpha[jl] * (S.zeta_im1[j] - S.zeta_i[j]);
-      double c2 = S.zeta_im1[j] * S.beta_im1[jl] * (1.0 + g->shifts[j] * S.beta_i[jl]);
+      double c2 = S.zeta_im1[j] * S.beta_im1[jl] * (1.0 + S.shifts[j] * S.beta_i[jl]);
       if (c1 + c2 != 0.0) S.zeta_ip1[j] /= c1 + c2;
       else S.zeta_ip1[j] = 0.0;
       if (S.zeta_i[j] != 0.0) {
@@ -126,7 +127,7 @@ __global__ void k_bms_alpha(const double* __restrict__ sums, BMSR* __restrict__ 
   if (g->halt_all) return;
   if (r < g->nr && !s[r].halt) {
     BMSR& S = s[r];
-    const int jl = g->j_low;
+    const int jl = S.j_low;
     double rsq = sums[r];
     S.oldrsq = S.rsq;
     S.rsq = rsq;
@@ -185,7 +186,7 @@ __global__ void __launch_bounds__(256) k_bms_r(const float* __restrict__ rprev, 
   if (g->halt_all) return;
   const int rhs = (blockIdx.x * blockDim.x + threadIdx.x) % NR;
   const bool live = !s[rhs].halt;
-  const float b = (float)s[rhs].beta_i[g->j_low];
+  const float b = (float)s[rhs].beta_i[s[rhs].j_low];
   double d[1] = {0.0};
   REC_LOOP(idx) {
     if (idx >= nrec || !live) continue;
@@ -208,7 +209,7 @@ __global__ void __launch_bounds__(256) k_bms_p(float* __restrict__ p, const floa
   if (g->halt_all) return;
   const int rhs = (blockIdx.x * blockDim.x + threadIdx.x) % NR;
   if (s[rhs].halt) return;
-  const float a = (float)s[rhs].alpha[g->j_low];
+  const float a = (float)s[rhs].alpha[s[rhs].j_low];
   REC_LOOP(idx) {
     if (idx >= nrec) continue;
     size_t o = (size_t)idx * 6;
@@ -485,11 +486,11 @@ int bconv(const void* in, void* out, int nr) {
   return 0;
 }
 
+// offs[j * NR + r]: offset of shift j for rhs r; resid[r]: base-system target of rhs r
 template <int NR>
-int bmultishift_nr(const void* b64, const std::vector<double>& offsets, const BMSParams& prm, std::vector<void*>& x64,
-                   BMSStats& st) {
+int bmultishift_nr(const void* b64, const double* offs, int n, const double* resid, const BMSParams& prm,
+                   std::vector<void*>& x64, BMSStats& st) {
   Context& c = ctx();
-  const int n = (int)offsets.size();
   if (n <= 0 || n > BMS_MAX || (int)x64.size() != n) return -2;
   const int pb = pbit(prm.parity);
   int rc;
@@ -529,20 +530,22 @@ int bmultishift_nr(const void* b64, const std::vector<double>& offsets, const BM
   BMSG hg;
   memset(hs.data(), 0, NR * sizeof(BMSR));
   memset(&hg, 0, sizeof hg);
-  int j_low = 0;
-  for (int j = 1; j < n; j++)
-    if (offsets[j] < offsets[j_low]) j_low = j;
-  for (int j = 0; j < n; j++) hg.shifts[j] = j == j_low ? offsets[j] : offsets[j] - offsets[j_low];
   hg.n = n;
-  hg.j_low = j_low;
   hg.nr = NR;
   int all = 1;
+  double base[BNR_MAX];
   for (int r = 0; r < NR; r++) {
     BMSR& S = hs[r];
+    int jl = 0;
+    for (int j = 1; j < n; j++)
+      if (offs[j * NR + r] < offs[jl * NR + r]) jl = j;
+    base[r] = offs[jl * NR + r];
+    for (int j = 0; j < n; j++) S.shifts[j] = j == jl ? base[r] : offs[j * NR + r] - base[r];
+    S.j_low = jl;
     S.num_now = n;
     S.source_norm = norms[r];
     S.rsq = norms[r];
-    S.rsqstop = prm.resid * prm.resid * norms[r];
+    S.rsqstop = resid[r] * resid[r] * norms[r];
     for (int j = 0; j < n; j++) {
       S.zeta_im1[j] = S.zeta_i[j] = 1.0;
       S.beta_im1[j] = -1.0;
@@ -585,7 +588,7 @@ int bmultishift_nr(const void* b64, const std::vector<double>& offsets, const BM
       d2.src = tmp.p;
       d2.dst = ttt.p;
       d2.p = p.p;
-      for (int r = 0; r < NR; r++) d2.coef[r] = -offsets[j_low];
+      for (int r = 0; r < NR; r++) d2.coef[r] = -base[r];
       int nb2 = 0;
       if ((r2 = bdslash(d2, &nb2))) return r2;
       if ((r2 = breduce(NR, nb2, dsum))) return r2;
@@ -750,7 +753,8 @@ int bcg_mixed_nr(const void* b64, void* x64, const double* offsets, const BCGPar
     iteration++;
     int all = 1;
     for (int r = 0; r < NR; r++) {
-      converged[r] = bn[r] == 0.0 || rt[r] < prm.resid * prm.resid * bn[r];
+      const double res = prm.resid_r ? prm.resid_r[r] : prm.resid;
+      converged[r] = bn[r] == 0.0 || rt[r] < res * res * bn[r];
       st.true_rel[r] = bn[r] > 0 ? std::sqrt(rt[r] / bn[r]) : 0.0;
       all &= converged[r];
     }
@@ -762,7 +766,8 @@ int bcg_mixed_nr(const void* b64, void* x64, const double* offsets, const BCGPar
       S.go = 0;
       S.iters = 0;
       S.rsq = rt[r];
-      S.rsq_stop = std::max(prm.resid * prm.resid * bn[r], delta * delta * rt[r]);
+      const double res = prm.resid_r ? prm.resid_r[r] : prm.resid;
+      S.rsq_stop = std::max(res * res * bn[r], delta * delta * rt[r]);
     }
     if ((rc = (int)hipMemcpyAsync(ds, hs.data(), NR * sizeof(BCGR), hipMemcpyHostToDevice, c.stream))) break;
     if ((rc = (int)hipMemcpyAsync(dg, &hg0, sizeof(BCGG), hipMemcpyHostToDevice, c.stream))) break;
@@ -792,25 +797,26 @@ int bcg_mixed_nr(const void* b64, void* x64, const double* offsets, const BCGPar
 
 }  // namespace
 
-int bmultishift(const void* b64, int nr, const std::vector<double>& offsets, const BMSParams& prm,
-                std::vector<void*>& x64, BMSStats& st) {
+int bmultishift_sets(const void* b64, int nr, int n, const double* offs, const double* stage_resid,
+                     const double* refine_resid, const BMSParams& prm, std::vector<void*>& x64, BMSStats& st) {
   int rc;
   st = BMSStats();
-  if (nr == 4) rc = bmultishift_nr<4>(b64, offsets, prm, x64, st);
-  else if (nr == 8) rc = bmultishift_nr<8>(b64, offsets, prm, x64, st);
-  else if (nr == 16) rc = bmultishift_nr<16>(b64, offsets, prm, x64, st);
+  if (nr == 4) rc = bmultishift_nr<4>(b64, offs, n, stage_resid, prm, x64, st);
+  else if (nr == 8) rc = bmultishift_nr<8>(b64, offs, n, stage_resid, prm, x64, st);
+  else if (nr == 16) rc = bmultishift_nr<16>(b64, offs, n, stage_resid, prm, x64, st);
   else return -2;
-  if (rc || prm.refine_resid <= 0) return rc;
-  // per-shift refinement to the FP64 true-residual target
+  if (rc || !refine_resid) return rc;
+  // per-shift refinement to the FP64 true-residual targets
   auto t0 = std::chrono::steady_clock::now();
   int conv = 1;
-  for (size_t j = 0; j < offsets.size(); j++) {
+  for (int j = 0; j < n; j++) {
     double off[BNR_MAX];
-    for (int r = 0; r < BNR_MAX; r++) off[r] = offsets[j];
+    for (int r = 0; r < nr; r++) off[r] = offs[j * nr + r];
     BCGParams cp;
     cp.parity = prm.parity;
     cp.max_iters = prm.max_iters;
-    cp.resid = prm.refine_resid;
+    cp.resid = refine_resid[0];
+    cp.resid_r = refine_resid;
     cp.delta = prm.delta;
     BCGStats cs;
     rc = bcg_mixed(b64, x64[j], nr, off, cp, cs);
@@ -827,6 +833,17 @@ int bmultishift(const void* b64, int nr, const std::vector<double>& offsets, con
   st.converged = conv;
   st.refine_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return 0;
+}
+
+int bmultishift(const void* b64, int nr, const std::vector<double>& offsets, const BMSParams& prm,
+                std::vector<void*>& x64, BMSStats& st) {
+  const int n = (int)offsets.size();
+  if (n <= 0 || n > BMS_MAX || nr <= 0 || nr > BNR_MAX) return -2;
+  std::vector<double> offs((size_t)n * nr), stage(nr, prm.resid), refine(nr, prm.refine_resid);
+  for (int j = 0; j < n; j++)
+    for (int r = 0; r < nr; r++) offs[(size_t)j * nr + r] = offsets[j];
+  return bmultishift_sets(b64, nr, n, offs.data(), stage.data(), prm.refine_resid > 0 ? refine.data() : nullptr,
+                          prm, x64, st);
 }
 
 int bcg_mixed(const void* b64, void* x64, int nr, const double* offsets, const BCGParams& prm, BCGStats& st) {

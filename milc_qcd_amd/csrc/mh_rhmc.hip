@@ -17,6 +17,7 @@
 #include <cstring>
 #include <vector>
 
+#include "mh_batch.h"
 #include "mh_internal.h"
 
 namespace mh {
@@ -129,6 +130,82 @@ static int rational_solve(const VecField& phi, int n, const double* offsets, dou
     st.iters += s2.iters;
     st.converged &= s2.converged;
   }
+  return 0;
+}
+
+// Several sources' rational solves at once (every pseudofermion's MD, GR or
+// FA rational): the batched multi-shift with a shift set per source, FP32
+// iterations sharing every stencil launch, then every shift of every source
+// refined to its FP64 true-residual target; x[s][t] allocated here.
+static int rational_solve_all(const std::vector<const VecField*>& src, const std::vector<std::vector<double>>& offs,
+                              const std::vector<double>& resid, std::vector<std::vector<VecField>>& x, long* iters) {
+  const int ns = (int)src.size();
+  if (ns < 1 || ns > BNR_MAX || (int)offs.size() != ns || (int)resid.size() != ns) return -2;
+  const int nr = ns <= 4 ? 4 : (ns <= 8 ? 8 : 16);
+  int n = 0;
+  for (auto& o : offs) {
+    if (o.empty() || (int)o.size() > BMS_MAX) return -2;
+    n = std::max(n, (int)o.size());
+  }
+  // a source with fewer shifts repeats its last one; the padding sources are
+  // zero and halt at once
+  std::vector<double> flat((size_t)n * nr), stage(nr, 1.0), refine(nr, 1.0);
+  for (int j = 0; j < n; j++)
+    for (int r = 0; r < nr; r++) {
+      const std::vector<double>& o = offs[r < ns ? r : 0];
+      flat[(size_t)j * nr + r] = o[std::min(j, (int)o.size() - 1)];
+    }
+  for (int r = 0; r < ns; r++) {
+    stage[r] = std::max(resid[r], 1e-7);
+    refine[r] = resid[r];
+  }
+  void* b = nullptr;
+  std::vector<void*> xb(n, nullptr);
+  auto release = [&]() {
+    dev_free(b);
+    for (void* p : xb) dev_free(p);
+  };
+  int rc = balloc(&b, P_F64, nr);
+  for (int j = 0; j < n && !rc; j++) rc = balloc(&xb[j], P_F64, nr);
+  for (int r = 0; r < ns && !rc; r++) rc = bpack(*src[r], b, P_F64, nr, r);
+  BMSParams prm;
+  prm.parity = EVEN;
+  prm.max_iters = 20000;
+  prm.resid = 0;
+  prm.refine_resid = 0;
+  prm.delta = 0.1;
+  BMSStats st;
+  if (!rc) rc = bmultishift_sets(b, nr, n, flat.data(), stage.data(), refine.data(), prm, xb, st);
+  x.assign(ns, {});
+  for (int r = 0; r < ns && !rc; r++) {
+    x[r].assign(offs[r].size(), VecField{});
+    for (int t = 0; t < (int)offs[r].size() && !rc; t++) {
+      rc = alloc_vec(x[r][t], P_F64, ctx().stride);
+      if (!rc) rc = bunpack(xb[t], P_F64, nr, r, x[r][t]);
+    }
+  }
+  (void)hipStreamSynchronize(ctx().stream);
+  release();
+  if (iters) {
+    long it = st.ms_iters;
+    for (int j = 0; j < n; j++) it += st.refine_iters[j];
+    *iters = it;
+  }
+  return rc;
+}
+
+// dst = res[0] src + sum_t res[t+1] x[t] (ks_rateval)
+static int rateval(const VecField& src, const std::vector<VecField>& x, const double* res, VecField& dst) {
+  Context& c = ctx();
+  if (x.size() > RAT_MAX) return -2;
+  RatSet rs{};
+  for (size_t t = 0; t < x.size(); t++) {
+    rs.x[t] = x[t];
+    rs.a[t] = res[t + 1];
+  }
+  hipLaunchKernelGGL(k_rateval, dim3((c.Vh + 255) / 256), dim3(256), 0, c.stream, src, res[0], rs, (int)x.size(), dst,
+                     c.Vh);
+  MH_CHECK(hipGetLastError());
   return 0;
 }
 
@@ -319,6 +396,24 @@ struct Traj {
     for (auto& q : pf) free_vec(q.phi);
   }
   size_t nmom() const { return (size_t)2 * ctx().Vh * 4 * 10; }
+  // several pseudofermions with mixed solves share one batched multi-shift
+  bool batched(int mixed) const { return mixed && p.n_pseudo >= 2; }
+  std::vector<const VecField*> sources() const {
+    std::vector<const VecField*> v;
+    for (auto& q : pf) v.push_back(&q.phi);
+    return v;
+  }
+  std::vector<std::vector<double>> offsets(double (Pseudo::*offs)[33], RhmcRat RhmcPseudo::*rat) const {
+    std::vector<std::vector<double>> v;
+    for (int i = 0; i < p.n_pseudo; i++)
+      v.emplace_back(pf[i].*offs + 1, pf[i].*offs + 1 + (p.pf[i].*rat).order);
+    return v;
+  }
+  std::vector<double> resids(double RhmcPseudo::*r) const {
+    std::vector<double> v;
+    for (int i = 0; i < p.n_pseudo; i++) v.push_back(p.pf[i].*r);
+    return v;
+  }
 
   // fat and Naik links of the current thin links, rebuilt only when used
   // (MILC invalidates them in update_u and rebuilds on demand)
@@ -393,14 +488,29 @@ struct Traj {
     if (p.mixed_md && (rc = ensure_links(P_F32))) return rc;
     std::vector<VecField> all;
     std::vector<double> res;
-    for (int i = 0; i < p.n_pseudo && !rc; i++) {
-      std::vector<VecField> x;
-      SolveStats st;
-      rc = rational_solve(pf[i].phi, p.pf[i].md.order, pf[i].offs_md + 1, p.pf[i].resid_md, p.mixed_md, EVEN, x, st);
-      md_iters += st.iters;
-      for (int t = 0; t < (int)x.size(); t++) {
-        all.push_back(x[t]);
-        res.push_back(p.pf[i].md.res[t + 1]);
+    if (batched(p.mixed_md)) {
+      // all pseudofermions' MD solves in one batched multi-shift
+      std::vector<std::vector<VecField>> xs;
+      long it = 0;
+      rc = rational_solve_all(sources(), offsets(&Pseudo::offs_md, &RhmcPseudo::md), resids(&RhmcPseudo::resid_md),
+                              xs, &it);
+      md_iters += it;
+      for (int i = 0; i < (int)xs.size(); i++)
+        for (int t = 0; t < (int)xs[i].size(); t++) {
+          all.push_back(xs[i][t]);
+          res.push_back(p.pf[i].md.res[t + 1]);
+        }
+    } else {
+      for (int i = 0; i < p.n_pseudo && !rc; i++) {
+        std::vector<VecField> x;
+        SolveStats st;
+        rc = rational_solve(pf[i].phi, p.pf[i].md.order, pf[i].offs_md + 1, p.pf[i].resid_md, p.mixed_md, EVEN, x,
+                            st);
+        md_iters += st.iters;
+        for (int t = 0; t < (int)x.size(); t++) {
+          all.push_back(x[t]);
+          res.push_back(p.pf[i].md.res[t + 1]);
+        }
       }
     }
     (void)hipStreamSynchronize(c.stream);
@@ -436,6 +546,27 @@ struct Traj {
     if (!rc) rc = fresh();
     if (!rc && p.mixed_action) rc = ensure_links(P_F32);
     double fs = 0;
+    if (!rc && !p.action_from_md && batched(p.mixed_action)) {
+      // sum over pseudofermions of |R_FA phi|^2 from one batched multi-shift
+      std::vector<std::vector<VecField>> xs;
+      long it = 0;
+      rc = rational_solve_all(sources(), offsets(&Pseudo::offs_fa, &RhmcPseudo::fa), resids(&RhmcPseudo::resid_fa),
+                              xs, &it);
+      fa_iters += it;
+      VecField sum{};
+      if (!rc) rc = alloc_vec(sum, P_F64, ctx().stride);
+      for (int i = 0; i < (int)xs.size() && !rc; i++) {
+        double a = 0;
+        rc = rateval(pf[i].phi, xs[i], p.pf[i].fa.res, sum);
+        if (!rc) rc = vec_norm2(sum, P_F64, &a);
+        fs += a;
+      }
+      free_vec(sum);
+      for (auto& v : xs)
+        for (auto& y : v) free_vec(y);
+      *f = fs;
+      return rc;
+    }
     for (int i = 0; i < p.n_pseudo && !rc; i++) {
       const RhmcPseudo& q = p.pf[i];
       if (p.action_from_md) {
@@ -514,8 +645,9 @@ int rhmc_trajectory(const RhmcParams& p, double* stats) {
     return rc;
   for (auto& P : T.pf)
     if ((rc = alloc_vec(P.phi, P_F64, c.stride))) return rc;
-  VecField chi{};
-  if ((rc = alloc_vec(chi, P_F64, c.stride))) return rc;
+  std::vector<VecField> chi(p.n_pseudo);
+  for (auto& v : chi)
+    if ((rc = alloc_vec(v, P_F64, c.stride))) return rc;
   (void)hipStreamSynchronize(c.stream);
   const double t_setup = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
   // ranmom, then grsource_imp_rhmc for each pseudofermion: phi = R_GR(M^dag M) chi
@@ -523,20 +655,36 @@ int rhmc_trajectory(const RhmcParams& p, double* stats) {
   int it_gr = 0;
   if (p.mixed_action && (rc = ensure_links(P_F32))) return rc;
   rc = p.site_rng ? ranmom_prn_dev(T.mom) : ranmom_dev(host_mix(p.seed), T.mom);
+  // the noise of every pseudofermion first, in MILC's order of draws
   for (int i = 0; i < p.n_pseudo && !rc; i++) {
     if (p.site_rng) {
-      rc = gaussian_prn_dev(0, chi, &grsum[i]);
+      rc = gaussian_prn_dev(0, chi[i], &grsum[i]);
     } else {
-      rc = gaussian_vec_dev(host_mix(p.seed + 1) + 0x632BE59BD9B4E019ULL * (uint64_t)i, 0, chi);
-      if (!rc) rc = vec_norm2(chi, P_F64, &grsum[i]);
+      rc = gaussian_vec_dev(host_mix(p.seed + 1) + 0x632BE59BD9B4E019ULL * (uint64_t)i, 0, chi[i]);
+      if (!rc) rc = vec_norm2(chi[i], P_F64, &grsum[i]);
     }
-    int it = 0;
-    if (!rc)
-      rc = rational_apply(chi, p.pf[i].gr.order, T.pf[i].offs_gr + 1, p.pf[i].gr.res, p.pf[i].resid_gr,
-                          p.mixed_action, T.pf[i].phi, &it);
-    it_gr += it;
   }
-  free_vec(chi);
+  if (!rc && T.batched(p.mixed_action)) {
+    std::vector<const VecField*> src;
+    for (auto& v : chi) src.push_back(&v);
+    std::vector<std::vector<VecField>> xs;
+    long it = 0;
+    rc = rational_solve_all(src, T.offsets(&Pseudo::offs_gr, &RhmcPseudo::gr), T.resids(&RhmcPseudo::resid_gr), xs,
+                            &it);
+    it_gr = (int)it;
+    for (int i = 0; i < (int)xs.size() && !rc; i++) rc = rateval(chi[i], xs[i], p.pf[i].gr.res, T.pf[i].phi);
+    (void)hipStreamSynchronize(c.stream);
+    for (auto& v : xs)
+      for (auto& y : v) free_vec(y);
+  } else {
+    for (int i = 0; i < p.n_pseudo && !rc; i++) {
+      int it = 0;
+      rc = rational_apply(chi[i], p.pf[i].gr.order, T.pf[i].offs_gr + 1, p.pf[i].gr.res, p.pf[i].resid_gr,
+                          p.mixed_action, T.pf[i].phi, &it);
+      it_gr += it;
+    }
+  }
+  for (auto& v : chi) free_vec(v);
   if (rc) return rc;
   (void)hipStreamSynchronize(c.stream);
   const double t_heatbath =
