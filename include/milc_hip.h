@@ -212,6 +212,9 @@ typedef struct {
   double epsilon;        /* microcanonical_time_step */
   double beta;           /* gauge coupling */
   double loop_coeff[3];  /* Symanzik loop coefficients (1x1, 1x2, 1x1x1) */
+  int action;            /* quark action: 0 HISQ (ks_imp_rhmc's su3_rhmc_hisq builds),
+                            1 asqtad with tadpole factor u0 (su3_rhmc / su3_omelyan_rhmc) */
+  double u0;             /* asqtad tadpole factor (unused for HISQ) */
   int n_pseudo;          /* 1..MILC_HIP_MAX_PSEUDO */
   milc_hip_pseudofermion pf[MILC_HIP_MAX_PSEUDO];
   int mixed_md;          /* MD solves: FP32 multi-shift + 16-bit mixed refinement */

@@ -1825,6 +1825,9 @@ extern "C" int milc_hip_dev_rhmc_trajectory(const milc_hip_rhmc_params* p, doubl
   q.epsilon = p->epsilon;
   q.beta = p->beta;
   for (int k = 0; k < 3; k++) q.loop_coeff[k] = p->loop_coeff[k];
+  q.action = p->action;
+  q.u0 = p->u0;
+  if (q.action != 0 && !(q.action == 1 && q.u0 > 0)) return -2;
   q.n_pseudo = p->n_pseudo;
   for (int i = 0; i < p->n_pseudo; i++) {
     const milc_hip_pseudofermion& a = p->pf[i];

@@ -174,6 +174,8 @@ struct RhmcParams {
   int steps;
   int integrator;  // MILC's int_alg_t: 0 leapfrog, 1 Omelyan, 4 2G1F, 5 3G1F, 6 5G1F, 7 6G1F
   double epsilon, beta, loop_coeff[3];
+  int action;  // 0 HISQ, 1 asqtad (tadpole factor u0)
+  double u0;
   int n_pseudo;
   RhmcPseudo pf[RHMC_MAX_PSEUDO];
   int mixed_md, mixed_action, action_from_md, accept, site_rng;
@@ -189,6 +191,9 @@ int ranmom_prn_dev(double* mom_dev);
 int gaussian_prn_dev(int pb, VecField& v, double* sum);
 double prn_node_uniform();
 void prn_release();
+// FN (asqtad) thin-link force from the level-0 outer products: mom = TA(scale U F)
+int fn_force_from_oprod(const LinkField& U, const LinkField& o1, const LinkField& o3, const double coeff[6],
+                        double scale, double* mom_dev);
 // reunit_su3 on every link (KS phases taken out and put back when phases_in)
 int reunitarize_dev(LinkField& U, int phases_in, double tol, int* errors, double* maxdev);
 

@@ -422,9 +422,14 @@ struct Traj {
     Context& c = ctx();
     auto t0 = std::chrono::steady_clock::now();
     DevLinks& L = c.links[P_F64];
-    int nsvd = 0;
-    int rc = hisq_links_dev(c.thin, L.fat, L.lng, &W, &nsvd, &V);
-    aux = rc == 0;
+    int nsvd = 0, rc;
+    if (p.action == 1) {
+      rc = asqtad_links_dev(c.thin, p.u0, L.fat, L.lng);
+      aux = false;
+    } else {
+      rc = hisq_links_dev(c.thin, L.fat, L.lng, &W, &nsvd, &V);
+      aux = rc == 0;
+    }
     for (int q = P_F32; q <= P_H16; q++)
       if (c.links[q].valid) {
         free_links(c.links[q].fat);
@@ -517,9 +522,18 @@ struct Traj {
     t_md_solve += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (!rc) rc = force_level0_from_solutions(all, res.data(), o1, o3);
     for (auto& v : all) free_vec(v);
-    if (!rc && !aux) rc = hisq_aux_dev(c.thin, V, W);
-    aux = rc == 0;
-    if (!rc) rc = hisq_force_from_oprod(c.thin, V, W, o1, o3, kHisqLevel2, kHisqLevel1, 2.0 * e, kForceFilter, mtmp);
+    if (p.action == 1) {
+      // asqtad: the FN force on the thin links (fermion_force_fn_multi)
+      const double u2 = p.u0 * p.u0, u4 = u2 * u2, u6 = u4 * u2;
+      const double co[6] = {5.0 / 8.0, -1.0 / 24.0 / u2, -1.0 / 16.0 / u2, 1.0 / 64.0 / u4, -1.0 / 384.0 / u6,
+                            -1.0 / 16.0 / u4};
+      if (!rc) rc = fn_force_from_oprod(c.thin, o1, o3, co, 2.0 * e, mtmp);
+    } else {
+      if (!rc && !aux) rc = hisq_aux_dev(c.thin, V, W);
+      aux = rc == 0;
+      if (!rc)
+        rc = hisq_force_from_oprod(c.thin, V, W, o1, o3, kHisqLevel2, kHisqLevel1, 2.0 * e, kForceFilter, mtmp);
+    }
     if (!rc) {
       hipLaunchKernelGGL(k_axpy_flat, dim3(1024), dim3(256), 0, c.stream, mom, mtmp, nmom());
       rc = hipGetLastError() == hipSuccess ? 0 : -1;
@@ -597,6 +611,10 @@ struct Traj {
   }
 };
 
+// links_version of the context's links when a trajectory last left asqtad
+// fat and Naik links there (every other builder makes HISQ links)
+unsigned g_asqtad_links_version = ~0u;
+
 uint64_t host_mix(uint64_t z) {
   z += 0x9E3779B97F4A7C15ULL;
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
@@ -618,6 +636,8 @@ int rhmc_trajectory(const RhmcParams& p, double* stats) {
   auto t_start = std::chrono::steady_clock::now();
   Traj T(p);
   T.pf.resize(p.n_pseudo);
+  // the fat and Naik links in place must be those of this quark action
+  T.dirty = (p.action == 1) != (c.links_version == g_asqtad_links_version);
   for (int i = 0; i < p.n_pseudo; i++) {
     const RhmcPseudo& q = p.pf[i];
     if (!rat_ok(q.md) || !rat_ok(q.gr) || !rat_ok(q.fa)) return -2;
@@ -653,6 +673,7 @@ int rhmc_trajectory(const RhmcParams& p, double* stats) {
   // ranmom, then grsource_imp_rhmc for each pseudofermion: phi = R_GR(M^dag M) chi
   double grsum[RHMC_MAX_PSEUDO] = {0};
   int it_gr = 0;
+  if ((rc = T.fresh())) return rc;
   if (p.mixed_action && (rc = ensure_links(P_F32))) return rc;
   rc = p.site_rng ? ranmom_prn_dev(T.mom) : ranmom_dev(host_mix(p.seed), T.mom);
   // the noise of every pseudofermion first, in MILC's order of draws
@@ -711,6 +732,7 @@ int rhmc_trajectory(const RhmcParams& p, double* stats) {
     T.dirty = true;
     T.aux = false;
     (void)T.fresh();
+    g_asqtad_links_version = p.action == 1 ? c.links_version : ~0u;
     return code;
   };
   if (rc) return fail(rc);
@@ -734,6 +756,7 @@ int rhmc_trajectory(const RhmcParams& p, double* stats) {
   }
   // leave the fat and Naik links of the final thin links in place
   if ((rc = T.fresh())) return rc;
+  g_asqtad_links_version = p.action == 1 ? c.links_version : ~0u;
   (void)hipStreamSynchronize(c.stream);
   double tot = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
   double out[32] = {g0, h0, f0, g1, h1, f1, dH, (double)accepted, tot, (double)T.md_iters, (double)T.n_ferm,

@@ -123,7 +123,8 @@ class Pseudofermion(ctypes.Structure):
 
 class RhmcParams(ctypes.Structure):
     _fields_ = [("steps", ctypes.c_int), ("integrator", ctypes.c_int), ("epsilon", ctypes.c_double),
-                ("beta", ctypes.c_double), ("loop_coeff", ctypes.c_double * 3), ("n_pseudo", ctypes.c_int),
+                ("beta", ctypes.c_double), ("loop_coeff", ctypes.c_double * 3), ("action", ctypes.c_int),
+                ("u0", ctypes.c_double), ("n_pseudo", ctypes.c_int),
                 ("pf", Pseudofermion * MAX_PSEUDO), ("mixed_md", ctypes.c_int), ("mixed_action", ctypes.c_int),
                 ("action_from_md", ctypes.c_int),
                 ("accept", ctypes.c_int), ("site_rng", ctypes.c_int), ("seed", ctypes.c_ulonglong)]
@@ -518,13 +519,16 @@ class Lattice:
         return a.value
 
     def dev_rhmc_trajectory(self, steps, epsilon, beta, loop_coeff, pseudofermions, integrator=INT_OMELYAN,
-                            mixed_md=1, action_from_md=False, accept=1, seed=1, site_rng=False, mixed_action=0):
+                            mixed_md=1, action_from_md=False, accept=1, seed=1, site_rng=False, mixed_action=0,
+                            asqtad_u0=None):
         """one RHMC trajectory on the device links (update(), ks_imp_rhmc/update_rhmc.c);
-        pseudofermions: list of Pseudofermion"""
+        pseudofermions: list of Pseudofermion; asqtad_u0: the asqtad quark action with
+        that tadpole factor instead of HISQ"""
         self._activate()
         if not 1 <= len(pseudofermions) <= MAX_PSEUDO:
             raise MilcHipError(f"1..{MAX_PSEUDO} pseudofermions")
         p = RhmcParams(steps=steps, integrator=integrator, epsilon=epsilon, beta=beta, n_pseudo=len(pseudofermions),
+                       action=0 if asqtad_u0 is None else 1, u0=0.0 if asqtad_u0 is None else asqtad_u0,
                        mixed_md=int(mixed_md), mixed_action=int(mixed_action),
                        action_from_md=int(bool(action_from_md)), accept=accept,
                        site_rng=int(bool(site_rng)), seed=seed)

@@ -76,6 +76,7 @@ IO_FILES = [
     "lat.sample.l4444", "lat.sample.l4444.scidac", "lat.sample.l4444.ildg",
     "lat.sample.l4448", "lat.sample.l4448.scidac", "lat.sample.l4448.foreign",
     "lat.sample.5p7", "lat.sample.l6666", "lat.sample.l6666.hisq", "ksvector.src.sample.l8",
+    "lat.sample.l6666b650m010m050",
 ]
 
 
@@ -85,9 +86,10 @@ def main(ref="/root/reference"):
         assert dims == [8, 8, 8, 8]
         np.save(os.path.join(HERE, dst), arr)
         print(dst, arr.shape, arr.dtype)
-    # MILC's rational functions for its HISQ RHMC test (MD / GR / FA per pseudofermion)
-    rat = open(os.path.join(ref, "ks_imp_rhmc", "test", "rationals.sample.su3_rhmc_hisq")).read()
-    open(os.path.join(HERE, "rationals.sample.su3_rhmc_hisq"), "w").write(rat)
+    # MILC's rational functions for its HISQ and asqtad RHMC tests (MD / GR / FA per pseudofermion)
+    for name in ("rationals.sample.su3_rhmc_hisq", "rationals.sample.su3_omelyan_rhmc"):
+        rat = open(os.path.join(ref, "ks_imp_rhmc", "test", name)).read()
+        open(os.path.join(HERE, name), "w").write(rat)
     os.makedirs(os.path.join(HERE, "io"), exist_ok=True)
     for name in IO_FILES:
         data = open(os.path.join(ref, "binary_samples", name), "rb").read()

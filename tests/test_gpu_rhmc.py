@@ -3,14 +3,16 @@ update() of ks_imp_rhmc/update_rhmc.c): gauge force + HISQ fermion force,
 link updates with the fat and Naik links rebuilt, reunitarisation, actions,
 accept/reject.
 
-* MILC's own regression trajectory, ks_imp_rhmc/test/su3_rhmc_hisq.1
+* MILC's own regression trajectories: the HISQ one, ks_imp_rhmc/test/su3_rhmc_hisq.1
   (lat.sample.l6666.hisq, iseed 86658, four pseudofermions of
   rationals.sample.su3_rhmc_hisq, INT_3G1F, 10 steps of 0.02, beta 7.4,
   u0 0.89): with MILC's per-site generators the heatbath noise reproduces the
   printed GRSOURCE sums to every digit and the start actions the printed
   ACTION line; dS and the final plaquette agree with the printed ones within
   the tolerances of MILC's own check (su3_rhmc_hisq.1.errtol: delta S 0.02,
-  PLAQ 4e-5);
+  PLAQ 4e-5); and the asqtad one, su3_omelyan_rhmc.1 (one pseudofermion,
+  Omelyan), whose start and end actions, delta S and plaquette agree within
+  su3_omelyan_rhmc.1.errtol;
 * the device generators against the oracle's restatement of MILC's: momenta
   identical, sources identical;
 * the starting actions against the oracle: beta/3 times the gauge-loop sum of
@@ -108,6 +110,52 @@ def test_milc_su3_rhmc_hisq_trajectory(mixed):
     assert abs(ss - MILC_PLAQ[0]) < 4e-5 and abs(st_ - MILC_PLAQ[1]) < 4e-5
     for k, ref in zip(("g1", "h1", "f1"), MILC_ACTION1):
         assert abs(st[k] - ref) < 0.05, (k, st[k], ref)
+
+
+# ---- MILC's su3_omelyan_rhmc.1 regression run (asqtad; sample-out lines cited) ----
+ASQ_ISEED = 3546789
+ASQ_GRSOURCE = 1.9073263760e+03                                                     # 237
+ASQ_ACTION0 = (1.67770842255353e+04, -9.36446401476860e+01, 1.90732694119215e+03)   # 241
+ASQ_ACTION1 = (1.68225951988426e+04, -1.32490214347839e+02, 1.90066177399457e+03)   # 257
+ASQ_DELTA_S = 2.319096e-04                                                           # 259
+ASQ_PLAQ = (1.649426, 1.643181)                                                      # 260
+# su3_omelyan_rhmc.1.errtol: ACTION g, h, f at the start and the end, delta S, PLAQ
+ASQ_TOL0, ASQ_TOL1, ASQ_TOL_DS, ASQ_TOL_PLAQ = (0.03, 0.0002, 0.004), (0.03, 0.004, 0.004), 0.01, 3e-6
+
+
+@pytest.mark.parametrize("mixed", [0, 1])
+def test_milc_su3_omelyan_rhmc_asqtad_trajectory(mixed):
+    """MILC's asqtad RHMC regression run ks_imp_rhmc/test/su3_omelyan_rhmc.1
+    (lat.sample.l6666b650m010m050, iseed 3546789, one pseudofermion of
+    rationals.sample.su3_omelyan_rhmc, INT_OMELYAN, 12 steps of 0.01,
+    beta 6.5, asqtad with u0 0.862, residuals 2e-6): the first trajectory
+    within every tolerance of MILC's own check"""
+    links, info = lio.reload_lattice(os.path.join(HERE, "golden", "io", "lat.sample.l6666b650m010m050"))
+    assert info["checksum_ok"]
+    dims = (6, 6, 6, 6)
+    ss, st0 = oio.plaquette(links[oio.node_index_of_lex(list(dims))], list(dims))
+    assert abs(ss - 1.658265) < 1e-6 and abs(st0 - 1.639891) < 1e-6                  # CHECK PLAQ, line 217
+    pfs = mq.pseudofermions_from_file(os.path.join(HERE, "golden", "rationals.sample.su3_omelyan_rhmc"),
+                                      2e-6, 2e-6, 2e-6)
+    coeff = mq.gauge_loop_coeffs(0.862, 3, hisq=False)
+    np.testing.assert_allclose(coeff[1:], [-7.355018e-02, -8.663654e-03], rtol=1e-6)  # lines 232-234
+    with mq.Lattice(dims) as G:
+        G.dev_load_thin_links(links, phases_in=False)
+        G.dev_seed_rng(ASQ_ISEED, precision=1)
+        st = G.dev_rhmc_trajectory(12, 0.01, 6.5, coeff, pfs, integrator=mq.INT_OMELYAN, mixed_md=mixed,
+                                   mixed_action=mixed, accept=1, site_rng=True, asqtad_u0=0.862)
+        U1 = G.dev_read_thin_links(with_phases=False)
+    print({k: v for k, v in st.items()})
+    assert st["grsource"][0] == pytest.approx(ASQ_GRSOURCE, rel=3e-11)
+    for k, ref, tol in zip(("g0", "h0", "f0"), ASQ_ACTION0, ASQ_TOL0):
+        assert abs(st[k] - ref) < tol, (k, st[k], ref)
+    for k, ref, tol in zip(("g1", "h1", "f1"), ASQ_ACTION1, ASQ_TOL1):
+        assert abs(st[k] - ref) < tol, (k, st[k], ref)
+    assert abs(st["dH"] - ASQ_DELTA_S) < ASQ_TOL_DS
+    assert st["accepted"] == 1 and st["fermion_forces"] == 12 and st["reunit_errors"] == 0
+    ss, st_ = oio.plaquette(U1[oio.node_index_of_lex(list(dims))], list(dims))
+    print("PLAQ", ss, st_)
+    assert abs(ss - ASQ_PLAQ[0]) < ASQ_TOL_PLAQ and abs(st_ - ASQ_PLAQ[1]) < ASQ_TOL_PLAQ
 
 
 def test_device_generators_match_oracle():
