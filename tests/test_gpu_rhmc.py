@@ -158,6 +158,50 @@ def test_milc_su3_omelyan_rhmc_asqtad_trajectory(mixed):
     assert abs(ss - ASQ_PLAQ[0]) < ASQ_TOL_PLAQ and abs(st_ - ASQ_PLAQ[1]) < ASQ_TOL_PLAQ
 
 
+# ---- MILC's su3_leapfrog.1 run (asqtad RHMD, two pseudofermions; sample-out lines) ----
+LF_GRSOURCE = [1.9073263760e+03, 1.9029227323e+03]                                  # 306, 308
+LF_ACTION0 = (1.67770842255353e+04, -9.36446401476860e+01, 3.81024987246096e+03)    # 313
+LF_ACTION1 = (1.68258343205252e+04, -1.56867322713137e+02, 3.82467601203918e+03)    # 358
+LF_DELTA_S = -4.644800e-02                                                            # 360
+LF_PLAQ = (1.649295, 1.642828)                                                        # 361
+# su3_leapfrog.1.errtol: start and end ACTION g, h, f; delta S; PLAQ
+LF_TOL0, LF_TOL1, LF_TOL_DS, LF_TOL_PLAQ = (3e-4, 2e-4, 0.01), (0.01, 0.008, 0.007), 0.008, (2e-6, 1e-7)
+
+
+@pytest.mark.parametrize("mixed", [0, 1])
+def test_milc_su3_leapfrog_asqtad_rhmd_trajectory(mixed):
+    """MILC's asqtad RHMD regression run ks_imp_rhmc/test/su3_leapfrog.1
+    (lat.sample.l6666b650m010m050, iseed 3546789, two pseudofermions of
+    rationals.sample.su3_leapfrog, INT_LEAPFROG, 12 steps of 0.01, beta 6.5,
+    asqtad with u0 0.862, residuals 2e-6, no accept/reject): the first
+    trajectory within every tolerance of MILC's own check"""
+    links, info = lio.reload_lattice(os.path.join(HERE, "golden", "io", "lat.sample.l6666b650m010m050"))
+    dims = (6, 6, 6, 6)
+    pfs = mq.pseudofermions_from_file(os.path.join(HERE, "golden", "rationals.sample.su3_leapfrog"),
+                                      2e-6, 2e-6, 2e-6)
+    assert len(pfs) == 2
+    coeff = mq.gauge_loop_coeffs(0.862, 3, hisq=False)
+    with mq.Lattice(dims) as G:
+        G.dev_load_thin_links(links, phases_in=False)
+        G.dev_seed_rng(ASQ_ISEED, precision=1)
+        st = G.dev_rhmc_trajectory(12, 0.01, 6.5, coeff, pfs, integrator=mq.INT_LEAPFROG, mixed_md=mixed,
+                                   mixed_action=mixed, accept=0, site_rng=True, asqtad_u0=0.862)
+        U1 = G.dev_read_thin_links(with_phases=False)
+    print({k: v for k, v in st.items()})
+    np.testing.assert_allclose(st["grsource"], LF_GRSOURCE, rtol=3e-11)
+    for k, ref, tol in zip(("g0", "h0", "f0"), LF_ACTION0, LF_TOL0):
+        assert abs(st[k] - ref) < tol, (k, st[k], ref)
+    for k, ref, tol in zip(("g1", "h1", "f1"), LF_ACTION1, LF_TOL1):
+        assert abs(st[k] - ref) < tol, (k, st[k], ref)
+    assert abs(st["dH"] - LF_DELTA_S) < LF_TOL_DS
+    assert st["fermion_forces"] == 12
+    ss, st_ = oio.plaquette(U1[oio.node_index_of_lex(list(dims))], list(dims))
+    print("PLAQ", ss, st_)
+    assert abs(ss - LF_PLAQ[0]) < LF_TOL_PLAQ[0]
+    # MILC prints 6 decimals; its 1e-7 tolerance is below that rounding
+    assert abs(st_ - LF_PLAQ[1]) < 5e-7 + LF_TOL_PLAQ[1]
+
+
 def test_device_generators_match_oracle():
     """milc_hip_dev_ranmom / dev_gaussian_source draw MILC's numbers: the
     oracle's restatement (pinned to MILC's printed sums) gives the same values"""
