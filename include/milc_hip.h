@@ -135,7 +135,11 @@ int milc_hip_fermion_force_fn_multi(int precision, double eps, const double* res
  * Level 0 outer products -> level-2 adjoint on W (the three-hop product enters
  * the Naik path with unit weight) -> derivative of W = V (V^dag V)^-1/2 ->
  * fat7 adjoint on U; TA(eps U F) is ADDED to momentum ([i][dir][10]).
- * num_naik_terms must be 0 (extra Naik-epsilon sets are rejected, -5). */
+ * num_naik_terms (0..num_terms): the last num_naik_terms fields belong to a
+ * second Naik set (a charm quark, links X_2 + eps_naik X_3); coeff then has
+ * num_terms + num_naik_terms rows, row num_terms + i = eps_naik (1/8, -1/24)
+ * 2 r of field num_terms - num_naik_terms + i (fermion_force_hisq_multi.c:
+ * 2195-2222), and those products take the "1-link + Naik" paths on W. */
 int milc_hip_hisq_force(int precision, int num_terms, int num_naik_terms, double eps, double** coeff,
                         void** quark_field, const double level2_coeff[6], const double fat7_coeff[6],
                         const void* w_link, const void* v_link, const void* u_link, void* momentum);
@@ -323,6 +327,13 @@ int milc_hip_multishift_invert_block(int external_precision, int nsrc, int num_o
  * fallback (the analytic result is kept for them). */
 int milc_hip_load_hisq_links(int precision, const void* thin, void* fatlink, void* longlink, void* W_link,
                              int* n_svd);
+/* The links of a Naik set with correction eps_naik (a charm quark of a 2+1+1
+ * run): X_2 + eps_naik X_3, X_3 = the "1-link + Naik" table on W (1/8, -1/24),
+ * as load_hisq_fn_links builds set inaik (fermion_links_hisq_load_milc.c:
+ * 574-640, the qudaLoadKSLink call of that table on the GPU path);
+ * eps_naik = 0 gives milc_hip_load_hisq_links. */
+int milc_hip_load_hisq_links_naik(int precision, const void* thin, double eps_naik, void* fatlink, void* longlink,
+                                  void* W_link, int* n_svd);
 
 /* ---- device-resident workloads (inputs generated in HBM) -------------------- */
 /* Thin links U = exp(i*scale*H) from `seed` (H traceless Hermitian, Gaussian

@@ -492,6 +492,11 @@ extern "C" int milc_hip_dev_read_thin_links(double* thin, int with_phases) {
 
 extern "C" int milc_hip_load_hisq_links(int precision, const void* thin, void* fatlink, void* longlink, void* W_link,
                                         int* n_svd) {
+  return milc_hip_load_hisq_links_naik(precision, thin, 0.0, fatlink, longlink, W_link, n_svd);
+}
+
+extern "C" int milc_hip_load_hisq_links_naik(int precision, const void* thin, double eps_naik, void* fatlink,
+                                             void* longlink, void* W_link, int* n_svd) {
   Context& c = ctx();
   if (!c.initialised) return -1;
   if (precision != 1 && precision != 2) return -2;
@@ -513,6 +518,14 @@ extern "C" int milc_hip_load_hisq_links(int precision, const void* thin, void* f
   MH_CHECK(hipMemcpyAsync(stage, h, n * sizeof(double), hipMemcpyHostToDevice, c.stream));
   if ((rc = links_from_milc_dev(stage, U, P_F64))) return rc;
   if ((rc = hisq_links_dev(U, fat, lng, &W, n_svd))) return rc;
+  if (eps_naik != 0.0) {
+    // set = X_2 + eps_naik X_3, X_3 = (1/8 W, -1/24 W W W): the Naik paths of
+    // both tables have the same coefficient, so X_3's long link is lng
+    // (fermion_links_hisq_load_milc.c:597-615)
+    if ((rc = link_axpy_dev(fat, W, eps_naik * kHisqNaikSet[0], 1.0, 0)) ||
+        (rc = link_axpy_dev(lng, lng, eps_naik * kHisqNaikSet[1] / kHisqLevel2[1], 1.0, 0)))
+      return rc;
+  }
   std::vector<double> out(n);
   void* dsts[3] = {fatlink, longlink, W_link};
   LinkField* srcs[3] = {&fat, &lng, &W};
@@ -1594,7 +1607,7 @@ extern "C" int milc_hip_hisq_force(int precision, int num_terms, int num_naik_te
   if ((precision != 1 && precision != 2) || num_terms <= 0 || num_terms > 32 || !coeff || !quark_field ||
       !level2_coeff || !fat7_coeff || !w_link || !v_link || !u_link || !momentum)
     return -2;
-  if (num_naik_terms != 0) return -5;  // extra Naik-epsilon sets (charm) are not supported
+  if (num_naik_terms < 0 || num_naik_terms > num_terms) return -2;
   HisqBuffers b;
   int rc;
   if ((rc = b.alloc()) || (rc = b.alloc_vw())) return rc;
@@ -1607,20 +1620,40 @@ extern "C" int milc_hip_hisq_force(int precision, int num_terms, int num_naik_te
     c1[t] = coeff[t][0];
     c3[t] = coeff[t][1];
   }
-  if ((rc = quark_fields_in(precision, quark_field, num_terms, xe, xo)) ||
-      (rc = force_oprod_coeff(xe, xo, c1, c3, b.o1, b.o3))) {
+  LinkField o1n{}, o3n{};
+  auto done = [&](int r) {
     release_fields(xe);
     release_fields(xo);
-    return rc;
+    free_links(o1n);
+    free_links(o3n);
+    return r;
+  };
+  if ((rc = quark_fields_in(precision, quark_field, num_terms, xe, xo)) ||
+      (rc = force_oprod_coeff(xe, xo, c1, c3, b.o1, b.o3)))
+    return done(rc);
+  if (num_naik_terms > 0) {
+    // the second Naik set: the last num_naik_terms fields again, with the
+    // weights coeff[num_terms + i] = eps_naik (1/8, -1/24) 2 r_t
+    // (fermion_force_hisq_multi.c:2195-2222); their three-hop products join
+    // the Naik path, their one-hop products go straight onto W
+    std::vector<VecField> ne(xe.end() - num_naik_terms, xe.end()), no(xo.end() - num_naik_terms, xo.end());
+    std::vector<double> n1(num_naik_terms), n3(num_naik_terms);
+    for (int i = 0; i < num_naik_terms; i++) {
+      n1[i] = coeff[num_terms + i][0];
+      n3[i] = coeff[num_terms + i][1];
+    }
+    if ((rc = alloc_links(o1n, P_F64, c.stride)) || (rc = alloc_links(o3n, P_F64, c.stride)) ||
+        (rc = force_oprod_coeff(ne, no, n1, n3, o1n, o3n)) || (rc = link_axpy_dev(b.o3, o3n, 1.0, 1.0, 0)))
+      return done(rc);
   }
   // the three-hop coefficients already carry the Naik weight
   // (fermion_force_hisq_multi.c:2190-2193): unit weight on the Naik path
   double l2[6];
   for (int i = 0; i < 6; i++) l2[i] = level2_coeff[i];
   l2[1] = 1.0;
-  rc = hisq_force_from_oprod(b.U, b.V, b.W, b.o1, b.o3, l2, fat7_coeff, eps, kForceFilter, b.mom);
-  release_fields(xe);
-  release_fields(xo);
+  rc = hisq_force_from_oprod(b.U, b.V, b.W, b.o1, b.o3, l2, fat7_coeff, eps, kForceFilter, b.mom,
+                             num_naik_terms > 0 ? &o1n : nullptr);
+  done(0);
   if (rc) return rc;
   return add_momentum(precision, b.mom, momentum);
 }

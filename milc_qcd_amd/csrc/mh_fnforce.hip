@@ -1390,9 +1390,39 @@ static int smear_adjoint_dev(const LinkField& L, const LinkField& F1, const Link
 // fn_fermion_force_multi_hisq_wrapper_mx_cpu, fermion_force_hisq_multi.c:1182-1475):
 // F_W = level-2 adjoint on W (coeff2, eps(x) O1 / O3), F_V = U(3) projection
 // adjoint at V, then the fat7 adjoint on U (coeff1), mom += TA(scale U F_U).
+// y = b y + a s x on FP64 link fields, s = +1 on even and -1 on odd sites
+// when signed_odd (the parity factor of the force's outer products)
+__global__ void __launch_bounds__(256) k_link_axpy(LinkField y, LinkField x, double a, double b, int signed_odd,
+                                                   int Vh) {
+  int cb = blockIdx.x * blockDim.x + threadIdx.x;
+  int pb = blockIdx.y >> 2, mu = blockIdx.y & 3;
+  if (cb >= Vh) return;
+  Mat3<double> my, mx;
+  load_link<P_F64>(y, pb, mu, cb, my);
+  load_link<P_F64>(x, pb, mu, cb, mx);
+  const double as = (signed_odd && pb == 1) ? -a : a;
+#pragma unroll
+  for (int e = 0; e < 9; e++) {
+    my.e[e / 3][e % 3].re = b * my.e[e / 3][e % 3].re + as * mx.e[e / 3][e % 3].re;
+    my.e[e / 3][e % 3].im = b * my.e[e / 3][e % 3].im + as * mx.e[e / 3][e % 3].im;
+  }
+  store_link<P_F64, double>(y, pb, mu, cb, my, 1.0f);
+}
+
+int link_axpy_dev(LinkField& y, const LinkField& x, double a, double b, int signed_odd) {
+  Context& c = ctx();
+  hipLaunchKernelGGL(k_link_axpy, dim3((c.Vh + 255) / 256, 8), dim3(256), 0, c.stream, y, x, a, b, signed_odd,
+                     c.Vh);
+  MH_CHECK(hipGetLastError());
+  return 0;
+}
+
+// one_link_extra: outer products of terms whose links carry a second Naik set,
+// already weighted by eps_naik times that set's one-link coefficient; their
+// one-link path adds them (with the parity factor) to the force on W
 int hisq_force_from_oprod(const LinkField& U, const LinkField& V, const LinkField& W, const LinkField& o1,
                           const LinkField& o3, const double coeff2[6], const double coeff1[6], double scale,
-                          double filter, double* mom_dev) {
+                          double filter, double* mom_dev, const LinkField* one_link_extra) {
   Context& c = ctx();
   LinkField FW{}, FV{};
   int rc;
@@ -1403,6 +1433,7 @@ int hisq_force_from_oprod(const LinkField& U, const LinkField& V, const LinkFiel
   LinkField none{nullptr, 1.0f, 0};
   if (use_staple_form(coeff2, true)) rc = smear_adjoint_dev(W, o1, o3, 1, coeff2, FW);
   else rc = path_adjoint_launch(0, W, o1, o3, 1, coeff2, FW, none, 0.0, nullptr);
+  if (!rc && one_link_extra) rc = link_axpy_dev(FW, *one_link_extra, 1.0, 1.0, 1);
   if (!rc) {
     hipLaunchKernelGGL(k_u3_adjoint, dim3((c.Vh + 127) / 128, 8), dim3(128), 0, c.stream, c.geom, V, FW, FV, filter);
     rc = hipGetLastError() == hipSuccess ? 0 : -1;

@@ -155,7 +155,11 @@ constexpr double kHisqLevel1[6] = {1.0 / 8.0, 0.0, -1.0 / 8.0 * 0.5, 1.0 / 8.0 *
 constexpr double kForceFilter = 5.0e-5;  // HISQ_FORCE_FILTER of ks_imp_rhmc/Make_template
 int hisq_force_from_oprod(const LinkField& U, const LinkField& V, const LinkField& W, const LinkField& o1,
                           const LinkField& o3, const double coeff2[6], const double coeff1[6], double scale,
-                          double filter, double* mom_dev);
+                          double filter, double* mom_dev, const LinkField* one_link_extra = nullptr);
+// y = b y + a x on FP64 link fields (x negated on odd sites when signed_odd)
+int link_axpy_dev(LinkField& y, const LinkField& x, double a, double b, int signed_odd);
+// "1-link + Naik" table of a second Naik set (imp_actions/hisq/hisq_u3_action.h:86-101)
+constexpr double kHisqNaikSet[2] = {1.0 / 8.0, -1.0 / 24.0};
 // thin-link force adjoint form: 1 staple form (default), 0 path-table gather
 void set_force_form(int form);
 // One RHMC trajectory on the device links (one HISQ pseudofermion): rational

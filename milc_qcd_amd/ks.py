@@ -9,7 +9,8 @@ returned control record, invalid arguments raise):
     ks_multicg_offset_field generic_ks/ks_multicg_offset_gpu.c:38 (-> milc_hip_multishift_invert)
     dslash_fn_field        generic_ks/dslash_fn.c:306  (-> milc_hip_dslash)
     compute_oprod          generic_ks/fermion_force_asqtad_gpu.c:60 (-> milc_hip_compute_oprod)
-    load_hisq_links        generic_ks/fermion_links_hisq_load_milc.c:546-702 (-> milc_hip_load_hisq_links)
+    load_hisq_links        generic_ks/fermion_links_hisq_load_milc.c:546-702 (-> milc_hip_load_hisq_links_naik;
+                           eps_naik != 0: the Naik set of load_hisq_fn_links, :574-640)
 
 Host fields are numpy float64 arrays in MILC's own layout and site order:
 vectors (V, 3, 2), links (V, 4, 3, 3, 2), even sites first.
@@ -41,7 +42,7 @@ class InvArgs(ctypes.Structure):
 EXPORTS = [
     "milc_hip_init", "milc_hip_finalize", "milc_hip_version", "milc_hip_invert",
     "milc_hip_multishift_invert", "milc_hip_dslash", "milc_hip_compute_oprod",
-    "milc_hip_load_hisq_links", "milc_hip_dev_gauge_hisq", "milc_hip_dev_source",
+    "milc_hip_load_hisq_links", "milc_hip_load_hisq_links_naik", "milc_hip_dev_gauge_hisq", "milc_hip_dev_source",
     "milc_hip_dev_invert", "milc_hip_dev_read", "milc_hip_dev_write", "milc_hip_dev_read_links",
     "milc_hip_dev_dslash_bench", "milc_hip_dev_copy_bench", "milc_hip_dev_diff",
     "milc_hip_dev_rhmc_action", "milc_hip_dev_force_level0", "milc_hip_dev_load_links",
@@ -172,6 +173,7 @@ def load_library(path=LIB_PATH):
     L.milc_hip_compute_oprod.argtypes = [I, I, ctypes.POINTER(ctypes.POINTER(D)), ctypes.POINTER(P),
                                          ctypes.POINTER(P)]
     L.milc_hip_load_hisq_links.argtypes = [I, P, P, P, P, ctypes.POINTER(I)]
+    L.milc_hip_load_hisq_links_naik.argtypes = [I, P, ctypes.c_double, P, P, P, ctypes.POINTER(I)]
     L.milc_hip_dev_gauge_hisq.argtypes = [ctypes.c_uint64, D, ctypes.POINTER(I)]
     L.milc_hip_dev_source.argtypes = [ctypes.c_uint64, I, I]
     L.milc_hip_dev_invert.argtypes = [I, I, D, I, D, I, I, ctypes.POINTER(D)]
@@ -322,14 +324,15 @@ class Lattice:
         self._fresh = False
         return n
 
-    def load_hisq_links(self, U_phased):
-        """HISQ U(3) fat and Naik links from thin links with phases in."""
+    def load_hisq_links(self, U_phased, eps_naik=0.0):
+        """HISQ U(3) fat and Naik links from thin links with phases in; with
+        eps_naik != 0 the links of that Naik set (X_2 + eps_naik X_3)."""
         self._activate()
         U = np.ascontiguousarray(U_phased, dtype=np.float64)
         fat, lng, W = np.zeros_like(U), np.zeros_like(U), np.zeros_like(U)
         nsvd = ctypes.c_int(0)
-        _check(self.lib.milc_hip_load_hisq_links(2, _p(U), _p(fat), _p(lng), _p(W), ctypes.byref(nsvd)),
-               "milc_hip_load_hisq_links")
+        _check(self.lib.milc_hip_load_hisq_links_naik(2, _p(U), ctypes.c_double(eps_naik), _p(fat), _p(lng), _p(W),
+                                                      ctypes.byref(nsvd)), "milc_hip_load_hisq_links_naik")
         return fat, lng, W, nsvd.value
 
     # ---------------------------------------------------------------- operators
@@ -593,14 +596,18 @@ class Lattice:
         return mom
 
     def hisq_force(self, eps, coeff, multi_x, level2_coeff, fat7_coeff, W, V, U, mom, num_naik_terms=0):
-        """qudaHisqForce form: coeff[t] = (one-hop, three-hop) weights."""
+        """qudaHisqForce form: coeff[t] = (one-hop, three-hop) weights; with
+        num_naik_terms > 0, len(multi_x) + num_naik_terms rows, the extra rows
+        for the last num_naik_terms fields (second Naik set)."""
         self._activate()
         n = len(multi_x)
+        if not 0 <= num_naik_terms <= n or len(coeff) != n + num_naik_terms:
+            raise MilcHipError("hisq_force: coeff needs len(multi_x) + num_naik_terms rows")
         xs = [np.ascontiguousarray(x, dtype=np.float64) for x in multi_x]
         qptr = (ctypes.c_void_p * n)(*[x.ctypes.data for x in xs])
         rows = [(ctypes.c_double * 2)(*c) for c in coeff]
-        cptr = (ctypes.POINTER(ctypes.c_double) * n)(*[ctypes.cast(r, ctypes.POINTER(ctypes.c_double))
-                                                       for r in rows])
+        cptr = (ctypes.POINTER(ctypes.c_double) * len(rows))(*[ctypes.cast(r, ctypes.POINTER(ctypes.c_double))
+                                                               for r in rows])
         l2 = (ctypes.c_double * 6)(*level2_coeff)
         l1 = (ctypes.c_double * 6)(*fat7_coeff)
         arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (W, V, U)]

@@ -481,6 +481,24 @@ int kso_u3_unitarize(const double *V, double *W) {
  * unitarisation with phases out (rephase_field_offset OFF/ON around
  * u3_unitarize_analytic, :251-270), U(3) group so W = Y (WeqY).
  * eps_naik = 0, a single Naik set. */
+/* load_hisq_fn_links with n_naiks = 2 (generic_ks/fermion_links_hisq_load_milc.c:
+ * 574-640): X_3 = the "1-link + Naik" table on W, set 1 = X_2 + eps_naik X_3;
+ * fat/lng hold X_2 on entry and set 1 on return */
+void kso_hisq_links_naik(const kso_geom *g, double eps_naik, const double *W, double *fat, double *lng) {
+  const size_t n = 4 * g->V * 18;
+  const double c3[6] = {1.0 / 8.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  double *f3 = (double *)malloc(sizeof(double) * n);
+  double *l3 = (double *)malloc(sizeof(double) * n);
+  kso_fat_links(g, c3, W, f3);
+  kso_long_links(g, -1.0 / 24.0, W, l3);
+  for (size_t k = 0; k < n; k++) {
+    fat[k] += eps_naik * f3[k];
+    lng[k] += eps_naik * l3[k];
+  }
+  free(f3);
+  free(l3);
+}
+
 int kso_hisq_links(const kso_geom *g, const double *U, double *fat, double *lng, double *W_out) {
   const size_t n = 4 * g->V * 18;
   double *Vl = (double *)malloc(sizeof(double) * n);
@@ -1264,7 +1282,19 @@ void kso_u3_adjoint(const double *Vm, const double *FW, double *FV) {
  * U has the KS phases in; V and W are rebuilt here as kso_hisq_links does. */
 void kso_hisq_force(const kso_geom *g, const double *U, const double *const *multi_x, const double *residues,
                     int nterms, double eps, double *mom) {
+  kso_hisq_force_naik(g, U, multi_x, residues, nterms, 0, 0.0, eps, mom);
+}
+
+/* With a second Naik set (fn_fermion_force_multi_hisq_wrapper_mx,
+ * generic_ks/fermion_force_hisq_multi.c:1284-1374): every term goes through
+ * the level-2 adjoint (ap->p2); the last n_naik terms, whose links carry
+ * eps_naik times the "1-link + Naik" set (ap->p3: 1/8, -1/24,
+ * imp_actions/hisq/hisq_u3_action.h:86-101), also through that table's
+ * adjoint, scaled by eps_naik, before the U(3) derivative. */
+void kso_hisq_force_naik(const kso_geom *g, const double *U, const double *const *multi_x, const double *residues,
+                         int nterms, int n_naik, double eps_naik, double eps, double *mom) {
   const size_t n = 4 * g->V * 18;
+  const double c3[6] = {1.0 / 8.0, -1.0 / 24.0, 0.0, 0.0, 0.0, 0.0};
   const double c1[6] = {1.0 / 8.0, 0.0, -1.0 / 8.0 * 0.5, 1.0 / 8.0 * 0.25 * 0.5,
                         -1.0 / 8.0 * 0.125 * (1.0 / 6.0), 0.0};
   const double c2[6] = {(1.0 / 8.0) + (2.0 * 6.0 / 16.0) + (1.0 / 8.0), -1.0 / 24.0,
@@ -1282,6 +1312,13 @@ void kso_hisq_force(const kso_geom *g, const double *U, const double *const *mul
   for (size_t l = 0; l < 4 * g->V; l++) kso_u3_unitarize(Vl + 18 * l, W + 18 * l);
   kso_force_oprod(g, multi_x, residues, nterms, o1, o3);
   force_adjoint(g, c2, W, o1, o3, 1, FW);
+  if (n_naik > 0) {
+    double *FE = (double *)calloc(n, sizeof(double));
+    kso_force_oprod(g, multi_x + (nterms - n_naik), residues + (nterms - n_naik), n_naik, o1, o3);
+    force_adjoint(g, c3, W, o1, o3, 1, FE);
+    for (size_t k = 0; k < n; k++) FW[k] += eps_naik * FE[k];
+    free(FE);
+  }
 #pragma omp parallel for
   for (size_t l = 0; l < 4 * g->V; l++) kso_u3_adjoint(Vl + 18 * l, FW + 18 * l, FV + 18 * l);
   force_adjoint(g, c1, U, FV, FV, 0, FU);

@@ -108,6 +108,15 @@ double kso_mom_action(const kso_geom *g, const double *mom);
 void kso_update_u(const kso_geom *g, double eps, double *U, const double *mom);
 void kso_hisq_force(const kso_geom *g, const double *U, const double *const *multi_x,
                     const double *residues, int nterms, double eps, double *mom);
+/* the same with the last n_naik of the nterms terms in a second Naik set
+ * (links X_2 + eps_naik X_3) */
+void kso_hisq_force_naik(const kso_geom *g, const double *U, const double *const *multi_x,
+                         const double *residues, int nterms, int n_naik, double eps_naik, double eps,
+                         double *mom);
+/* links of a second Naik set from the level-2 links and W:
+ * fat += eps_naik/8 W, lng += eps_naik lng (the Naik paths of both tables
+ * have coefficient -1/24) */
+void kso_hisq_links_naik(const kso_geom *g, double eps_naik, const double *W, double *fat, double *lng);
 /* 0: force adjoints through the path tables (default), 1: staple form */
 void kso_set_force_form(int form);
 /* force through W = V (V^dag V)^-1/2 at one link: F_V from F_W */

@@ -67,6 +67,9 @@ def lib():
         L.kso_force_oprod.argtypes = [G, P, P, ctypes.c_int, P, P]
         L.kso_fn_force.argtypes = [G, P, P, P, P, ctypes.c_int, ctypes.c_double, P]
         L.kso_hisq_force.argtypes = [G, P, P, P, ctypes.c_int, ctypes.c_double, P]
+        L.kso_hisq_force_naik.argtypes = [G, P, P, P, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                          ctypes.c_double, P]
+        L.kso_hisq_links_naik.argtypes = [G, ctypes.c_double, P, P, P]
         L.kso_u3_adjoint.argtypes = [P, P, P]
         L.kso_set_force_form.argtypes = [ctypes.c_int]
         L.kso_gauge_coeffs.argtypes = [ctypes.c_double, ctypes.c_int, ctypes.c_int, P]
@@ -276,15 +279,23 @@ class Lattice:
         n = lib().kso_reunitarize(self.g, tol, _p(U), ctypes.byref(md))
         return n, md.value
 
-    def hisq_force(self, U_phased, multi_x, residues, eps=1.0):
-        """HISQ thin-link force TA(2 eps U F_U), full 3x3 matrices (V, 4, 3, 3, 2)."""
+    def hisq_force(self, U_phased, multi_x, residues, eps=1.0, n_naik=0, eps_naik=0.0):
+        """HISQ thin-link force TA(2 eps U F_U), full 3x3 matrices (V, 4, 3, 3, 2);
+        the last n_naik terms belong to a second Naik set with eps_naik."""
         n = len(multi_x)
         xs = [np.ascontiguousarray(x, dtype=np.float64) for x in multi_x]
         ptrs = (ctypes.c_void_p * n)(*[x.ctypes.data for x in xs])
         res = np.ascontiguousarray(residues, dtype=np.float64)
         mom = self.new_links()
-        lib().kso_hisq_force(self.g, _p(np.ascontiguousarray(U_phased)), ptrs, _p(res), n, eps, _p(mom))
+        lib().kso_hisq_force_naik(self.g, _p(np.ascontiguousarray(U_phased)), ptrs, _p(res), n, n_naik, eps_naik,
+                                  eps, _p(mom))
         return mom
+
+    def hisq_links_naik(self, U_phased, eps_naik):
+        """links of a second Naik set: X_2 + eps_naik X_3 (fat, lng)"""
+        fat, lng, _, W = self.hisq_links(U_phased, want_W=True)
+        lib().kso_hisq_links_naik(self.g, eps_naik, _p(W), _p(fat), _p(lng))
+        return fat, lng
 
     def force_oprod(self, multi_x, residues):
         n = len(multi_x)

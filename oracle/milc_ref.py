@@ -39,6 +39,8 @@ def lib():
         L.ref_multicg.argtypes = [P, ctypes.POINTER(P), P, I, I, I, D, ctypes.POINTER(D)]
         L.ref_force_oprod.argtypes = [ctypes.POINTER(P), P, I, P, P]
         L.ref_hisq_force.argtypes = [ctypes.POINTER(P), P, I, ctypes.c_double, P]
+        L.ref_hisq_links_naik.argtypes = [D, P, P, P, P]
+        L.ref_hisq_force_naik.argtypes = [ctypes.POINTER(P), P, I, I, ctypes.c_double, P]
         _lib = L
     return _lib
 
@@ -70,6 +72,13 @@ def hisq_links(V):
     lng = np.zeros((V, 4, 3, 3, 2))
     lib().ref_hisq_links(_p(fat), _p(lng))
     return fat, lng
+
+
+def hisq_links_naik(V, eps):
+    """MILC's links with two Naik sets, eps_naik = {0, eps}: (fat0, lng0, fat1, lng1)"""
+    out = [np.zeros((V, 4, 3, 3, 2)) for _ in range(4)]
+    lib().ref_hisq_links_naik(eps, *[_p(a) for a in out])
+    return tuple(out)
 
 
 def set_fn_links(fat, lng):
@@ -115,15 +124,17 @@ def force_oprod(multi_x, residues, V):
     return o1, o3
 
 
-def hisq_force(multi_x, residues, eps, V):
-    """MILC's HISQ thin-link force on the links of the last hisq_links call;
-    returns the momentum added to zero, (V, 4, 10) packed anti_hermitmat."""
+def hisq_force(multi_x, residues, eps, V, n_naik=0):
+    """MILC's HISQ thin-link force on the links of the last hisq_links call
+    (n_naik = 0) or hisq_links_naik call (the last n_naik terms in the second
+    Naik set); returns the momentum added to zero, (V, 4, 10) packed
+    anti_hermitmat."""
     n = len(multi_x)
     xs = [np.ascontiguousarray(x, dtype=np.float64) for x in multi_x]
     ptrs = (ctypes.c_void_p * n)(*[x.ctypes.data for x in xs])
     res = np.ascontiguousarray(residues, dtype=np.float64)
     mom = np.zeros((V, 4, 10))
-    rc = lib().ref_hisq_force(ptrs, _p(res), n, eps, _p(mom))
+    rc = lib().ref_hisq_force_naik(ptrs, _p(res), n - n_naik, n_naik, eps, _p(mom))
     if rc != 0:
         raise RuntimeError("ref_hisq_force: no HISQ links (call hisq_links first)")
     return mom

@@ -86,6 +86,23 @@ int ref_hisq_links(double *fat, double *lng) {
   return 0;
 }
 
+/* the same with a second Naik set (n_naiks = 2, eps_naik = {0, eps}, as a
+ * 2+1+1 run with a charm quark): set 0 into fat/lng, set 1 into fat1/lng1
+ * (load_hisq_fn_links, fermion_links_hisq_load_milc.c:574-640) */
+int ref_hisq_links_naik(double eps, double *fat, double *lng, double *fat1, double *lng1) {
+  double eps_naik[2] = {0.0, eps};
+  imp_ferm_links_t **fn;
+  if (g_fl) destroy_fermion_links_hisq(g_fl);
+  g_fl = create_fermion_links_from_site(MILC_PRECISION, 2, eps_naik);
+  fn = get_fm_links(g_fl);
+  memcpy(fat, get_fatlinks(fn[0]), sizeof(su3_matrix) * 4 * sites_on_node);
+  memcpy(lng, get_lnglinks(fn[0]), sizeof(su3_matrix) * 4 * sites_on_node);
+  memcpy(fat1, get_fatlinks(fn[1]), sizeof(su3_matrix) * 4 * sites_on_node);
+  memcpy(lng1, get_lnglinks(fn[1]), sizeof(su3_matrix) * 4 * sites_on_node);
+  g_cur = fn[0];
+  return 0;
+}
+
 /* use caller-supplied fat/long links (MILC layout) for the solver entry points */
 int ref_set_fn_links(const double *fat, const double *lng) {
   if (!g_ext) g_ext = create_fn_links();

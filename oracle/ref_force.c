@@ -32,6 +32,7 @@ void ref_force_oprod(double **multi_x, const double *res, int n, double *o1, dou
 }
 
 fermion_links_t *ref_get_fl(void);
+int ref_hisq_force_naik(double **multi_x, const double *res, int n0, int n1, double eps, double *mom);
 
 /* MILC's full HISQ thin-link force (eo_fermion_force_multi ->
  * fn_fermion_force_multi_hisq_wrapper_mx_cpu with KS_MULTIFF = FNMAT and
@@ -39,14 +40,21 @@ fermion_links_t *ref_get_fl(void);
  * multi_x[t] = X_t on all sites, one Naik set; mom: the anti_hermitmat
  * added to a zero momentum, [i][dir][10] doubles. */
 int ref_hisq_force(double **multi_x, const double *res, int n, double eps, double *mom) {
+  return ref_hisq_force_naik(multi_x, res, n, 0, eps, mom);
+}
+
+/* ... with the last n1 of the n0 + n1 terms in the second Naik set (links of
+ * the last ref_hisq_links_naik call, or n1 = 0) */
+int ref_hisq_force_naik(double **multi_x, const double *res, int n0, int n1, double eps, double *mom) {
   fermion_links_t *fl = ref_get_fl();
   Real r[64];
   size_t i;
-  int dir, t;
-  if (!fl || n > 64) return -1;
+  int dir, t, n = n0 + n1;
+  if (!fl || n > 64 || n1 < 0 || (n1 > 0 && fermion_links_get_n_naiks(fl) != 2)) return -1;
   for (t = 0; t < n; t++) r[t] = res[t];
   n_order_naik_total = n;
-  n_orders_naik[0] = n;
+  n_orders_naik[0] = n0;
+  n_orders_naik[1] = n1;
   for (i = 0; i < sites_on_node; i++)
     for (dir = 0; dir < 4; dir++) memset(&lattice[i].mom[dir], 0, sizeof(anti_hermitmat));
   eo_fermion_force_multi(eps, r, (su3_vector **)multi_x, n, MILC_PRECISION, fl);

@@ -428,6 +428,17 @@ def test_hisq_force_thin_link_finite_differences():
                   lambda Ul, Z, r, e: L.hisq_force(Ul, Z, r, eps=e), seed=6)
 
 
+def test_hisq_force_second_naik_set_finite_differences():
+    """Links of a second Naik set (X_2 + eps_naik X_3, eps_naik = -0.3) and
+    kso_hisq_force_naik with every term in that set, against central
+    differences of the action built on those links."""
+    L = ko.Lattice((4, 4, 4, 4))
+    U = L.rephase(L.gauge_random(79, 0.3))
+    epsn = -0.3
+    _thin_link_fd(L, U, lambda Ul: L.hisq_links_naik(Ul, epsn),
+                  lambda Ul, Z, r, e: L.hisq_force(Ul, Z, r, eps=e, n_naik=len(Z), eps_naik=epsn), seed=7)
+
+
 def test_staple_form_force_equals_path_table_form():
     """The reverse-mode staple form of the smearing adjoint gives the same
     thin-link momentum as the path-table form (asqtad FN and HISQ chains) on a

@@ -104,3 +104,28 @@ def test_hisq_force_matches_milc(setup):
     e = rel(m_orc, m_milc)
     print(f"HISQ force oracle vs MILC: {e:.2e}")
     assert e < 1e-12
+
+
+def test_second_naik_set_links_and_force_match_milc(setup):
+    """A second Naik set (n_naiks = 2, eps_naik = {0, -0.2}: a charm quark of a
+    2+1+1 run): MILC's set-1 links (X_2 + eps X_3) and its force with the last
+    two of four terms in that set, against the oracle"""
+    L, V, fat, lng, f_ref, l_ref = setup
+    U = L.gauge_random(1000)
+    epsn = -0.2
+    f0, l0, f1, l1 = mr.hisq_links_naik(V, epsn)
+    assert rel(f0, f_ref) < 1e-14 and rel(l0, l_ref) < 1e-14
+    fo, lo = L.hisq_links_naik(L.rephase(U), epsn)
+    assert rel(fo, f1) < 1e-14 and rel(lo, l1) < 1e-14
+    assert rel(lo, (1 + epsn) * l_ref) < 1e-14
+    xs = [L.gaussian_vector(300 + t, ko.EVENANDODD) for t in range(4)]
+    res = [0.25, -0.8, 1.3, 0.6]
+    eps = 0.35
+    m_milc = mr.hisq_force(xs, res, eps, V, n_naik=2)
+    m_orc = _pack_ah(L.hisq_force(L.rephase(U), xs, res, eps=eps, n_naik=2, eps_naik=epsn))
+    e = rel(m_orc, m_milc)
+    print(f"HISQ force with a second Naik set, oracle vs MILC: {e:.2e}")
+    assert e < 1e-12
+    # the second set changes the force: the eps_naik term is really there
+    m_plain = _pack_ah(L.hisq_force(L.rephase(U), xs, res, eps=eps))
+    assert rel(m_plain, m_milc) > 1e-3

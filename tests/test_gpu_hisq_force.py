@@ -6,7 +6,11 @@ central differences of the action):
   V and W rebuilt on the device from the thin links);
 * milc_hip_hisq_force (qudaHisqForce form: per-term one-hop / three-hop
   weights 2 r_t and naik 2 r_t, W / V / U supplied);
-* the momentum is added to, not overwritten.
+* the momentum is added to, not overwritten;
+* a second Naik set (a charm quark of a 2+1+1 run, eps_naik = -0.2): its
+  links from milc_hip_load_hisq_links_naik and the force with the last two of
+  four terms in that set (oracle pinned to MILC's compiled code with
+  n_naiks = 2 and to central differences on those links).
 
 Workload: 8^4 and 6x4x8x10 lattices, thin links from gauge seeds 1020/1021
 (scale 0.4), three terms on all sites from seeds 300+, FP64, tolerance 1e-12.
@@ -149,3 +153,42 @@ def test_non_unitary_links_exact_by_default(form):
             assert rel(mom, ref_f) < 1e-12
         finally:
             G.set_force_form(1)
+
+
+EPS_NAIK = -0.2
+
+
+def test_load_hisq_links_second_naik_set_vs_oracle():
+    dims = (8, 8, 8, 8)
+    L = ko.Lattice(dims)
+    U = L.rephase(L.gauge_random(1023))
+    fo, lo = L.hisq_links_naik(U, EPS_NAIK)
+    with mq.Lattice(dims) as G:
+        gf, gl, gW, _ = G.load_hisq_links(U, eps_naik=EPS_NAIK)
+        gf0, gl0, _, _ = G.load_hisq_links(U)
+    assert rel(gf, fo) < 1e-13 and rel(gl, lo) < 1e-13
+    f0, l0, _ = L.hisq_links(U)
+    assert rel(gf0, f0) < 1e-13 and rel(gl0, l0) < 1e-13
+    assert rel(gf, f0) > 1e-3
+
+
+def test_hisq_force_second_naik_set_vs_oracle():
+    dims = (8, 8, 8, 8)
+    L = ko.Lattice(dims)
+    U = L.rephase(L.gauge_random(1024))
+    _, _, _, W = L.hisq_links(U, want_W=True)
+    Vf = L.fat_links(C1, U)
+    xs = [L.gaussian_vector(330 + t, ko.EVENANDODD) for t in range(4)]
+    res = [0.6, 0.9, -0.4, 1.1]
+    eps = 0.2
+    ref = pack_ah(L.hisq_force(U, xs, res, eps=eps, n_naik=2, eps_naik=EPS_NAIK))
+    coeff = [(2 * r, C2[1] * 2 * r) for r in res]
+    coeff += [(EPS_NAIK * 2 * r / 8, -EPS_NAIK * 2 * r / 24) for r in res[2:]]
+    with mq.Lattice(dims) as G:
+        mom = np.zeros((L.V, 4, 10))
+        G.hisq_force(eps, coeff, xs, C2, C1, W, Vf, U, mom, num_naik_terms=2)
+    e = rel(mom, ref)
+    print(f"second Naik set force vs oracle: {e:.2e}")
+    assert e < 1e-12
+    plain = pack_ah(L.hisq_force(U, xs, res, eps=eps))
+    assert rel(plain, ref) > 1e-3
