@@ -204,6 +204,9 @@ typedef struct {
                                            poles already hold it, as in MILC's files */
   milc_hip_rational md, gr, fa;         /* force, heatbath, action */
   double resid_md, resid_gr, resid_fa;  /* solver targets |r|/|b| (MILC: sqrt(rsqmin)) */
+  double naik_epsilon;                  /* naik_term_epsilon of the rational file (HISQ only):
+                                           != 0 puts the pseudofermion on the links
+                                           X_2 + eps X_3 of its Naik set */
 } milc_hip_pseudofermion;
 
 typedef struct {

@@ -1876,6 +1876,8 @@ extern "C" int milc_hip_dev_rhmc_trajectory(const milc_hip_rhmc_params* p, doubl
     b.resid_md = a.resid_md;
     b.resid_gr = a.resid_gr;
     b.resid_fa = a.resid_fa;
+    b.naik_eps = a.naik_epsilon;
+    if (b.naik_eps != 0.0 && q.action != 0) return -2;  // Naik sets are HISQ's
   }
   q.mixed_md = p->mixed_md;
   q.mixed_action = p->mixed_action;

@@ -172,6 +172,7 @@ struct RhmcPseudo {
   double mass;  // 4 mass^2 is added to every pole
   RhmcRat md, gr, fa;
   double resid_md, resid_gr, resid_fa;
+  double naik_eps;  // naik_term_epsilon: links X_2 + naik_eps X_3
 };
 #define RHMC_MAX_PSEUDO 8
 struct RhmcParams {

@@ -381,6 +381,13 @@ struct Pseudo {
 struct Traj {
   const RhmcParams& p;
   LinkField Ub{}, V{}, W{}, o1{}, o3{}, Uold{};
+  // Naik sets (load_hisq_fn_links, fermion_links_hisq_load_milc.c:574-640):
+  // the distinct naik_term_epsilon values, 0 first; with more than one, the
+  // set-0 links X_2 are kept in fat0/lng0 and the context's links are made
+  // X_2 + eps X_3 for the solves of that set's pseudofermions
+  std::vector<double> sets{0.0};
+  double cur_eps = 0;
+  LinkField fat0{}, lng0{}, o1s{}, o3s{}, o1x{};
   double *mom = nullptr, *mtmp = nullptr;
   std::vector<Pseudo> pf;
   bool dirty = false;  // thin links changed since the fat and Naik links were built
@@ -392,27 +399,64 @@ struct Traj {
   explicit Traj(const RhmcParams& prm) : p(prm) {}
   ~Traj() {
     free_links(Ub); free_links(V); free_links(W); free_links(o1); free_links(o3); free_links(Uold);
+    free_links(fat0); free_links(lng0); free_links(o1s); free_links(o3s); free_links(o1x);
     dev_free(mom); dev_free(mtmp);
     for (auto& q : pf) free_vec(q.phi);
   }
   size_t nmom() const { return (size_t)2 * ctx().Vh * 4 * 10; }
-  // several pseudofermions with mixed solves share one batched multi-shift
-  bool batched(int mixed) const { return mixed && p.n_pseudo >= 2; }
-  std::vector<const VecField*> sources() const {
-    std::vector<const VecField*> v;
-    for (auto& q : pf) v.push_back(&q.phi);
-    return v;
-  }
-  std::vector<std::vector<double>> offsets(double (Pseudo::*offs)[33], RhmcRat RhmcPseudo::*rat) const {
-    std::vector<std::vector<double>> v;
+  bool naik() const { return sets.size() > 1; }
+  // the pseudofermions of the Naik set with correction eps, in their order
+  std::vector<int> members(double eps) const {
+    std::vector<int> v;
     for (int i = 0; i < p.n_pseudo; i++)
-      v.emplace_back(pf[i].*offs + 1, pf[i].*offs + 1 + (p.pf[i].*rat).order);
+      if (p.pf[i].naik_eps == eps) v.push_back(i);
     return v;
   }
-  std::vector<double> resids(double RhmcPseudo::*r) const {
-    std::vector<double> v;
-    for (int i = 0; i < p.n_pseudo; i++) v.push_back(p.pf[i].*r);
+  // several pseudofermions with mixed solves share one batched multi-shift
+  static bool batched(int mixed, size_t n) { return mixed && n >= 2; }
+  std::vector<const VecField*> sources(const std::vector<int>& idx) const {
+    std::vector<const VecField*> v;
+    for (int i : idx) v.push_back(&pf[i].phi);
     return v;
+  }
+  std::vector<std::vector<double>> offsets(const std::vector<int>& idx, double (Pseudo::*offs)[33],
+                                           RhmcRat RhmcPseudo::*rat) const {
+    std::vector<std::vector<double>> v;
+    for (int i : idx) v.emplace_back(pf[i].*offs + 1, pf[i].*offs + 1 + (p.pf[i].*rat).order);
+    return v;
+  }
+  std::vector<double> resids(const std::vector<int>& idx, double RhmcPseudo::*r) const {
+    std::vector<double> v;
+    for (int i : idx) v.push_back(p.pf[i].*r);
+    return v;
+  }
+  // the FP32 and 16-bit copies follow the FP64 links: drop them
+  static void drop_derived() {
+    Context& c = ctx();
+    for (int q = P_F32; q <= P_H16; q++)
+      if (c.links[q].valid) {
+        free_links(c.links[q].fat);
+        free_links(c.links[q].lng);
+        c.links[q].valid = false;
+      }
+    c.links_version++;
+  }
+  // make the context's FP64 links those of the Naik set eps: X_2 + eps X_3,
+  // X_3 = (1/8 W, -1/24 W W W), whose long link is X_2's times the ratio of
+  // the two tables' Naik coefficients; W is the current thin links' (fresh())
+  int use_set(double eps) {
+    if (eps == cur_eps) return 0;
+    Context& c = ctx();
+    DevLinks& L = c.links[P_F64];
+    const size_t nb = link_bytes(P_F64, c.stride);
+    MH_CHECK(hipMemcpyAsync(L.fat.data, fat0.data, nb, hipMemcpyDeviceToDevice, c.stream));
+    MH_CHECK(hipMemcpyAsync(L.lng.data, lng0.data, nb, hipMemcpyDeviceToDevice, c.stream));
+    int rc = 0;
+    if (eps != 0.0 && !(rc = link_axpy_dev(L.fat, W, eps * kHisqNaikSet[0], 1.0, 0)))
+      rc = link_axpy_dev(L.lng, L.lng, eps * kHisqNaikSet[1] / kHisqLevel2[1], 1.0, 0);
+    drop_derived();
+    cur_eps = eps;
+    return rc;
   }
 
   // fat and Naik links of the current thin links, rebuilt only when used
@@ -430,13 +474,14 @@ struct Traj {
       rc = hisq_links_dev(c.thin, L.fat, L.lng, &W, &nsvd, &V);
       aux = rc == 0;
     }
-    for (int q = P_F32; q <= P_H16; q++)
-      if (c.links[q].valid) {
-        free_links(c.links[q].fat);
-        free_links(c.links[q].lng);
-        c.links[q].valid = false;
-      }
-    c.links_version++;
+    drop_derived();
+    if (!rc && naik()) {
+      // keep the set-0 links
+      const size_t nb = link_bytes(P_F64, c.stride);
+      MH_CHECK(hipMemcpyAsync(fat0.data, L.fat.data, nb, hipMemcpyDeviceToDevice, c.stream));
+      MH_CHECK(hipMemcpyAsync(lng0.data, L.lng.data, nb, hipMemcpyDeviceToDevice, c.stream));
+    }
+    cur_eps = 0.0;
     (void)hipStreamSynchronize(c.stream);
     t_links += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     n_rebuild++;
@@ -489,39 +534,63 @@ struct Traj {
     int rc;
     if ((rc = fresh())) return rc;
     auto t0 = std::chrono::steady_clock::now();
-    // the sloppy links of the mixed solve were dropped by the last link update
-    if (p.mixed_md && (rc = ensure_links(P_F32))) return rc;
-    std::vector<VecField> all;
-    std::vector<double> res;
-    if (batched(p.mixed_md)) {
-      // all pseudofermions' MD solves in one batched multi-shift
-      std::vector<std::vector<VecField>> xs;
-      long it = 0;
-      rc = rational_solve_all(sources(), offsets(&Pseudo::offs_md, &RhmcPseudo::md), resids(&RhmcPseudo::resid_md),
-                              xs, &it);
-      md_iters += it;
-      for (int i = 0; i < (int)xs.size(); i++)
-        for (int t = 0; t < (int)xs[i].size(); t++) {
-          all.push_back(xs[i][t]);
-          res.push_back(p.pf[i].md.res[t + 1]);
-        }
-    } else {
-      for (int i = 0; i < p.n_pseudo && !rc; i++) {
-        std::vector<VecField> x;
-        SolveStats st;
-        rc = rational_solve(pf[i].phi, p.pf[i].md.order, pf[i].offs_md + 1, p.pf[i].resid_md, p.mixed_md, EVEN, x,
-                            st);
-        md_iters += st.iters;
-        for (int t = 0; t < (int)x.size(); t++) {
-          all.push_back(x[t]);
-          res.push_back(p.pf[i].md.res[t + 1]);
+    if (naik()) {
+      const size_t nb = link_bytes(P_F64, c.stride);
+      MH_CHECK(hipMemsetAsync(o1.data, 0, nb, c.stream));
+      MH_CHECK(hipMemsetAsync(o3.data, 0, nb, c.stream));
+      MH_CHECK(hipMemsetAsync(o1x.data, 0, nb, c.stream));
+    }
+    // update_h_fermion: each Naik set's pseudofermions on that set's links,
+    // X_t and D_oe X_t with them (update_h_rhmc.c:59-83)
+    for (size_t s = 0; s < sets.size() && !rc; s++) {
+      const double eps = sets[s];
+      const std::vector<int> idx = members(eps);
+      auto ts = std::chrono::steady_clock::now();
+      if ((rc = use_set(eps))) break;
+      // the sloppy links of the mixed solve were dropped by the last link update
+      if (p.mixed_md && (rc = ensure_links(P_F32))) break;
+      std::vector<VecField> all;
+      std::vector<double> res;
+      if (batched(p.mixed_md, idx.size())) {
+        // the set's MD solves in one batched multi-shift
+        std::vector<std::vector<VecField>> xs;
+        long it = 0;
+        rc = rational_solve_all(sources(idx), offsets(idx, &Pseudo::offs_md, &RhmcPseudo::md),
+                                resids(idx, &RhmcPseudo::resid_md), xs, &it);
+        md_iters += it;
+        for (int j = 0; j < (int)xs.size(); j++)
+          for (int t = 0; t < (int)xs[j].size(); t++) {
+            all.push_back(xs[j][t]);
+            res.push_back(p.pf[idx[j]].md.res[t + 1]);
+          }
+      } else {
+        for (int i : idx) {
+          if (rc) break;
+          std::vector<VecField> x;
+          SolveStats st;
+          rc = rational_solve(pf[i].phi, p.pf[i].md.order, pf[i].offs_md + 1, p.pf[i].resid_md, p.mixed_md, EVEN,
+                              x, st);
+          md_iters += st.iters;
+          for (int t = 0; t < (int)x.size(); t++) {
+            all.push_back(x[t]);
+            res.push_back(p.pf[i].md.res[t + 1]);
+          }
         }
       }
+      (void)hipStreamSynchronize(c.stream);
+      t_md_solve += std::chrono::duration<double>(std::chrono::steady_clock::now() - ts).count();
+      if (!rc) rc = force_level0_from_solutions(all, res.data(), naik() ? o1s : o1, naik() ? o3s : o3);
+      for (auto& v : all) free_vec(v);
+      if (!rc && naik()) {
+        // every term takes the level-2 paths; a set with eps != 0 also eps
+        // times the "1-link + Naik" paths: its three-hop products join the
+        // Naik path, its one-hop products go onto W
+        // (fermion_force_hisq_multi.c:1284-1374)
+        rc = link_axpy_dev(o1, o1s, 1.0, 1.0, 0);
+        if (!rc) rc = link_axpy_dev(o3, o3s, 1.0 + eps * kHisqNaikSet[1] / kHisqLevel2[1], 1.0, 0);
+        if (!rc && eps != 0.0) rc = link_axpy_dev(o1x, o1s, eps * kHisqNaikSet[0], 1.0, 0);
+      }
     }
-    (void)hipStreamSynchronize(c.stream);
-    t_md_solve += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    if (!rc) rc = force_level0_from_solutions(all, res.data(), o1, o3);
-    for (auto& v : all) free_vec(v);
     if (p.action == 1) {
       // asqtad: the FN force on the thin links (fermion_force_fn_multi)
       const double u2 = p.u0 * p.u0, u4 = u2 * u2, u6 = u4 * u2;
@@ -532,7 +601,8 @@ struct Traj {
       if (!rc && !aux) rc = hisq_aux_dev(c.thin, V, W);
       aux = rc == 0;
       if (!rc)
-        rc = hisq_force_from_oprod(c.thin, V, W, o1, o3, kHisqLevel2, kHisqLevel1, 2.0 * e, kForceFilter, mtmp);
+        rc = hisq_force_from_oprod(c.thin, V, W, o1, o3, kHisqLevel2, kHisqLevel1, 2.0 * e, kForceFilter, mtmp,
+                                   naik() ? &o1x : nullptr);
     }
     if (!rc) {
       hipLaunchKernelGGL(k_axpy_flat, dim3(1024), dim3(256), 0, c.stream, mom, mtmp, nmom());
@@ -558,30 +628,44 @@ struct Traj {
     *g = p.beta / 3.0 * ga;
     if (!rc) rc = mom_action_dev(mom, h);
     if (!rc) rc = fresh();
-    if (!rc && p.mixed_action) rc = ensure_links(P_F32);
     double fs = 0;
-    if (!rc && !p.action_from_md && batched(p.mixed_action)) {
-      // sum over pseudofermions of |R_FA phi|^2 from one batched multi-shift
-      std::vector<std::vector<VecField>> xs;
-      long it = 0;
-      rc = rational_solve_all(sources(), offsets(&Pseudo::offs_fa, &RhmcPseudo::fa), resids(&RhmcPseudo::resid_fa),
-                              xs, &it);
-      fa_iters += it;
-      VecField sum{};
-      if (!rc) rc = alloc_vec(sum, P_F64, ctx().stride);
-      for (int i = 0; i < (int)xs.size() && !rc; i++) {
-        double a = 0;
-        rc = rateval(pf[i].phi, xs[i], p.pf[i].fa.res, sum);
-        if (!rc) rc = vec_norm2(sum, P_F64, &a);
-        fs += a;
+    // each Naik set's pseudofermions on that set's links
+    for (size_t s = 0; s < sets.size() && !rc; s++) {
+      const std::vector<int> idx = members(sets[s]);
+      rc = use_set(sets[s]);
+      if (!rc && p.mixed_action) rc = ensure_links(P_F32);
+      if (!rc && !p.action_from_md && batched(p.mixed_action, idx.size())) {
+        // sum over the set's pseudofermions of |R_FA phi|^2 from one batched multi-shift
+        std::vector<std::vector<VecField>> xs;
+        long it = 0;
+        rc = rational_solve_all(sources(idx), offsets(idx, &Pseudo::offs_fa, &RhmcPseudo::fa),
+                                resids(idx, &RhmcPseudo::resid_fa), xs, &it);
+        fa_iters += it;
+        VecField sum{};
+        if (!rc) rc = alloc_vec(sum, P_F64, ctx().stride);
+        for (int j = 0; j < (int)xs.size() && !rc; j++) {
+          double a = 0;
+          rc = rateval(pf[idx[j]].phi, xs[j], p.pf[idx[j]].fa.res, sum);
+          if (!rc) rc = vec_norm2(sum, P_F64, &a);
+          fs += a;
+        }
+        free_vec(sum);
+        for (auto& v : xs)
+          for (auto& y : v) free_vec(y);
+        continue;
       }
-      free_vec(sum);
-      for (auto& v : xs)
-        for (auto& y : v) free_vec(y);
-      *f = fs;
-      return rc;
+      for (int i : idx) {
+        if (rc) break;
+        rc = pseudo_action(i, fs);
+      }
     }
-    for (int i = 0; i < p.n_pseudo && !rc; i++) {
+    *f = fs;
+    return rc;
+  }
+  // one pseudofermion's action on the current links, added to fs
+  int pseudo_action(int i, double& fs) {
+    int rc = 0;
+    {
       const RhmcPseudo& q = p.pf[i];
       if (p.action_from_md) {
         // phi^dag R_MD phi: the action whose derivative the MD force is
@@ -606,7 +690,6 @@ struct Traj {
         fs += a;
       }
     }
-    *f = fs;
     return rc;
   }
 };
@@ -636,6 +719,11 @@ int rhmc_trajectory(const RhmcParams& p, double* stats) {
   auto t_start = std::chrono::steady_clock::now();
   Traj T(p);
   T.pf.resize(p.n_pseudo);
+  for (int i = 0; i < p.n_pseudo; i++)
+    if (std::find(T.sets.begin(), T.sets.end(), p.pf[i].naik_eps) == T.sets.end()) {
+      if (p.action != 0) return -2;
+      T.sets.push_back(p.pf[i].naik_eps);
+    }
   // the fat and Naik links in place must be those of this quark action
   T.dirty = (p.action == 1) != (c.links_version == g_asqtad_links_version);
   for (int i = 0; i < p.n_pseudo; i++) {
@@ -663,6 +751,13 @@ int rhmc_trajectory(const RhmcParams& p, double* stats) {
       (rc = dev_alloc((void**)&T.mom, T.nmom() * sizeof(double))) ||
       (rc = dev_alloc((void**)&T.mtmp, T.nmom() * sizeof(double))))
     return rc;
+  if (T.naik() &&
+      ((rc = alloc_links(T.fat0, P_F64, c.stride)) || (rc = alloc_links(T.lng0, P_F64, c.stride)) ||
+       (rc = alloc_links(T.o1s, P_F64, c.stride)) || (rc = alloc_links(T.o3s, P_F64, c.stride)) ||
+       (rc = alloc_links(T.o1x, P_F64, c.stride))))
+    return rc;
+  // with Naik sets, W and the set-0 links of the links in place come from a rebuild
+  if (T.naik()) T.dirty = true;
   for (auto& P : T.pf)
     if ((rc = alloc_vec(P.phi, P_F64, c.stride))) return rc;
   std::vector<VecField> chi(p.n_pseudo);
@@ -685,24 +780,32 @@ int rhmc_trajectory(const RhmcParams& p, double* stats) {
       if (!rc) rc = vec_norm2(chi[i], P_F64, &grsum[i]);
     }
   }
-  if (!rc && T.batched(p.mixed_action)) {
-    std::vector<const VecField*> src;
-    for (auto& v : chi) src.push_back(&v);
-    std::vector<std::vector<VecField>> xs;
-    long it = 0;
-    rc = rational_solve_all(src, T.offsets(&Pseudo::offs_gr, &RhmcPseudo::gr), T.resids(&RhmcPseudo::resid_gr), xs,
-                            &it);
-    it_gr = (int)it;
-    for (int i = 0; i < (int)xs.size() && !rc; i++) rc = rateval(chi[i], xs[i], p.pf[i].gr.res, T.pf[i].phi);
-    (void)hipStreamSynchronize(c.stream);
-    for (auto& v : xs)
-      for (auto& y : v) free_vec(y);
-  } else {
-    for (int i = 0; i < p.n_pseudo && !rc; i++) {
-      int it = 0;
-      rc = rational_apply(chi[i], p.pf[i].gr.order, T.pf[i].offs_gr + 1, p.pf[i].gr.res, p.pf[i].resid_gr,
-                          p.mixed_action, T.pf[i].phi, &it);
-      it_gr += it;
+  // each Naik set's pseudofermions on that set's links (grsource_imp_rhmc with fn[inaik])
+  for (size_t s = 0; s < T.sets.size() && !rc; s++) {
+    const std::vector<int> idx = T.members(T.sets[s]);
+    rc = T.use_set(T.sets[s]);
+    if (!rc && p.mixed_action) rc = ensure_links(P_F32);
+    if (!rc && Traj::batched(p.mixed_action, idx.size())) {
+      std::vector<const VecField*> src;
+      for (int i : idx) src.push_back(&chi[i]);
+      std::vector<std::vector<VecField>> xs;
+      long it = 0;
+      rc = rational_solve_all(src, T.offsets(idx, &Pseudo::offs_gr, &RhmcPseudo::gr),
+                              T.resids(idx, &RhmcPseudo::resid_gr), xs, &it);
+      it_gr += (int)it;
+      for (int j = 0; j < (int)xs.size() && !rc; j++)
+        rc = rateval(chi[idx[j]], xs[j], p.pf[idx[j]].gr.res, T.pf[idx[j]].phi);
+      (void)hipStreamSynchronize(c.stream);
+      for (auto& v : xs)
+        for (auto& y : v) free_vec(y);
+    } else {
+      for (int i : idx) {
+        if (rc) break;
+        int it = 0;
+        rc = rational_apply(chi[i], p.pf[i].gr.order, T.pf[i].offs_gr + 1, p.pf[i].gr.res, p.pf[i].resid_gr,
+                            p.mixed_action, T.pf[i].phi, &it);
+        it_gr += it;
+      }
     }
   }
   for (auto& v : chi) free_vec(v);
@@ -754,8 +857,8 @@ int rhmc_trajectory(const RhmcParams& p, double* stats) {
     T.dirty = true;
     T.aux = false;
   }
-  // leave the fat and Naik links of the final thin links in place
-  if ((rc = T.fresh())) return rc;
+  // leave the fat and Naik links (set 0) of the final thin links in place
+  if ((rc = T.fresh()) || (rc = T.use_set(0.0))) return rc;
   g_asqtad_links_version = p.action == 1 ? c.links_version : ~0u;
   (void)hipStreamSynchronize(c.stream);
   double tot = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();

@@ -75,17 +75,23 @@ class Rational(ctypes.Structure):
 
 def load_rhmc_params(path):
     """MILC's rational-function parameter file (ks_imp_rhmc/load_rhmc_params.c):
-    one dict per pseudofermion, {"MD"|"GR"|"FA": Rational}, res[0] = constant"""
-    out, cur, raw = [], None, {}
+    one dict per pseudofermion, {"MD"|"GR"|"FA": Rational, "naik_epsilon": float},
+    res[0] = constant; naik_term_epsilon precedes each pseudofermion's rationals"""
+    out, cur, raw, eps = [], None, {}, []
+    pending = 0.0
     for line in open(path):
         t = line.split()
         if not t or t[0].startswith("#"):
             continue
+        if t[0] == "naik_term_epsilon":
+            pending = float(t[1])
         for kind in ("MD", "GR", "FA"):
             if t[0] == "order_" + kind:
                 if kind == "MD":
                     raw = {}
                     out.append(raw)
+                    eps.append(pending)
+                    pending = 0.0
                 raw[kind] = {"order": int(t[1]), "res": [], "pole": []}
             elif t[0] == "res_" + kind:
                 raw[kind]["res"].append(float(t[1]))
@@ -96,6 +102,8 @@ def load_rhmc_params(path):
             if len(v["res"]) != v["order"] + 1 or len(v["pole"]) != v["order"] + 1:
                 raise ValueError(f"{path}: {kind} has order {v['order']} but {len(v['res'])} residues")
             raw[kind] = Rational.make(v["res"][0], v["res"][1:], v["pole"][1:])
+    for raw, e in zip(out, eps):
+        raw["naik_epsilon"] = e
     return out
 
 
@@ -119,7 +127,8 @@ class Pseudofermion(ctypes.Structure):
     """one pseudofermion: mass (4 m^2 added to the poles; 0 for MILC's files,
     whose poles hold it), MD / GR / FA rationals, solver targets"""
     _fields_ = [("mass", ctypes.c_double), ("md", Rational), ("gr", Rational), ("fa", Rational),
-                ("resid_md", ctypes.c_double), ("resid_gr", ctypes.c_double), ("resid_fa", ctypes.c_double)]
+                ("resid_md", ctypes.c_double), ("resid_gr", ctypes.c_double), ("resid_fa", ctypes.c_double),
+                ("naik_epsilon", ctypes.c_double)]
 
 
 class RhmcParams(ctypes.Structure):
@@ -149,7 +158,7 @@ def pseudofermions_from_file(path, resid_md, resid_gr, resid_fa, mass=0.0):
         return list(v) if hasattr(v, "__len__") else [v] * n
     rm, rg, rf, ms = per(resid_md), per(resid_gr), per(resid_fa), per(mass)
     return [Pseudofermion(mass=ms[i], md=r["MD"], gr=r["GR"], fa=r["FA"], resid_md=rm[i], resid_gr=rg[i],
-                          resid_fa=rf[i]) for i, r in enumerate(rats)]
+                          resid_fa=rf[i], naik_epsilon=r["naik_epsilon"]) for i, r in enumerate(rats)]
 
 
 def load_library(path=LIB_PATH):

@@ -76,7 +76,7 @@ IO_FILES = [
     "lat.sample.l4444", "lat.sample.l4444.scidac", "lat.sample.l4444.ildg",
     "lat.sample.l4448", "lat.sample.l4448.scidac", "lat.sample.l4448.foreign",
     "lat.sample.5p7", "lat.sample.l6666", "lat.sample.l6666.hisq", "ksvector.src.sample.l8",
-    "lat.sample.l6666b650m010m050",
+    "lat.sample.l6666b650m010m050", "lat.sample.hisq.f211b670.l6666",
 ]
 
 
@@ -88,7 +88,7 @@ def main(ref="/root/reference"):
         print(dst, arr.shape, arr.dtype)
     # MILC's rational functions for its HISQ and asqtad RHMC tests (MD / GR / FA per pseudofermion)
     for name in ("rationals.sample.su3_rhmc_hisq", "rationals.sample.su3_omelyan_rhmc",
-                 "rationals.sample.su3_leapfrog"):
+                 "rationals.sample.su3_leapfrog", "rationals.sample.su3_rhmc_hisq_f211"):
         rat = open(os.path.join(ref, "ks_imp_rhmc", "test", name)).read()
         open(os.path.join(HERE, name), "w").write(rat)
     os.makedirs(os.path.join(HERE, "io"), exist_ok=True)

@@ -202,6 +202,54 @@ def test_milc_su3_leapfrog_asqtad_rhmd_trajectory(mixed):
     assert abs(st_ - LF_PLAQ[1]) < 5e-7 + LF_TOL_PLAQ[1]
 
 
+# ---- MILC's su3_rhmc_hisq.f211.1 run (2+1+1, the charm pseudofermion on a second Naik set) ----
+F211_ISEED = 4563421
+F211_GRSOURCE = [1.9606615161e+03, 1.9419062139e+03, 1.9778946018e+03, 2.0146820996e+03,
+                 1.9699683368e+03]                                                               # 635-643
+F211_ACTION0 = (2.08587740172183e+04, -2.05282869219780e+02, 9.86511273326725e+03)            # 651
+F211_ACTION1 = (1.98409032770018e+04, 8.18777145951986e+02, 9.85889961154759e+03)             # 770
+F211_DELTA_S = -2.384676e-02                                                                    # 772
+F211_PLAQ = (1.746504, 1.737158)                                                                # 773
+F211_RESID = [(5e-4, 1e-4, 1e-4)] + [(5e-6, 1e-6, 1e-6)] * 4
+
+
+@pytest.mark.parametrize("mixed", [0, 1])
+def test_milc_su3_rhmc_hisq_f211_trajectory(mixed):
+    """MILC's 2+1+1 HISQ regression run ks_imp_rhmc/test/su3_rhmc_hisq.f211.1
+    (lat.sample.hisq.f211b670.l6666, iseed 4563421, five pseudofermions of
+    rationals.sample.su3_rhmc_hisq_f211, the last with naik_term_epsilon
+    -0.16875, INT_3G1F, 12 steps of 0.04, beta 6.7, u0 0.89 with four
+    flavours): GRSOURCE to every digit, start actions, delta S and the final
+    plaquette within su3_rhmc_hisq.f211.1.errtol (delta S 0.01, PLAQ 4e-6)"""
+    links, info = lio.reload_lattice(os.path.join(HERE, "golden", "io", "lat.sample.hisq.f211b670.l6666"))
+    assert info["checksum_ok"]
+    dims = (6, 6, 6, 6)
+    pfs = mq.pseudofermions_from_file(os.path.join(HERE, "golden", "rationals.sample.su3_rhmc_hisq_f211"),
+                                      [r[0] for r in F211_RESID], [r[2] for r in F211_RESID],
+                                      [r[1] for r in F211_RESID])
+    assert len(pfs) == 5 and pfs[4].naik_epsilon == -0.16875 and pfs[0].naik_epsilon == 0.0
+    coeff = mq.gauge_loop_coeffs(0.89, 4)
+    with mq.Lattice(dims) as G:
+        G.dev_load_thin_links(links, phases_in=False)
+        G.dev_seed_rng(F211_ISEED, precision=1)
+        st = G.dev_rhmc_trajectory(12, 0.04, 6.7, coeff, pfs, integrator=mq.INT_3G1F, mixed_md=mixed,
+                                   mixed_action=mixed, accept=1, site_rng=True)
+        U1 = G.dev_read_thin_links(with_phases=False)
+    print({k: v for k, v in st.items()})
+    np.testing.assert_allclose(st["grsource"], F211_GRSOURCE, rtol=3e-11)
+    assert st["g0"] == pytest.approx(F211_ACTION0[0], rel=2e-7)
+    assert st["h0"] == pytest.approx(F211_ACTION0[1], abs=1e-4)
+    assert st["f0"] == pytest.approx(F211_ACTION0[2], abs=0.05)
+    assert st["fermion_forces"] == 12 and st["reunit_errors"] == 0
+    assert st["accepted"] == 1
+    assert abs(st["dH"] - F211_DELTA_S) < 0.01                      # su3_rhmc_hisq.f211.1.errtol
+    ss, st_ = oio.plaquette(U1[oio.node_index_of_lex(list(dims))], list(dims))
+    print("PLAQ", ss, st_)
+    assert abs(ss - F211_PLAQ[0]) < 4e-6 + 5e-7 and abs(st_ - F211_PLAQ[1]) < 4e-6 + 5e-7
+    for k, ref in zip(("g1", "h1", "f1"), F211_ACTION1):
+        assert abs(st[k] - ref) < 0.05, (k, st[k], ref)
+
+
 def test_device_generators_match_oracle():
     """milc_hip_dev_ranmom / dev_gaussian_source draw MILC's numbers: the
     oracle's restatement (pinned to MILC's printed sums) gives the same values"""
