@@ -88,7 +88,8 @@ def main(ref="/root/reference"):
         print(dst, arr.shape, arr.dtype)
     # MILC's rational functions for its HISQ and asqtad RHMC tests (MD / GR / FA per pseudofermion)
     for name in ("rationals.sample.su3_rhmc_hisq", "rationals.sample.su3_omelyan_rhmc",
-                 "rationals.sample.su3_leapfrog", "rationals.sample.su3_rhmc_hisq_f211"):
+                 "rationals.sample.su3_leapfrog", "rationals.sample.su3_rhmc_hisq_f211",
+                 "rationals.sample.su3_rhmc_hisq_f2111"):
         rat = open(os.path.join(ref, "ks_imp_rhmc", "test", name)).read()
         open(os.path.join(HERE, name), "w").write(rat)
     os.makedirs(os.path.join(HERE, "io"), exist_ok=True)

@@ -250,6 +250,47 @@ def test_milc_su3_rhmc_hisq_f211_trajectory(mixed):
         assert abs(st[k] - ref) < 0.05, (k, st[k], ref)
 
 
+# ---- MILC's su3_rhmc_hisq.f2111.1 run (six pseudofermions on three Naik sets) ----
+F2111_GRSOURCE = F211_GRSOURCE + [1.9529017883e+03]                                           # 741-751
+F2111_ACTION0 = (2.16927145638990e+04, -2.05282869219780e+02, 1.18180142822787e+04)          # 760
+F2111_ACTION1 = (2.02631810822886e+04, 1.23237197795510e+03, 1.18098377667516e+04)           # 892
+F2111_DELTA_S = -5.514996e-02                                                                  # 894
+F2111_PLAQ = (1.769942, 1.759599)                                                              # 895
+
+
+@pytest.mark.parametrize("mixed", [0, 1])
+def test_milc_su3_rhmc_hisq_f2111_trajectory(mixed):
+    """MILC's ks_imp_rhmc/test/su3_rhmc_hisq.f2111.1 (the f211 lattice and
+    seed, six pseudofermions of rationals.sample.su3_rhmc_hisq_f2111, the last
+    two with naik_term_epsilon -0.1520 and -0.2712: three Naik sets; five
+    flavours in the gauge action): within su3_rhmc_hisq.f2111.1.errtol
+    (delta S 0.01, PLAQ 2e-5)"""
+    links, info = lio.reload_lattice(os.path.join(HERE, "golden", "io", "lat.sample.hisq.f211b670.l6666"))
+    dims = (6, 6, 6, 6)
+    res = [(5e-4, 1e-4, 1e-4)] + [(5e-6, 1e-6, 1e-6)] * 5
+    pfs = mq.pseudofermions_from_file(os.path.join(HERE, "golden", "rationals.sample.su3_rhmc_hisq_f2111"),
+                                      [r[0] for r in res], [r[2] for r in res], [r[1] for r in res])
+    assert len(pfs) == 6 and len({p.naik_epsilon for p in pfs}) == 3
+    coeff = mq.gauge_loop_coeffs(0.89, 5)
+    with mq.Lattice(dims) as G:
+        G.dev_load_thin_links(links, phases_in=False)
+        G.dev_seed_rng(F211_ISEED, precision=1)
+        st = G.dev_rhmc_trajectory(12, 0.04, 6.7, coeff, pfs, integrator=mq.INT_3G1F, mixed_md=mixed,
+                                   mixed_action=mixed, accept=1, site_rng=True)
+        U1 = G.dev_read_thin_links(with_phases=False)
+    print({k: v for k, v in st.items()})
+    np.testing.assert_allclose(st["grsource"], F2111_GRSOURCE, rtol=3e-11)
+    assert st["g0"] == pytest.approx(F2111_ACTION0[0], rel=2e-7)
+    assert st["f0"] == pytest.approx(F2111_ACTION0[2], abs=0.05)
+    assert st["accepted"] == 1 and st["reunit_errors"] == 0
+    assert abs(st["dH"] - F2111_DELTA_S) < 0.01                     # su3_rhmc_hisq.f2111.1.errtol
+    ss, st_ = oio.plaquette(U1[oio.node_index_of_lex(list(dims))], list(dims))
+    print("PLAQ", ss, st_)
+    assert abs(ss - F2111_PLAQ[0]) < 2e-5 and abs(st_ - F2111_PLAQ[1]) < 2e-5
+    for k, ref in zip(("g1", "h1", "f1"), F2111_ACTION1):
+        assert abs(st[k] - ref) < 0.05, (k, st[k], ref)
+
+
 def test_device_generators_match_oracle():
     """milc_hip_dev_ranmom / dev_gaussian_source draw MILC's numbers: the
     oracle's restatement (pinned to MILC's printed sums) gives the same values"""
