@@ -256,23 +256,45 @@ __global__ void __launch_bounds__(256) k_bms_flush(float* __restrict__ ring, siz
 #pragma unroll
     for (int l = 0; l < BKW; l++)
 #pragma unroll
-      for (int k = 0; k < 6; k++) rv[l][k] = l < md ? ring[(size_t)(l + 1) * ring_stride + o + k] : 0.f;
+      for (int h = 0; h < 3; h++) {
+        // ring_stride and o are even: 8-byte aligned pairs
+        const float2 t = l < md ? ((const float2*)(ring + (size_t)(l + 1) * ring_stride + o))[h] : make_float2(0.f, 0.f);
+        rv[l][2 * h] = t.x;
+        rv[l][2 * h + 1] = t.y;
+      }
+    // 8-byte accesses: records start at even float offsets (6 floats each)
+    // and coefficient rows hold an even number (BMS_NC) of floats
     for (int j = 0; j < nn; j++) {
-      const float* c = cr + (size_t)j * BMS_NC;
-      float* pmj = pm.v[j] + o;
-      float* xj = psim.v[j] + o;
+      const float2* c2 = (const float2*)(cr + (size_t)j * BMS_NC);
+      float c[BMS_NC];
 #pragma unroll
-      for (int k = 0; k < 6; k++) {
-        float pv = pmj[k];
-        float x = xj[k] + c[0] * pv;
-        float q = c[1 + BKW] * pv;
+      for (int l = 0; l < BMS_NC / 2; l++) {
+        const float2 t = c2[l];
+        c[2 * l] = t.x;
+        c[2 * l + 1] = t.y;
+      }
+      float2* pmj = (float2*)(pm.v[j] + o);
+      float2* xj = (float2*)(psim.v[j] + o);
 #pragma unroll
-        for (int l = 0; l < BKW; l++) {
-          x += c[1 + l] * rv[l][k];
-          q += c[2 + BKW + l] * rv[l][k];
+      for (int h = 0; h < 3; h++) {
+        const float2 p2 = pmj[h], x2 = xj[h];
+        const float pv[2] = {p2.x, p2.y};
+        float xv[2] = {x2.x, x2.y}, qv[2];
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+          const int k = 2 * h + u;
+          float x = xv[u] + c[0] * pv[u];
+          float q = c[1 + BKW] * pv[u];
+#pragma unroll
+          for (int l = 0; l < BKW; l++) {
+            x += c[1 + l] * rv[l][k];
+            q += c[2 + BKW + l] * rv[l][k];
+          }
+          xv[u] = x;
+          qv[u] = q;
         }
-        xj[k] = x;
-        pmj[k] = q;
+        xj[h] = make_float2(xv[0], xv[1]);
+        pmj[h] = make_float2(qv[0], qv[1]);
       }
     }
 #pragma unroll
