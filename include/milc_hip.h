@@ -73,7 +73,13 @@ const char* milc_hip_version(void);
  * the links from the previous call is reused when the pointers match.
  * internal_precision 2 with mixed_precision 0 reproduces MILC's FP64
  * restarted CG (d_congrad5_fn_milc.c:60-407).  final_resid = |r|/|b| of the
- * FP64 true residual.  target_relresid must be 0. */
+ * FP64 true residual.  target_relresid > 0 adds MILC's Fermilab relative
+ * residual test (qic->relresid, d_congrad5_fn_milc.c:37-56,92): the solve also
+ * runs until my_relative_residue(r, x) = sqrt(sum_i |r_i|^2/|x_i|^2 / (V/2)) <
+ * target_relresid^2, tested where the CPU path tests it (after every update and
+ * at every restart); 0 = no such test.  final_rel_resid =
+ * sqrt(my_relative_residue) of the solution, so that the glue's square
+ * (d_congrad5_fn_gpu.c:151) gives the CPU path's final_relrsq. */
 int milc_hip_invert(int external_precision, int internal_precision, double mass, milc_hip_inv_args args,
                     double target_resid, double target_relresid, const void* fatlink, const void* longlink,
                     const void* source, void* solution, double* final_resid, double* final_rel_resid,
@@ -84,7 +90,9 @@ int milc_hip_invert(int external_precision, int internal_precision, double mass,
  * (-D_eo D_oe + offset_j) x_j = b, all offsets > 0.  Solutions are zeroed
  * first.  The base system (smallest offset) stops at target_resid[0];
  * internal_precision 1 runs the vector updates in FP32 with FP64
- * recurrences, 2 runs everything in FP64. */
+ * recurrences, 2 runs everything in FP64.  A nonzero target_relresid[j]
+ * returns -4: the multi-shift solve has no relative-residual test (the
+ * reference glue stops on it, ks_multicg_offset_gpu.c:55-58). */
 int milc_hip_multishift_invert(int external_precision, int internal_precision, int num_offsets,
                                const double* offsets, milc_hip_inv_args args, const double* target_resid,
                                const double* target_relresid, const void* fatlink, const void* longlink,
@@ -258,6 +266,10 @@ typedef struct {
  * terminates there); after any error the device links are the ones the call
  * started with. */
 int milc_hip_dev_rhmc_trajectory(const milc_hip_rhmc_params* p, double* stats);
+/* Test hook for the error contract above, no counterpart in the reference:
+ * the next trajectory fails with -90 at `stage` (1 after the heatbath, 2 after
+ * the starting actions, 3 after the molecular dynamics; 0 = none), once. */
+int milc_hip_debug_fail_trajectory(int stage);
 
 /* MILC's per-site random number generators on the device: seeded as
  * initialize_site_prn_from_seed(iseed) (generic/ranstuff.c:117-131, index =

@@ -190,6 +190,7 @@ int milc_hip_finalize(void) {
   dev_free(c.ws.partials);
   dev_free(c.ws.scal);
   if (c.ws.scal_host) (void)hipHostFree(c.ws.scal_host);
+  if (c.ws.pinned) (void)hipHostFree(c.ws.pinned);
   dev_free(c.staging);
   c.staging = nullptr;
   c.staging_elems = 0;
@@ -209,14 +210,14 @@ int milc_hip_invert(int external_precision, int internal_precision, double mass,
   Context& c = ctx();
   if (!c.initialised) return -1;
   int parity = parity_from_evenodd(args.evenodd);
-  if (parity < 0) return -2;
-  if (target_relresid != 0.0) return -4;
+  if (parity < 0 || target_relresid < 0.0) return -2;
   int rc = refresh_links(external_precision, fatlink, longlink, num_iters);
   if (rc) return rc;
   int ps = sloppy_of(internal_precision, args.mixed_precision);
   if (ps != P_F64 && (rc = ensure_links(ps))) return rc;
   int pb = pbit(parity);
   VecField b, x;
+  VecScope owned{&b, &x};
   if ((rc = alloc_vec(b, P_F64, c.stride)) || (rc = alloc_vec(x, P_F64, c.stride))) return rc;
   if ((rc = host_vec_in(external_precision, source, b, pb)) || (rc = host_vec_in(external_precision, solution, x, pb)))
     return rc;
@@ -226,18 +227,18 @@ int milc_hip_invert(int external_precision, int internal_precision, double mass,
   sp.max_iters = per > 0 ? per : 1;
   sp.max_restarts = std::max(1, (args.max_iter + sp.max_iters - 1) / sp.max_iters);
   sp.resid = target_resid;
-  sp.relresid = 0;
+  sp.relresid = target_relresid;
   sp.sloppy_prec = ps;
   sp.delta = kReliableDelta;
   sp.batch = 16;
   SolveStats st;
   rc = cg_solve(b, x, mass, sp, st);
   if (!rc) rc = host_vec_out(external_precision, x, pb, solution);
-  free_vec(b);
-  free_vec(x);
   if (rc) return rc;
   if (final_resid) *final_resid = std::sqrt(st.final_rsq);
-  if (final_rel_resid) *final_rel_resid = 0.0;
+  // the glue squares it into qic->final_relrsq (d_congrad5_fn_gpu.c:151), which
+  // then holds my_relative_residue as on the CPU path (d_congrad5_fn_milc.c:205)
+  if (final_rel_resid) *final_rel_resid = std::sqrt(st.final_relrsq);
   if (num_iters) *num_iters = st.iters;
   return st.iters;
 }
@@ -254,6 +255,7 @@ int milc_hip_dslash(int external_precision, int internal_precision, milc_hip_inv
   if ((rc = ensure_links(prec))) return rc;
   int pb = pbit(parity);
   VecField s, d, s64, d64;
+  VecScope owned{&s, &d, &s64, &d64};
   if ((rc = alloc_vec(s64, P_F64, c.stride)) || (rc = alloc_vec(d64, P_F64, c.stride))) return rc;
   if ((rc = host_vec_in(external_precision, src, s64, pb ^ 1))) return rc;
   if (prec == P_F64) {
@@ -262,12 +264,8 @@ int milc_hip_dslash(int external_precision, int internal_precision, milc_hip_inv
     if ((rc = alloc_vec(s, prec, c.stride)) || (rc = alloc_vec(d, prec, c.stride))) return rc;
     if (!(rc = convert_vec(s64, P_F64, s, prec)) && !(rc = dslash(c.links[prec], s, d, pb, prec)))
       rc = convert_vec(d, prec, d64, P_F64);
-    free_vec(s);
-    free_vec(d);
   }
   if (!rc) rc = host_vec_out(external_precision, d64, pb, dst);
-  free_vec(s64);
-  free_vec(d64);
   return rc;
 }
 
@@ -351,14 +349,18 @@ int milc_hip_dev_dslash_bench(int prec, int parity, int reps, double* usec) {
   if (prec < P_F64 || prec > P_H16) return -2;
   if ((rc = ensure_links(prec))) return rc;
   int pb = pbit(parity);
-  VecField s, d;
+  VecField s, d, s64;
+  VecScope owned{&s, &d, &s64};
   if ((rc = alloc_vec(s, prec, c.stride)) || (rc = alloc_vec(d, prec, c.stride))) return rc;
-  VecField s64;
   if ((rc = alloc_vec(s64, P_F64, c.stride))) return rc;
   if ((rc = gaussian_vec_dev(12345, pb ^ 1, s64)) || (rc = convert_vec(s64, P_F64, s, prec))) return rc;
   for (int i = 0; i < 3; i++)
     if ((rc = dslash(c.links[prec], s, d, pb, prec))) return rc;
-  hipEvent_t e0, e1;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  OnExit events([&] {
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+  });
   MH_CHECK(hipEventCreate(&e0));
   MH_CHECK(hipEventCreate(&e1));
   MH_CHECK(hipEventRecord(e0, c.stream));
@@ -369,11 +371,6 @@ int milc_hip_dev_dslash_bench(int prec, int parity, int reps, double* usec) {
   float ms = 0;
   MH_CHECK(hipEventElapsedTime(&ms, e0, e1));
   *usec = 1000.0 * ms / reps;
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  free_vec(s);
-  free_vec(d);
-  free_vec(s64);
   return 0;
 }
 
@@ -394,12 +391,18 @@ extern "C" int milc_hip_dev_gauge_hisq(uint64_t seed, double scale, int* n_svd) 
   }
   c.link_key = c.link_key2 = nullptr;
   LinkField U;
-  int rc;
-  if ((rc = alloc_links(U, P_F64, c.stride))) return rc;
   DevLinks& L = c.links[P_F64];
-  if ((rc = alloc_links(L.fat, P_F64, c.stride)) || (rc = alloc_links(L.lng, P_F64, c.stride))) return rc;
-  if ((rc = gauge_random_dev(seed, scale, U)) || (rc = rephase_dev(U))) return rc;
-  if ((rc = hisq_links_dev(U, L.fat, L.lng, nullptr, n_svd))) return rc;
+  int rc;
+  if (!(rc = alloc_links(U, P_F64, c.stride)) && !(rc = alloc_links(L.fat, P_F64, c.stride)) &&
+      !(rc = alloc_links(L.lng, P_F64, c.stride)) && !(rc = gauge_random_dev(seed, scale, U)) &&
+      !(rc = rephase_dev(U)))
+    rc = hisq_links_dev(U, L.fat, L.lng, nullptr, n_svd);
+  if (rc) {
+    free_links(U);
+    free_links(L.fat);
+    free_links(L.lng);
+    return rc;
+  }
   if (c.thin_valid) free_links(c.thin);
   c.thin = U;  // kept for the thin-link force
   c.thin_valid = true;
@@ -509,11 +512,19 @@ extern "C" int milc_hip_load_hisq_links_naik(int precision, const void* thin, do
     h = conv.data();
   }
   LinkField U, fat, lng, W;
+  double* stage = nullptr;
+  OnExit cleanup([&] {
+    (void)hipStreamSynchronize(c.stream);
+    dev_free(stage);
+    free_links(U);
+    free_links(fat);
+    free_links(lng);
+    free_links(W);
+  });
   int rc;
   if ((rc = alloc_links(U, P_F64, c.stride)) || (rc = alloc_links(fat, P_F64, c.stride)) ||
       (rc = alloc_links(lng, P_F64, c.stride)) || (rc = alloc_links(W, P_F64, c.stride)))
     return rc;
-  double* stage = nullptr;
   if ((rc = dev_alloc((void**)&stage, n * sizeof(double)))) return rc;
   MH_CHECK(hipMemcpyAsync(stage, h, n * sizeof(double), hipMemcpyHostToDevice, c.stream));
   if ((rc = links_from_milc_dev(stage, U, P_F64))) return rc;
@@ -538,11 +549,6 @@ extern "C" int milc_hip_load_hisq_links_naik(int precision, const void* thin, do
     else
       for (size_t i = 0; i < n; i++) ((float*)dsts[k])[i] = (float)out[i];
   }
-  dev_free(stage);
-  free_links(U);
-  free_links(fat);
-  free_links(lng);
-  free_links(W);
   return 0;
 }
 
@@ -565,7 +571,11 @@ extern "C" int milc_hip_multishift_invert(int external_precision, int internal_p
   if (parity < 0 || num_offsets <= 0 || num_offsets > 32) return -2;
   for (int j = 0; j < num_offsets; j++)
     if (!(offsets[j] > 0)) return -2;  // ks_multicg_offset.c:139-145
-  (void)target_relresid;
+  // no Fermilab relative residual in the multi-shift solve: the reference
+  // glue stops on it too (ks_multicg_offset_gpu.c:55-58)
+  if (target_relresid)
+    for (int j = 0; j < num_offsets; j++)
+      if (target_relresid[j] != 0.0) return -4;
   int rc = refresh_links(external_precision, fatlink, longlink, num_iters);
   if (rc) return rc;
   int pb = pbit(parity);
@@ -574,6 +584,8 @@ extern "C" int milc_hip_multishift_invert(int external_precision, int internal_p
   if ((rc = ensure_links(bulk))) return rc;
   VecField b;
   std::vector<VecField> xs(num_offsets);
+  VecScope owned{&b};
+  for (auto& v : xs) owned.add(&v);
   if ((rc = alloc_vec(b, P_F64, c.stride))) return rc;
   for (auto& v : xs)
     if ((rc = alloc_vec(v, P_F64, c.stride))) return rc;
@@ -619,8 +631,6 @@ extern "C" int milc_hip_multishift_invert(int external_precision, int internal_p
     if (final_resid) final_resid[j] = fres[j];
     if (final_rel_resid) final_rel_resid[j] = 0.0;
   }
-  free_vec(b);
-  for (auto& v : xs) free_vec(v);
   if (num_iters) *num_iters = total_iters;
   return total_iters;
 }
@@ -633,6 +643,18 @@ extern "C" int milc_hip_compute_oprod(int precision, int num_terms, double** coe
   int rc;
   std::vector<VecField> xe(num_terms), xo(num_terms);
   std::vector<double> c1(num_terms), c3(num_terms);
+  LinkField o1, o3;
+  double* stage = nullptr;
+  OnExit cleanup([&] {
+    (void)hipStreamSynchronize(c.stream);
+    dev_free(stage);
+    free_links(o1);
+    free_links(o3);
+    for (int t = 0; t < num_terms; t++) {
+      free_vec(xe[t]);
+      free_vec(xo[t]);
+    }
+  });
   for (int t = 0; t < num_terms; t++) {
     c1[t] = coeff[t][0];
     c3[t] = coeff[t][1];
@@ -640,12 +662,10 @@ extern "C" int milc_hip_compute_oprod(int precision, int num_terms, double** coe
     if ((rc = host_vec_in(precision, quark_field[t], xe[t], 0)) || (rc = host_vec_in(precision, quark_field[t], xo[t], 1)))
       return rc;
   }
-  LinkField o1, o3;
   if ((rc = alloc_links(o1, P_F64, c.stride)) || (rc = alloc_links(o3, P_F64, c.stride))) return rc;
   if ((rc = force_oprod_coeff(xe, xo, c1, c3, o1, o3))) return rc;
   // device layout -> MILC AoS (4*i+dir) -> the caller's per-direction fields
   size_t n = (size_t)2 * c.Vh * 4 * 18;
-  double* stage = nullptr;
   if ((rc = dev_alloc((void**)&stage, n * sizeof(double)))) return rc;
   std::vector<double> h(n);
   LinkField* src[2] = {&o1, &o3};
@@ -665,13 +685,6 @@ extern "C" int milc_hip_compute_oprod(int precision, int num_terms, double** coe
           for (int e = 0; e < 18; e++) d[18 * i + e] = (float)h[18 * (4 * i + mu) + e];
       }
     }
-  }
-  dev_free(stage);
-  free_links(o1);
-  free_links(o3);
-  for (int t = 0; t < num_terms; t++) {
-    free_vec(xe[t]);
-    free_vec(xo[t]);
   }
   return 0;
 }
@@ -806,6 +819,13 @@ extern "C" int milc_hip_dev_force_level0(int slot_phi, int n, const double* offs
   if (!c.links[P_F64].valid) return -3;
   if (mixed_precision && (rc = ensure_links(P_F32))) return rc;
   LinkField o1, o3;
+  double* stage = nullptr;
+  OnExit cleanup([&] {
+    (void)hipStreamSynchronize(c.stream);
+    dev_free(stage);
+    free_links(o1);
+    free_links(o3);
+  });
   if ((rc = alloc_links(o1, P_F64, c.stride)) || (rc = alloc_links(o3, P_F64, c.stride))) return rc;
   int iters = 0;
   if ((rc = rhmc_force_level0(g_slots[slot_phi], n, offsets, residues, resid, mixed_precision ? 1 : 0, S, o1, o3,
@@ -813,7 +833,6 @@ extern "C" int milc_hip_dev_force_level0(int slot_phi, int n, const double* offs
     return rc;
   if (oprod1 || oprod3) {
     size_t nn = (size_t)2 * c.Vh * 4 * 18;
-    double* stage = nullptr;
     if ((rc = dev_alloc((void**)&stage, nn * sizeof(double)))) return rc;
     if (oprod1) {
       if ((rc = links_to_milc_dev(o1, P_F64, stage))) return rc;
@@ -825,10 +844,7 @@ extern "C" int milc_hip_dev_force_level0(int slot_phi, int n, const double* offs
       MH_CHECK(hipMemcpyAsync(oprod3, stage, nn * sizeof(double), hipMemcpyDeviceToHost, c.stream));
       MH_CHECK(hipStreamSynchronize(c.stream));
     }
-    dev_free(stage);
   }
-  free_links(o1);
-  free_links(o3);
   return iters;
 }
 
@@ -1844,6 +1860,12 @@ extern "C" int milc_hip_mom_action(int precision, const void* momentum, double* 
   if (!(rc = mom_in(precision, momentum, mom))) rc = mom_action_dev(mom, action);
   dev_free(mom);
   return rc;
+}
+
+extern "C" int milc_hip_debug_fail_trajectory(int stage) {
+  if (stage < 0 || stage > 3) return -2;
+  rhmc_fail_at(stage);
+  return 0;
 }
 
 extern "C" int milc_hip_dev_rhmc_trajectory(const milc_hip_rhmc_params* p, double* stats) {

@@ -227,9 +227,9 @@ inline int link_site_bytes(int prec) { return prec == P_F64 ? 144 : (prec == P_F
 
 // A colour-vector field of one parity (stride = sites per plane, >= Vh).
 struct VecField {
-  void* data;
-  float* norm;   // unused (the 16-bit norm lives in the site record)
-  int stride;
+  void* data = nullptr;
+  float* norm = nullptr;   // unused (the 16-bit norm lives in the site record)
+  int stride = 0;
 };
 
 // A link field (fat or long) for both parities and all four directions.
@@ -237,9 +237,9 @@ struct VecField {
 // L = c U): rows 0 and 1 are stored, row 2 = z conj(row0 x row1) with the
 // per-link z = det(U) / c, i.e. 28 bytes per link instead of 36.
 struct LinkField {
-  void* data;
-  float scale;         // H16: value = q * scale / 32767
-  int stride;
+  void* data = nullptr;
+  float scale = 1.0f;  // H16: value = q * scale / 32767
+  int stride = 0;
   float zscale = 1.0f; // H16 recon: z = q * zscale / 32767
   int recon = 0;
 };

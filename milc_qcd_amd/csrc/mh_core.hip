@@ -113,6 +113,25 @@ void dev_free(void* p) {
 
 void dev_pool_release() { pool_release_spare(0); }
 
+void* pinned_scratch(size_t bytes) {
+  Workspace& w = ctx().ws;
+  if (w.pinned_bytes >= bytes) return w.pinned;
+  if (w.pinned) {
+    if (ctx().stream) (void)hipStreamSynchronize(ctx().stream);
+    (void)hipHostFree(w.pinned);
+  }
+  w.pinned = nullptr;
+  w.pinned_bytes = 0;
+  size_t n = std::max<size_t>(bytes, 64 << 10);
+  if (hipHostMalloc(&w.pinned, n, hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    w.pinned = nullptr;
+    return nullptr;
+  }
+  w.pinned_bytes = n;
+  return w.pinned;
+}
+
 int alloc_vec(VecField& v, int prec, int stride) {
   v.stride = stride;
   v.norm = nullptr;
@@ -615,12 +634,13 @@ __global__ void __launch_bounds__(256) k_dslash_resid(Geom g, LinkField fat, Lin
   typedef typename PrecTraits<P>::Real R;
   int bid = xcd_block(blockIdx.x, gridDim.x);
   int cb = bid * blockDim.x + threadIdx.x;
-  double s[1] = {0};
+  double s[2] = {0, 0};
   if (cb < g.Vh) {
     Vec3<R> acc, xv, bv, r;
     dslash_site<P>(g, fat, lng, tmp, cb, pb, acc);
     load_vec<P>(x, cb, xv);
     load_vec<P>(src, cb, bv);
+    double xx = 0;
 #pragma unroll
     for (int k = 0; k < 3; k++) {
       R tre = acc.c[k].re + (R)(-msq_x4) * xv.c[k].re;
@@ -628,19 +648,25 @@ __global__ void __launch_bounds__(256) k_dslash_resid(Geom g, LinkField fat, Lin
       r.c[k].re = bv.c[k].re + tre;
       r.c[k].im = bv.c[k].im + tim;
       s[0] += (double)r.c[k].re * (double)r.c[k].re + (double)r.c[k].im * (double)r.c[k].im;
+      xx += (double)xv.c[k].re * (double)xv.c[k].re + (double)xv.c[k].im * (double)xv.c[k].im;
     }
+    // per-site term of the Fermilab relative residual (d_congrad5_fn_milc.c:45-47)
+    s[1] = xx == 0.0 ? 1.0 : s[0] / xx;
     store_vec<P>(resid, cb, r);
     if (write_p) store_vec<P>(p, cb, r);
   }
-  block_sum_store<1>(s, partials_rr);
+  block_sum_store<2>(s, partials_rr);
 }
 
-// after a restart: rsq = actual_rsq = |resid|^2, reset the device loop state
-__global__ void k_restart_scalars(const double* __restrict__ partials_rr, int nblocks, CGScalars* sc) {
-  double t[1];
-  reduce_records<1>(partials_rr, nblocks, t);
+// after a restart: rsq = actual_rsq = |resid|^2 and the relative residual
+// sqrt(sum_i |r_i|^2/|x_i|^2 / Vh) (d_congrad5_fn_milc.c:52-55, one parity);
+// reset the device loop state
+__global__ void k_restart_scalars(const double* __restrict__ partials_rr, int nblocks, int Vh, CGScalars* sc) {
+  double t[2];
+  reduce_records<2>(partials_rr, nblocks, t);
   if (threadIdx.x == 0) {
     double rr = t[0];
+    sc->relrsq = sqrt(t[1] / (double)Vh);
     sc->rsq = rr;
     sc->actual_rsq = rr;
     sc->rsq_first = rr;
@@ -680,20 +706,38 @@ __global__ void k_cg_scalars(const double* __restrict__ partials, const double* 
     sc->go = 1;
     sc->iters += 1;
     if (rsq > sc->rsq_max) sc->rsq_max = rsq;
-    if (rsq < sc->rsq_stop) sc->halt = 1;
+    if (!sc->rel_mode && rsq < sc->rsq_stop) sc->halt = 1;
+  }
+}
+
+// With a relative-residual target the halt test waits for the update: the
+// recurrence residual and my_relative_residue(resid, dest) of the updated
+// fields must both be below their targets (d_congrad5_fn_milc.c:177-180,353-354).
+__global__ void k_cg_relcheck(const double* __restrict__ partials_rr, int nblocks, int Vh, CGScalars* sc) {
+  if (!sc->go) return;
+  double t[2];
+  reduce_records<2>(partials_rr, nblocks, t);
+  if (threadIdx.x == 0) {
+    double rel = sqrt(t[1] / (double)Vh);
+    sc->relrsq = rel;
+    bool r_ok = sc->rsq_stop <= 0.0 || sc->rsq < sc->rsq_stop;
+    bool rel_ok = sc->relrsq_stop <= 0.0 || rel < sc->relrsq_stop;
+    if (r_ok && rel_ok) sc->halt = 1;
   }
 }
 
 // dest += a p; resid += a ttt; |resid|^2 partials; p = resid + b p
 // (d_congrad5_fn_milc.c:318-367)
-template <int PP, int PA>
+// With REL the per-site terms |r_i|^2/|x_i|^2 of the relative residual of the
+// updated fields go to the second record slot (d_congrad5_fn_milc.c:351-352).
+template <int PP, int PA, int REL>
 __global__ void __launch_bounds__(256) k_cg_update(VecField x, VecField p, VecField r, VecField ttt, int Vh,
                                                    const CGScalars* __restrict__ sc, double* __restrict__ partials_rr) {
   if (!sc->go) return;
   typedef typename PrecTraits<PA>::Real RA;
   typedef typename PrecTraits<PP>::Real RP;
   int cb = blockIdx.x * blockDim.x + threadIdx.x;
-  double s[1] = {0};
+  double s[1 + REL] = {0};
   if (cb < Vh) {
     RA a = (RA)sc->a, b = (RA)sc->b;
     Vec3<RA> xv, rv, tv;
@@ -713,11 +757,18 @@ __global__ void __launch_bounds__(256) k_cg_update(VecField x, VecField p, VecFi
       pn.c[k].re = (RP)(rv.c[k].re + b * (RA)pv.c[k].re);
       pn.c[k].im = (RP)(rv.c[k].im + b * (RA)pv.c[k].im);
     }
+    if (REL) {
+      double xx = 0;
+#pragma unroll
+      for (int k = 0; k < 3; k++)
+        xx += (double)xv.c[k].re * (double)xv.c[k].re + (double)xv.c[k].im * (double)xv.c[k].im;
+      s[REL] = xx == 0.0 ? 1.0 : s[0] / xx;
+    }
     store_vec<PA>(x, cb, xv);
     store_vec<PA>(r, cb, rv);
     store_vec<PP>(p, cb, pn);
   }
-  block_sum_store<1>(s, partials_rr);
+  block_sum_store<1 + REL>(s, partials_rr);
 }
 
 // Reliable update, vector part: x_hi += x_s; x_s = 0
@@ -799,8 +850,10 @@ int vec_norm2(const VecField& v, int prec, double* out) {
   else if (prec == P_F32) hipLaunchKernelGGL(k_norm2<P_F32>, dim3(nb), dim3(256), 0, c.stream, v, c.Vh, c.ws.partials);
   else hipLaunchKernelGGL(k_norm2<P_H16>, dim3(nb), dim3(256), 0, c.stream, v, c.Vh, c.ws.partials);
   MH_CHECK(hipGetLastError());
-  std::vector<double> h((size_t)nb * 4);
-  MH_CHECK(hipMemcpyAsync(h.data(), c.ws.partials, h.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  size_t bytes = (size_t)nb * 4 * sizeof(double);
+  double* h = (double*)pinned_scratch(bytes);
+  if (!h) return -5;
+  MH_CHECK(hipMemcpyAsync(h, c.ws.partials, bytes, hipMemcpyDeviceToHost, c.stream));
   MH_CHECK(hipStreamSynchronize(c.stream));
   double s = 0;
   for (int i = 0; i < nb; i++) s += h[(size_t)i * 4];
@@ -819,15 +872,20 @@ int vec_copy(const VecField& in, VecField& out, int prec) {
   return 0;
 }
 
+// The scalar block moves through its pinned host mirror in both directions.
 static int read_scalars(CGScalars* h) {
   Context& c = ctx();
-  MH_CHECK(hipMemcpyAsync(h, c.ws.scal, sizeof(CGScalars), hipMemcpyDeviceToHost, c.stream));
+  MH_CHECK(hipMemcpyAsync(c.ws.scal_host, c.ws.scal, sizeof(CGScalars), hipMemcpyDeviceToHost, c.stream));
   MH_CHECK(hipStreamSynchronize(c.stream));
+  *h = *c.ws.scal_host;
   return 0;
 }
 static int write_scalars(const CGScalars* h) {
   Context& c = ctx();
-  MH_CHECK(hipMemcpyAsync(c.ws.scal, h, sizeof(CGScalars), hipMemcpyHostToDevice, c.stream));
+  // the mirror may still feed an earlier copy
+  MH_CHECK(hipStreamSynchronize(c.stream));
+  *c.ws.scal_host = *h;
+  MH_CHECK(hipMemcpyAsync(c.ws.scal, c.ws.scal_host, sizeof(CGScalars), hipMemcpyHostToDevice, c.stream));
   return 0;
 }
 
@@ -844,14 +902,15 @@ static int true_residual(const VecField& src, const VecField& x, VecField& tmp_o
   launch_dslash<P_F64>(L, x, tmp_other, pb ^ 1, nullptr);
   hipLaunchKernelGGL(k_dslash_resid<P_F64>, dim3(dslash_grid()), dim3(256), 0, c.stream, c.geom, L.fat, L.lng,
                      tmp_other, x, src, resid, p, write_p, pb, msq_x4, partials_rr());
-  hipLaunchKernelGGL(k_restart_scalars, dim3(1), dim3(1024), 0, c.stream, partials_rr(), dslash_grid(), c.ws.scal);
+  hipLaunchKernelGGL(k_restart_scalars, dim3(1), dim3(1024), 0, c.stream, partials_rr(), dslash_grid(), c.Vh,
+                     c.ws.scal);
   MH_CHECK(hipGetLastError());
   return 0;
 }
 
 template <int PL, int PP, int PA>
 static void enqueue_cg_iteration(const DevLinks& L, VecField& p, VecField& tmp, VecField& r, VecField& ttt,
-                                 VecField& x, int pb, double msq_x4) {
+                                 VecField& x, int pb, double msq_x4, bool rel = false) {
   Context& c = ctx();
   int nb = dslash_grid();
   const int* halt = &c.ws.scal->halt;
@@ -859,11 +918,21 @@ static void enqueue_cg_iteration(const DevLinks& L, VecField& p, VecField& tmp, 
   hipLaunchKernelGGL((k_dslash_mdagm<PL, PP, PA>), dim3(nb), dim3(256), 0, c.stream, c.geom, L.fat, L.lng, tmp, p,
                      r, ttt, pb, msq_x4, c.ws.partials, halt);
   hipLaunchKernelGGL(k_cg_scalars, dim3(1), dim3(1024), 0, c.stream, c.ws.partials, partials_rr(), nb, c.ws.scal);
-  hipLaunchKernelGGL((k_cg_update<PP, PA>), dim3(nblocks_for(c.Vh, 256)), dim3(256), 0, c.stream, x, p, r, ttt,
-                     c.Vh, c.ws.scal, partials_rr());
+  int nu = nblocks_for(c.Vh, 256);
+  if (!rel) {
+    hipLaunchKernelGGL((k_cg_update<PP, PA, 0>), dim3(nu), dim3(256), 0, c.stream, x, p, r, ttt, c.Vh, c.ws.scal,
+                       partials_rr());
+  } else {
+    hipLaunchKernelGGL((k_cg_update<PP, PA, 1>), dim3(nu), dim3(256), 0, c.stream, x, p, r, ttt, c.Vh, c.ws.scal,
+                       partials_rr());
+    hipLaunchKernelGGL(k_cg_relcheck, dim3(1), dim3(1024), 0, c.stream, partials_rr(), nu, c.Vh, c.ws.scal);
+  }
 }
 
-// Plain FP64 CG, MILC's restart/iteration semantics (d_congrad5_fn_milc.c:60-407)
+// Plain FP64 CG, MILC's restart/iteration semantics (d_congrad5_fn_milc.c:60-407),
+// with the optional Fermilab relative residual: its target is met when
+// my_relative_residue < relresid^2 (relrsqmin, :92), tested beside the
+// residual at every restart block (:213-220) and after every update (:351-354).
 static int cg_solve_f64(const VecField& src, VecField& x, double mass, const SolveParams& sp, SolveStats& st) {
   Context& c = ctx();
   const DevLinks& L = c.links[P_F64];
@@ -871,6 +940,8 @@ static int cg_solve_f64(const VecField& src, VecField& x, double mass, const Sol
   int pb = pbit(sp.parity);
   double msq_x4 = 4.0 * mass * mass;
   double rsqmin = sp.resid * sp.resid;
+  double relrsqmin = sp.relresid * sp.relresid;
+  const bool rel = relrsqmin > 0;
   int niter = sp.max_iters, max_restarts = sp.max_restarts;
   long max_cg = (long)max_restarts * niter;
   st = SolveStats();
@@ -882,6 +953,7 @@ static int cg_solve_f64(const VecField& src, VecField& x, double mass, const Sol
     return vec_zero(x, P_F64);
   }
   VecField ttt, p, r, tmp;
+  VecScope owned{&ttt, &p, &r, &tmp};
   if ((rc = alloc_vec(ttt, P_F64, c.stride)) || (rc = alloc_vec(p, P_F64, c.stride)) ||
       (rc = alloc_vec(r, P_F64, c.stride)) || (rc = alloc_vec(tmp, P_F64, c.stride)))
     return rc;
@@ -889,18 +961,24 @@ static int cg_solve_f64(const VecField& src, VecField& x, double mass, const Sol
   memset(&h, 0, sizeof h);
   h.source_norm = source_norm;
   h.rsq_stop = rsqmin * source_norm;
-  write_scalars(&h);
+  h.relrsq_stop = relrsqmin;
+  h.rel_mode = rel ? 1 : 0;
+  if ((rc = write_scalars(&h))) return rc;
   long iteration = 0;
   int nrestart = 0;
-  double final_rsq = 0, size_r = 0;
+  double final_rsq = 0, final_relrsq = 0, size_r = 0;
   int batch = sp.batch > 0 ? sp.batch : 16;
+  auto met = [&](double r2, double rel2) {
+    return (rsqmin <= 0 || rsqmin > r2) && (relrsqmin <= 0 || relrsqmin > rel2);
+  };
   for (;;) {
-    // restart block: true residual
+    // restart block: true residual (and relative residual)
     if ((rc = true_residual(src, x, tmp, r, p, 1, pb, msq_x4))) break;
     if ((rc = read_scalars(&h))) break;
     final_rsq = h.rsq / source_norm;
+    final_relrsq = h.relrsq;
     iteration++;
-    if (iteration >= max_cg || nrestart >= max_restarts || rsqmin > final_rsq) break;
+    if (iteration >= max_cg || nrestart >= max_restarts || met(final_rsq, final_relrsq)) break;
     nrestart++;
     // normal iterations until the next restart point or the recurrence target
     long left = niter - (iteration % niter);
@@ -908,7 +986,7 @@ static int cg_solve_f64(const VecField& src, VecField& x, double mass, const Sol
     left = std::min(left, max_cg - iteration);
     while (left > 0) {
       int nb = (int)std::min<long>(batch, left);
-      for (int k = 0; k < nb; k++) enqueue_cg_iteration<P_F64, P_F64, P_F64>(L, p, tmp, r, ttt, x, pb, msq_x4);
+      for (int k = 0; k < nb; k++) enqueue_cg_iteration<P_F64, P_F64, P_F64>(L, p, tmp, r, ttt, x, pb, msq_x4, rel);
       MH_CHECK(hipGetLastError());
       int before = h.iters;
       if ((rc = read_scalars(&h))) break;
@@ -921,20 +999,20 @@ static int cg_solve_f64(const VecField& src, VecField& x, double mass, const Sol
     if (rc) break;
   }
   st.final_rsq = final_rsq;
+  st.final_relrsq = final_relrsq;
   st.size_r = size_r;
   st.iters = (int)iteration;
   st.restarts = nrestart;
-  // converged = the FP64 true residual met the target (the GPU glue's test,
-  // d_congrad5_fn_gpu.c:153)
-  st.converged = rsqmin > final_rsq ? 1 : 0;
-  free_vec(ttt); free_vec(p); free_vec(r); free_vec(tmp);
+  // converged = the FP64 true residuals met the targets (the GPU glue's test,
+  // d_congrad5_fn_gpu.c:153, with the relative residual of :92 when asked for)
+  st.converged = met(final_rsq, final_relrsq) ? 1 : 0;
   return rc;
 }
 
 // Mixed precision CG: sloppy (PS = F32 or H16 links and p, F32 r/x/Ap)
 // iterations with the FEWSUMS recurrence, FP64 reliable updates when the
 // sloppy residual has fallen by delta (or reached the target), FP64 true
-// residual for the stopping test.
+// residual (and relative residual, when asked for) for the stopping test.
 template <int PS>
 static int cg_solve_mixed(const VecField& src, VecField& x, double mass, const SolveParams& sp, SolveStats& st) {
   Context& c = ctx();
@@ -943,6 +1021,7 @@ static int cg_solve_mixed(const VecField& src, VecField& x, double mass, const S
   int pb = pbit(sp.parity);
   double msq_x4 = 4.0 * mass * mass;
   double rsqmin = sp.resid * sp.resid;
+  double relrsqmin = sp.relresid * sp.relresid;
   long max_cg = (long)sp.max_restarts * sp.max_iters;
   double delta = sp.delta > 0 ? sp.delta : kReliableDelta;
   st = SolveStats();
@@ -954,6 +1033,7 @@ static int cg_solve_mixed(const VecField& src, VecField& x, double mass, const S
     return vec_zero(x, P_F64);
   }
   VecField rhi, tmp64, ps, tmps, ttts, rs, xs, dummy{};
+  VecScope owned{&rhi, &tmp64, &ps, &tmps, &ttts, &rs, &xs};
   if ((rc = alloc_vec(rhi, P_F64, c.stride)) || (rc = alloc_vec(tmp64, P_F64, c.stride)) ||
       (rc = alloc_vec(ps, PS, c.stride)) || (rc = alloc_vec(tmps, PS, c.stride)) ||
       (rc = alloc_vec(ttts, P_F32, c.stride)) || (rc = alloc_vec(rs, P_F32, c.stride)) ||
@@ -962,13 +1042,16 @@ static int cg_solve_mixed(const VecField& src, VecField& x, double mass, const S
   CGScalars h;
   memset(&h, 0, sizeof h);
   h.source_norm = source_norm;
-  write_scalars(&h);
+  if ((rc = write_scalars(&h))) return rc;
   int nb = nblocks_for(c.Vh, 256);
   long iteration = 0;
   int nru = 0;
-  double final_rsq = 0;
+  double final_rsq = 0, final_relrsq = 0;
   int batch = sp.batch > 0 ? sp.batch : 16;
   bool first = true;
+  auto met = [&](double r2, double rel2) {
+    return (rsqmin <= 0 || rsqmin > r2) && (relrsqmin <= 0 || relrsqmin > rel2);
+  };
   for (;;) {
     // reliable update (or the initial residual): FP64 true residual
     if (!first) hipLaunchKernelGGL(k_accum_x, dim3(nb), dim3(256), 0, c.stream, x, xs, c.Vh);
@@ -982,10 +1065,15 @@ static int cg_solve_mixed(const VecField& src, VecField& x, double mass, const S
     first = false;
     if ((rc = read_scalars(&h))) break;
     final_rsq = h.rsq / source_norm;
+    final_relrsq = h.relrsq;
     iteration++;
-    if (rsqmin > final_rsq || iteration >= max_cg) break;
+    if (met(final_rsq, final_relrsq) || iteration >= max_cg) break;
     // sloppy iterations until |r|^2 fell by delta^2 or reached the target
-    h.rsq_stop = std::max(rsqmin * source_norm, delta * delta * h.rsq);
+    // (once the residual target is met and only the relative residual is
+    // left, by delta^2 alone)
+    bool r_done = rsqmin <= 0 || rsqmin > final_rsq;
+    h.rsq_stop = r_done ? delta * delta * h.rsq : std::max(rsqmin * source_norm, delta * delta * h.rsq);
+    h.rel_mode = 0;
     h.halt = 0;
     h.iters = 0;
     if ((rc = write_scalars(&h))) break;
@@ -1004,19 +1092,18 @@ static int cg_solve_mixed(const VecField& src, VecField& x, double mass, const S
     if (rc) break;
   }
   st.final_rsq = final_rsq;
+  st.final_relrsq = final_relrsq;
   st.size_r = final_rsq;
   st.iters = (int)iteration;
   st.restarts = nru;
   st.reliable_updates = nru;
-  st.converged = rsqmin > final_rsq ? 1 : 0;
-  free_vec(rhi); free_vec(tmp64); free_vec(ps); free_vec(tmps); free_vec(ttts); free_vec(rs); free_vec(xs);
+  st.converged = met(final_rsq, final_relrsq) ? 1 : 0;
   return rc;
 }
 
 int cg_solve(const VecField& src, VecField& x, double mass, const SolveParams& sp, SolveStats& st) {
   auto t0 = std::chrono::steady_clock::now();
   int rc;
-  if (sp.relresid > 0) return -4;  // Fermilab relative residual is not supported
   if (sp.sloppy_prec == P_F64) rc = cg_solve_f64(src, x, mass, sp, st);
   else if (sp.sloppy_prec == P_F32) rc = cg_solve_mixed<P_F32>(src, x, mass, sp, st);
   else rc = cg_solve_mixed<P_H16>(src, x, mass, sp, st);

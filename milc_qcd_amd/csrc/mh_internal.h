@@ -1,6 +1,7 @@
 // mh_internal.h -- host-side declarations shared by the library's translation units.
 #pragma once
 #include <functional>
+#include <initializer_list>
 #include <vector>
 #include "mh_common.h"
 
@@ -19,12 +20,16 @@ struct CGScalars {
   double rsq_stop;     // stop when rsq < rsq_stop (absolute)
   double rsq_first;    // |r|^2 right after the last reliable update
   double rsq_max;      // max |r|^2 since the last reliable update
+  double relrsq;       // Fermilab relative residual of the last update or
+                       // restart: sqrt(sum_i |r_i|^2/|x_i|^2 / Vh), the value
+                       // of my_relative_residue (d_congrad5_fn_milc.c:37-56)
+  double relrsq_stop;  // with rel_mode: stop once relrsq < relrsq_stop too
+  int rel_mode;        // 1: the halt test needs relrsq, taken after the update
   int halt;            // set by the scalar step once the target is met: later
                        // iterations' kernels return immediately
   int go;              // written by every scalar step: 1 if this iteration's
                        // vector update must run
   int iters;           // scalar steps executed since the last reset
-  int pad;
 };
 
 // Reduction scratch: partial sums per block (FP64) and the scalar block.
@@ -33,6 +38,8 @@ struct Workspace {
   int max_blocks = 0;
   CGScalars* scal = nullptr;   // device
   CGScalars* scal_host = nullptr;  // pinned host mirror
+  void* pinned = nullptr;          // pinned host scratch for small read-backs
+  size_t pinned_bytes = 0;
 };
 
 // A set of links on the device in one precision.
@@ -85,6 +92,32 @@ void dev_free(void* p);
 void dev_pool_release();
 int alloc_vec(VecField& v, int prec, int stride);
 void free_vec(VecField& v);
+// Frees the vector fields it holds when it goes out of scope, so that every
+// return path of a solver hands its blocks back to the pool.
+struct VecScope {
+  std::vector<VecField*> v;
+  VecScope() = default;
+  VecScope(std::initializer_list<VecField*> l) : v(l) {}
+  VecScope(const VecScope&) = delete;
+  VecScope& operator=(const VecScope&) = delete;
+  void add(VecField* f) { v.push_back(f); }
+  ~VecScope() {
+    for (VecField* f : v) free_vec(*f);
+  }
+};
+// Runs a clean-up action when it goes out of scope (every return path).
+template <class F>
+struct OnExit {
+  F f;
+  explicit OnExit(F fn) : f(fn) {}
+  OnExit(const OnExit&) = delete;
+  OnExit& operator=(const OnExit&) = delete;
+  ~OnExit() { f(); }
+};
+// Pinned host scratch of at least `bytes` (grown on demand, freed with the
+// context): small device-to-host read-backs land in pinned memory, so the
+// runtime copies them by DMA and never through a pageable bounce.
+void* pinned_scratch(size_t bytes);
 int alloc_links(LinkField& l, int prec, int stride);
 void free_links(LinkField& l);
 size_t vec_bytes(int prec, int stride);
@@ -110,8 +143,10 @@ struct SolveParams {
   int parity;          // MILC code
   int max_iters;       // per restart (qic->max)
   int max_restarts;    // qic->nrestart
-  double resid;        // target |r|/|b|
-  double relresid;     // unsupported (must be 0)
+  double resid;        // target |r|/|b| (0: no test on it)
+  double relresid;     // Fermilab relative residual target (qic->relresid, 0:
+                       // no test); met when my_relative_residue < relresid^2,
+                       // the test of d_congrad5_fn_milc.c:92,213-220
   int sloppy_prec;     // P_F64 (plain), P_F32 or P_H16
   double delta;        // reliable-update threshold (mixed)
   int batch;           // iterations enqueued per host check
@@ -188,6 +223,8 @@ struct RhmcParams {
 };
 // stats: 32 doubles (see milc_hip_dev_rhmc_trajectory)
 int rhmc_trajectory(const RhmcParams& p, double* stats);
+// the next trajectory fails with -90 at this stage (1 heatbath, 2 start actions, 3 molecular dynamics)
+void rhmc_fail_at(int stage);
 
 // MILC's per-site generators (SITERAND) on the device
 int prn_seed_dev(int iseed, int single);

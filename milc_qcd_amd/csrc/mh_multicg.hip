@@ -38,10 +38,6 @@ struct MSScalars {
   int max_iter, pad;
 };
 
-struct VecSet {
-  VecField v[MS_MAX];
-};
-
 template <int N>
 __device__ __forceinline__ void block_sum4(double (&v)[N], double* __restrict__ partials) {
 #pragma unroll
@@ -181,14 +177,17 @@ __global__ void __launch_bounds__(256) k_ms_r(VecField rprev, VecField rcur, Vec
 }
 
 #define MS_RING (MS_KW + 1)
-struct VecRing {
-  VecField v[MS_RING];
-};
+// Field table of one solve, in device memory: the residual ring, then psim_j,
+// then pm_j (the flush kernel reads it instead of taking ~1.7 KB of by-value
+// arguments).
+#define MS_TAB_PSIM MS_RING
+#define MS_TAB_PM (MS_RING + MS_MAX)
+#define MS_TAB_N (MS_RING + 2 * MS_MAX)
 
 // window flush: psim_j and pm_j brought up to date from the ring of residuals;
 // ring[0] = the window's last residual
 template <int P>
-__global__ void __launch_bounds__(256) k_ms_flush(VecRing ring, VecSet psim, VecSet pm, int Vh,
+__global__ void __launch_bounds__(256) k_ms_flush(const VecField* __restrict__ tab, int Vh,
                                                   const MSScalars* __restrict__ sc) {
   typedef typename PrecTraits<P>::Real R;
   int cb = blockIdx.x * blockDim.x + threadIdx.x;
@@ -197,14 +196,15 @@ __global__ void __launch_bounds__(256) k_ms_flush(VecRing ring, VecSet psim, Vec
   Vec3<R> rv[MS_KW];
 #pragma unroll
   for (int l = 0; l < MS_KW; l++) {
-    if (l < md) load_vec<P>(ring.v[l + 1], cb, rv[l]);
+    if (l < md) load_vec<P>(tab[l + 1], cb, rv[l]);
     else zero(rv[l]);
   }
   const int nn = sc->now_ws;
   for (int j = 0; j < nn; j++) {
     Vec3<R> d, x;
-    load_vec<P>(pm.v[j], cb, d);
-    load_vec<P>(psim.v[j], cb, x);
+    const VecField pmj = tab[MS_TAB_PM + j], psimj = tab[MS_TAB_PSIM + j];
+    load_vec<P>(pmj, cb, d);
+    load_vec<P>(psimj, cb, x);
     R cx = (R)sc->wx[j], ca = (R)sc->wa[j];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
@@ -223,12 +223,12 @@ __global__ void __launch_bounds__(256) k_ms_flush(VecRing ring, VecSet psim, Vec
       d.c[k].re = qr;
       d.c[k].im = qi;
     }
-    store_vec<P>(psim.v[j], cb, x);
-    store_vec<P>(pm.v[j], cb, d);
+    store_vec<P>(psimj, cb, x);
+    store_vec<P>(pmj, cb, d);
   }
 #pragma unroll
   for (int l = 0; l < MS_KW; l++)
-    if (l + 1 == md) store_vec<P>(ring.v[0], cb, rv[l]);
+    if (l + 1 == md) store_vec<P>(tab[0], cb, rv[l]);
 }
 
 // convergence test, alpha recurrences and scalar scroll (ks_multicg_offset.c:358-463)
@@ -311,20 +311,43 @@ static int multicg_run(const VecField& src64, std::vector<VecField>& x64, const 
     st.converged = 1;
     return 0;
   }
-  VecSet psim{}, pm{};
+  // tab[0] is r itself; tab[1..MS_KW] the rest of the ring
+  VecField tab[MS_TAB_N]{};
   VecField p, r, ttt, tmp;
+  MSScalars* d_sc = nullptr;
+  VecField* d_tab = nullptr;
+  VecScope owned{&p, &r, &ttt, &tmp};
+  for (int l = 1; l < MS_RING; l++) owned.add(&tab[l]);
   for (int j = 0; j < n; j++) {
-    if ((rc = alloc_vec(psim.v[j], P, c.stride))) return rc;
-    if ((rc = alloc_vec(pm.v[j], P, c.stride))) return rc;
-    if ((rc = convert_vec(src64, P_F64, pm.v[j], P))) return rc;
+    owned.add(&tab[MS_TAB_PSIM + j]);
+    owned.add(&tab[MS_TAB_PM + j]);
+  }
+  OnExit scalars([&] {
+    (void)hipStreamSynchronize(c.stream);
+    dev_free(d_sc);
+    dev_free(d_tab);
+  });
+  for (int j = 0; j < n; j++) {
+    if ((rc = alloc_vec(tab[MS_TAB_PSIM + j], P, c.stride))) return rc;
+    if ((rc = alloc_vec(tab[MS_TAB_PM + j], P, c.stride))) return rc;
+    if ((rc = convert_vec(src64, P_F64, tab[MS_TAB_PM + j], P))) return rc;
   }
   if ((rc = alloc_vec(p, P, c.stride)) || (rc = alloc_vec(r, P, c.stride)) || (rc = alloc_vec(ttt, P, c.stride)) ||
       (rc = alloc_vec(tmp, P, c.stride)))
     return rc;
   if ((rc = convert_vec(src64, P_F64, p, P)) || (rc = convert_vec(src64, P_F64, r, P))) return rc;
+  tab[0] = r;
+  for (int l = 1; l < MS_RING; l++)
+    if ((rc = alloc_vec(tab[l], P, c.stride))) return rc;
 
-  MSScalars h;
+  // scalars and the field table go up through pinned memory (and come back
+  // through it once per window)
+  const size_t tab_bytes = sizeof(VecField) * MS_TAB_N;
+  char* hp = (char*)pinned_scratch(sizeof(MSScalars) + tab_bytes);
+  if (!hp) return -5;
+  MSScalars& h = *(MSScalars*)hp;
   memset(&h, 0, sizeof h);
+  memcpy(hp + sizeof(MSScalars), tab, tab_bytes);
   double offset_low = 1.0e20;
   int j_low = -1;
   for (int j = 0; j < n; j++) {
@@ -349,13 +372,10 @@ static int multicg_run(const VecField& src64, std::vector<VecField>& x64, const 
   }
   h.max_iter = (int)std::min<long>(mp.max_iters, 1L << 30);
   // MILC stores shift0 = -shifts[j_low] with shifts[j_low] = offset_low
-  MSScalars* d_sc = nullptr;
-  if ((rc = dev_alloc((void**)&d_sc, sizeof(MSScalars)))) return rc;
+  if ((rc = dev_alloc((void**)&d_sc, sizeof(MSScalars))) || (rc = dev_alloc((void**)&d_tab, tab_bytes))) return rc;
   MH_CHECK(hipMemcpyAsync(d_sc, &h, sizeof h, hipMemcpyHostToDevice, c.stream));
-  VecRing ring{};
-  ring.v[0] = r;
-  for (int l = 1; l < MS_RING; l++)
-    if ((rc = alloc_vec(ring.v[l], P, c.stride))) return rc;
+  MH_CHECK(hipMemcpyAsync(d_tab, hp + sizeof(MSScalars), tab_bytes, hipMemcpyHostToDevice, c.stream));
+  MH_CHECK(hipStreamSynchronize(c.stream));
   const int nb = (c.Vh + 255) / 256;
   // one window: reset, MS_KW iterations (ks_multicg_offset.c:291-460), flush
   auto window = [&]() -> int {
@@ -366,12 +386,12 @@ static int multicg_run(const VecField& src64, std::vector<VecField>& x64, const 
       hipLaunchKernelGGL(k_ms_dslash2<P>, dim3(nb), dim3(256), 0, c.stream, c.geom, L.fat, L.lng, tmp, p, ttt, pb,
                          d_sc, c.ws.partials);
       hipLaunchKernelGGL(k_ms_beta, dim3(1), dim3(1024), 0, c.stream, c.ws.partials, nb, d_sc);
-      hipLaunchKernelGGL(k_ms_r<P>, dim3(nb), dim3(256), 0, c.stream, ring.v[m - 1], ring.v[m], ttt, c.Vh, d_sc,
+      hipLaunchKernelGGL(k_ms_r<P>, dim3(nb), dim3(256), 0, c.stream, tab[m - 1], tab[m], ttt, c.Vh, d_sc,
                          c.ws.partials);
       hipLaunchKernelGGL(k_ms_alpha, dim3(1), dim3(1024), 0, c.stream, c.ws.partials, nb, d_sc);
-      hipLaunchKernelGGL(k_ms_p<P>, dim3(nb), dim3(256), 0, c.stream, p, ring.v[m], c.Vh, d_sc);
+      hipLaunchKernelGGL(k_ms_p<P>, dim3(nb), dim3(256), 0, c.stream, p, tab[m], c.Vh, d_sc);
     }
-    hipLaunchKernelGGL(k_ms_flush<P>, dim3(nb), dim3(256), 0, c.stream, ring, psim, pm, c.Vh, d_sc);
+    hipLaunchKernelGGL(k_ms_flush<P>, dim3(nb), dim3(256), 0, c.stream, (const VecField*)d_tab, c.Vh, d_sc);
     MH_CHECK(hipGetLastError());
     return 0;
   };
@@ -385,20 +405,13 @@ static int multicg_run(const VecField& src64, std::vector<VecField>& x64, const 
     iteration = 1 + h.iters;
     if (h.halt) break;
   }
-  for (int l = 1; l < MS_RING; l++) free_vec(ring.v[l]);
   for (int j = 0; j < n; j++)
-    if ((rc = convert_vec(psim.v[j], P, x64[j], P_F64))) return rc;
+    if ((rc = convert_vec(tab[MS_TAB_PSIM + j], P, x64[j], P_F64))) return rc;
   MH_CHECK(hipStreamSynchronize(c.stream));
   st.iters = (int)iteration;
   st.converged = h.converged;
   st.final_rsq = h.rsq / source_norm;
   st.size_r = st.final_rsq;
-  for (int j = 0; j < n; j++) {
-    free_vec(psim.v[j]);
-    free_vec(pm.v[j]);
-  }
-  free_vec(p); free_vec(r); free_vec(ttt); free_vec(tmp);
-  dev_free(d_sc);
   return 0;
 }
 

@@ -707,7 +707,17 @@ uint64_t host_mix(uint64_t z) {
 
 bool rat_ok(const RhmcRat& r) { return r.order > 0 && r.order <= 32; }
 
+int g_fail_stage = 0;
+// the injected failure of stage `stage`, once
+int injected(int stage) {
+  if (g_fail_stage != stage) return 0;
+  g_fail_stage = 0;
+  return -90;
+}
+
 }  // namespace
+
+void rhmc_fail_at(int stage) { g_fail_stage = stage; }
 
 int rhmc_trajectory(const RhmcParams& p, double* stats) {
   Context& c = ctx();
@@ -761,15 +771,31 @@ int rhmc_trajectory(const RhmcParams& p, double* stats) {
   for (auto& P : T.pf)
     if ((rc = alloc_vec(P.phi, P_F64, c.stride))) return rc;
   std::vector<VecField> chi(p.n_pseudo);
+  VecScope chi_owned;
+  for (auto& v : chi) chi_owned.add(&v);
   for (auto& v : chi)
     if ((rc = alloc_vec(v, P_F64, c.stride))) return rc;
+  // the starting links, restored by every failure from here on
+  MH_CHECK(hipMemcpyAsync(T.Uold.data, c.thin.data, link_bytes(P_F64, c.stride), hipMemcpyDeviceToDevice, c.stream));
   (void)hipStreamSynchronize(c.stream);
   const double t_setup = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+  // A failed trajectory leaves the starting thin links in place with their
+  // fat and Naik links (set 0) of this quark action, and records which
+  // action those links belong to.
+  auto fail = [&](int code) {
+    (void)hipMemcpyAsync(c.thin.data, T.Uold.data, link_bytes(P_F64, c.stride), hipMemcpyDeviceToDevice, c.stream);
+    T.dirty = true;
+    T.aux = false;
+    if (T.fresh() == 0) (void)T.use_set(0.0);
+    g_asqtad_links_version = p.action == 1 ? c.links_version : ~0u;
+    (void)hipStreamSynchronize(c.stream);
+    return code;
+  };
   // ranmom, then grsource_imp_rhmc for each pseudofermion: phi = R_GR(M^dag M) chi
   double grsum[RHMC_MAX_PSEUDO] = {0};
   int it_gr = 0;
-  if ((rc = T.fresh())) return rc;
-  if (p.mixed_action && (rc = ensure_links(P_F32))) return rc;
+  if ((rc = T.fresh())) return fail(rc);
+  if (p.mixed_action && (rc = ensure_links(P_F32))) return fail(rc);
   rc = p.site_rng ? ranmom_prn_dev(T.mom) : ranmom_dev(host_mix(p.seed), T.mom);
   // the noise of every pseudofermion first, in MILC's order of draws
   for (int i = 0; i < p.n_pseudo && !rc; i++) {
@@ -808,16 +834,16 @@ int rhmc_trajectory(const RhmcParams& p, double* stats) {
       }
     }
   }
+  if (!rc) rc = injected(1);
+  if (rc) return fail(rc);
   for (auto& v : chi) free_vec(v);
-  if (rc) return rc;
   (void)hipStreamSynchronize(c.stream);
   const double t_heatbath =
       std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count() - t_setup;
   double g0, h0, f0, g1, h1, f1;
   auto t_a0 = std::chrono::steady_clock::now();
-  if ((rc = T.actions(&g0, &h0, &f0))) return rc;
+  if ((rc = T.actions(&g0, &h0, &f0)) || (rc = injected(2))) return fail(rc);
   double t_act = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_a0).count();
-  MH_CHECK(hipMemcpyAsync(T.Uold.data, c.thin.data, link_bytes(P_F64, c.stride), hipMemcpyDeviceToDevice, c.stream));
   const double e = p.epsilon;
   for (int cyc = 0; cyc < p.steps / sch.steps && !rc; cyc++) {
     double now = 0;
@@ -829,15 +855,7 @@ int rhmc_trajectory(const RhmcParams& p, double* stats) {
     if (!rc && sch.length - now > 1e-12) rc = T.update_u(e * (sch.length - now));
     if (!rc) rc = T.reunitarize();
   }
-  // a failed trajectory leaves the starting links (and their fat and Naik links) in place
-  auto fail = [&](int code) {
-    (void)hipMemcpyAsync(c.thin.data, T.Uold.data, link_bytes(P_F64, c.stride), hipMemcpyDeviceToDevice, c.stream);
-    T.dirty = true;
-    T.aux = false;
-    (void)T.fresh();
-    g_asqtad_links_version = p.action == 1 ? c.links_version : ~0u;
-    return code;
-  };
+  if (!rc) rc = injected(3);
   if (rc) return fail(rc);
   auto t_a1 = std::chrono::steady_clock::now();
   if ((rc = T.actions(&g1, &h1, &f1))) return fail(rc);
@@ -853,12 +871,14 @@ int rhmc_trajectory(const RhmcParams& p, double* stats) {
     accepted = 0;
   }
   if (!accepted) {
-    MH_CHECK(hipMemcpyAsync(c.thin.data, T.Uold.data, link_bytes(P_F64, c.stride), hipMemcpyDeviceToDevice, c.stream));
+    if ((rc = hipMemcpyAsync(c.thin.data, T.Uold.data, link_bytes(P_F64, c.stride), hipMemcpyDeviceToDevice,
+                             c.stream)) != hipSuccess)
+      return fail(-1000 - rc);
     T.dirty = true;
     T.aux = false;
   }
   // leave the fat and Naik links (set 0) of the final thin links in place
-  if ((rc = T.fresh()) || (rc = T.use_set(0.0))) return rc;
+  if ((rc = T.fresh()) || (rc = T.use_set(0.0))) return fail(rc);
   g_asqtad_links_version = p.action == 1 ? c.links_version : ~0u;
   (void)hipStreamSynchronize(c.stream);
   double tot = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();

@@ -12,8 +12,10 @@ returned control record, invalid arguments raise):
     load_hisq_links        generic_ks/fermion_links_hisq_load_milc.c:546-702 (-> milc_hip_load_hisq_links_naik;
                            eps_naik != 0: the Naik set of load_hisq_fn_links, :574-640)
 
-Host fields are numpy float64 arrays in MILC's own layout and site order:
-vectors (V, 3, 2), links (V, 4, 3, 3, 2), even sites first.
+Host fields are numpy arrays in MILC's own layout and site order: vectors
+(V, 3, 2), links (V, 4, 3, 3, 2), even sites first; float64 (MILC_PRECISION
+2) throughout, and float32 (MILC_PRECISION 1) for the links and fields of the
+Dslash, CG and multi-shift calls.
 """
 import ctypes
 import os
@@ -55,6 +57,7 @@ EXPORTS = [
     "milc_hip_update_u", "milc_hip_mom_action", "milc_hip_dev_load_thin_links", "milc_hip_dev_read_thin_links",
     "milc_hip_dev_dslash", "milc_hip_set_link_compression", "milc_hip_link_compression", "milc_hip_dev_hisq_force",
     "milc_hip_reunitarize", "milc_hip_dev_seed_rng", "milc_hip_dev_ranmom", "milc_hip_dev_gaussian_source",
+    "milc_hip_debug_fail_trajectory",
 ]
 
 
@@ -214,6 +217,7 @@ def load_library(path=LIB_PATH):
     L.milc_hip_eo_fermion_force_multi.argtypes = [I, D, PD, ctypes.POINTER(P), I, P, P]
     L.milc_hip_set_force_form.argtypes = [I]
     L.milc_hip_dev_rhmc_trajectory.argtypes = [ctypes.POINTER(RhmcParams), ctypes.POINTER(D)]
+    L.milc_hip_debug_fail_trajectory.argtypes = [I]
     L.milc_hip_dev_seed_rng.argtypes = [I, I]
     L.milc_hip_dev_ranmom.argtypes = [P]
     L.milc_hip_dev_gaussian_source.argtypes = [I, I, ctypes.POINTER(D)]
@@ -230,6 +234,23 @@ def load_library(path=LIB_PATH):
     L.milc_hip_dev_hisq_force.argtypes = [I, I, P, ctypes.c_double, P, P]
     _lib = L
     return L
+
+
+def _real(a):
+    """MILC_PRECISION of a host field: 1 for float32 arrays, 2 for float64."""
+    if a.dtype == np.float32:
+        return 1
+    if a.dtype == np.float64:
+        return 2
+    raise MilcHipError("fields must be float32 (MILC_PRECISION 1) or float64 (MILC_PRECISION 2) arrays")
+
+
+def _pr(a):
+    """Pointer to a C-contiguous float32 or float64 host field."""
+    _real(a)
+    if not a.flags.c_contiguous:
+        raise MilcHipError("fields must be C-contiguous")
+    return a.ctypes.data_as(ctypes.c_void_p)
 
 
 def _p(a):
@@ -323,10 +344,18 @@ class Lattice:
     # ---------------------------------------------------------------- links
     def set_links(self, fat, lng):
         """Hold the (fat, long) pair as the current fn links; the next call
-        uploads them (MILC's refresh signal, d_congrad5_fn_gpu.c:120-126)."""
-        self.fat = np.ascontiguousarray(fat, dtype=np.float64)
-        self.lng = np.ascontiguousarray(lng, dtype=np.float64)
+        uploads them (MILC's refresh signal, d_congrad5_fn_gpu.c:120-126).
+        float32 links are MILC_PRECISION 1 arrays: the operator and solver
+        calls then take float32 fields."""
+        fat, lng = np.asarray(fat), np.asarray(lng)
+        dt = np.float32 if fat.dtype == np.float32 and lng.dtype == np.float32 else np.float64
+        self.fat = np.ascontiguousarray(fat, dtype=dt)
+        self.lng = np.ascontiguousarray(lng, dtype=dt)
         self._fresh = True
+
+    def _field(self, a):
+        """A host field in the precision of the current links."""
+        return np.ascontiguousarray(a, dtype=self.fat.dtype)
 
     def _refresh_flag(self):
         n = ctypes.c_int(-1 if getattr(self, "_fresh", True) else 0)
@@ -348,30 +377,33 @@ class Lattice:
     def dslash_fn_field(self, src, parity, prec=2):
         """dest = D src on `parity` (src read on the other parity)."""
         self._activate()
-        src = np.ascontiguousarray(src, dtype=np.float64)
+        src = self._field(src)
         dst = np.zeros_like(src)
         args = InvArgs(0, _evenodd(parity), 0, 0.0, 1.0, 0)
-        _check(self.lib.milc_hip_dslash(2, prec, args, _p(self.fat), _p(self.lng), _p(src), _p(dst),
+        _check(self.lib.milc_hip_dslash(_real(src), prec, args, _pr(self.fat), _pr(self.lng), _pr(src), _pr(dst),
                                         ctypes.byref(self._refresh_flag())), "milc_hip_dslash")
         return dst
 
     def ks_congrad_parity(self, src, dest, qic: QuarkInvertControl, mass):
-        """Solve (M^dag M) dest = src on qic.parity; dest is the initial guess."""
+        """Solve (M^dag M) dest = src on qic.parity; dest is the initial guess
+        (ks_congrad_parity_gpu, d_congrad5_fn_gpu.c:35-166).  Fields are in the
+        precision of the links (float32 = MILC_PRECISION 1).  qic.relresid > 0
+        adds MILC's Fermilab relative residual test (d_congrad5_fn_milc.c:92):
+        final_relrsq is then my_relative_residue of the solution."""
         self._activate()
-        if qic.relresid != 0.0:
-            raise MilcHipError("Fermilab relative residual is not supported")
-        src = np.ascontiguousarray(src, dtype=np.float64)
-        if dest.dtype != np.float64 or not dest.flags.c_contiguous:
-            raise MilcHipError("dest must be a C-contiguous float64 array")
+        src = self._field(src)
+        if dest.dtype != self.fat.dtype or not dest.flags.c_contiguous:
+            raise MilcHipError(f"dest must be a C-contiguous {self.fat.dtype} array")
         args = InvArgs(qic.max * qic.nrestart, _evenodd(qic.parity), qic.mixed, 0.0, 1.0, qic.max)
         res, rel = ctypes.c_double(0), ctypes.c_double(0)
         n = self._refresh_flag()
-        it = _check(self.lib.milc_hip_invert(2, qic.prec, mass, args, qic.resid, 0.0, _p(self.fat),
-                                             _p(self.lng), _p(src), _p(dest), ctypes.byref(res),
+        it = _check(self.lib.milc_hip_invert(_real(src), qic.prec, mass, args, qic.resid, qic.relresid,
+                                             _pr(self.fat), _pr(self.lng), _pr(src), _pr(dest), ctypes.byref(res),
                                              ctypes.byref(rel), ctypes.byref(n)), "milc_hip_invert")
         qic.final_rsq = res.value ** 2
         qic.final_relrsq = rel.value ** 2
         qic.final_iters = it
+        # the glue's convergence flag (d_congrad5_fn_gpu.c:156)
         qic.converged = int(res.value < qic.resid)
         return it
 
@@ -434,20 +466,20 @@ class Lattice:
     def ks_multicg_offset_field(self, src, ksp, qic, mixed=0, prec=2):
         """Multi-shift CG; returns the list of solutions psim[j]."""
         self._activate()
-        src = np.ascontiguousarray(src, dtype=np.float64)
+        src = self._field(src)
         n = len(ksp)
         psim = [np.zeros_like(src) for _ in range(n)]
         offs = (ctypes.c_double * n)(*[k.offset for k in ksp])
         resid = (ctypes.c_double * n)(*[q.resid for q in qic])
-        relr = (ctypes.c_double * n)(*[0.0] * n)
+        relr = (ctypes.c_double * n)(*[q.relresid for q in qic])
         fres = (ctypes.c_double * n)()
         frel = (ctypes.c_double * n)()
         ptrs = (ctypes.c_void_p * n)(*[p.ctypes.data for p in psim])
         args = InvArgs(qic[0].max * qic[0].nrestart, _evenodd(qic[0].parity), mixed, 0.0, 1.0, 0)
         it = ctypes.c_int(-1 if getattr(self, "_fresh", True) else 0)
         self._fresh = False
-        _check(self.lib.milc_hip_multishift_invert(2, prec, n, offs, args, resid, relr, _p(self.fat),
-                                                   _p(self.lng), _p(src), ptrs, fres, frel, ctypes.byref(it)),
+        _check(self.lib.milc_hip_multishift_invert(_real(src), prec, n, offs, args, resid, relr, _pr(self.fat),
+                                                   _pr(self.lng), _pr(src), ptrs, fres, frel, ctypes.byref(it)),
                "milc_hip_multishift_invert")
         for j, q in enumerate(qic):
             q.final_rsq = fres[j] ** 2
@@ -555,6 +587,12 @@ class Lattice:
         out["action_iters"] = st[28]
         out["setup_seconds"], out["heatbath_seconds"], out["md_solve_seconds"] = st[29], st[30], st[31]
         return out
+
+    def debug_fail_trajectory(self, stage):
+        """the next trajectory fails with -90 at stage 1 (heatbath), 2 (start
+        actions) or 3 (molecular dynamics): a test hook of the error contract"""
+        self._activate()
+        _check(self.lib.milc_hip_debug_fail_trajectory(stage), "milc_hip_debug_fail_trajectory")
 
     def dev_seed_rng(self, iseed, precision=1):
         """MILC's per-site generators (initialize_site_prn_from_seed) and node

@@ -36,6 +36,8 @@ def lib():
         L.ref_set_fn_links.argtypes = [P, P]
         L.ref_dslash.argtypes = [P, P, I]
         L.ref_congrad.argtypes = [P, P, I, D, I, I, D, ctypes.POINTER(D), ctypes.POINTER(I), ctypes.POINTER(I)]
+        L.ref_congrad_rel.argtypes = [P, P, I, D, I, I, D, D, ctypes.POINTER(D), ctypes.POINTER(D),
+                                      ctypes.POINTER(I)]
         L.ref_multicg.argtypes = [P, ctypes.POINTER(P), P, I, I, I, D, ctypes.POINTER(D)]
         L.ref_force_oprod.argtypes = [ctypes.POINTER(P), P, I, P, P]
         L.ref_hisq_force.argtypes = [ctypes.POINTER(P), P, I, ctypes.c_double, P]
@@ -100,6 +102,16 @@ def congrad(src, mass, resid, parity=EVEN, niter=1000, nrestart=5, dest=None):
     it = lib().ref_congrad(_p(src), _p(dest), parity, mass, niter, nrestart, resid, ctypes.byref(frsq),
                            ctypes.byref(rs), ctypes.byref(conv))
     return dest, it, {"final_rsq": frsq.value, "restarts": rs.value, "converged": conv.value}
+
+
+def congrad_rel(src, mass, resid, relresid, parity=EVEN, niter=1000, nrestart=5):
+    """ks_congrad_parity_cpu with the Fermilab relative residual test (qic.relresid)."""
+    src = np.ascontiguousarray(src, dtype=np.float64)
+    dest = np.zeros_like(src)
+    frsq, frel, conv = ctypes.c_double(0), ctypes.c_double(0), ctypes.c_int(0)
+    it = lib().ref_congrad_rel(_p(src), _p(dest), parity, mass, niter, nrestart, resid, relresid, ctypes.byref(frsq),
+                               ctypes.byref(frel), ctypes.byref(conv))
+    return dest, it, {"final_rsq": frsq.value, "final_relrsq": frel.value, "converged": conv.value}
 
 
 def multicg(src, offsets, resid, parity=EVEN, niter=2000):

@@ -139,6 +139,27 @@ int ref_congrad(const double *src, double *dest, int parity, double mass, int ni
   return it;
 }
 
+/* the same with MILC's Fermilab relative residual test (qic.relresid,
+ * d_congrad5_fn_milc.c:37-56,92); *final_relrsq = qic.final_relrsq */
+int ref_congrad_rel(const double *src, double *dest, int parity, double mass, int niter, int nrestart,
+                    double resid, double relresid, double *final_rsq, double *final_relrsq, int *converged) {
+  quark_invert_control qic;
+  int it;
+  if (!g_cur) return -1;
+  memset(&qic, 0, sizeof qic);
+  qic.prec = MILC_PRECISION;
+  qic.max = niter;
+  qic.nrestart = nrestart;
+  qic.parity = parity;
+  qic.resid = resid;
+  qic.relresid = relresid;
+  it = ks_congrad_parity_cpu((su3_vector *)src, (su3_vector *)dest, &qic, mass, g_cur);
+  if (final_rsq) *final_rsq = qic.final_rsq;
+  if (final_relrsq) *final_relrsq = qic.final_relrsq;
+  if (converged) *converged = qic.converged;
+  return it;
+}
+
 /* ks_multicg_offset_field_cpu (ks_multicg_offset.c:63) */
 int ref_multicg(const double *src, double **psim, const double *offsets, int n, int parity, int niter, double resid,
                 double *final_rsq) {

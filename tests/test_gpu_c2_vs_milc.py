@@ -1,0 +1,100 @@
+"""The headline workload against compiled MILC: 32^3x64, m = 0.01.
+
+Thin links U = exp(0.4 i H) from gauge seed 1001 and HISQ links built on the
+device, the even-site Gaussian source of seed 2100 (the first source of the
+CG benchmark).  On the device: the FP64 CG (MILC's restarted FEWSUMS
+algorithm, d_congrad5_fn_milc.c:60-407) to resid 1e-14 and the 16-bit mixed
+CG with FP64 reliable updates to 1e-10.  In a subprocess (tests/milc_cpu_c2.py)
+MILC's own CPU code builds the HISQ links from the same U, runs
+ks_congrad_parity_cpu to 1e-14 and recomputes every residual with its own
+dslash_fn_field.
+
+Bars: device links vs MILC's 1e-13; FP64 solution vs MILC's 1e-12 relative;
+iteration counts +-1 of MILC's; the mixed solution's true residual, taken
+with MILC's Dslash, <= 1e-10.  Its error against MILC's solution is reported
+and bounded by kappa * (1e-10 + 1e-14), kappa = lambda_max / 4m^2 with
+lambda_max measured here by power iteration on the device (4m^2 is the exact
+lower end of the spectrum of M^dag M = 4m^2 - D_eo D_oe).
+"""
+import json
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import milc_qcd_amd as mq
+import milc_ref as mr
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DIMS = (32, 32, 32, 64)
+MASS = 0.01
+
+pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not mr.available(), reason="oracle/_ref not built")]
+
+
+def lambda_max(G, iters=40):
+    """Largest eigenvalue of M^dag M on even sites by power iteration with the
+    device FP64 Dslash (slots 10-12)."""
+    V = G.V
+    Vh = V // 2
+    v = np.random.default_rng(5).standard_normal((V, 3, 2))
+    v[Vh:] = 0
+    v /= np.linalg.norm(v)
+    lam = 0.0
+    for _ in range(iters):
+        G.dev_write(10, mq.EVEN, v)
+        G.dev_dslash(0, 10, mq.ODD, 11)
+        G.dev_dslash(0, 11, mq.EVEN, 12)
+        u = G.dev_read(12, mq.EVEN)
+        w = 4 * MASS * MASS * v - u
+        w[Vh:] = 0
+        lam = float(np.vdot(v[:Vh].ravel(), w[:Vh].ravel()))
+        v = w / np.linalg.norm(w)
+    return lam
+
+
+@pytest.mark.timeout(900)
+def test_c2_fp64_and_mixed_vs_milc(tmp_path):
+    print("C2 32^3x64 m=0.01: device links, FP64 and mixed solves", flush=True)
+    G = mq.Lattice(DIMS)
+    try:
+        G.dev_gauge_hisq(1001, 0.4)
+        U = G.dev_read_thin_links(with_phases=False)
+        fat, lng = G.dev_read_links()
+        G.dev_source(2100, mq.EVEN, 0)
+        b = G.dev_read(0, mq.EVEN)
+        lam = lambda_max(G)
+        G.set_links(fat, lng)
+        x64 = np.zeros_like(b)
+        q64 = mq.QuarkInvertControl(max=1000, nrestart=5, parity=mq.EVEN, resid=1e-14)
+        it64 = G.ks_congrad_parity(b, x64, q64, MASS)
+        xm = np.zeros_like(b)
+        qm = mq.QuarkInvertControl(max=4000, nrestart=5, parity=mq.EVEN, resid=1e-10, mixed=2)
+        itm = G.ks_congrad_parity(b, xm, qm, MASS)
+    finally:
+        G.close()
+    print(f"device: FP64 {it64} iterations (final_rsq {q64.final_rsq:.3e}), mixed {itm} iterations "
+          f"(final_rsq {qm.final_rsq:.3e}); lambda_max {lam:.4f}", flush=True)
+    for name, arr in (("U", U), ("fat", fat), ("lng", lng), ("b", b), ("x64", x64), ("xmix", xm)):
+        np.save(tmp_path / f"{name}.npy", arr)
+    del U, fat, lng
+    # the MILC side prints its progress straight to this process's output
+    r = subprocess.run([sys.executable, "-u", os.path.join(HERE, "milc_cpu_c2.py"), "--dir", str(tmp_path),
+                        "--mass", str(MASS), "--resid", "1e-14", "--niter", "1000", "--nrestart", "5"],
+                       timeout=800)
+    assert r.returncode == 0, f"MILC side failed ({r.returncode})"
+    with open(tmp_path / "result.json") as f:
+        res = json.load(f)
+    kappa = lam / (4 * MASS * MASS)
+    bound = kappa * (1e-10 + 1e-14)
+    print(f"kappa = {kappa:.4e}; mixed error {res['rel_xmix_vs_milc']:.3e} (bound {bound:.2e}); "
+          f"FP64 error {res['rel_x64_vs_milc']:.3e}; iterations device {it64} MILC {res['milc_iters']}")
+    assert res["rel_fat_gpu_vs_milc"] < 1e-13 and res["rel_lng_gpu_vs_milc"] < 1e-13
+    assert res["milc_converged"] == 1 and q64.converged == 1
+    assert res["true_resid_milc"] < 1e-14 * 1.01 and res["true_resid_x64"] < 1e-14 * 1.01
+    assert abs(it64 - res["milc_iters"]) <= 1
+    assert res["rel_x64_vs_milc"] <= 1e-12
+    assert qm.converged == 1 and res["true_resid_xmix"] <= 1e-10
+    assert res["rel_xmix_vs_milc"] <= bound

@@ -385,6 +385,39 @@ def test_energy_conservation_and_restore():
         assert np.abs(np.linalg.det(Uc) - 1).max() < 1e-14
 
 
+def test_failed_trajectory_leaves_starting_links_of_its_action():
+    """Error contract of milc_hip_dev_rhmc_trajectory: a failure at any stage
+    (injected from the host: after the heatbath, the start actions, the MD)
+    leaves the starting thin links and the fat and Naik links of that quark
+    action in place, and a following trajectory of the other action rebuilds
+    its own links."""
+    with mq.Lattice(DIMS) as G:
+        G.dev_gauge_hisq(1200, 0.4)
+        U0 = G.dev_read_thin_links(with_phases=True)
+        hisq = G.dev_read_links()
+        # an asqtad trajectory that is rejected leaves the asqtad links of U0
+        G.dev_rhmc_trajectory(2, 0.02, BETA, COEFF, [pf()], integrator=mq.INT_LEAPFROG, mixed_md=0, accept=2,
+                              seed=3, asqtad_u0=0.86)
+        asq = G.dev_read_links()
+        assert np.array_equal(G.dev_read_thin_links(with_phases=True), U0)
+        assert not np.allclose(asq[0], hisq[0])
+        for stage in (1, 2, 3):
+            # the HISQ links of U0 first (rejected HISQ trajectory)
+            traj(G, 2, 0.02)
+            assert np.array_equal(G.dev_read_links()[0], hisq[0])
+            G.debug_fail_trajectory(stage)
+            with pytest.raises(mq.MilcHipError):
+                G.dev_rhmc_trajectory(2, 0.02, BETA, COEFF, [pf()], integrator=mq.INT_LEAPFROG, mixed_md=0,
+                                      accept=0, seed=3, asqtad_u0=0.86)
+            assert np.array_equal(G.dev_read_thin_links(with_phases=True), U0), stage
+            f, l = G.dev_read_links()
+            assert np.array_equal(f, asq[0]) and np.array_equal(l, asq[1]), stage
+        # the next HISQ trajectory rebuilds the HISQ links of the starting links
+        traj(G, 2, 0.02)
+        f, l = G.dev_read_links()
+        assert np.array_equal(f, hisq[0]) and np.array_equal(l, hisq[1])
+
+
 @pytest.mark.parametrize("alg,ngauge", [(mq.INT_2G1F, 2), (mq.INT_3G1F, 3), (mq.INT_5G1F, 5), (mq.INT_6G1F, 6)])
 def test_multi_timescale_integrators(alg, ngauge):
     """INT_nG1F: n gauge kicks per fermion kick; dH falls about 4x when eps
