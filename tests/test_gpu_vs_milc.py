@@ -89,7 +89,11 @@ def test_cg_relative_residual_gpu_vs_milc(env, resid, relresid):
           f"{qic.final_relrsq:.6e} milc {q_ref['final_relrsq']:.6e}; rel {err:.3e}")
     assert q_ref["converged"] == 1 and q_ref["final_relrsq"] < relresid ** 2
     assert abs(it - it_ref) <= 1
-    assert abs(qic.final_relrsq - q_ref["final_relrsq"]) <= 1e-6 * q_ref["final_relrsq"]
+    # the relative residual is built from r = b - Ax recomputed in FP64, whose
+    # elements carry rounding of order u|b| (u = 2.2e-16): relative to |r| that
+    # is u / (|r|/|b|), 2.5e-4 at the 1e-12 residual; 1e-6 where r is larger
+    tol = max(1e-6, 2.2e-16 / np.sqrt(q_ref["final_rsq"]))
+    assert abs(qic.final_relrsq - q_ref["final_relrsq"]) <= tol * q_ref["final_relrsq"]
     assert err < 1e-9
 
 
