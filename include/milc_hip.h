@@ -373,6 +373,22 @@ int milc_hip_dev_source(uint64_t seed, int parity, int slot);
  * updates, converged, iterations, final_rsq. */
 int milc_hip_dev_invert(int slot_b, int slot_x, double mass, int parity, double resid, int mixed_precision,
                         int max_iter, double* stats);
+/* Allocation-free form of milc_hip_dev_invert; no counterpart in the
+ * reference.  For callers that cannot allocate device memory (tight solve
+ * loops, harnesses that forbid hipMalloc): the work vectors come from one
+ * static device array of 256 MiB per device (held until the process ends),
+ * and the call makes no hipMalloc, hipFree or other device allocation.
+ * Preconditions (else -3): slot_x already exists (written by
+ * milc_hip_dev_write, milc_hip_dev_source or an earlier solve) and the links of
+ * the sloppy precision are resident (one milc_hip_dev_invert with the same
+ * mixed_precision after the links were loaded builds them).  Work vectors
+ * needing more than 256 MiB (FP64: 4 x 48 B, mixed: 200 B per site of one
+ * parity, i.e. up to about 2.6M lattice sites mixed, 2.8M FP64) return -6
+ * with a message.  parity must be EVEN or ODD (else -2).  Same results and
+ * stats as milc_hip_dev_invert.  Calls share the array, so calls that overlap
+ * on one device race (the library's calls must not overlap anyway). */
+int milc_hip_dev_invert_noalloc(int slot_b, int slot_x, double mass, int parity, double resid, int mixed_precision,
+                                int max_iter, double* stats);
 /* RHMC fermion action (ks_ratinv + ks_rateval + FP64 sum, d_action_rhmc.c
  * fermion_action): action = sum_even |residues[0] phi + sum_k residues[k] x_k|^2,
  * x_k = (M^dag M + roots[k] - 4m^2)^-1 phi (roots already include 4m^2, k = 1..order).

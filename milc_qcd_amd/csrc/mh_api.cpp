@@ -155,16 +155,20 @@ int milc_hip_init(const int dims[4], int device) {
     if (dims[i] < 4) return -2;  // the Naik hop must not wrap onto itself
   MH_CHECK(hipSetDevice(device));
   c.device = device;
+  int rc = scratch_init(device);
+  if (rc) return rc;
   for (int i = 0; i < 4; i++) c.dims[i] = dims[i];
   c.geom = make_geom(dims);
   c.Vh = c.geom.Vh;
   c.stride = (c.Vh + 63) / 64 * 64;
   MH_CHECK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
   c.ws.max_blocks = (c.Vh + 255) / 256 + 1024;
-  int rc = dev_alloc((void**)&c.ws.partials, (size_t)c.ws.max_blocks * 4 * 2 * sizeof(double));
+  rc = dev_alloc((void**)&c.ws.partials, (size_t)c.ws.max_blocks * 4 * 2 * sizeof(double));
   if (rc) return rc;
   if ((rc = dev_alloc((void**)&c.ws.scal, sizeof(CGScalars)))) return rc;
   MH_CHECK(hipHostMalloc((void**)&c.ws.scal_host, sizeof(CGScalars), hipHostMallocDefault));
+  // pinned read-back space for one partial record per block (vec_norm2)
+  if (!pinned_scratch((size_t)c.ws.max_blocks * 4 * sizeof(double))) return -5;
   c.initialised = true;
   return 0;
 }
@@ -293,15 +297,29 @@ int milc_hip_dev_write(int slot, int parity, const double* host) {
   return rc;
 }
 
-int milc_hip_dev_invert(int slot_b, int slot_x, double mass, int parity, double resid, int mixed_precision,
-                        int max_iter, double* stats) {
+// Solve on device slots; with `noalloc` every work vector comes from the
+// static scratch array and nothing is allocated (the solution slot and the
+// sloppy links must already exist).
+static int dev_invert_impl(int slot_b, int slot_x, double mass, int parity, double resid, int mixed_precision,
+                           int max_iter, double* stats, bool noalloc) {
   Context& c = ctx();
   if (!c.initialised || slot_b < 0 || slot_b >= kSlots || slot_x < 0 || slot_x >= kSlots || !g_slots[slot_b].data)
     return -1;
   int rc;
-  if (!g_slots[slot_x].data && (rc = alloc_vec(g_slots[slot_x], P_F64, c.stride))) return rc;
   int ps = sloppy_of(2, mixed_precision);
-  if (ps != P_F64 && (rc = ensure_links(ps))) return rc;
+  if (noalloc) {
+    if (parity != EVEN && parity != ODD) return -2;
+    if (!g_slots[slot_x].data || !c.links[P_F64].valid || !c.links[ps].valid) return -3;
+    size_t need = cg_scratch_bytes(ps, c.stride);
+    if (need > kScratchBytes) {
+      fprintf(stderr, "milc_hip_dev_invert_noalloc: %zu bytes of work vectors needed, scratch holds %zu\n", need,
+              kScratchBytes);
+      return -6;
+    }
+  } else {
+    if (!g_slots[slot_x].data && (rc = alloc_vec(g_slots[slot_x], P_F64, c.stride))) return rc;
+    if (ps != P_F64 && (rc = ensure_links(ps))) return rc;
+  }
   SolveParams sp;
   sp.parity = parity;
   sp.max_iters = max_iter;
@@ -312,8 +330,15 @@ int milc_hip_dev_invert(int slot_b, int slot_x, double mass, int parity, double 
   sp.delta = kReliableDelta;
   sp.batch = 16;
   SolveStats st;
-  if ((rc = vec_zero(g_slots[slot_x], P_F64))) return rc;
-  rc = cg_solve(g_slots[slot_b], g_slots[slot_x], mass, sp, st);
+  if (noalloc) {
+    ScratchArena arena;
+    if (!arena.ok) return -5;
+    if ((rc = vec_zero(g_slots[slot_x], P_F64))) return rc;
+    rc = cg_solve(g_slots[slot_b], g_slots[slot_x], mass, sp, st);
+  } else {
+    if ((rc = vec_zero(g_slots[slot_x], P_F64))) return rc;
+    rc = cg_solve(g_slots[slot_b], g_slots[slot_x], mass, sp, st);
+  }
   if (rc) return rc;
   if (stats) {
     stats[0] = std::sqrt(st.final_rsq);
@@ -324,6 +349,16 @@ int milc_hip_dev_invert(int slot_b, int slot_x, double mass, int parity, double 
     stats[5] = st.final_rsq;
   }
   return st.iters;
+}
+
+int milc_hip_dev_invert(int slot_b, int slot_x, double mass, int parity, double resid, int mixed_precision,
+                        int max_iter, double* stats) {
+  return dev_invert_impl(slot_b, slot_x, mass, parity, resid, mixed_precision, max_iter, stats, false);
+}
+
+int milc_hip_dev_invert_noalloc(int slot_b, int slot_x, double mass, int parity, double resid, int mixed_precision,
+                                int max_iter, double* stats) {
+  return dev_invert_impl(slot_b, slot_x, mass, parity, resid, mixed_precision, max_iter, stats, true);
 }
 
 int milc_hip_dev_read_links(double* fat, double* lng) {

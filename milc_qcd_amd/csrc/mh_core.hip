@@ -68,10 +68,76 @@ void pool_release_spare(size_t keep) {
 }
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free solve (milc_hip_dev_invert_noalloc): one
+// file-scope device array of kScratchBytes.  Every device the library runs on
+// holds its own copy until the process ends; its address on the current
+// device is looked up once per device at init (hipGetSymbolAddress) and only
+// that address is handed to kernels.  While an arena over it is active, the
+// solver's work vectors are carved from it and any other device allocation is
+// refused.  Calls that overlap on one device would share it: the library's
+// calls must not overlap (one context per process).
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+static constexpr int kMaxDevices = 64;
+static void* g_scratch_ptr[kMaxDevices];
+
+int scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices) return -2;
+  if (g_scratch_ptr[device]) return 0;
+  void* p = nullptr;
+  MH_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  if (!p) return -5;
+  g_scratch_ptr[device] = p;
+  return 0;
+}
+
+namespace {
+struct Arena {
+  char* base = nullptr;
+  size_t size = 0, used = 0;
+};
+Arena g_arena;
+}  // namespace
+
+ScratchArena::ScratchArena() {
+  Context& c = ctx();
+  void* p = (c.device >= 0 && c.device < kMaxDevices) ? g_scratch_ptr[c.device] : nullptr;
+  ok = p != nullptr;
+  g_arena.base = (char*)p;
+  g_arena.size = ok ? kScratchBytes : 0;
+  g_arena.used = 0;
+}
+ScratchArena::~ScratchArena() { g_arena = Arena(); }
+
+static bool in_arena(const void* p) {
+  return g_arena.base && (const char*)p >= g_arena.base && (const char*)p < g_arena.base + g_arena.size;
+}
+
+__global__ void k_zero16(uint4* __restrict__ p, size_t n) {
+  size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  const size_t step = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += step) p[i] = make_uint4(0u, 0u, 0u, 0u);
+}
+
+// Zero `bytes` (a multiple of 16) with a fill kernel on the context stream.
+static int zero_fill(void* p, size_t bytes) {
+  size_t n = bytes / 16;
+  if (!n) return 0;
+  unsigned nb = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_zero16, dim3(nb), dim3(256), 0, ctx().stream, (uint4*)p, n);
+  MH_CHECK(hipGetLastError());
+  return 0;
+}
+
 int dev_alloc(void** p, size_t bytes) {
   *p = nullptr;
   if (bytes == 0) return 0;
   bytes = (bytes + 255) & ~(size_t)255;
+  if (g_arena.base) {
+    fprintf(stderr, "milc_hip: device allocation of %zu bytes inside the allocation-free path\n", bytes);
+    return -7;
+  }
   auto it = g_pool.spare.find(bytes);
   if (it != g_pool.spare.end()) {
     *p = it->second;
@@ -95,7 +161,7 @@ int dev_alloc(void** p, size_t bytes) {
 }
 
 void dev_free(void* p) {
-  if (!p) return;
+  if (!p || in_arena(p)) return;
   auto it = g_pool.live.find(p);
   if (it == g_pool.live.end()) {
     (void)hipFree(p);
@@ -116,6 +182,7 @@ void dev_pool_release() { pool_release_spare(0); }
 void* pinned_scratch(size_t bytes) {
   Workspace& w = ctx().ws;
   if (w.pinned_bytes >= bytes) return w.pinned;
+  if (g_arena.base) return nullptr;  // no allocation in the allocation-free path
   if (w.pinned) {
     if (ctx().stream) (void)hipStreamSynchronize(ctx().stream);
     (void)hipHostFree(w.pinned);
@@ -135,6 +202,19 @@ void* pinned_scratch(size_t bytes) {
 int alloc_vec(VecField& v, int prec, int stride) {
   v.stride = stride;
   v.norm = nullptr;
+  if (g_arena.base) {
+    // carve from the scratch arena; zeroed with a fill kernel (no memset node)
+    size_t bytes = (vec_bytes(prec, stride) + 255) & ~(size_t)255;
+    if (g_arena.used + bytes > g_arena.size) {
+      v.data = nullptr;
+      fprintf(stderr, "milc_hip: scratch of %zu bytes exhausted (%zu more needed)\n", g_arena.size,
+              g_arena.used + bytes - g_arena.size);
+      return -6;
+    }
+    v.data = g_arena.base + g_arena.used;
+    g_arena.used += bytes;
+    return zero_fill(v.data, bytes);
+  }
   int rc = dev_alloc(&v.data, vec_bytes(prec, stride));
   if (rc) return rc;
   MH_CHECK(hipMemsetAsync(v.data, 0, vec_bytes(prec, stride), ctx().stream));
@@ -1099,6 +1179,13 @@ static int cg_solve_mixed(const VecField& src, VecField& x, double mass, const S
   st.reliable_updates = nru;
   st.converged = met(final_rsq, final_relrsq) ? 1 : 0;
   return rc;
+}
+
+// The work vectors cg_solve_f64 / cg_solve_mixed allocate, 256-byte aligned.
+size_t cg_scratch_bytes(int ps, int stride) {
+  auto a = [&](int prec) { return (vec_bytes(prec, stride) + 255) & ~(size_t)255; };
+  if (ps == P_F64) return 4 * a(P_F64);
+  return 2 * a(P_F64) + 2 * a(ps) + 3 * a(P_F32);
 }
 
 int cg_solve(const VecField& src, VecField& x, double mass, const SolveParams& sp, SolveStats& st) {

@@ -88,6 +88,23 @@ int capture_graph(HipGraph& gr, const std::function<int()>& enqueue);
 // device memory helpers
 int dev_alloc(void** p, size_t bytes);
 void dev_free(void* p);
+
+// Scratch of the allocation-free solve: a static device array of
+// kScratchBytes per device.  scratch_init looks up its address on `device`
+// (the current device) once; while a ScratchArena is alive, alloc_vec carves
+// zeroed fields from that array, dev_alloc refuses (-7) and the pinned host
+// scratch does not grow.
+constexpr size_t kScratchBytes = (size_t)256 << 20;
+int scratch_init(int device);
+struct ScratchArena {
+  bool ok = false;  // false: no scratch address for the current device
+  ScratchArena();
+  ~ScratchArena();
+  ScratchArena(const ScratchArena&) = delete;
+  ScratchArena& operator=(const ScratchArena&) = delete;
+};
+// Work-vector bytes cg_solve takes from the arena for sloppy precision ps.
+size_t cg_scratch_bytes(int ps, int stride);
 // hand the pooled free blocks back to the device (end of context)
 void dev_pool_release();
 int alloc_vec(VecField& v, int prec, int stride);

@@ -189,6 +189,7 @@ def load_library(path=LIB_PATH):
     L.milc_hip_dev_gauge_hisq.argtypes = [ctypes.c_uint64, D, ctypes.POINTER(I)]
     L.milc_hip_dev_source.argtypes = [ctypes.c_uint64, I, I]
     L.milc_hip_dev_invert.argtypes = [I, I, D, I, D, I, I, ctypes.POINTER(D)]
+    L.milc_hip_dev_invert_noalloc.argtypes = [I, I, D, I, D, I, I, ctypes.POINTER(D)]
     L.milc_hip_dev_read.argtypes = [I, I, P]
     L.milc_hip_dev_write.argtypes = [I, I, P]
     L.milc_hip_dev_read_links.argtypes = [P, P]
