@@ -72,6 +72,28 @@ class stdout_to_stderr:
         os.close(self.saved)
 
 
+def host_info(lib=None):
+    """nproc, CPU model and, for a compiled CPU baseline, the compiler that
+    built it (the GCC identification string in the library's .comment data)."""
+    info = {"nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0))}
+    try:
+        with open("/proc/cpuinfo") as f:
+            info["cpu_model"] = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), None)
+    except OSError:
+        info["cpu_model"] = None
+    if lib:
+        info["compiler"] = None
+        try:
+            with open(os.path.join(ROOT, lib), "rb") as f:
+                data = f.read()
+            k = data.find(b"GCC: (")
+            if k >= 0:
+                info["compiler"] = data[k:data.index(b"\0", k)].decode(errors="replace")
+        except (OSError, ValueError):
+            pass
+    return info
+
+
 def cpu_baseline(G, fp64_iters, budget_s=15.0):
     """MILC's own FP64 CG (compiled reference, ks_congrad_parity_cpu) on this
     host's cores over a bounded number of iterations; falls back to the FP64
@@ -101,6 +123,7 @@ def cpu_baseline(G, fp64_iters, budget_s=15.0):
             "unit": "s/solve",
             "cores": threads,
             "kind": "reference",
+            **host_info("oracle/_ref/libmilc_ref.so"),
             "sample": (f"MILC 7.8.1 ks_congrad_parity_cpu (FP64, FEWSUMS, DBLSTORE_FN, OpenMP {threads} threads), "
                        f"compiled from the reference sources, on the same 32^3x64 HISQ links and source: "
                        f"{it} iterations timed ({dt:.1f} s, {s_per_iter * 1e3:.1f} ms/iter), scaled by the "
@@ -127,6 +150,7 @@ def cpu_baseline(G, fp64_iters, budget_s=15.0):
         "unit": "s/solve",
         "cores": threads,
         "kind": "port",
+        **host_info("oracle/libks_oracle.so"),
         "sample": (f"FP64 oracle CG (MILC ks_congrad_parity_cpu algorithm, OpenMP) on the same 32^3x64 HISQ links "
                    f"and source: {it} iterations timed ({dt:.1f} s, {s_per_iter * 1e3:.1f} ms/iter), "
                    f"scaled by the GPU FP64 iteration count {fp64_iters}"),
@@ -170,6 +194,7 @@ def cpu_multishift_baseline(G, dims, offs, fp64_iters, nsrc, budget_s=15.0):
         "unit": "s/solve",
         "cores": threads,
         "kind": "reference",
+        **host_info("oracle/_ref/libmilc_ref.so"),
         "sample": (f"MILC 7.8.1 ks_multicg_offset_field_cpu (FP64, {len(offs)} shifts, OpenMP {threads} threads), "
                    f"compiled from the reference sources, same {'x'.join(map(str, dims))} HISQ links and source 0: "
                    f"{it} iterations timed ({dt:.1f} s, {s_per_iter * 1e3:.1f} ms/iter), scaled by the GPU FP64 "
@@ -379,6 +404,7 @@ def cpu_force_baseline(res, eps):
         "unit": "s/force",
         "cores": threads,
         "kind": "reference",
+        **host_info("oracle/_ref/libmilc_ref.so"),
         "sample": (f"MILC 7.8.1 eo_fermion_force_multi (HISQ, FP64, {len(res)} terms, OpenMP {threads} threads), "
                    f"compiled from the reference sources, one call on a {'x'.join(map(str, FORCE_CPU_DIMS))} lattice "
                    f"(thin links from gauge seed {HMC_GAUGE_SEED}, Gaussian Z_t): {dt:.2f} s, scaled by the volume "
@@ -523,6 +549,7 @@ def cpu_io_baseline(path, nbytes):
     dt = time.perf_counter() - t1
     ok = f["suma"] == f["calca"] and f["sumb"] == f["calcb"]
     return {"value": round(nbytes / dt / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
+            **host_info(), "compiler": f"numpy {np.__version__} (Python)",
             "sample": (f"numpy restatement of the SciDAC reader (oracle/ks_io.py) on the same "
                        f"{nbytes / 1e6:.0f} MB file: {dt:.2f} s, checksum {'ok' if ok else 'MISMATCH'}")}
 

@@ -644,7 +644,7 @@ __global__ void __launch_bounds__(256) k_dslash(Geom g, LinkField fat, LinkField
   if (cb >= g.Vh) return;
   Vec3<typename PrecTraits<P>::Real> acc;
   dslash_site<P>(g, fat, lng, src, cb, pb, acc);
-  store_vec<P>(dst, cb, acc);
+  store_vec_nt<P>(dst, cb, acc);
 }
 
 // Block reduction of N doubles into partials[blockIdx.x * 4 + k].
@@ -674,6 +674,10 @@ __device__ __forceinline__ void block_sum_store(double (&v)[N], double* __restri
 // (d_congrad5_fn_milc.c:281-308):
 //   ttt = D tmp - msq_x4 * p   (= -M^dag M p)
 //   pkp = <p,ttt>, c_tr = <ttt,r>, c_tt = <ttt,ttt>   (real parts)
+// The scalar step runs in its own one-block kernel (k_cg_scalars): folding it
+// into the last block of this kernel needs a device-scope release fence in
+// every block, which writes back the XCD's L2 and was measured to triple the
+// kernel time (16-bit, 32^3x64: 312 us against 107 us).
 template <int PL, int PP, int PA>
 __global__ void __launch_bounds__(256) k_dslash_mdagm(Geom g, LinkField fat, LinkField lng, VecField tmp, VecField p,
                                                       VecField r, VecField ttt, int pb, double msq_x4,
@@ -698,7 +702,7 @@ __global__ void __launch_bounds__(256) k_dslash_mdagm(Geom g, LinkField fat, Lin
       s[1] += (double)t.c[k].re * (double)rv.c[k].re + (double)t.c[k].im * (double)rv.c[k].im;
       s[2] += (double)t.c[k].re * (double)t.c[k].re + (double)t.c[k].im * (double)t.c[k].im;
     }
-    store_vec<PA>(ttt, cb, t);
+    store_vec_nt<PA>(ttt, cb, t);
   }
   block_sum_store<3>(s, partials);
 }
@@ -757,37 +761,42 @@ __global__ void k_restart_scalars(const double* __restrict__ partials_rr, int nb
   }
 }
 
-// FEWSUMS scalar step (d_congrad5_fn_milc.c:309-362):
+// FEWSUMS scalar step (d_congrad5_fn_milc.c:309-362), thread 0 of the
+// calling block, from the reduced sums (pkp, c_tr, c_tt) and |r|^2 of the
+// last update:
 //   oldrsq = actual_rsq; a = -rsq/pkp; rsq = oldrsq + 2a c_tr + a^2 c_tt; b = rsq/oldrsq
 // Halts the loop once rsq < rsq_stop (the target, or for the mixed solver the
 // reliable-update trigger).
+__device__ __forceinline__ void cg_scalar_step(const double (&d3)[3], const double (&d1)[1], CGScalars* sc) {
+  double pkp = d3[0], c_tr = d3[1], c_tt = d3[2], oldrsq = d1[0];
+  double a = -sc->rsq / pkp;
+  double rsq = oldrsq + 2.0 * a * c_tr + a * a * c_tt;
+  double b = rsq / oldrsq;
+  sc->oldrsq = oldrsq;
+  sc->actual_rsq = oldrsq;
+  sc->pkp = pkp;
+  sc->c_tr = c_tr;
+  sc->c_tt = c_tt;
+  sc->a = a;
+  sc->b = b;
+  sc->rsq = rsq;
+  sc->go = 1;
+  sc->iters += 1;
+  if (rsq > sc->rsq_max) sc->rsq_max = rsq;
+  if (!sc->rel_mode && rsq < sc->rsq_stop) sc->halt = 1;
+}
+
+// Both record arrays are read in one pass (their loads in flight together)
+// before the halt flag is looked at, so the kernel waits for memory once.
 __global__ void k_cg_scalars(const double* __restrict__ partials, const double* __restrict__ partials_rr,
                              int nblocks, CGScalars* sc) {
+  double d3[3], d1[1];
+  reduce_records_pair<3, 1>(partials, partials_rr, nblocks, d3, d1);
   if (sc->halt) {
     if (threadIdx.x == 0) sc->go = 0;
     return;
   }
-  double d3[3], d1[1];
-  reduce_records<3>(partials, nblocks, d3);
-  reduce_records<1>(partials_rr, nblocks, d1);
-  if (threadIdx.x == 0) {
-    double pkp = d3[0], c_tr = d3[1], c_tt = d3[2], oldrsq = d1[0];
-    double a = -sc->rsq / pkp;
-    double rsq = oldrsq + 2.0 * a * c_tr + a * a * c_tt;
-    double b = rsq / oldrsq;
-    sc->oldrsq = oldrsq;
-    sc->actual_rsq = oldrsq;
-    sc->pkp = pkp;
-    sc->c_tr = c_tr;
-    sc->c_tt = c_tt;
-    sc->a = a;
-    sc->b = b;
-    sc->rsq = rsq;
-    sc->go = 1;
-    sc->iters += 1;
-    if (rsq > sc->rsq_max) sc->rsq_max = rsq;
-    if (!sc->rel_mode && rsq < sc->rsq_stop) sc->halt = 1;
-  }
+  if (threadIdx.x == 0) cg_scalar_step(d3, d1, sc);
 }
 
 // With a relative-residual target the halt test waits for the update: the

@@ -5,8 +5,7 @@
 // fermion_force_asqtad_gpu.c:22-69:
 //   O1_mu(x) = sum_t c1_t Z_t(x+mu)  Z_t(x)^dag
 //   O3_mu(x) = sum_t c3_t Z_t(x+3mu) Z_t(x)^dag
-// over all sites, with Z_t given on both parities.  One thread per
-// (site, direction); the two accumulators stay in registers across terms.
+// over all sites, with Z_t given on both parities.
 #include <vector>
 
 #include "mh_internal.h"
@@ -21,12 +20,19 @@ struct TermSet {
   double c3[OP_MAX];
 };
 
+// Four lanes per site, one per direction mu (lane = 4 * site + mu), 64 sites
+// of one parity per block; the blocks of a site range are dealt to one XCD
+// with both parities side by side (xcd_block), so the here-vector Z_t(x) is
+// one request for the four lanes and the x/y neighbours come from that XCD's
+// L2.  Each lane keeps its O1_mu and O3_mu (two FP64 3x3 accumulators) over
+// all terms and writes them once.
 __global__ void __launch_bounds__(256) k_oprod(Geom g, TermSet ts, int nterms, LinkField o1, LinkField o3) {
-  int bx, pm;
-  if (!xcd_block((g.Vh + blockDim.x - 1) / blockDim.x, 8, bx, pm)) return;
-  int cb = bx * blockDim.x + threadIdx.x;
+  const int nbx = (g.Vh + 63) / 64;
+  int bx, pb;
+  if (!xcd_block(nbx, 2, bx, pb)) return;
+  const int mu = threadIdx.x & 3;
+  const int cb = bx * 64 + (threadIdx.x >> 2);
   if (cb >= g.Vh) return;
-  int pb = pm >> 2, mu = pm & 3;
   Coord c = site_coords(g, cb, pb);
   int n1 = neighbour(g, c, cb, mu, 1);
   int n3 = neighbour(g, c, cb, mu, 3);
@@ -58,6 +64,7 @@ __global__ void __launch_bounds__(256) k_oprod(Geom g, TermSet ts, int nterms, L
         a3.e[i][j].im += w3 * im3;
       }
   }
+  const int pm = pb * 4 + mu;
   double2* p1 = (double2*)o1.data + (size_t)(pm * 9) * o1.stride + cb;
   double2* p3 = (double2*)o3.data + (size_t)(pm * 9) * o3.stride + cb;
 #pragma unroll
@@ -79,7 +86,7 @@ int force_oprod_coeff(const std::vector<VecField>& xe, const std::vector<VecFiel
     ts.c1[t] = c1[t];
     ts.c3[t] = c3[t];
   }
-  hipLaunchKernelGGL(k_oprod, xcd_grid((c.Vh + 255) / 256, 8), dim3(256), 0, c.stream, c.geom, ts, n, o1, o3);
+  hipLaunchKernelGGL(k_oprod, xcd_grid((c.Vh + 63) / 64, 2), dim3(256), 0, c.stream, c.geom, ts, n, o1, o3);
   MH_CHECK(hipGetLastError());
   return 0;
 }

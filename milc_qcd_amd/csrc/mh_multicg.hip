@@ -75,7 +75,7 @@ __global__ void __launch_bounds__(256) k_ms_dslash1(Geom g, LinkField fat, LinkF
   if (cb >= g.Vh) return;
   Vec3<typename PrecTraits<P>::Real> acc;
   dslash_site<P>(g, fat, lng, src, cb, pb, acc);
-  store_vec<P>(dst, cb, acc);
+  store_vec_nt<P>(dst, cb, acc);
 }
 
 // ttt = D tmp + shift0 * cg_p ; pkp = <cg_p, ttt>   (ks_multicg_offset.c:296-318)
@@ -99,7 +99,7 @@ __global__ void __launch_bounds__(256) k_ms_dslash2(Geom g, LinkField fat, LinkF
       acc.c[k].im = acc.c[k].im + shift0 * pv.c[k].im;
       s[0] += (double)pv.c[k].re * (double)acc.c[k].re + (double)pv.c[k].im * (double)acc.c[k].im;
     }
-    store_vec<P>(ttt, cb, acc);
+    store_vec_nt<P>(ttt, cb, acc);
   }
   block_sum4<1>(s, partials);
 }
