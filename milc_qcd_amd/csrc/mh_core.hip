@@ -20,6 +20,12 @@ namespace mh {
 static Context g_ctx;
 Context& ctx() { return g_ctx; }
 
+// Storage of r_s, x_s and M^dag M p_s in the 16-bit mixed CG: FP32.  Keeping
+// them in 16-bit as well saves 40 B per site and iteration but was measured
+// slower at 32^3x64, m = 0.01 (95.8 against 94.6 ms per solve on one box):
+// the 16-bit residual recurrence needs 369-401 iterations instead of 365-368.
+static constexpr int kH16Arith = P_F32;
+
 size_t vec_bytes(int prec, int stride) { return (size_t)vec_site_bytes(prec) * stride; }
 size_t link_bytes(int prec, int stride) { return (size_t)8 * link_site_bytes(prec) * stride; }
 
@@ -860,14 +866,15 @@ __global__ void __launch_bounds__(256) k_cg_update(VecField x, VecField p, VecFi
   block_sum_store<1 + REL>(s, partials_rr);
 }
 
-// Reliable update, vector part: x_hi += x_s; x_s = 0
+// Reliable update, vector part: x_hi += x_s; x_s = 0 (x_s in precision PA)
+template <int PA>
 __global__ void k_accum_x(VecField xhi, VecField xs, int Vh) {
   int cb = blockIdx.x * blockDim.x + threadIdx.x;
   if (cb >= Vh) return;
   Vec3<double> a;
   Vec3<float> b, z;
   load_vec<P_F64>(xhi, cb, a);
-  load_vec<P_F32>(xs, cb, b);
+  load_vec<PA>(xs, cb, b);
 #pragma unroll
   for (int k = 0; k < 3; k++) {
     a.c[k].re += (double)b.c[k].re;
@@ -875,11 +882,12 @@ __global__ void k_accum_x(VecField xhi, VecField xs, int Vh) {
     z.c[k].re = z.c[k].im = 0.f;
   }
   store_vec<P_F64>(xhi, cb, a);
-  store_vec<P_F32>(xs, cb, z);
+  store_vec<PA>(xs, cb, z);
 }
 // Reliable update, sloppy vectors: p_s += r_hi - r_s; r_s = r_hi, so that p_s
-// equals r_hi + b p_old as if the true residual had been used in the last step.
-template <int PS>
+// equals r_hi + b p_old as if the true residual had been used in the last step
+// (r_s in precision PA, p_s in PS).
+template <int PS, int PA>
 __global__ void k_ru_vectors(VecField rhi, VecField rs, VecField ps, int Vh) {
   int cb = blockIdx.x * blockDim.x + threadIdx.x;
   if (cb >= Vh) return;
@@ -887,7 +895,7 @@ __global__ void k_ru_vectors(VecField rhi, VecField rs, VecField ps, int Vh) {
   Vec3<float> r;
   Vec3<typename PrecTraits<PS>::Real> p;
   load_vec<P_F64>(rhi, cb, rh);
-  load_vec<P_F32>(rs, cb, r);
+  load_vec<PA>(rs, cb, r);
   load_vec<PS>(ps, cb, p);
 #pragma unroll
   for (int k = 0; k < 3; k++) {
@@ -896,7 +904,7 @@ __global__ void k_ru_vectors(VecField rhi, VecField rs, VecField ps, int Vh) {
     r.c[k].re = (float)rh.c[k].re;
     r.c[k].im = (float)rh.c[k].im;
   }
-  store_vec<P_F32>(rs, cb, r);
+  store_vec<PA>(rs, cb, r);
   store_vec<PS>(ps, cb, p);
 }
 
@@ -1102,7 +1110,9 @@ static int cg_solve_f64(const VecField& src, VecField& x, double mass, const Sol
 // iterations with the FEWSUMS recurrence, FP64 reliable updates when the
 // sloppy residual has fallen by delta (or reached the target), FP64 true
 // residual (and relative residual, when asked for) for the stopping test.
-template <int PS>
+// PA: precision of the sloppy solution, residual and M^dag M p (r_s, x_s,
+// ttt_s).
+template <int PS, int PA>
 static int cg_solve_mixed(const VecField& src, VecField& x, double mass, const SolveParams& sp, SolveStats& st) {
   Context& c = ctx();
   const DevLinks& LS = c.links[PS];
@@ -1125,8 +1135,8 @@ static int cg_solve_mixed(const VecField& src, VecField& x, double mass, const S
   VecScope owned{&rhi, &tmp64, &ps, &tmps, &ttts, &rs, &xs};
   if ((rc = alloc_vec(rhi, P_F64, c.stride)) || (rc = alloc_vec(tmp64, P_F64, c.stride)) ||
       (rc = alloc_vec(ps, PS, c.stride)) || (rc = alloc_vec(tmps, PS, c.stride)) ||
-      (rc = alloc_vec(ttts, P_F32, c.stride)) || (rc = alloc_vec(rs, P_F32, c.stride)) ||
-      (rc = alloc_vec(xs, P_F32, c.stride)))
+      (rc = alloc_vec(ttts, PA, c.stride)) || (rc = alloc_vec(rs, PA, c.stride)) ||
+      (rc = alloc_vec(xs, PA, c.stride)))
     return rc;
   CGScalars h;
   memset(&h, 0, sizeof h);
@@ -1143,12 +1153,12 @@ static int cg_solve_mixed(const VecField& src, VecField& x, double mass, const S
   };
   for (;;) {
     // reliable update (or the initial residual): FP64 true residual
-    if (!first) hipLaunchKernelGGL(k_accum_x, dim3(nb), dim3(256), 0, c.stream, x, xs, c.Vh);
+    if (!first) hipLaunchKernelGGL(k_accum_x<PA>, dim3(nb), dim3(256), 0, c.stream, x, xs, c.Vh);
     if ((rc = true_residual(src, x, tmp64, rhi, dummy, 0, pb, msq_x4))) break;
     if (first) {
-      if ((rc = convert_vec(rhi, P_F64, rs, P_F32)) || (rc = convert_vec(rhi, P_F64, ps, PS))) break;
+      if ((rc = convert_vec(rhi, P_F64, rs, PA)) || (rc = convert_vec(rhi, P_F64, ps, PS))) break;
     } else {
-      hipLaunchKernelGGL(k_ru_vectors<PS>, dim3(nb), dim3(256), 0, c.stream, rhi, rs, ps, c.Vh);
+      hipLaunchKernelGGL((k_ru_vectors<PS, PA>), dim3(nb), dim3(256), 0, c.stream, rhi, rs, ps, c.Vh);
       nru++;
     }
     first = false;
@@ -1169,7 +1179,7 @@ static int cg_solve_mixed(const VecField& src, VecField& x, double mass, const S
     long left = max_cg - iteration;
     while (left > 0) {
       int n = (int)std::min<long>(batch, left);
-      for (int k = 0; k < n; k++) enqueue_cg_iteration<PS, PS, P_F32>(LS, ps, tmps, rs, ttts, xs, pb, msq_x4);
+      for (int k = 0; k < n; k++) enqueue_cg_iteration<PS, PS, PA>(LS, ps, tmps, rs, ttts, xs, pb, msq_x4);
       MH_CHECK(hipGetLastError());
       int before = h.iters;
       if ((rc = read_scalars(&h))) break;
@@ -1194,15 +1204,15 @@ static int cg_solve_mixed(const VecField& src, VecField& x, double mass, const S
 size_t cg_scratch_bytes(int ps, int stride) {
   auto a = [&](int prec) { return (vec_bytes(prec, stride) + 255) & ~(size_t)255; };
   if (ps == P_F64) return 4 * a(P_F64);
-  return 2 * a(P_F64) + 2 * a(ps) + 3 * a(P_F32);
+  return 2 * a(P_F64) + 2 * a(ps) + 3 * a(ps == P_H16 ? kH16Arith : P_F32);
 }
 
 int cg_solve(const VecField& src, VecField& x, double mass, const SolveParams& sp, SolveStats& st) {
   auto t0 = std::chrono::steady_clock::now();
   int rc;
   if (sp.sloppy_prec == P_F64) rc = cg_solve_f64(src, x, mass, sp, st);
-  else if (sp.sloppy_prec == P_F32) rc = cg_solve_mixed<P_F32>(src, x, mass, sp, st);
-  else rc = cg_solve_mixed<P_H16>(src, x, mass, sp, st);
+  else if (sp.sloppy_prec == P_F32) rc = cg_solve_mixed<P_F32, P_F32>(src, x, mass, sp, st);
+  else rc = cg_solve_mixed<P_H16, kH16Arith>(src, x, mass, sp, st);
   (void)hipStreamSynchronize(ctx().stream);
   st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return rc;

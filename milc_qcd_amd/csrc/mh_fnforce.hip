@@ -527,23 +527,6 @@ __global__ void __launch_bounds__(128) k_sf_init(Geom g, MF F1, int with_eps, MF
   if (c1 != 0.0) mfa(G, pb, cb, c1, m);
 }
 
-// out = c in
-__global__ void __launch_bounds__(128) k_sf_scale(Geom g, MF in, double c, MF out) {
-  int cb = blockIdx.x * blockDim.x + threadIdx.x;
-  int pb = blockIdx.y;
-  if (cb >= g.Vh) return;
-  Mat3<double> m;
-  mfl(in, pb, cb, m);
-#pragma unroll
-  for (int i = 0; i < 3; i++)
-#pragma unroll
-    for (int j = 0; j < 3; j++) {
-      m.e[i][j].re *= c;
-      m.e[i][j].im *= c;
-    }
-  mfs(out, pb, cb, m);
-}
-
 // out = S_nu(A) for the mu link
 __global__ void __launch_bounds__(128) k_sf_fwd(Geom g, LinkField U, int mu, int nu, MF A, MF out) {
   int bx, pb;
@@ -574,8 +557,11 @@ __global__ void __launch_bounds__(128) k_sf_fwd(Geom g, LinkField U, int mu, int
 //   FA(y)    += cg [U_nu(y-nu+mu)^dag G(y-nu) U_nu(y-nu) + U_nu(y+mu) G(y+nu) U_nu(y)^dag]
 //   FU_nu(y) += cg [A(y+nu) U_nu(y+mu)^dag G(y) + (G(y-mu) U_nu(y-mu) A(y-mu+nu))^dag
 //                   + (A(y) U_nu(y+mu) G(y+nu))^dag + G(y-mu+nu) U_nu(y-mu)^dag A(y-mu)]
+// With fa_init the FA term is not accumulated but initialised:
+//   FA(y) = ci FI(y) + cg [...]
+// which stands for a preceding FA = ci FI pass (the scaled staple force).
 __global__ void __launch_bounds__(128) k_sf_adj(Geom g, LinkField U, int mu, int nu, MF A, MF Gs, double cg, MF FA,
-                                                int want_fa, MF FU) {
+                                                int want_fa, MF FU, int fa_init, MF FI, double ci) {
   int bx, pb;
   if (!xcd_block((g.Vh + blockDim.x - 1) / blockDim.x, 2, bx, pb)) return;
   int cb = bx * blockDim.x + threadIdx.x;
@@ -600,7 +586,20 @@ __global__ void __launch_bounds__(128) k_sf_adj(Geom g, LinkField U, int mu, int
     mload(U, pb, nu, cb, u);
     mmul_na(t, u, a);
     madd(acc, a);
-    mfa(FA, pb, cb, cg, acc);
+    if (fa_init) {
+      mfl(FI, pb, cb, a);
+#pragma unroll
+      for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+          double re = ci * a.e[i][j].re, im = ci * a.e[i][j].im;
+          a.e[i][j].re = re + cg * acc.e[i][j].re;
+          a.e[i][j].im = im + cg * acc.e[i][j].im;
+        }
+      mfs(FA, pb, cb, a);
+    } else {
+      mfa(FA, pb, cb, cg, acc);
+    }
   }
   // A(y+nu) U_nu(y+mu)^dag G(y)
   mfl(A, po, ypn, a);
@@ -1347,6 +1346,14 @@ static int smear_adjoint_dev(const LinkField& L, const LinkField& F1, const Link
   const bool staples = c3 != 0.0 || cl != 0.0 || c5 != 0.0 || c7 != 0.0;
   // init and scale: plain grid; the neighbour-reading kernels: XCD block order
   dim3 grid((c.Vh + 127) / 128, 2), xgrid = xcd_grid((c.Vh + 127) / 128, 2), blk(128);
+  // The scaled copies G1 = c3 F and G2 = c5 F are never formed: the first
+  // adjoint pass into G1 (G2) initialises it as c3 F (c5 F) plus its own
+  // term, and where nothing accumulates into it, its reader takes F with the
+  // coefficient folded into cg.
+  auto adj = [&](int mu, int nu, const MF& A, const MF& Gs, double cg, const MF& FA, const MF& FU, int init,
+                 double ci) {
+    hipLaunchKernelGGL(k_sf_adj, xgrid, blk, 0, c.stream, c.geom, L, mu, nu, A, Gs, cg, FA, 1, FU, init, mF, ci);
+  };
   for (int mu = 0; mu < 4; mu++) {
     const MF Gmu = mf_link(G, mu);
     hipLaunchKernelGGL(k_sf_init, grid, blk, 0, c.stream, c.geom, mf_link(F1, mu), with_eps, mF, Gmu, c1);
@@ -1354,27 +1361,28 @@ static int smear_adjoint_dev(const LinkField& L, const LinkField& F1, const Link
       for (int nu = 0; nu < 4; nu++) {
         if (nu == mu) continue;
         hipLaunchKernelGGL(k_sf_fwd, xgrid, blk, 0, c.stream, c.geom, L, mu, nu, mf_link(L, mu), mS1);
-        hipLaunchKernelGGL(k_sf_scale, grid, blk, 0, c.stream, c.geom, mF, c3, mG1);
-        if (cl != 0.0)
-          hipLaunchKernelGGL(k_sf_adj, xgrid, blk, 0, c.stream, c.geom, L, mu, nu, mS1, mF, cl, mG1, 1,
-                             mf_link(G, nu));
+        bool g1_set = false;
+        if (cl != 0.0) {
+          adj(mu, nu, mS1, mF, cl, mG1, mf_link(G, nu), 1, c3);
+          g1_set = true;
+        }
         if (c5 != 0.0 || c7 != 0.0) {
           for (int rho = 0; rho < 4; rho++) {
             if (rho == mu || rho == nu) continue;
             hipLaunchKernelGGL(k_sf_fwd, xgrid, blk, 0, c.stream, c.geom, L, mu, rho, mS1, mS2);
-            hipLaunchKernelGGL(k_sf_scale, grid, blk, 0, c.stream, c.geom, mF, c5, mG2);
+            // G2 = c5 F + the 7-staple terms (one sigma in four dimensions)
+            bool g2_set = false;
             if (c7 != 0.0)
               for (int sig = 0; sig < 4; sig++) {
                 if (sig == mu || sig == nu || sig == rho) continue;
-                hipLaunchKernelGGL(k_sf_adj, xgrid, blk, 0, c.stream, c.geom, L, mu, sig, mS2, mF, c7, mG2, 1,
-                                   mf_link(G, sig));
+                adj(mu, sig, mS2, mF, c7, mG2, mf_link(G, sig), g2_set ? 0 : 1, c5);
+                g2_set = true;
               }
-            hipLaunchKernelGGL(k_sf_adj, xgrid, blk, 0, c.stream, c.geom, L, mu, rho, mS1, mG2, 1.0, mG1, 1,
-                               mf_link(G, rho));
+            adj(mu, rho, mS1, g2_set ? mG2 : mF, g2_set ? 1.0 : c5, mG1, mf_link(G, rho), g1_set ? 0 : 1, c3);
+            g1_set = true;
           }
         }
-        hipLaunchKernelGGL(k_sf_adj, xgrid, blk, 0, c.stream, c.geom, L, mu, nu, mf_link(L, mu), mG1, 1.0, Gmu, 1,
-                           mf_link(G, nu));
+        adj(mu, nu, mf_link(L, mu), g1_set ? mG1 : mF, g1_set ? 1.0 : c3, Gmu, mf_link(G, nu), 0, 0.0);
       }
     }
     if (cN != 0.0)
