@@ -1026,6 +1026,23 @@ static void enqueue_cg_iteration(const DevLinks& L, VecField& p, VecField& tmp, 
   }
 }
 
+// Iterations to enqueue before the next host check.  The recurrence residual
+// of a CG phase falls roughly geometrically, so from the rate of the batches
+// so far (log rsq per iteration, < 0) the iterations still needed to reach
+// rsq_stop are estimated; one more than that is enqueued, between 4 and 64,
+// so a phase ends after few host round trips and few iterations that return
+// at once after the halt.  Without a usable rate the default batch is kept.
+static int next_batch(int dflt, double log_rate, double rsq_now, double rsq_stop) {
+  if (!(log_rate < 0) || !(rsq_now > 0) || !(rsq_stop > 0)) return dflt;
+  double need = std::log(rsq_stop / rsq_now) / log_rate;
+  if (!(need > 0)) return 4;
+  return (int)std::max(4.0, std::min(64.0, std::ceil(need) + 1.0));
+}
+static double batch_rate(double log_rate, double rsq_before, double rsq_now, int done_now) {
+  if (done_now <= 0 || !(rsq_now > 0) || !(rsq_before > rsq_now)) return log_rate;
+  return std::log(rsq_now / rsq_before) / done_now;
+}
+
 // Plain FP64 CG, MILC's restart/iteration semantics (d_congrad5_fn_milc.c:60-407),
 // with the optional Fermilab relative residual: its target is met when
 // my_relative_residue < relresid^2 (relrsqmin, :92), tested beside the
@@ -1065,6 +1082,7 @@ static int cg_solve_f64(const VecField& src, VecField& x, double mass, const Sol
   int nrestart = 0;
   double final_rsq = 0, final_relrsq = 0, size_r = 0;
   int batch = sp.batch > 0 ? sp.batch : 16;
+  double log_rate = 0.0;  // log of the recurrence residual's fall per iteration
   auto met = [&](double r2, double rel2) {
     return (rsqmin <= 0 || rsqmin > r2) && (relrsqmin <= 0 || relrsqmin > rel2);
   };
@@ -1081,17 +1099,21 @@ static int cg_solve_f64(const VecField& src, VecField& x, double mass, const Sol
     long left = niter - (iteration % niter);
     if (left == niter) left = 0;
     left = std::min(left, max_cg - iteration);
+    int nb_next = next_batch(batch, log_rate, h.rsq, h.rsq_stop);
     while (left > 0) {
-      int nb = (int)std::min<long>(batch, left);
+      int nb = (int)std::min<long>(nb_next, left);
       for (int k = 0; k < nb; k++) enqueue_cg_iteration<P_F64, P_F64, P_F64>(L, p, tmp, r, ttt, x, pb, msq_x4, rel);
       MH_CHECK(hipGetLastError());
       int before = h.iters;
+      double rsq_before = h.rsq;
       if ((rc = read_scalars(&h))) break;
       int done_now = h.iters - before;
       iteration += done_now;
       left -= done_now;
       size_r = h.rsq / source_norm;
       if (h.halt) break;
+      log_rate = batch_rate(log_rate, rsq_before, h.rsq, done_now);
+      nb_next = next_batch(batch, log_rate, h.rsq, h.rsq_stop);
     }
     if (rc) break;
   }
@@ -1147,6 +1169,7 @@ static int cg_solve_mixed(const VecField& src, VecField& x, double mass, const S
   int nru = 0;
   double final_rsq = 0, final_relrsq = 0;
   int batch = sp.batch > 0 ? sp.batch : 16;
+  double log_rate = 0.0;  // log of the recurrence residual's fall per iteration
   bool first = true;
   auto met = [&](double r2, double rel2) {
     return (rsqmin <= 0 || rsqmin > r2) && (relrsqmin <= 0 || relrsqmin > rel2);
@@ -1177,16 +1200,20 @@ static int cg_solve_mixed(const VecField& src, VecField& x, double mass, const S
     h.iters = 0;
     if ((rc = write_scalars(&h))) break;
     long left = max_cg - iteration;
+    int n_next = next_batch(batch, log_rate, h.rsq, h.rsq_stop);
     while (left > 0) {
-      int n = (int)std::min<long>(batch, left);
+      int n = (int)std::min<long>(n_next, left);
       for (int k = 0; k < n; k++) enqueue_cg_iteration<PS, PS, PA>(LS, ps, tmps, rs, ttts, xs, pb, msq_x4);
       MH_CHECK(hipGetLastError());
       int before = h.iters;
+      double rsq_before = h.rsq;
       if ((rc = read_scalars(&h))) break;
       int done_now = h.iters - before;
       iteration += done_now;
       left -= done_now;
       if (h.halt) break;
+      log_rate = batch_rate(log_rate, rsq_before, h.rsq, done_now);
+      n_next = next_batch(batch, log_rate, h.rsq, h.rsq_stop);
     }
     if (rc) break;
   }
