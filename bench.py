@@ -49,8 +49,8 @@ HBM_PEAK_GBS = 8000.0
 # algorithmic bytes per output site of one single-parity Dslash:
 # 16 link matrices + 1 source + 1 destination vector
 DSLASH_BYTES = {0: 16 * 144 + 48 + 48, 1: 16 * 72 + 24 + 24, 2: 16 * 36 + 16 + 16}
-# 16-bit stencil with the Naik links in the two-row form (rows 0, 1 and z: 28 B)
-DSLASH_BYTES_H16_TWO_ROW = 8 * 36 + 8 * 28 + 16 + 16
+# 16-bit stencil with the Naik links in the two-row form (rows 0, 1 and the det phase: 26 B)
+DSLASH_BYTES_H16_TWO_ROW = 8 * 36 + 8 * 26 + 16 + 16
 DSLASH_FLOPS = 1146
 
 

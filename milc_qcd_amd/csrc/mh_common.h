@@ -235,12 +235,13 @@ struct VecField {
 // A link field (fat or long) for both parities and all four directions.
 // H16 with recon = 1 (long links that are a real multiple of a U(3) matrix,
 // L = c U): rows 0 and 1 are stored, row 2 = z conj(row0 x row1) with the
-// per-link z = det(U) / c, i.e. 28 bytes per link instead of 36.
+// per-link z = det(U) / c kept as its phase (16 bits; the modulus 1/c follows
+// from the rows), i.e. 26 bytes per link instead of 36.
 struct LinkField {
   void* data = nullptr;
   float scale = 1.0f;  // H16: value = q * scale / 32767
   int stride = 0;
-  float zscale = 1.0f; // H16 recon: z = q * zscale / 32767
+  float zscale = 1.0f; // H16 recon: largest |Re z|, |Im z| (reported only)
   int recon = 0;
 };
 
@@ -404,21 +405,31 @@ __device__ __forceinline__ void load_link_h16f(const LinkField& f, int pb, int m
 }
 
 // 16-bit long link in the two-row form: planes of int4 (row-0 elements and
-// element (1,0)), int2 (elements (1,1), (1,2)) and int (z), each plane
-// 8 * stride records; row 2 = z conj(row0 x row1).
+// element (1,0)), int2 (elements (1,1), (1,2)) and short (the phase of z),
+// each plane 8 * stride records; row 2 = z conj(row0 x row1) with
+// z = exp(i phi) / c, phi = q pi / 32767 and 1/c = (|row0|^2 + |row1|^2)/2)^-1/2
+// (the rows of L = c U both have norm c): 26 bytes per link.
 __device__ __forceinline__ void load_link_h16r(const LinkField& f, int pb, int mu, int cb, Mat3<float>& m) {
   const size_t S = (size_t)f.stride, k = (size_t)(pb * 4 + mu) * S + cb;
   const int4* p4 = (const int4*)f.data;
   const int2* p2 = (const int2*)(p4 + 8 * S);
-  const int* p1 = (const int*)(p2 + 8 * S);
+  const short* ph = (const short*)(p2 + 8 * S);
   nat_i4 a = ldnt(p4 + k);
   nat_i2 b = ldnt(p2 + k);
-  int zq = ldnt(p1 + k);
-  const float s = f.scale * (1.0f / 32767.0f), zs = f.zscale * (1.0f / 32767.0f);
+  const short zq = __builtin_nontemporal_load(ph + k);
+  const float s = f.scale * (1.0f / 32767.0f);
   const int w[6] = {a.x, a.y, a.z, a.w, b.x, b.y};
+  float n2 = 0.f;
 #pragma unroll
-  for (int e = 0; e < 6; e++) { m.e[e / 3][e % 3].re = q16lo(w[e]) * s; m.e[e / 3][e % 3].im = q16hi(w[e]) * s; }
-  const float zr = q16lo(zq) * zs, zi = q16hi(zq) * zs;
+  for (int e = 0; e < 6; e++) {
+    m.e[e / 3][e % 3].re = q16lo(w[e]) * s;
+    m.e[e / 3][e % 3].im = q16hi(w[e]) * s;
+    n2 += m.e[e / 3][e % 3].re * m.e[e / 3][e % 3].re + m.e[e / 3][e % 3].im * m.e[e / 3][e % 3].im;
+  }
+  const float inv_c = __frsqrt_rn(0.5f * n2);
+  float sn, cs;
+  __sincosf((float)zq * (3.14159265358979f / 32767.0f), &sn, &cs);
+  const float zr = cs * inv_c, zi = sn * inv_c;
 #pragma unroll
   for (int k3 = 0; k3 < 3; k3++) {
     const int i = (k3 + 1) % 3, j = (k3 + 2) % 3;

@@ -437,10 +437,14 @@ __global__ void k_store_h16r(LinkField in, LinkField out, int Vh, float inv_scal
   const size_t S = (size_t)out.stride, k = (size_t)pm * S + cb;
   int4* p4 = (int4*)out.data;
   int2* p2 = (int2*)(p4 + 8 * S);
-  int* p1 = (int*)(p2 + 8 * S);
+  short* ph = (short*)(p2 + 8 * S);
   p4[k] = make_int4(w[0], w[1], w[2], w[3]);
   p2[k] = make_int2(w[4], w[5]);
-  p1[k] = q16pack((float)(zr * inv_zscale), (float)(zi * inv_zscale));
+  // z = det(U) / c: its modulus 1/c follows from the stored rows, its phase
+  // arg det(U) is kept as 16-bit fixed point in units of pi / 32767
+  (void)inv_zscale;
+  const double phi = atan2(zi, zr);
+  ph[k] = (short)__double2int_rn(fmin(fmax(phi * (32767.0 / M_PI), -32767.0), 32767.0));
 }
 
 template <int PI, int PO>
@@ -1029,14 +1033,16 @@ static void enqueue_cg_iteration(const DevLinks& L, VecField& p, VecField& tmp, 
 // Iterations to enqueue before the next host check.  The recurrence residual
 // of a CG phase falls roughly geometrically, so from the rate of the batches
 // so far (log rsq per iteration, < 0) the iterations still needed to reach
-// rsq_stop are estimated; one more than that is enqueued, between 4 and 64,
-// so a phase ends after few host round trips and few iterations that return
-// at once after the halt.  Without a usable rate the default batch is kept.
-static int next_batch(int dflt, double log_rate, double rsq_now, double rsq_stop) {
-  if (!(log_rate < 0) || !(rsq_now > 0) || !(rsq_stop > 0)) return dflt;
+// rsq_stop are estimated.  Three quarters of that (plus one) is enqueued,
+// between 4 and `cap`: an iteration queued after the halt still costs its
+// four dispatches (about 13 us at 32^3x64), an extra host check about twice
+// that, and the residual's fall is not smooth, so the estimate is taken low.
+// Without a usable rate the default batch is kept.
+static int next_batch(int dflt, double log_rate, double rsq_now, double rsq_stop, int cap = 64) {
+  if (!(log_rate < 0) || !(rsq_now > 0) || !(rsq_stop > 0)) return std::min(dflt, cap);
   double need = std::log(rsq_stop / rsq_now) / log_rate;
   if (!(need > 0)) return 4;
-  return (int)std::max(4.0, std::min(64.0, std::ceil(need) + 1.0));
+  return (int)std::max(4.0, std::min((double)cap, std::ceil(0.75 * need) + 1.0));
 }
 static double batch_rate(double log_rate, double rsq_before, double rsq_now, int done_now) {
   if (done_now <= 0 || !(rsq_now > 0) || !(rsq_before > rsq_now)) return log_rate;
@@ -1099,7 +1105,7 @@ static int cg_solve_f64(const VecField& src, VecField& x, double mass, const Sol
     long left = niter - (iteration % niter);
     if (left == niter) left = 0;
     left = std::min(left, max_cg - iteration);
-    int nb_next = next_batch(batch, log_rate, h.rsq, h.rsq_stop);
+    int nb_next = next_batch(batch, log_rate, h.rsq, h.rsq_stop, 16);
     while (left > 0) {
       int nb = (int)std::min<long>(nb_next, left);
       for (int k = 0; k < nb; k++) enqueue_cg_iteration<P_F64, P_F64, P_F64>(L, p, tmp, r, ttt, x, pb, msq_x4, rel);
@@ -1200,7 +1206,7 @@ static int cg_solve_mixed(const VecField& src, VecField& x, double mass, const S
     h.iters = 0;
     if ((rc = write_scalars(&h))) break;
     long left = max_cg - iteration;
-    int n_next = next_batch(batch, log_rate, h.rsq, h.rsq_stop);
+    int n_next = next_batch(batch, log_rate, h.rsq, h.rsq_stop, 16);
     while (left > 0) {
       int n = (int)std::min<long>(n_next, left);
       for (int k = 0; k < n; k++) enqueue_cg_iteration<PS, PS, PA>(LS, ps, tmps, rs, ttts, xs, pb, msq_x4);
