@@ -16,6 +16,8 @@ kernel (the 16-bit Dslash) and the CPU baseline (the FP64 oracle CG timed on
 this host's cores over a bounded number of iterations).
 
 Usage: python bench.py [--gpus 1] [--steps 10] [--warmup 1] [--no-cpu]
+       python bench.py --workload c1   (the 8^4, m = 0.05 case: FP64 and mixed GPU solves to 1e-8
+           against MILC's compiled CG on the same links; prints its own JSON line)
        python bench.py --workload multishift   (batched RHMC multi-shift: 24^3 x 48,
            m = 0.02, 12 shifts sigma_i = 1e-4 10^(4i/11), 16 sources, per-shift
            FP64 true residual 1e-9; prints its own JSON line)
@@ -629,6 +631,89 @@ def io_workload(args):
     print(json.dumps(out), flush=True)
 
 
+C1_DIMS = (8, 8, 8, 8)
+C1_MASS = 0.05
+C1_RESID = 1e-8
+
+
+def c1_workload(args):
+    """The CPU-runnable case: 8^4, m = 0.05, U = exp(0.4 i H) from seed 1001,
+    HISQ links built on the device, even-site Gaussian sources 2100 + j,
+    (M^dag M) x = b to 1e-8.  FP64 and 16-bit mixed solves on the GPU, the
+    FP64 solution against the oracle-grade FP64 solve of MILC's own CG
+    (compiled reference, oracle/_ref) on the same links and source, and that
+    CG timed on this host's cores.  The lattice (4096 sites per parity) lives in
+    cache: the GPU times are launch latency, not bandwidth."""
+    import milc_qcd_amd as mq
+    G = mq.Lattice(C1_DIMS)
+    G.dev_gauge_hisq(GAUGE_SEED, 0.4)
+    for j in range(4):
+        G.dev_source(SRC_SEEDS[j], mq.EVEN, j)
+    res = {}
+    for mixed, name in ((0, "fp64"), (2, "mixed16")):
+        for i in range(args.warmup):
+            G.dev_invert(0, 10 + (mixed > 0), C1_MASS, mq.EVEN, C1_RESID, mixed)
+        its, rr = [], []
+        t1 = time.perf_counter()
+        for i in range(args.steps):
+            it, st = G.dev_invert(i % 4, 10 + (mixed > 0), C1_MASS, mq.EVEN, C1_RESID, mixed)
+            its.append(it)
+            rr.append(st["resid"])
+        res[name] = {"s_per_solve": (time.perf_counter() - t1) / args.steps, "iters": its, "true_rel_resid": max(rr)}
+        log(f"{name}: {res[name]}")
+    # both GPU solutions of the last source against each other
+    err_mixed = G.dev_diff(11, 10)
+    cpu, err_fp64 = None, None
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import milc_ref as mr
+    if mr.available():
+        with stdout_to_stderr():
+            mr.init(C1_DIMS)
+            fat, lng = G.dev_read_links()
+            mr.set_fn_links(fat, lng)
+            last = (args.steps - 1) % 4
+            src = G.dev_read(last, mq.EVEN)
+            x_gpu = G.dev_read(10, mq.EVEN)
+            t1 = time.perf_counter()
+            reps = 5
+            for _ in range(reps):
+                x_ref, it_ref, _ = mr.congrad(src, C1_MASS, C1_RESID, mr.EVEN, 1000, 5)
+            dt = (time.perf_counter() - t1) / reps
+        Vh = G.Vh
+        err_fp64 = float(np.linalg.norm(x_gpu[:Vh] - x_ref[:Vh]) / np.linalg.norm(x_ref[:Vh]))
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        cpu = {"value": dt, "unit": "s/solve", "cores": threads, "kind": "reference",
+               **host_info("oracle/_ref/libmilc_ref.so"),
+               "sample": f"MILC 7.8.1 ks_congrad_parity_cpu, full solve to 1e-8 ({it_ref} iterations), mean of {reps}"}
+    G.close()
+    out = {
+        "metric": "HISQ CG time-to-1e-8 on 8^4 (FP64 and mixed, 1 GPU; cache-resident)",
+        "value": round(res["mixed16"]["s_per_solve"], 6),
+        "unit": "s/solve",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(res["mixed16"]["s_per_solve"] * 1e3, 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "16-bit storage / f32 arithmetic / f64 reliable updates",
+        "data": "synthetic",
+        "config": {"workload": "hisq_cg_8x8x8x8_m0.05", "lattice": list(C1_DIMS), "mass": C1_MASS,
+                   "target_true_rel_resid": C1_RESID, "gauge_seed": GAUGE_SEED},
+        "fp64_s_per_solve": round(res["fp64"]["s_per_solve"], 6),
+        "iters_fp64": res["fp64"]["iters"],
+        "iters_mixed": res["mixed16"]["iters"],
+        "true_rel_resid_fp64": res["fp64"]["true_rel_resid"],
+        "true_rel_resid_mixed": res["mixed16"]["true_rel_resid"],
+        "rel_err_fp64_vs_milc_solution": err_fp64,
+        "rel_err_mixed_vs_fp64_solution": err_mixed,
+        "roofline": None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -637,7 +722,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--kernel-only", action="store_true",
                     help="only the Dslash and copy bursts (for rocprofv3 counter runs)")
-    ap.add_argument("--workload", choices=["cg", "multishift", "hmc", "force", "io", "trajectory"], default="cg")
+    ap.add_argument("--workload", choices=["cg", "c1", "multishift", "hmc", "force", "io", "trajectory"], default="cg")
     ap.add_argument("--no-sequential", action="store_true",
                     help="multishift: leave out the 16 one-source comparison solves")
     args = ap.parse_args()
@@ -645,6 +730,9 @@ def main():
         print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); "
               "the multi-GPU domain decomposition is a design, not a measured path", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "c1":
+        c1_workload(args)
+        return
     if args.workload == "multishift":
         multishift_workload(args)
         return
