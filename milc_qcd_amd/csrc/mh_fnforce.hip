@@ -470,6 +470,39 @@ __device__ __forceinline__ void mfa(const MF& f, int pb, int cb, double c, const
     p[(size_t)e * f.stride] = a;
   }
 }
+// Non-temporal forms for the own-site accumulators of the staple kernels,
+// which no other site reads in the same pass: their lines are not kept in L2,
+// leaving it to the neighbour-read links and forces.
+__device__ __forceinline__ void mfl_nt(const MF& f, int pb, int cb, Mat3<double>& m) {
+  const double2* p = f.data + (size_t)(pb * f.pstride + f.off) * f.stride + cb;
+#pragma unroll
+  for (int e = 0; e < 9; e++) {
+    const nat_d2 a = ldnt(p + (size_t)e * f.stride);
+    m.e[e / 3][e % 3].re = a.x;
+    m.e[e / 3][e % 3].im = a.y;
+  }
+}
+__device__ __forceinline__ void mfs_nt(const MF& f, int pb, int cb, const Mat3<double>& m) {
+  double2* p = f.data + (size_t)(pb * f.pstride + f.off) * f.stride + cb;
+#pragma unroll
+  for (int e = 0; e < 9; e++) {
+    nat_d2 a;
+    a.x = m.e[e / 3][e % 3].re;
+    a.y = m.e[e / 3][e % 3].im;
+    __builtin_nontemporal_store(a, (nat_d2*)(p + (size_t)e * f.stride));
+  }
+}
+// f += c m, non-temporal
+__device__ __forceinline__ void mfa_nt(const MF& f, int pb, int cb, double c, const Mat3<double>& m) {
+  double2* p = f.data + (size_t)(pb * f.pstride + f.off) * f.stride + cb;
+#pragma unroll
+  for (int e = 0; e < 9; e++) {
+    nat_d2 a = ldnt(p + (size_t)e * f.stride);
+    a.x += c * m.e[e / 3][e % 3].re;
+    a.y += c * m.e[e / 3][e % 3].im;
+    __builtin_nontemporal_store(a, (nat_d2*)(p + (size_t)e * f.stride));
+  }
+}
 __device__ __forceinline__ void mmul_na(const Mat3<double>& a, const Mat3<double>& b, Mat3<double>& c) {
   Mat3<double> t = b;
   madj(t);
@@ -527,27 +560,73 @@ __global__ void __launch_bounds__(128) k_sf_init(Geom g, MF F1, int with_eps, MF
   if (c1 != 0.0) mfa(G, pb, cb, c1, m);
 }
 
+// Sites of the staple kernels, with the two directions (MU, NU) fixed at
+// compile time: one coordinate decode, then every neighbour cb from
+// coordinates.  Site order: TT = 0 is the XCD-contiguous cb order of
+// xcd_block; TT = 1 (pairs with NU or MU = t) gives each wavefront 16
+// consecutive cb of four consecutive t-slices, so that the +-t neighbours of
+// three of every four sites are read by the same wavefront and the t-hop
+// rows come through L1/L2 instead of being fetched again from HBM (needs
+// st % 16 == 0 and nt % 4 == 0; the block is 128 threads, two wavefronts).
+struct SfSites {
+  int ypn, ymn, ypm, ymm, ymnpm, ymmpn;  // y+nu, y-nu, y+mu, y-mu, y-nu+mu, y-mu+nu
+};
+__device__ __forceinline__ int sf_wrap(int v, int L) { return v < 0 ? v + L : (v >= L ? v - L : v); }
+template <int MU, int NU>
+__device__ __forceinline__ SfSites sf_sites(const Geom& g, int cb, int pb) {
+  const Coord c = site_coords(g, cb, pb);
+  const int x[4] = {c.x, c.y, c.z, c.t};
+  auto at = [&](int dm, int dn) {
+    int v[4] = {x[0], x[1], x[2], x[3]};
+    v[MU] = sf_wrap(v[MU] + dm, g.X[MU]);
+    v[NU] = sf_wrap(v[NU] + dn, g.X[NU]);
+    return (v[0] + g.X[0] * (v[1] + g.X[1] * (v[2] + g.X[2] * v[3]))) >> 1;
+  };
+  SfSites n;
+  n.ypn = at(0, 1);
+  n.ymn = at(0, -1);
+  n.ypm = at(1, 0);
+  n.ymm = at(-1, 0);
+  n.ymnpm = at(1, -1);
+  n.ymmpn = at(-1, 1);
+  return n;
+}
+template <int TT>
+__device__ __forceinline__ bool sf_site(const Geom& g, int& cb, int& pb) {
+  int bx;
+  if (!xcd_block((g.Vh + 127) / 128, 2, bx, pb)) return false;
+  if (TT == 0) {
+    cb = bx * 128 + threadIdx.x;
+    return cb < g.Vh;
+  }
+  const int lane = threadIdx.x & 63;
+  const int w = bx * 2 + (threadIdx.x >> 6);  // wavefront index
+  const int ns = g.st >> 4;                   // 16-site chunks per t-slice
+  const int s = w % ns, t = (w / ns) * 4 + (lane >> 4);
+  cb = t * g.st + s * 16 + (lane & 15);
+  return t < g.X[3];
+}
+__host__ inline bool sf_tiled(const Geom& g, int mu, int nu) {
+  return (mu == 3 || nu == 3) && g.st % 16 == 0 && g.X[3] % 4 == 0;
+}
+
 // out = S_nu(A) for the mu link
-__global__ void __launch_bounds__(128) k_sf_fwd(Geom g, LinkField U, int mu, int nu, MF A, MF out) {
-  int bx, pb;
-  if (!xcd_block((g.Vh + blockDim.x - 1) / blockDim.x, 2, bx, pb)) return;
-  int cb = bx * blockDim.x + threadIdx.x;
-  if (cb >= g.Vh) return;
-  int po = pb ^ 1;
-  Coord c = site_coords(g, cb, pb);
-  int xn = neighbour(g, c, cb, nu, 1), xm = neighbour(g, c, cb, mu, 1), xd = neighbour(g, c, cb, nu, -1);
-  int dpb;
-  int xdm = hop2(g, cb, pb, nu, -1, mu, 1, dpb);  // x - nu + mu, parity pb
+template <int MU, int NU, int TT>
+__global__ void __launch_bounds__(128) k_sf_fwd(Geom g, LinkField U, MF A, MF out) {
+  int cb, pb;
+  if (!sf_site<TT>(g, cb, pb)) return;
+  const int po = pb ^ 1;
+  const SfSites n = sf_sites<MU, NU>(g, cb, pb);
   Mat3<double> a, b, t, up, lo;
-  mfl(A, po, xn, a);
-  mload(U, po, nu, xm, b);
+  mfl(A, po, n.ypn, a);
+  mload(U, po, NU, n.ypm, b);
   mmul_na(a, b, t);
-  mload(U, pb, nu, cb, b);
+  mload(U, pb, NU, cb, b);
   mmul(b, t, up);
-  mload(U, po, nu, xd, a);
-  mfl(A, po, xd, b);
+  mload(U, po, NU, n.ymn, a);
+  mfl(A, po, n.ymn, b);
   mmul_an(a, b, t);
-  mload(U, dpb, nu, xdm, b);
+  mload(U, pb, NU, n.ymnpm, b);
   mmul(t, b, lo);
   madd(up, lo);
   mfs(out, pb, cb, up);
@@ -560,75 +639,93 @@ __global__ void __launch_bounds__(128) k_sf_fwd(Geom g, LinkField U, int mu, int
 // With fa_init the FA term is not accumulated but initialised:
 //   FA(y) = ci FI(y) + cg [...]
 // which stands for a preceding FA = ci FI pass (the scaled staple force).
-__global__ void __launch_bounds__(128) k_sf_adj(Geom g, LinkField U, int mu, int nu, MF A, MF Gs, double cg, MF FA,
-                                                int want_fa, MF FU, int fa_init, MF FI, double ci) {
-  int bx, pb;
-  if (!xcd_block((g.Vh + blockDim.x - 1) / blockDim.x, 2, bx, pb)) return;
-  int cb = bx * blockDim.x + threadIdx.x;
-  if (cb >= g.Vh) return;
-  int po = pb ^ 1;
-  Coord c = site_coords(g, cb, pb);
-  int ypn = neighbour(g, c, cb, nu, 1), ymn = neighbour(g, c, cb, nu, -1);
-  int ypm = neighbour(g, c, cb, mu, 1), ymm = neighbour(g, c, cb, mu, -1);
-  int q;
-  int ymnpm = hop2(g, cb, pb, nu, -1, mu, 1, q);  // parity pb
-  int ymmpn = hop2(g, cb, pb, mu, -1, nu, 1, q);  // parity pb
+template <int MU, int NU, int TT>
+__global__ void __launch_bounds__(128) k_sf_adj(Geom g, LinkField U, MF A, MF Gs, double cg, MF FA, MF FU,
+                                                int fa_init, MF FI, double ci) {
+  int cb, pb;
+  if (!sf_site<TT>(g, cb, pb)) return;
+  const int po = pb ^ 1;
+  const SfSites n = sf_sites<MU, NU>(g, cb, pb);
   Mat3<double> a, b, t, u, acc;
-  if (want_fa) {
-    mload(U, pb, nu, ymnpm, u);
-    mfl(Gs, po, ymn, b);
-    mmul_an(u, b, t);
-    mload(U, po, nu, ymn, u);
-    mmul(t, u, acc);
-    mload(U, po, nu, ypm, u);
-    mfl(Gs, po, ypn, b);
-    mmul(u, b, t);
-    mload(U, pb, nu, cb, u);
-    mmul_na(t, u, a);
-    madd(acc, a);
-    if (fa_init) {
-      mfl(FI, pb, cb, a);
+  mload(U, pb, NU, n.ymnpm, u);
+  mfl(Gs, po, n.ymn, b);
+  mmul_an(u, b, t);
+  mload(U, po, NU, n.ymn, u);
+  mmul(t, u, acc);
+  mload(U, po, NU, n.ypm, u);
+  mfl(Gs, po, n.ypn, b);
+  mmul(u, b, t);
+  mload(U, pb, NU, cb, u);
+  mmul_na(t, u, a);
+  madd(acc, a);
+  if (fa_init) {
+    mfl_nt(FI, pb, cb, a);
 #pragma unroll
-      for (int i = 0; i < 3; i++)
+    for (int i = 0; i < 3; i++)
 #pragma unroll
-        for (int j = 0; j < 3; j++) {
-          double re = ci * a.e[i][j].re, im = ci * a.e[i][j].im;
-          a.e[i][j].re = re + cg * acc.e[i][j].re;
-          a.e[i][j].im = im + cg * acc.e[i][j].im;
-        }
-      mfs(FA, pb, cb, a);
-    } else {
-      mfa(FA, pb, cb, cg, acc);
-    }
+      for (int j = 0; j < 3; j++) {
+        double re = ci * a.e[i][j].re, im = ci * a.e[i][j].im;
+        a.e[i][j].re = re + cg * acc.e[i][j].re;
+        a.e[i][j].im = im + cg * acc.e[i][j].im;
+      }
+    mfs_nt(FA, pb, cb, a);
+  } else {
+    mfa_nt(FA, pb, cb, cg, acc);
   }
   // A(y+nu) U_nu(y+mu)^dag G(y)
-  mfl(A, po, ypn, a);
-  mload(U, po, nu, ypm, u);
+  mfl(A, po, n.ypn, a);
+  mload(U, po, NU, n.ypm, u);
   mmul_na(a, u, t);
   mfl(Gs, pb, cb, b);
   mmul(t, b, acc);
   // (G(y-mu) U_nu(y-mu) A(y-mu+nu))^dag
-  mfl(Gs, po, ymm, b);
-  mload(U, po, nu, ymm, u);
+  mfl(Gs, po, n.ymm, b);
+  mload(U, po, NU, n.ymm, u);
   mmul(b, u, t);
-  mfl(A, pb, ymmpn, a);
+  mfl(A, pb, n.ymmpn, a);
   mmul(t, a, b);
   madd_adj(acc, b);
   // (A(y) U_nu(y+mu) G(y+nu))^dag
   mfl(A, pb, cb, a);
-  mload(U, po, nu, ypm, u);
+  mload(U, po, NU, n.ypm, u);
   mmul(a, u, t);
-  mfl(Gs, po, ypn, b);
+  mfl(Gs, po, n.ypn, b);
   mmul(t, b, a);
   madd_adj(acc, a);
   // G(y-mu+nu) U_nu(y-mu)^dag A(y-mu)
-  mfl(Gs, pb, ymmpn, b);
-  mload(U, po, nu, ymm, u);
+  mfl(Gs, pb, n.ymmpn, b);
+  mload(U, po, NU, n.ymm, u);
   mmul_na(b, u, t);
-  mfl(A, po, ymm, a);
+  mfl(A, po, n.ymm, a);
   mmul(t, a, b);
   madd(acc, b);
-  mfa(FU, pb, cb, cg, acc);
+  mfa_nt(FU, pb, cb, cg, acc);
+}
+
+typedef void (*SfFwdFn)(Geom, LinkField, MF, MF);
+typedef void (*SfAdjFn)(Geom, LinkField, MF, MF, double, MF, MF, int, MF, double);
+template <int MU, int NU>
+static void sf_pick(int tiled, SfFwdFn& f, SfAdjFn& a) {
+  f = tiled ? k_sf_fwd<MU, NU, 1> : k_sf_fwd<MU, NU, 0>;
+  a = tiled ? k_sf_adj<MU, NU, 1> : k_sf_adj<MU, NU, 0>;
+}
+// kernels of the direction pair (mu, nu), mu != nu
+static void sf_kernels(const Geom& g, int mu, int nu, SfFwdFn& f, SfAdjFn& a) {
+  const int tt = sf_tiled(g, mu, nu) ? 1 : 0;
+  switch (mu * 4 + nu) {
+    case 1: sf_pick<0, 1>(tt, f, a); break;
+    case 2: sf_pick<0, 2>(tt, f, a); break;
+    case 3: sf_pick<0, 3>(tt, f, a); break;
+    case 4: sf_pick<1, 0>(tt, f, a); break;
+    case 6: sf_pick<1, 2>(tt, f, a); break;
+    case 7: sf_pick<1, 3>(tt, f, a); break;
+    case 8: sf_pick<2, 0>(tt, f, a); break;
+    case 9: sf_pick<2, 1>(tt, f, a); break;
+    case 11: sf_pick<2, 3>(tt, f, a); break;
+    case 12: sf_pick<3, 0>(tt, f, a); break;
+    case 13: sf_pick<3, 1>(tt, f, a); break;
+    default: sf_pick<3, 2>(tt, f, a); break;
+  }
 }
 
 // Naik: G_mu(y) += cN [U(y+mu) U(y+2mu) F3(y) + U(y+mu) F3(y-mu) U(y-mu) + F3(y-2mu) U(y-2mu) U(y-mu)],
@@ -1352,7 +1449,16 @@ static int smear_adjoint_dev(const LinkField& L, const LinkField& F1, const Link
   // coefficient folded into cg.
   auto adj = [&](int mu, int nu, const MF& A, const MF& Gs, double cg, const MF& FA, const MF& FU, int init,
                  double ci) {
-    hipLaunchKernelGGL(k_sf_adj, xgrid, blk, 0, c.stream, c.geom, L, mu, nu, A, Gs, cg, FA, 1, FU, init, mF, ci);
+    SfFwdFn kf;
+    SfAdjFn ka;
+    sf_kernels(c.geom, mu, nu, kf, ka);
+    hipLaunchKernelGGL(ka, xgrid, blk, 0, c.stream, c.geom, L, A, Gs, cg, FA, FU, init, mF, ci);
+  };
+  auto fwd = [&](int mu, int nu, const MF& A, const MF& out) {
+    SfFwdFn kf;
+    SfAdjFn ka;
+    sf_kernels(c.geom, mu, nu, kf, ka);
+    hipLaunchKernelGGL(kf, xgrid, blk, 0, c.stream, c.geom, L, A, out);
   };
   for (int mu = 0; mu < 4; mu++) {
     const MF Gmu = mf_link(G, mu);
@@ -1360,7 +1466,7 @@ static int smear_adjoint_dev(const LinkField& L, const LinkField& F1, const Link
     if (staples) {
       for (int nu = 0; nu < 4; nu++) {
         if (nu == mu) continue;
-        hipLaunchKernelGGL(k_sf_fwd, xgrid, blk, 0, c.stream, c.geom, L, mu, nu, mf_link(L, mu), mS1);
+        fwd(mu, nu, mf_link(L, mu), mS1);
         bool g1_set = false;
         if (cl != 0.0) {
           adj(mu, nu, mS1, mF, cl, mG1, mf_link(G, nu), 1, c3);
@@ -1369,7 +1475,7 @@ static int smear_adjoint_dev(const LinkField& L, const LinkField& F1, const Link
         if (c5 != 0.0 || c7 != 0.0) {
           for (int rho = 0; rho < 4; rho++) {
             if (rho == mu || rho == nu) continue;
-            hipLaunchKernelGGL(k_sf_fwd, xgrid, blk, 0, c.stream, c.geom, L, mu, rho, mS1, mS2);
+            fwd(mu, rho, mS1, mS2);
             // G2 = c5 F + the 7-staple terms (one sigma in four dimensions)
             bool g2_set = false;
             if (c7 != 0.0)
