@@ -51,14 +51,14 @@ struct BMSR {  // one right-hand side of the multi-shift solve
   int num_now, now_ws, m, halt, iters, converged, j_low, pad;
 };
 struct BMSG {  // shared by all right-hand sides
-  int n, nr, halt_all, iters_max, pad[4];
+  int n, nr, halt_all, iters_max, max_iter, pad[3];
 };
 struct BCGR {
   double rsq, oldrsq, a, b, rsq_stop, pad;
   int halt, go, iters, pad2;
 };
 struct BCGG {
-  int halt_all, iters_max, any_go, pad;
+  int halt_all, iters_max, any_go, iter_cap;  // iter_cap: iterations left to this sloppy phase
 };
 struct FPtrs {
   float* v[BMS_MAX];
@@ -156,6 +156,8 @@ __global__ void k_bms_alpha(const double* __restrict__ sums, BMSR* __restrict__ 
         S.zeta_im1[j] = S.zeta_i[j];
         S.zeta_i[j] = S.zeta_ip1[j];
       }
+      // the loop of ks_multicg_offset.c ends when iteration reaches niter
+      if (1 + S.iters >= g->max_iter) S.halt = 1;
     }
   }
   __syncthreads();
@@ -403,7 +405,7 @@ __global__ void k_bcg_scalars(const double* __restrict__ sums, const double* __r
       S.rsq = rsq;
       S.go = 1;
       S.iters += 1;
-      if (rsq < S.rsq_stop) S.halt = 1;
+      if (rsq < S.rsq_stop || S.iters >= g->iter_cap) S.halt = 1;
     }
   }
   __syncthreads();
@@ -554,6 +556,7 @@ int bmultishift_nr(const void* b64, const double* offs, int n, const double* res
   memset(&hg, 0, sizeof hg);
   hg.n = n;
   hg.nr = NR;
+  hg.max_iter = (int)std::min<long>(prm.max_iters, 1L << 30);
   int all = 1;
   double base[BNR_MAX];
   for (int r = 0; r < NR; r++) {
@@ -792,6 +795,7 @@ int bcg_mixed_nr(const void* b64, void* x64, const double* offsets, const BCGPar
       S.rsq_stop = std::max(res * res * bn[r], delta * delta * rt[r]);
     }
     if ((rc = (int)hipMemcpyAsync(ds, hs.data(), NR * sizeof(BCGR), hipMemcpyHostToDevice, c.stream))) break;
+    hg0.iter_cap = (int)std::min<long>(prm.max_iters - iteration, 1L << 30);
     if ((rc = (int)hipMemcpyAsync(dg, &hg0, sizeof(BCGG), hipMemcpyHostToDevice, c.stream))) break;
     hipLaunchKernelGGL(k_bru<NR>, dim3(rg.nb), dim3(256), 0, c.stream, (const double*)r64.p, (float*)rs.p,
                        (float*)ps.p, ds, first ? 1 : 0, rg.nrec, rg.loop, part, pstride);

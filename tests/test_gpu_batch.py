@@ -151,3 +151,28 @@ def test_invert_block_vs_oracle(l8):
     for s in range(6):
         x, _, _ = L.congrad(fat, lng, srcs[s], m, 1e-14, parity=ko.EVEN, max_iters=5000, nrestart=5)
         assert rel(dests[s][sl], x[sl]) < 1e-8, s
+
+
+def test_batched_solvers_stop_at_max_iters(l8):
+    """The iteration cap holds inside the captured windows: the batched
+    multi-shift stops at max_iters (ks_multicg_offset.c counts the set-up as
+    one), the batched CG at max_iters plus the closing true-residual
+    evaluation, which MILC also counts (d_congrad5_fn_milc.c:223-234); both
+    report no convergence.  Caps that are not multiples of the 8-iteration
+    window."""
+    L, fat, lng = l8
+    srcs = [L.gaussian_vector(1400 + s, ko.EVEN) for s in range(4)]
+    with mq.Lattice(L.dims) as G:
+        G.dev_load_links(fat, lng)
+        for s, v in enumerate(srcs):
+            G.dev_write(s, ko.EVEN, v)
+        for cap in (13, 30):
+            st = G.dev_multishift_block(list(range(4)), OFFS, 1e-20, 0.0, parity=ko.EVEN, max_iters=cap)
+            assert st["ms_iters"] == cap and st["converged"] == 0, (cap, st["ms_iters"])
+        G.set_links(fat, lng)
+        for cap in (11, 21):
+            dests = [L.new_vector() for _ in range(4)]
+            q = mq.QuarkInvertControl(max=cap, nrestart=1, parity=ko.EVEN, resid=1e-14)
+            it, fres = G.ks_congrad_block_parity(srcs, dests, q, 0.05)
+            assert it <= cap + 1 and q.converged == 0, (cap, it)
+            assert max(fres) > 1e-14
