@@ -454,12 +454,15 @@ int milc_hip_dev_multishift(int slot_src, int parity, int num_offsets, const dou
  * coefficients residues[t]; thin_force != 0 (links built by
  * milc_hip_dev_gauge_hisq) carries them through the HISQ chain to the
  * thin-link momentum TA(2 U F_U); then S_act = sum_even |phi + sum_t
- * residues[t] X_t|^2 in FP64 (target min(resid, 1e-10)).  compare_fp64 != 0
- * repeats the force in FP64 (target min(resid, 1e-12)) and reports relative
- * differences.  stats[0..10] = force seconds, force iterations,
- * sum_t r_t Re<phi, X_t>, action, action seconds, action iterations, rel.
- * error O1, rel. error O3, FP64 force seconds, thin-link force seconds,
- * rel. error of the thin-link momentum (-1 when not computed). */
+ * residues[t] X_t|^2 summed in FP64, its solve in the precision of the force
+ * solve to the FP64 true residual min(resid, 1e-10).  compare_fp64 != 0
+ * repeats the force in FP64 (target min(resid, 1e-12)) and the action with an
+ * FP64 solve, and reports relative differences.  stats must hold 13 values:
+ * stats[0..12] = force seconds, force iterations, sum_t r_t Re<phi, X_t>,
+ * action, action seconds, action iterations, rel. error O1, rel. error O3,
+ * FP64 force seconds, thin-link force seconds, rel. error of the thin-link
+ * momentum, FP64-solve action, rel. error of the action (-1 when not
+ * computed). */
 int milc_hip_dev_hmc_step(int slot_phi, int n, const double* offsets, const double* residues, double resid,
                           int mixed_precision, int compare_fp64, int thin_force, double* stats);
 

@@ -1558,7 +1558,9 @@ extern "C" int milc_hip_dev_hmc_step(int slot_phi, int n, const double* offsets,
       return rc;
     }
   }
-  // FP64 action sum_even |a0 phi + sum_k a_k x_k|^2 (d_action_rhmc.c), a0 = 1
+  // action sum_even |a0 phi + sum_k a_k x_k|^2 (d_action_rhmc.c), a0 = 1, summed
+  // in FP64 from solutions at the FP64 true-residual target (the solve in the
+  // precision of the force solve)
   std::vector<double> roots(n + 1), res(n + 1);
   roots[0] = 0;
   res[0] = 1.0;
@@ -1569,15 +1571,23 @@ extern "C" int milc_hip_dev_hmc_step(int slot_phi, int n, const double* offsets,
   t0 = std::chrono::steady_clock::now();
   double action = 0;
   int it_act = 0;
-  rc = rhmc_action(g_slots[slot_phi], n, roots.data(), res.data(), std::min(resid, 1e-10), 0, &action, &it_act);
+  rc = rhmc_action(g_slots[slot_phi], n, roots.data(), res.data(), std::min(resid, 1e-10), mixed_precision ? 1 : 0,
+                   &action, &it_act);
   (void)hipStreamSynchronize(c.stream);
   double t_action = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   if (rc) {
     release();
     return rc;
   }
-  double e1 = -1, e3 = -1, em = -1, t_ref = 0;
+  double e1 = -1, e3 = -1, em = -1, t_ref = 0, action64 = -1, ea = -1;
   if (compare_fp64) {
+    int it64;
+    if ((rc = rhmc_action(g_slots[slot_phi], n, roots.data(), res.data(), std::min(resid, 1e-10), 0, &action64,
+                          &it64))) {
+      release();
+      return rc;
+    }
+    ea = std::fabs(action - action64) / std::fabs(action64);
     if ((rc = alloc_links(r1, P_F64, c.stride)) || (rc = alloc_links(r3, P_F64, c.stride))) {
       release();
       return rc;
@@ -1612,6 +1622,8 @@ extern "C" int milc_hip_dev_hmc_step(int slot_phi, int n, const double* offsets,
   stats[8] = t_ref;
   stats[9] = t_thin;
   stats[10] = em;
+  stats[11] = action64;
+  stats[12] = ea;
   return 0;
 }
 

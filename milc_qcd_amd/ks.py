@@ -754,12 +754,13 @@ class Lattice:
         n = len(offsets)
         offs = (ctypes.c_double * n)(*offsets)
         res = (ctypes.c_double * n)(*residues)
-        st = (ctypes.c_double * 11)()
+        st = (ctypes.c_double * 13)()
         _check(self.lib.milc_hip_dev_hmc_step(slot_phi, n, offs, res, resid, mixed, int(bool(compare_fp64)),
                                               int(bool(thin_force)), st), "milc_hip_dev_hmc_step")
         return {"force_seconds": st[0], "force_iters": int(st[1]), "S": st[2], "action": st[3],
                 "action_seconds": st[4], "action_iters": int(st[5]), "rel_err_O1": st[6], "rel_err_O3": st[7],
-                "fp64_force_seconds": st[8], "thin_force_seconds": st[9], "rel_err_mom": st[10]}
+                "fp64_force_seconds": st[8], "thin_force_seconds": st[9], "rel_err_mom": st[10],
+                "action_fp64": st[11], "rel_err_action": st[12]}
 
     def dev_hisq_force(self, slot0, residues, eps, mom=None):
         """HISQ thin-link force from the even-site solutions in slots slot0..;

@@ -8,7 +8,8 @@ oracle on a small lattice.
 * 48^3 x 96, m = 0.01, same shifts, residues 0.1 (i+1), source seed 2400,
   gauge 1004: one RHMC force (mixed) against the FP64 force on the device,
   relative error below 1e-7 for the outer products and for the HISQ
-  thin-link momentum; the FP64 action positive and finite.
+  thin-link momentum; the action (mixed solve, FP64 sum) within 1e-9 of the
+  action from an FP64 solve.
 * 8^4: milc_hip_dev_hmc_step's sum_t r_t <phi, X_t> and action against the oracle.
 """
 import math
@@ -53,6 +54,7 @@ def test_hmc_step_48x48x48x96_mixed_vs_fp64_force():
         assert st["rel_err_O1"] < 1e-7 and st["rel_err_O3"] < 1e-7
         assert 0 <= st["rel_err_mom"] < 1e-7
         assert math.isfinite(st["action"]) and st["action"] > 0
+        assert 0 <= st["rel_err_action"] < 1e-9
         assert st["S"] > 0
 
 
@@ -78,4 +80,5 @@ def test_hmc_step_vs_oracle_8x8x8x8():
         st = G.dev_hmc_step(0, offs, res, 1e-11, mixed=1, compare_fp64=True)
     assert abs(st["S"] - S_ref) < 1e-9 * abs(S_ref)
     assert abs(st["action"] - act_ref) < 1e-10 * act_ref
+    assert abs(st["action_fp64"] - act_ref) < 1e-10 * act_ref
     assert st["rel_err_O1"] < 1e-8 and st["rel_err_O3"] < 1e-8
