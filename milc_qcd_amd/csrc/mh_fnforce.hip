@@ -610,9 +610,9 @@ __host__ inline bool sf_tiled(const Geom& g, int mu, int nu) {
   return (mu == 3 || nu == 3) && g.st % 16 == 0 && g.X[3] % 4 == 0;
 }
 
-// out = S_nu(A) for the mu link
+// out = S_nu(A) for the mu link (acc != 0: out += S_nu(A))
 template <int MU, int NU, int TT>
-__global__ void __launch_bounds__(128) k_sf_fwd(Geom g, LinkField U, MF A, MF out) {
+__global__ void __launch_bounds__(128) k_sf_fwd(Geom g, LinkField U, MF A, MF out, int acc) {
   int cb, pb;
   if (!sf_site<TT>(g, cb, pb)) return;
   const int po = pb ^ 1;
@@ -629,7 +629,8 @@ __global__ void __launch_bounds__(128) k_sf_fwd(Geom g, LinkField U, MF A, MF ou
   mload(U, pb, NU, n.ymnpm, b);
   mmul(t, b, lo);
   madd(up, lo);
-  mfs(out, pb, cb, up);
+  if (acc) mfa(out, pb, cb, 1.0, up);
+  else mfs(out, pb, cb, up);
 }
 
 // reverse of O = S_nu(A) for the force Gs on O scaled by cg:
@@ -639,7 +640,10 @@ __global__ void __launch_bounds__(128) k_sf_fwd(Geom g, LinkField U, MF A, MF ou
 // With fa_init the FA term is not accumulated but initialised:
 //   FA(y) = ci FI(y) + cg [...]
 // which stands for a preceding FA = ci FI pass (the scaled staple force).
-template <int MU, int NU, int TT>
+// MODE 0 computes both terms; MODE 1 only the FA term, MODE 2 only the FU term
+// (FA depends on Gs alone, FU is linear in A: the 7-staple driver forms each
+// FA once per direction pair and the FU of two 5-staples from their sum).
+template <int MU, int NU, int TT, int MODE>
 __global__ void __launch_bounds__(128) k_sf_adj(Geom g, LinkField U, MF A, MF Gs, double cg, MF FA, MF FU,
                                                 int fa_init, MF FI, double ci) {
   int cb, pb;
@@ -647,6 +651,7 @@ __global__ void __launch_bounds__(128) k_sf_adj(Geom g, LinkField U, MF A, MF Gs
   const int po = pb ^ 1;
   const SfSites n = sf_sites<MU, NU>(g, cb, pb);
   Mat3<double> a, b, t, u, acc;
+  if (MODE != 2) {
   mload(U, pb, NU, n.ymnpm, u);
   mfl(Gs, po, n.ymn, b);
   mmul_an(u, b, t);
@@ -672,6 +677,8 @@ __global__ void __launch_bounds__(128) k_sf_adj(Geom g, LinkField U, MF A, MF Gs
   } else {
     mfa_nt(FA, pb, cb, cg, acc);
   }
+  }
+  if (MODE == 1) return;
   // A(y+nu) U_nu(y+mu)^dag G(y)
   mfl(A, po, n.ypn, a);
   mload(U, po, NU, n.ypm, u);
@@ -702,29 +709,32 @@ __global__ void __launch_bounds__(128) k_sf_adj(Geom g, LinkField U, MF A, MF Gs
   mfa_nt(FU, pb, cb, cg, acc);
 }
 
-typedef void (*SfFwdFn)(Geom, LinkField, MF, MF);
+typedef void (*SfFwdFn)(Geom, LinkField, MF, MF, int);
 typedef void (*SfAdjFn)(Geom, LinkField, MF, MF, double, MF, MF, int, MF, double);
-template <int MU, int NU>
-static void sf_pick(int tiled, SfFwdFn& f, SfAdjFn& a) {
-  f = tiled ? k_sf_fwd<MU, NU, 1> : k_sf_fwd<MU, NU, 0>;
-  a = tiled ? k_sf_adj<MU, NU, 1> : k_sf_adj<MU, NU, 0>;
+struct SfKernels {
+  SfFwdFn fwd;
+  SfAdjFn adj[3];  // by MODE
+};
+template <int MU, int NU, int TT>
+static SfKernels sf_pick() {
+  return SfKernels{k_sf_fwd<MU, NU, TT>, {k_sf_adj<MU, NU, TT, 0>, k_sf_adj<MU, NU, TT, 1>, k_sf_adj<MU, NU, TT, 2>}};
 }
 // kernels of the direction pair (mu, nu), mu != nu
-static void sf_kernels(const Geom& g, int mu, int nu, SfFwdFn& f, SfAdjFn& a) {
-  const int tt = sf_tiled(g, mu, nu) ? 1 : 0;
+static SfKernels sf_kernels(const Geom& g, int mu, int nu) {
+  const bool tt = sf_tiled(g, mu, nu);
   switch (mu * 4 + nu) {
-    case 1: sf_pick<0, 1>(tt, f, a); break;
-    case 2: sf_pick<0, 2>(tt, f, a); break;
-    case 3: sf_pick<0, 3>(tt, f, a); break;
-    case 4: sf_pick<1, 0>(tt, f, a); break;
-    case 6: sf_pick<1, 2>(tt, f, a); break;
-    case 7: sf_pick<1, 3>(tt, f, a); break;
-    case 8: sf_pick<2, 0>(tt, f, a); break;
-    case 9: sf_pick<2, 1>(tt, f, a); break;
-    case 11: sf_pick<2, 3>(tt, f, a); break;
-    case 12: sf_pick<3, 0>(tt, f, a); break;
-    case 13: sf_pick<3, 1>(tt, f, a); break;
-    default: sf_pick<3, 2>(tt, f, a); break;
+    case 1: return tt ? sf_pick<0, 1, 1>() : sf_pick<0, 1, 0>();
+    case 2: return tt ? sf_pick<0, 2, 1>() : sf_pick<0, 2, 0>();
+    case 3: return tt ? sf_pick<0, 3, 1>() : sf_pick<0, 3, 0>();
+    case 4: return tt ? sf_pick<1, 0, 1>() : sf_pick<1, 0, 0>();
+    case 6: return tt ? sf_pick<1, 2, 1>() : sf_pick<1, 2, 0>();
+    case 7: return tt ? sf_pick<1, 3, 1>() : sf_pick<1, 3, 0>();
+    case 8: return tt ? sf_pick<2, 0, 1>() : sf_pick<2, 0, 0>();
+    case 9: return tt ? sf_pick<2, 1, 1>() : sf_pick<2, 1, 0>();
+    case 11: return tt ? sf_pick<2, 3, 1>() : sf_pick<2, 3, 0>();
+    case 12: return tt ? sf_pick<3, 0, 1>() : sf_pick<3, 0, 0>();
+    case 13: return tt ? sf_pick<3, 1, 1>() : sf_pick<3, 1, 0>();
+    default: return tt ? sf_pick<3, 2, 1>() : sf_pick<3, 2, 0>();
   }
 }
 
@@ -1422,73 +1432,101 @@ int mom_action_dev(const double* mom_dev, double* action) {
 // Staple-form adjoint on the device: G (FP64 link field, zeroed here) = force on
 // the links L of the smearing with path coefficients coeff, from the force
 // F1 (non-Naik) / F3 (Naik) on its result (eps(x) applied when with_eps).
+//
+// Per direction mu, with F = eps F1_mu:
+//   S1[nu] = S_nu(L_mu)                      (3-staples)
+//   G2[sig] = c5 F + c7 FA_sig(F)            (the 7-staple force seen by a 5-staple)
+//   G[sig] += c7 FU_sig(S_rho(S1[nu]) + S_nu(S1[rho]), F)   ({nu, rho, sig} the other three)
+//   per nu: G1 = c3 F + cl FA_nu(F)  [Lepage] + sum_rho FA_rho(G2[sig]) (or c5 FA_rho(F)),
+//           G[nu] += cl FU_nu(S1[nu], F), G[rho] += FU_rho(S1[nu], G2[sig]),
+//           G_mu += FA_nu(G1), G[nu] += FU_nu(L_mu, G1)
+// FA_nu(X) and FU_nu(A, X) are the two halves of the reverse of S_nu (k_sf_adj).
+// A 7-staple force term depends on the 5-staple only through FU, linearly,
+// and its FA part only on F and sig, so the two orders (nu, rho) and
+// (rho, nu) of one sig share one FA pass and one FU pass.
 static int smear_adjoint_dev(const LinkField& L, const LinkField& F1, const LinkField& F3, int with_eps,
                              const double coeff[6], LinkField& G) {
   Context& c = ctx();
   const int st = c.stride;
-  double2 *F = nullptr, *S1 = nullptr, *S2 = nullptr, *G1 = nullptr, *G2 = nullptr;
-  int rc;
-  size_t bytes = (size_t)2 * 9 * st * sizeof(double2);
-  if ((rc = dev_alloc((void**)&F, bytes)) || (rc = dev_alloc((void**)&S1, bytes)) ||
-      (rc = dev_alloc((void**)&S2, bytes)) || (rc = dev_alloc((void**)&G1, bytes)) ||
-      (rc = dev_alloc((void**)&G2, bytes))) {
-    dev_free(F); dev_free(S1); dev_free(S2); dev_free(G1);
-    return rc;
-  }
-  MH_CHECK(hipMemsetAsync(G.data, 0, link_bytes(P_F64, st), c.stream));
-  const MF mF = mf_site(F, st), mS1 = mf_site(S1, st), mS2 = mf_site(S2, st), mG1 = mf_site(G1, st),
-           mG2 = mf_site(G2, st);
   const double c1 = coeff[0] - 6.0 * coeff[5], cN = coeff[1], c3 = coeff[2], c5 = coeff[3], c7 = coeff[4],
                cl = coeff[5];
   const bool staples = c3 != 0.0 || cl != 0.0 || c5 != 0.0 || c7 != 0.0;
-  // init and scale: plain grid; the neighbour-reading kernels: XCD block order
+  const bool five = c5 != 0.0 || c7 != 0.0, seven = c7 != 0.0;
+  // F, G1, S1[4], G2[4], SS: the mu-th entries of S1 and G2 are not used
+  double2 *F = nullptr, *G1 = nullptr, *SS = nullptr, *S1[4] = {}, *G2[4] = {};
+  OnExit bufs([&] {
+    dev_free(F);
+    dev_free(G1);
+    dev_free(SS);
+    for (int d = 0; d < 4; d++) {
+      dev_free(S1[d]);
+      dev_free(G2[d]);
+    }
+  });
+  const size_t bytes = (size_t)2 * 9 * st * sizeof(double2);
+  int rc;
+  if ((rc = dev_alloc((void**)&F, bytes))) return rc;
+  if (staples) {
+    if ((rc = dev_alloc((void**)&G1, bytes))) return rc;
+    for (int d = 0; d < 3; d++)
+      if ((rc = dev_alloc((void**)&S1[d], bytes))) return rc;
+    if (seven) {
+      if ((rc = dev_alloc((void**)&SS, bytes))) return rc;
+      for (int d = 0; d < 3; d++)
+        if ((rc = dev_alloc((void**)&G2[d], bytes))) return rc;
+    }
+  }
+  MH_CHECK(hipMemsetAsync(G.data, 0, link_bytes(P_F64, st), c.stream));
+  const MF mF = mf_site(F, st), mG1 = mf_site(G1, st), mSS = mf_site(SS, st);
+  // init: plain grid; the neighbour-reading kernels: XCD block order
   dim3 grid((c.Vh + 127) / 128, 2), xgrid = xcd_grid((c.Vh + 127) / 128, 2), blk(128);
-  // The scaled copies G1 = c3 F and G2 = c5 F are never formed: the first
-  // adjoint pass into G1 (G2) initialises it as c3 F (c5 F) plus its own
-  // term, and where nothing accumulates into it, its reader takes F with the
-  // coefficient folded into cg.
-  auto adj = [&](int mu, int nu, const MF& A, const MF& Gs, double cg, const MF& FA, const MF& FU, int init,
+  const MF none{};
+  auto adj = [&](int mode, int mu, int nu, const MF& A, const MF& Gs, double cg, const MF& FA, const MF& FU, int init,
                  double ci) {
-    SfFwdFn kf;
-    SfAdjFn ka;
-    sf_kernels(c.geom, mu, nu, kf, ka);
-    hipLaunchKernelGGL(ka, xgrid, blk, 0, c.stream, c.geom, L, A, Gs, cg, FA, FU, init, mF, ci);
+    const SfKernels k = sf_kernels(c.geom, mu, nu);
+    hipLaunchKernelGGL(k.adj[mode], xgrid, blk, 0, c.stream, c.geom, L, A, Gs, cg, FA, FU, init, mF, ci);
   };
-  auto fwd = [&](int mu, int nu, const MF& A, const MF& out) {
-    SfFwdFn kf;
-    SfAdjFn ka;
-    sf_kernels(c.geom, mu, nu, kf, ka);
-    hipLaunchKernelGGL(kf, xgrid, blk, 0, c.stream, c.geom, L, A, out);
+  auto fwd = [&](int mu, int nu, const MF& A, const MF& out, int acc) {
+    const SfKernels k = sf_kernels(c.geom, mu, nu);
+    hipLaunchKernelGGL(k.fwd, xgrid, blk, 0, c.stream, c.geom, L, A, out, acc);
   };
   for (int mu = 0; mu < 4; mu++) {
     const MF Gmu = mf_link(G, mu);
     hipLaunchKernelGGL(k_sf_init, grid, blk, 0, c.stream, c.geom, mf_link(F1, mu), with_eps, mF, Gmu, c1);
     if (staples) {
-      for (int nu = 0; nu < 4; nu++) {
-        if (nu == mu) continue;
-        fwd(mu, nu, mf_link(L, mu), mS1);
+      // the three other directions and their buffers
+      int od[3], k = 0;
+      for (int d = 0; d < 4; d++)
+        if (d != mu) od[k++] = d;
+      MF mS1[4], mG2[4];
+      for (int i = 0; i < 3; i++) {
+        mS1[od[i]] = mf_site(S1[i], st);
+        mG2[od[i]] = mf_site(G2[i], st);
+      }
+      for (int i = 0; i < 3; i++) fwd(mu, od[i], mf_link(L, mu), mS1[od[i]], 0);
+      if (seven)
+        for (int i = 0; i < 3; i++) {
+          const int sig = od[i], nu = od[(i + 1) % 3], rho = od[(i + 2) % 3];
+          adj(1, mu, sig, none, mF, c7, mG2[sig], none, 1, c5);
+          fwd(mu, rho, mS1[nu], mSS, 0);
+          fwd(mu, nu, mS1[rho], mSS, 1);
+          adj(2, mu, sig, mSS, mF, c7, none, mf_link(G, sig), 0, 0.0);
+        }
+      for (int i = 0; i < 3; i++) {
+        const int nu = od[i];
         bool g1_set = false;
         if (cl != 0.0) {
-          adj(mu, nu, mS1, mF, cl, mG1, mf_link(G, nu), 1, c3);
+          adj(0, mu, nu, mS1[nu], mF, cl, mG1, mf_link(G, nu), 1, c3);
           g1_set = true;
         }
-        if (c5 != 0.0 || c7 != 0.0) {
-          for (int rho = 0; rho < 4; rho++) {
-            if (rho == mu || rho == nu) continue;
-            fwd(mu, rho, mS1, mS2);
-            // G2 = c5 F + the 7-staple terms (one sigma in four dimensions)
-            bool g2_set = false;
-            if (c7 != 0.0)
-              for (int sig = 0; sig < 4; sig++) {
-                if (sig == mu || sig == nu || sig == rho) continue;
-                adj(mu, sig, mS2, mF, c7, mG2, mf_link(G, sig), g2_set ? 0 : 1, c5);
-                g2_set = true;
-              }
-            adj(mu, rho, mS1, g2_set ? mG2 : mF, g2_set ? 1.0 : c5, mG1, mf_link(G, rho), g1_set ? 0 : 1, c3);
+        if (five)
+          for (int j = 1; j < 3; j++) {
+            const int rho = od[(i + j) % 3], sig = od[(i + 3 - j) % 3];
+            adj(0, mu, rho, mS1[nu], seven ? mG2[sig] : mF, seven ? 1.0 : c5, mG1, mf_link(G, rho), g1_set ? 0 : 1,
+                c3);
             g1_set = true;
           }
-        }
-        adj(mu, nu, mf_link(L, mu), g1_set ? mG1 : mF, g1_set ? 1.0 : c3, Gmu, mf_link(G, nu), 0, 0.0);
+        adj(0, mu, nu, mf_link(L, mu), g1_set ? mG1 : mF, g1_set ? 1.0 : c3, Gmu, mf_link(G, nu), 0, 0.0);
       }
     }
     if (cN != 0.0)
@@ -1496,7 +1534,6 @@ static int smear_adjoint_dev(const LinkField& L, const LinkField& F1, const Link
   }
   MH_CHECK(hipGetLastError());
   MH_CHECK(hipStreamSynchronize(c.stream));
-  dev_free(F); dev_free(S1); dev_free(S2); dev_free(G1); dev_free(G2);
   return 0;
 }
 
