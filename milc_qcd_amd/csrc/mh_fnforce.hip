@@ -633,6 +633,69 @@ __global__ void __launch_bounds__(128) k_sf_fwd(Geom g, LinkField U, MF A, MF ou
   else mfs(out, pb, cb, up);
 }
 
+// out = S_P(AP) + S_Q(AQ) for the mu link: the two 5-staples of one 7-staple
+// direction in one pass
+template <int MU, int P, int Q, int TT>
+__global__ void __launch_bounds__(128) k_sf_fwd2(Geom g, LinkField U, MF AP, MF AQ, MF out) {
+  int cb, pb;
+  if (!sf_site<TT>(g, cb, pb)) return;
+  const int po = pb ^ 1;
+  Mat3<double> a, b, t, up, lo;
+  {
+    const SfSites n = sf_sites<MU, P>(g, cb, pb);
+    mfl(AP, po, n.ypn, a);
+    mload(U, po, P, n.ypm, b);
+    mmul_na(a, b, t);
+    mload(U, pb, P, cb, b);
+    mmul(b, t, up);
+    mload(U, po, P, n.ymn, a);
+    mfl(AP, po, n.ymn, b);
+    mmul_an(a, b, t);
+    mload(U, pb, P, n.ymnpm, b);
+    mmul(t, b, lo);
+    madd(up, lo);
+  }
+  {
+    const SfSites n = sf_sites<MU, Q>(g, cb, pb);
+    mfl(AQ, po, n.ypn, a);
+    mload(U, po, Q, n.ypm, b);
+    mmul_na(a, b, t);
+    mload(U, pb, Q, cb, b);
+    mmul(b, t, lo);
+    madd(up, lo);
+    mload(U, po, Q, n.ymn, a);
+    mfl(AQ, po, n.ymn, b);
+    mmul_an(a, b, t);
+    mload(U, pb, Q, n.ymnpm, b);
+    mmul(t, b, lo);
+    madd(up, lo);
+  }
+  mfs(out, pb, cb, up);
+}
+typedef void (*SfFwd2Fn)(Geom, LinkField, MF, MF, MF);
+template <int MU, int P, int Q>
+static SfFwd2Fn sf_fwd2_pick(bool tt) {
+  return tt ? k_sf_fwd2<MU, P, Q, 1> : k_sf_fwd2<MU, P, Q, 0>;
+}
+// the fused kernel of mu and the pair p < q (both != mu)
+static SfFwd2Fn sf_fwd2_kernel(const Geom& g, int mu, int p, int q) {
+  const bool tt = (mu == 3 || q == 3) && g.st % 16 == 0 && g.X[3] % 4 == 0;
+  switch (mu * 16 + p * 4 + q) {
+    case 0 * 16 + 1 * 4 + 2: return sf_fwd2_pick<0, 1, 2>(tt);
+    case 0 * 16 + 1 * 4 + 3: return sf_fwd2_pick<0, 1, 3>(tt);
+    case 0 * 16 + 2 * 4 + 3: return sf_fwd2_pick<0, 2, 3>(tt);
+    case 1 * 16 + 0 * 4 + 2: return sf_fwd2_pick<1, 0, 2>(tt);
+    case 1 * 16 + 0 * 4 + 3: return sf_fwd2_pick<1, 0, 3>(tt);
+    case 1 * 16 + 2 * 4 + 3: return sf_fwd2_pick<1, 2, 3>(tt);
+    case 2 * 16 + 0 * 4 + 1: return sf_fwd2_pick<2, 0, 1>(tt);
+    case 2 * 16 + 0 * 4 + 3: return sf_fwd2_pick<2, 0, 3>(tt);
+    case 2 * 16 + 1 * 4 + 3: return sf_fwd2_pick<2, 1, 3>(tt);
+    case 3 * 16 + 0 * 4 + 1: return sf_fwd2_pick<3, 0, 1>(tt);
+    case 3 * 16 + 0 * 4 + 2: return sf_fwd2_pick<3, 0, 2>(tt);
+    default: return sf_fwd2_pick<3, 1, 2>(tt);
+  }
+}
+
 // reverse of O = S_nu(A) for the force Gs on O scaled by cg:
 //   FA(y)    += cg [U_nu(y-nu+mu)^dag G(y-nu) U_nu(y-nu) + U_nu(y+mu) G(y+nu) U_nu(y)^dag]
 //   FU_nu(y) += cg [A(y+nu) U_nu(y+mu)^dag G(y) + (G(y-mu) U_nu(y-mu) A(y-mu+nu))^dag
@@ -1508,8 +1571,9 @@ static int smear_adjoint_dev(const LinkField& L, const LinkField& F1, const Link
         for (int i = 0; i < 3; i++) {
           const int sig = od[i], nu = od[(i + 1) % 3], rho = od[(i + 2) % 3];
           adj(1, mu, sig, none, mF, c7, mG2[sig], none, 1, c5);
-          fwd(mu, rho, mS1[nu], mSS, 0);
-          fwd(mu, nu, mS1[rho], mSS, 1);
+          const int p = std::min(nu, rho), q = std::max(nu, rho);
+          hipLaunchKernelGGL(sf_fwd2_kernel(c.geom, mu, p, q), xgrid, blk, 0, c.stream, c.geom, L, mS1[q], mS1[p],
+                             mSS);
           adj(2, mu, sig, mSS, mF, c7, none, mf_link(G, sig), 0, 0.0);
         }
       for (int i = 0; i < 3; i++) {
