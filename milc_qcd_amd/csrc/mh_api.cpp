@@ -160,7 +160,7 @@ int milc_hip_init(const int dims[4], int device) {
   for (int i = 0; i < 4; i++) c.dims[i] = dims[i];
   c.geom = make_geom(dims);
   c.Vh = c.geom.Vh;
-  c.stride = (c.Vh + 63) / 64 * 64;
+  c.stride = plane_stride(c.Vh);
   MH_CHECK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
   c.ws.max_blocks = (c.Vh + 255) / 256 + 1024;
   rc = dev_alloc((void**)&c.ws.partials, (size_t)c.ws.max_blocks * 4 * 2 * sizeof(double));
