@@ -165,7 +165,7 @@ MS_SIGMAS = [1e-4 * 10 ** (4 * i / 11) for i in range(12)]
 MS_GAUGE_SEED = 1002
 MS_SRC_SEEDS = [2200 + j for j in range(16)]
 MS_RESID = 1e-9        # per-shift FP64 true residual
-MS_STAGE_RESID = 1e-7  # FP32 multi-shift stage (base system), then per-shift refinement
+MS_STAGE_RESID = 1e-8  # FP32 multi-shift stage (base system), then per-shift refinement
 
 
 def cpu_multishift_baseline(G, dims, offs, fp64_iters, nsrc, budget_s=15.0):

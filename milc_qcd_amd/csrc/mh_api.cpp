@@ -1165,7 +1165,7 @@ extern "C" int milc_hip_dev_multishift_block(int nsrc, const int* src_slots, int
   prm.max_iters = max_iters > 0 ? max_iters : 20000;
   prm.resid = resid;
   prm.refine_resid = refine_resid;
-  prm.delta = 0.1;
+  prm.delta = kBatchReliableDelta;
   BMSStats st;
   std::vector<double> offs(offsets, offsets + num_offsets);
   if ((rc = bmultishift(b.p, nr, offs, prm, xp, st))) return rc;
@@ -1236,7 +1236,7 @@ extern "C" int milc_hip_multishift_invert_block(int external_precision, int nsrc
     prm.max_iters = args.max_iter > 0 ? args.max_iter : 20000;
     prm.resid = target_residual[0];
     prm.refine_resid = refine_residual;
-    prm.delta = 0.1;
+    prm.delta = kBatchReliableDelta;
     BMSStats st;
     std::vector<double> offs(offsets, offsets + num_offsets);
     if ((rc = bmultishift(b.p, nr, offs, prm, xp, st))) {
@@ -1296,7 +1296,7 @@ extern "C" int milc_hip_invert_block(int external_precision, int nsrc, double ma
     cp.parity = parity;
     cp.max_iters = args.max_iter > 0 ? args.max_iter : 20000;
     cp.resid = target_residual;
-    cp.delta = 0.1;
+    cp.delta = kBatchReliableDelta;
     BCGStats cs;
     if ((rc = bcg_mixed(b.p, x.p, nr, off, cp, cs))) {
       free_vec(v);

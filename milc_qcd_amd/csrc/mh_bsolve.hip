@@ -724,7 +724,7 @@ int bcg_mixed_nr(const void* b64, void* x64, const double* offsets, const BCGPar
   memset(hs.data(), 0, NR * sizeof(BCGR));
   BCGG hg0;
   memset(&hg0, 0, sizeof hg0);
-  const double delta = prm.delta > 0 ? prm.delta : 0.1;
+  const double delta = prm.delta > 0 ? prm.delta : kBatchReliableDelta;
   // K sloppy iterations (d_congrad5_fn_milc.c:281-367, FEWSUMS)
   auto iterations = [&]() -> int {
     int r2;

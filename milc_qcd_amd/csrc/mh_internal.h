@@ -187,6 +187,17 @@ struct SolveParams {
 // box: 1e-3 gives 365-368 iterations and 93.0-94.4 ms per solve, 3e-4
 // 359-365 iterations and 91.4 ms, 1e-4 368-375 iterations and 96.3 ms.)
 constexpr double kReliableDelta = 3e-4;
+// The same threshold for the batched mixed CG, whose sloppy phase runs in
+// FP32 (24^3x48, 16 sources x 12 shifts refined to 1e-9: 0.1 takes 0.471 s of
+// refinement, 1e-2 0.438 s, 1e-3 0.433 s).
+constexpr double kBatchReliableDelta = 1e-3;
+// Base-system residual of an FP32 multi-shift stage that per-shift refinement
+// takes to an FP64 target below it: the refinement starts from the stage
+// solutions, and a deeper stage is cheaper than more refinement (same
+// workload: stage 1e-7 with 176 multi-shift iterations, 84 refinement
+// iterations for the lightest shift, 0.749 s; 1e-8 with 214 and 58, 0.706 s;
+// 3e-9 stalls at the FP32 floor: 219 and 57).
+constexpr double kFp32StageResid = 1e-8;
 
 struct SolveStats {
   double final_rsq = 0, final_relrsq = 0, size_r = 0;

@@ -156,7 +156,7 @@ static int rational_solve_all(const std::vector<const VecField*>& src, const std
       flat[(size_t)j * nr + r] = o[std::min(j, (int)o.size() - 1)];
     }
   for (int r = 0; r < ns; r++) {
-    stage[r] = std::max(resid[r], 1e-7);
+    stage[r] = std::max(resid[r], kFp32StageResid);
     refine[r] = resid[r];
   }
   void* b = nullptr;
@@ -173,7 +173,7 @@ static int rational_solve_all(const std::vector<const VecField*>& src, const std
   prm.max_iters = 20000;
   prm.resid = 0;
   prm.refine_resid = 0;
-  prm.delta = 0.1;
+  prm.delta = kBatchReliableDelta;
   BMSStats st;
   if (!rc) rc = bmultishift_sets(b, nr, n, flat.data(), stage.data(), refine.data(), prm, xb, st);
   x.assign(ns, {});
