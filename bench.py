@@ -207,7 +207,8 @@ def cpu_multishift_baseline(G, dims, offs, fp64_iters, nsrc, budget_s=15.0):
 def multishift_workload(args):
     """16 sources x 12 shifts in one batched solve (FP32 multi-shift with FP64
     recurrences and deferred shift updates, captured window graph, autotuned
-    batched stencil; per-shift mixed refinement), against 16 one-source solves
+    batched stencil; per-shift mixed refinement with the 16-bit batched
+    stencil and FP64 reliable updates), against 16 one-source solves
     of the same solver family and an FP64 reference solve of source 0."""
     import milc_qcd_amd as mq
     offs = [4 * MS_MASS ** 2 + s for s in MS_SIGMAS]
@@ -267,7 +268,7 @@ def multishift_workload(args):
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 multi-shift + f64 recurrences, f32/f64 mixed refinement",
+        "dtype": "f32 multi-shift + f64 recurrences, 16-bit/f64 mixed refinement (batched 16-bit stencil)",
         "data": "synthetic",
         "config": {"workload": "hisq_multishift_batched_24x24x24x48_m0.02_12shifts_16src", "lattice": list(MS_DIMS),
                    "mass": MS_MASS, "sigmas": MS_SIGMAS, "sources": 16, "gauge_seed": MS_GAUGE_SEED,

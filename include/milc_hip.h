@@ -418,7 +418,8 @@ int milc_hip_dev_dslash_bench(int prec, int parity, int reps, double* usec);
  * the measured HBM reference next to the vendor peak. */
 int milc_hip_dev_copy_bench(size_t bytes, int reps, double* usec);
 /* Batched stencil autotuner: times every variant (VALU threads-per-site
- * shapes and the MFMA 4x4x1 form) for prec (1 float, 2 double) and nrhs
+ * shapes and the MFMA 4x4x1 form) for prec (1 float, 2 double, 3 the 16-bit
+ * records and links of the batched CG's sloppy operator) and nrhs
  * (4, 8, 16) right-hand sides and keeps the fastest; returns the number of
  * variants, the chosen index, its time and a description. */
 int milc_hip_dev_block_tune(int prec, int nrhs, int* variant, double* usec, char* desc, int ndesc);

@@ -1073,11 +1073,15 @@ extern "C" int milc_hip_dev_dslash(int prec, int slot_src, int parity_out, int s
   return rc;
 }
 
+// precision code of the batched-stencil entry points: 1 float, 2 double,
+// 3 the 16-bit records and links of the batched CG's sloppy operator
+static int block_prec(int prec) { return prec == 2 ? P_F64 : (prec == 3 ? P_H16 : P_F32); }
+
 extern "C" int milc_hip_dev_block_tune(int prec, int nrhs, int* variant, double* usec, char* desc, int ndesc) {
   Context& c = ctx();
   if (!c.initialised || !c.links[P_F64].valid) return -1;
-  int ip = prec == 2 ? P_F64 : P_F32;
-  if (ip == P_F32 && ensure_links(P_F32)) return -3;
+  int ip = block_prec(prec);
+  if (ip != P_F64 && ensure_links(ip)) return -3;
   int v;
   double us;
   int rc = btune(ip, nrhs, &v, &us);
@@ -1091,7 +1095,7 @@ extern "C" int milc_hip_dev_block_tune(int prec, int nrhs, int* variant, double*
 extern "C" int milc_hip_dev_block_describe(int prec, int nrhs, int variant, char* desc, int ndesc) {
   Context& c = ctx();
   if (!c.initialised || !desc || ndesc <= 0) return -1;
-  int ip = prec == 2 ? P_F64 : P_F32;
+  int ip = block_prec(prec);
   int n = (int)bvariants(ip, nrhs).size();
   if (variant < 0 || variant >= n) return -2;
   bdescribe(ip, nrhs, variant, desc, ndesc);
@@ -1101,8 +1105,8 @@ extern "C" int milc_hip_dev_block_describe(int prec, int nrhs, int variant, char
 extern "C" int milc_hip_dev_block_dslash_bench(int prec, int nrhs, int variant, int reps, double* usec) {
   Context& c = ctx();
   if (!c.initialised || !c.links[P_F64].valid) return -1;
-  int ip = prec == 2 ? P_F64 : P_F32;
-  if (ip == P_F32 && ensure_links(P_F32)) return -3;
+  int ip = block_prec(prec);
+  if (ip != P_F64 && ensure_links(ip)) return -3;
   return btime_variant(ip, nrhs, variant, reps, usec);
 }
 
@@ -1111,9 +1115,9 @@ extern "C" int milc_hip_dev_block_dslash(int prec, int nrhs, int variant, int ns
   Context& c = ctx();
   if (!c.initialised || !c.links[P_F64].valid) return -1;
   if (nsrc <= 0 || nsrc > nrhs || (nrhs != 4 && nrhs != 8 && nrhs != 16)) return -2;
-  int ip = prec == 2 ? P_F64 : P_F32;
+  int ip = block_prec(prec);
   int rc;
-  if (ip == P_F32 && (rc = ensure_links(P_F32))) return rc;
+  if (ip != P_F64 && (rc = ensure_links(ip))) return rc;
   for (int s = 0; s < nsrc; s++)
     if (!slot_ok(src_slots[s], true) || !slot_ok(dst_slots[s], false)) return -2;
   BBuf a, b;

@@ -3,7 +3,8 @@
 //
 // Batched field of one parity, NR right-hand sides (NR = 4, 8 or 16):
 //   record (site cb, rhs r) = 6 reals  [re c0, re c1, re c2, im c0, im c1, im c2]
-//   at offset (cb * NR + r) * 6; R = float (F32) or double (F64).
+//   at offset (cb * NR + r) * 6; R = float (F32) or double (F64);
+//   16-bit (P_H16): one int4 per record at cb * NR + r (mh_bkern.h).
 // All NR records of a site are contiguous, so a wavefront that spreads the
 // right-hand sides of a few sites over its lanes reads whole cache lines, and
 // the link matrices of a site are fetched once for all NR vectors.
@@ -64,12 +65,19 @@ int bpack(const VecField& in, void* batched, int prec, int nr, int r);
 int bunpack(const void* batched, int prec, int nr, int r, VecField& out);
 
 // ---- solvers ------------------------------------------------------------------
+// Sloppy precision of the batched mixed CG (per-shift refinement after a
+// batched multi-shift, block solves): 16-bit records and links.  24^3x48, 16
+// sources x 12 shifts refined from the FP32 stage to 1e-9: same iteration
+// counts as FP32 sloppy, 0.347-0.351 s of refinement against 0.426-0.432 s.
+constexpr int kBatchSloppy = P_H16;
+
 struct BMSParams {
   int parity;          // MILC code of the solve parity
   int max_iters;
   double resid;        // base-system |r|/|b| target of the multi-shift stage
   double refine_resid; // true |r|/|b| target per shift after refinement (0: none)
   double delta;        // reliable-update threshold of the refinement
+  int refine_sloppy = kBatchSloppy;  // sloppy precision of the refinement CG (BCGParams::sloppy)
 };
 struct BMSStats {
   int ms_iters = 0;                 // multi-shift iterations (max over rhs)
@@ -94,6 +102,7 @@ struct BCGParams {
   double resid;       // true |r|/|b| target (all rhs)
   double delta;       // reliable-update threshold
   const double* resid_r = nullptr;  // per-rhs targets (host array of nr), overriding resid
+  int sloppy = kBatchSloppy;  // sloppy operator and search direction: P_F32 or P_H16 (16-bit records)
 };
 struct BCGStats {
   int iters = 0;               // max over rhs
@@ -102,8 +111,9 @@ struct BCGStats {
   int converged[BNR_MAX] = {};
   double seconds = 0;
 };
-// Mixed-precision batched CG: FP32 sloppy iterations, FP64 reliable updates;
-// x64 holds the initial guess; offsets[r] = 4 m^2 (+ sigma) per rhs.
+// Mixed-precision batched CG: FP32 (or 16-bit operator and search direction:
+// prm.sloppy) sloppy iterations, FP64 reliable updates; x64 holds the initial
+// guess; offsets[r] = 4 m^2 (+ sigma) per rhs.
 int bcg_mixed(const void* b64, void* x64, int nr, const double* offsets, const BCGParams& prm, BCGStats& st);
 
 }  // namespace mh

@@ -28,6 +28,7 @@
 
 #include "mh_batch.h"
 #include "mh_bkern.h"
+#include "mh_dslash.h"
 
 namespace mh {
 
@@ -125,6 +126,94 @@ __global__ void __launch_bounds__(256) k_bds_valu(Geom g, LinkField fat, LinkFie
           d[2 % NV][i] += (double)t[i].c[k].re * (double)t[i].c[k].re + (double)t[i].c[k].im * (double)t[i].c[k].im;
         }
     }
+  }
+  if (KIND != BK_STORE) bsum_valu<NV, NR, TPS>(d, partials, pstride);
+}
+
+// 16-bit form for the sloppy operator of the batched mixed CG: 16-bit links
+// (fat nine elements; LR = 1: Naik links in the two-row form), 16-bit source
+// records, FP32 arithmetic.  BK_STORE writes 16-bit records; BK_CG reads the
+// 16-bit p and the FP32 r and writes the FP32 dst.
+template <int NR, int TPS, int KIND, int LR>
+__global__ void __launch_bounds__(256) k_bds_h16(Geom g, LinkField fat, LinkField lng, const int4* __restrict__ src,
+                                                 void* __restrict__ dst, const int4* __restrict__ pv,
+                                                 const float* __restrict__ rv, BCoef<float> coef, int pb, int loop,
+                                                 BOrder ord, double* __restrict__ partials, int pstride,
+                                                 const int* __restrict__ halt_all) {
+  static_assert(KIND == BK_STORE || KIND == BK_CG, "16-bit batched stencil: store or CG form");
+  if (halt_all && *halt_all) return;
+  constexpr int RPT = NR / TPS;
+  constexpr int NV = 3;
+  const int q = threadIdx.x % TPS;
+  const int gid = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  const int per_pass = (gridDim.x * blockDim.x) / TPS;
+  double d[NV][RPT];
+#pragma unroll
+  for (int v = 0; v < NV; v++)
+#pragma unroll
+    for (int i = 0; i < RPT; i++) d[v][i] = 0.0;
+  const int po = pb ^ 1;
+  for (int it = 0; it < loop; it++) {
+    int li = it * per_pass + gid / TPS;
+    bool valid = li < g.Vh;
+    if (!valid) li = g.Vh - 1;
+    Coord c;
+    const int cb = border_site(g, ord, li, pb, c);
+    Vec3<float> acc[RPT];
+#pragma unroll
+    for (int i = 0; i < RPT; i++) zero(acc[i]);
+    const int r0 = RPT * q;
+#pragma unroll 1
+    for (int mu = 0; mu < 4; mu++) {
+      Mat3<float> m;
+      Vec3<float> v[RPT];
+      int n = neighbour(g, c, cb, mu, 1);
+      load_stencil_link<P_H16, 0>(fat, pb, mu, cb, m);
+      bld_blk_h16<NR, RPT>(src, n, r0, v);
+#pragma unroll
+      for (int i = 0; i < RPT; i++) mv_acc(m, v[i], acc[i]);
+      n = neighbour(g, c, cb, mu, 3);
+      load_stencil_link<P_H16, LR>(lng, pb, mu, cb, m);
+      bld_blk_h16<NR, RPT>(src, n, r0, v);
+#pragma unroll
+      for (int i = 0; i < RPT; i++) mv_acc(m, v[i], acc[i]);
+      n = neighbour(g, c, cb, mu, -1);
+      load_stencil_link<P_H16, 0>(fat, po, mu, n, m);
+      bld_blk_h16<NR, RPT>(src, n, r0, v);
+#pragma unroll
+      for (int i = 0; i < RPT; i++) amv_sub(m, v[i], acc[i]);
+      n = neighbour(g, c, cb, mu, -3);
+      load_stencil_link<P_H16, LR>(lng, po, mu, n, m);
+      bld_blk_h16<NR, RPT>(src, n, r0, v);
+#pragma unroll
+      for (int i = 0; i < RPT; i++) amv_sub(m, v[i], acc[i]);
+    }
+    if (KIND == BK_STORE) {
+      if (valid) bst_blk_h16<NR, RPT>((int4*)dst, cb, r0, acc);
+      continue;
+    }
+    Vec3<float> pp[RPT], t[RPT], rr[RPT];
+    bld_blk_h16<NR, RPT>(pv, cb, r0, pp);
+#pragma unroll
+    for (int i = 0; i < RPT; i++) {
+      const float a = coef.c[r0 + i];
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        t[i].c[k].re = acc[i].c[k].re + a * pp[i].c[k].re;
+        t[i].c[k].im = acc[i].c[k].im + a * pp[i].c[k].im;
+      }
+    }
+    if (!valid) continue;
+    bst_blk<float, NR, RPT>((float*)dst, cb, r0, t);
+    bld_blk<float, NR, RPT>(rv, cb, r0, rr);
+#pragma unroll
+    for (int i = 0; i < RPT; i++)
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        d[0][i] += (double)pp[i].c[k].re * (double)t[i].c[k].re + (double)pp[i].c[k].im * (double)t[i].c[k].im;
+        d[1][i] += (double)rr[i].c[k].re * (double)t[i].c[k].re + (double)rr[i].c[k].im * (double)t[i].c[k].im;
+        d[2][i] += (double)t[i].c[k].re * (double)t[i].c[k].re + (double)t[i].c[k].im * (double)t[i].c[k].im;
+      }
   }
   if (KIND != BK_STORE) bsum_valu<NV, NR, TPS>(d, partials, pstride);
 }
@@ -283,6 +372,35 @@ __global__ void k_bunpack(const R* __restrict__ in, int r, VecField out, int Vh)
   store_vec<P_F64>(out, cb, v);
 }
 
+template <int NR>
+__global__ void k_bpack_h16(VecField in, int4* __restrict__ out, int r, int Vh) {
+  int cb = blockIdx.x * blockDim.x + threadIdx.x;
+  if (cb >= Vh) return;
+  Vec3<double> v;
+  load_vec<P_F64>(in, cb, v);
+  Vec3<float> w;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    w.c[k].re = (float)v.c[k].re;
+    w.c[k].im = (float)v.c[k].im;
+  }
+  out[(size_t)cb * NR + r] = h16_pack(w);
+}
+template <int NR>
+__global__ void k_bunpack_h16(const int4* __restrict__ in, int r, VecField out, int Vh) {
+  int cb = blockIdx.x * blockDim.x + threadIdx.x;
+  if (cb >= Vh) return;
+  Vec3<float> w;
+  h16_unpack(in[(size_t)cb * NR + r], w);
+  Vec3<double> v;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    v.c[k].re = (double)w.c[k].re;
+    v.c[k].im = (double)w.c[k].im;
+  }
+  store_vec<P_F64>(out, cb, v);
+}
+
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
@@ -294,10 +412,10 @@ struct BWork {
   double* partials = nullptr;
   float* W = nullptr;
   unsigned W_version = ~0u;
-  int tuned[2][3];      // [prec][nr index] -> variant, -1 untuned
-  double tuned_us[2][3];
+  int tuned[3][3];      // [prec][nr index] -> variant, -1 untuned
+  double tuned_us[3][3];
   BWork() {
-    for (int a = 0; a < 2; a++)
+    for (int a = 0; a < 3; a++)
       for (int b = 0; b < 3; b++) tuned[a][b] = -1, tuned_us[a][b] = 0;
   }
 };
@@ -311,7 +429,7 @@ BOrder order_of(int k) {
   if (k <= 0 || k > 3) return BOrder{};
   return make_border(ctx().geom, kOrders[k][0], kOrders[k][1], kOrders[k][2], kOrders[k][3]);
 }
-int prec_index(int prec) { return prec == P_F64 ? 0 : (prec == P_F32 ? 1 : -1); }
+int prec_index(int prec) { return prec == P_F64 ? 0 : (prec == P_F32 ? 1 : (prec == P_H16 ? 2 : -1)); }
 
 int ensure_partials() {
   if (g_bw.partials) return 0;
@@ -369,6 +487,54 @@ int launch_valu(const BDOp& op, int block, int loop_min, int order, int* nb_out)
   return 0;
 }
 
+template <int NR, int TPS>
+int launch_h16(const BDOp& op, int block, int loop_min, int order, int* nb_out) {
+  Context& c = ctx();
+  if (op.kind != BK_STORE && op.kind != BK_CG) return -2;
+  long threads1 = (long)c.Vh * TPS;
+  long blocks1 = (threads1 + block - 1) / block;
+  int loop = loop_for(blocks1, loop_min);
+  int nb = (int)((blocks1 + loop - 1) / loop);
+  const BOrder ord = order_of(order);
+  BCoef<float> cf;
+  for (int r = 0; r < BNR_MAX; r++) cf.c[r] = (float)op.coef[r];
+  const DevLinks& L = c.links[P_H16];
+  if (!L.valid) return -3;
+  const int4* src = (const int4*)op.src;
+  const int4* pv = (const int4*)op.p;
+  const float* rv = (const float*)op.r;
+  const bool lr = L.lng.recon != 0;
+#define MH_H16_LAUNCH(KIND, LR)                                                                                   \
+  hipLaunchKernelGGL((k_bds_h16<NR, TPS, KIND, LR>), dim3(nb), dim3(block), 0, c.stream, c.geom, L.fat, L.lng, src, \
+                     op.dst, pv, rv, cf, op.pb, loop, ord, g_bw.partials, kBlockCap, op.halt_all)
+  if (op.kind == BK_STORE) {
+    if (lr) MH_H16_LAUNCH(BK_STORE, 1);
+    else MH_H16_LAUNCH(BK_STORE, 0);
+  } else {
+    if (lr) MH_H16_LAUNCH(BK_CG, 1);
+    else MH_H16_LAUNCH(BK_CG, 0);
+  }
+#undef MH_H16_LAUNCH
+  MH_CHECK(hipGetLastError());
+  if (nb_out) *nb_out = nb;
+  return 0;
+}
+
+template <int NR>
+int launch_h16_tps(const BDOp& op, const BVariant& v, int* nb) {
+  switch (v.tps) {
+    case 2:
+      if constexpr (NR / 2 <= 8) return launch_h16<NR, 2>(op, v.block, v.loop, v.order, nb);
+      break;
+    case 4:
+      return launch_h16<NR, 4>(op, v.block, v.loop, v.order, nb);
+    case 8:
+      if constexpr (NR >= 8) return launch_h16<NR, 8>(op, v.block, v.loop, v.order, nb);
+      break;
+  }
+  return -2;
+}
+
 template <int NR>
 int launch_mfma(const BDOp& op, int block, int loop_min, int order, int* nb_out) {
   Context& c = ctx();
@@ -423,6 +589,12 @@ int launch_valu_tps(const BDOp& op, const BVariant& v, int* nb) {
 }
 
 int launch_variant(const BDOp& op, const BVariant& v, int* nb) {
+  if (op.prec == P_H16) {
+    if (v.mfma) return -2;
+    if (op.nr == 4) return launch_h16_tps<4>(op, v, nb);
+    if (op.nr == 8) return launch_h16_tps<8>(op, v, nb);
+    return launch_h16_tps<16>(op, v, nb);
+  }
   if (v.mfma) {
     if (op.prec != P_F32) return -2;
     if (op.nr == 4) return launch_mfma<4>(op, v.block, v.loop, v.order, nb);
@@ -442,6 +614,7 @@ int launch_variant(const BDOp& op, const BVariant& v, int* nb) {
 }  // namespace
 
 size_t bvec_bytes(int prec, int nr) {
+  if (prec == P_H16) return (size_t)ctx().Vh * nr * sizeof(int4);
   return (size_t)ctx().Vh * nr * 6 * (prec == P_F64 ? sizeof(double) : sizeof(float));
 }
 
@@ -462,6 +635,7 @@ std::vector<BVariant> bvariants(int prec, int nr) {
   for (int order : orders)
     for (int tps : {1, 2, 4, 8}) {
       if (tps > nr || nr / tps > rpt_max) continue;
+      if (prec == P_H16 && tps == 1) continue;
       for (int block : {128, 256})
         for (int loop : {1, 2}) {
           if (order && loop > 1) continue;
@@ -494,8 +668,8 @@ void bdescribe(int prec, int nr, int variant, char* buf, int n) {
   if (v.mfma)
     snprintf(buf, n, "mfma_f32_4x4x1 nr=%d sites/wave=%d block=%d loop=%d %s", nr, 32 / nr, v.block, v.loop, ord);
   else
-    snprintf(buf, n, "valu %s nr=%d threads/site=%d rhs/thread=%d block=%d loop=%d %s", prec == P_F64 ? "f64" : "f32",
-             nr, v.tps, nr / v.tps, v.block, v.loop, ord);
+    snprintf(buf, n, "valu %s nr=%d threads/site=%d rhs/thread=%d block=%d loop=%d %s",
+             prec == P_F64 ? "f64" : (prec == P_H16 ? "16-bit" : "f32"), nr, v.tps, nr / v.tps, v.block, v.loop, ord);
 }
 
 int bpartials(double** p, int* pstride) {
@@ -593,7 +767,11 @@ int bdslash(const BDOp& op, int* nblocks_out) {
 int bpack(const VecField& in, void* batched, int prec, int nr, int r) {
   Context& c = ctx();
   dim3 grid((c.Vh + 255) / 256);
-  if (prec == P_F64) {
+  if (prec == P_H16) {
+    if (nr == 4) hipLaunchKernelGGL((k_bpack_h16<4>), grid, dim3(256), 0, c.stream, in, (int4*)batched, r, c.Vh);
+    else if (nr == 8) hipLaunchKernelGGL((k_bpack_h16<8>), grid, dim3(256), 0, c.stream, in, (int4*)batched, r, c.Vh);
+    else hipLaunchKernelGGL((k_bpack_h16<16>), grid, dim3(256), 0, c.stream, in, (int4*)batched, r, c.Vh);
+  } else if (prec == P_F64) {
     if (nr == 4) hipLaunchKernelGGL((k_bpack<double, 4>), grid, dim3(256), 0, c.stream, in, (double*)batched, r, c.Vh);
     else if (nr == 8) hipLaunchKernelGGL((k_bpack<double, 8>), grid, dim3(256), 0, c.stream, in, (double*)batched, r, c.Vh);
     else hipLaunchKernelGGL((k_bpack<double, 16>), grid, dim3(256), 0, c.stream, in, (double*)batched, r, c.Vh);
@@ -609,7 +787,11 @@ int bpack(const VecField& in, void* batched, int prec, int nr, int r) {
 int bunpack(const void* batched, int prec, int nr, int r, VecField& out) {
   Context& c = ctx();
   dim3 grid((c.Vh + 255) / 256);
-  if (prec == P_F64) {
+  if (prec == P_H16) {
+    if (nr == 4) hipLaunchKernelGGL((k_bunpack_h16<4>), grid, dim3(256), 0, c.stream, (const int4*)batched, r, out, c.Vh);
+    else if (nr == 8) hipLaunchKernelGGL((k_bunpack_h16<8>), grid, dim3(256), 0, c.stream, (const int4*)batched, r, out, c.Vh);
+    else hipLaunchKernelGGL((k_bunpack_h16<16>), grid, dim3(256), 0, c.stream, (const int4*)batched, r, out, c.Vh);
+  } else if (prec == P_F64) {
     if (nr == 4) hipLaunchKernelGGL((k_bunpack<double, 4>), grid, dim3(256), 0, c.stream, (const double*)batched, r, out, c.Vh);
     else if (nr == 8) hipLaunchKernelGGL((k_bunpack<double, 8>), grid, dim3(256), 0, c.stream, (const double*)batched, r, out, c.Vh);
     else hipLaunchKernelGGL((k_bunpack<double, 16>), grid, dim3(256), 0, c.stream, (const double*)batched, r, out, c.Vh);

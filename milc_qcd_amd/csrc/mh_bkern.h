@@ -161,6 +161,45 @@ __device__ __forceinline__ void bst_blk(R* __restrict__ f, int cb, int r0, const
   }
 }
 
+// 16-bit batched records (the sloppy vectors of the batched mixed CG): record
+// (site cb, rhs r) = one int4 at cb * NR + r holding (re, im) of colours 0, 1,
+// 2 as 16-bit fixed point of the record's norm (its largest |component|), the
+// norm itself as a float in the fourth word: the single-rhs 16-bit vector
+// record, so a site's NR records are NR * 16 contiguous bytes.
+__device__ __forceinline__ void h16_unpack(const int4& w, Vec3<float>& v) {
+  const float n = __int_as_float(w.w) * (1.0f / 32767.0f);
+  v.c[0].re = q16lo(w.x) * n;
+  v.c[0].im = q16hi(w.x) * n;
+  v.c[1].re = q16lo(w.y) * n;
+  v.c[1].im = q16hi(w.y) * n;
+  v.c[2].re = q16lo(w.z) * n;
+  v.c[2].im = q16hi(w.z) * n;
+}
+__device__ __forceinline__ int4 h16_pack(const Vec3<float>& v) {
+  float m = 0.f;
+#pragma unroll
+  for (int k = 0; k < 3; k++) m = fmaxf(m, fmaxf(fabsf(v.c[k].re), fabsf(v.c[k].im)));
+  const float inv = m > 0.f ? 1.0f / m : 0.f;
+  int4 o;
+  o.x = q16pack(v.c[0].re * inv, v.c[0].im * inv);
+  o.y = q16pack(v.c[1].re * inv, v.c[1].im * inv);
+  o.z = q16pack(v.c[2].re * inv, v.c[2].im * inv);
+  o.w = __float_as_int(m);
+  return o;
+}
+template <int NR, int RPT>
+__device__ __forceinline__ void bld_blk_h16(const int4* __restrict__ f, int cb, int r0, Vec3<float> (&v)[RPT]) {
+  const int4* a = f + (size_t)cb * NR + r0;
+#pragma unroll
+  for (int i = 0; i < RPT; i++) h16_unpack(a[i], v[i]);
+}
+template <int NR, int RPT>
+__device__ __forceinline__ void bst_blk_h16(int4* __restrict__ f, int cb, int r0, const Vec3<float> (&v)[RPT]) {
+  int4* a = f + (size_t)cb * NR + r0;
+#pragma unroll
+  for (int i = 0; i < RPT; i++) a[i] = h16_pack(v[i]);
+}
+
 // Per-rhs block sums when rhs = lane % NR: written to LDS, then one value per
 // (dot, rhs) channel to partials[ch * pstride + blockIdx.x].
 template <int NV, int NR>

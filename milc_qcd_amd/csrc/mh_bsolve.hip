@@ -449,6 +449,73 @@ __global__ void __launch_bounds__(256) k_bcg_update(float* __restrict__ x, float
   bsum_lanes<1, NR>(d, partials, pstride);
 }
 
+// The same with the search direction p in 16-bit records.
+template <int NR>
+__global__ void __launch_bounds__(256) k_bcg_update_h16(float* __restrict__ x, int4* __restrict__ p,
+                                                        float* __restrict__ r, const float* __restrict__ ttt,
+                                                        const BCGR* __restrict__ s, const BCGG* __restrict__ g,
+                                                        long nrec, int loop, double* __restrict__ partials,
+                                                        int pstride) {
+  if (!g->any_go) return;
+  const int rhs = (blockIdx.x * blockDim.x + threadIdx.x) % NR;
+  const bool live = s[rhs].go != 0;
+  const float a = (float)s[rhs].a, b = (float)s[rhs].b;
+  double d[1] = {0.0};
+  REC_LOOP(idx) {
+    if (idx >= nrec || !live) continue;
+    size_t o = (size_t)idx * 6;
+    Vec3<float> pv, pn;
+    h16_unpack(p[idx], pv);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      x[o + k] += a * pv.c[k].re;
+      x[o + 3 + k] += a * pv.c[k].im;
+      const float rre = r[o + k] + a * ttt[o + k], rim = r[o + 3 + k] + a * ttt[o + 3 + k];
+      r[o + k] = rre;
+      r[o + 3 + k] = rim;
+      pn.c[k].re = rre + b * pv.c[k].re;
+      pn.c[k].im = rim + b * pv.c[k].im;
+      d[0] += (double)rre * rre + (double)rim * rim;
+    }
+    p[idx] = h16_pack(pn);
+  }
+  bsum_lanes<1, NR>(d, partials, pstride);
+}
+
+// Reliable update with a 16-bit search direction: r_s = r (FP64 true residual
+// rounded), p = r_s on the first call, else p += r - r_s(old).
+template <int NR>
+__global__ void __launch_bounds__(256) k_bru_h16(const double* __restrict__ r64, float* __restrict__ rs,
+                                                 int4* __restrict__ ps, const BCGR* __restrict__ s, int first,
+                                                 long nrec, int loop, double* __restrict__ partials, int pstride) {
+  const int rhs = (blockIdx.x * blockDim.x + threadIdx.x) % NR;
+  const bool live = !s[rhs].halt;
+  double d[1] = {0.0};
+  REC_LOOP(idx) {
+    if (idx >= nrec || !live) continue;
+    size_t o = (size_t)idx * 6;
+    Vec3<float> pv;
+    if (!first) h16_unpack(ps[idx], pv);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const double hre = r64[o + k], him = r64[o + 3 + k];
+      const float nre = (float)hre, nim = (float)him;
+      if (first) {
+        pv.c[k].re = nre;
+        pv.c[k].im = nim;
+      } else {
+        pv.c[k].re = (float)((double)pv.c[k].re + (hre - (double)rs[o + k]));
+        pv.c[k].im = (float)((double)pv.c[k].im + (him - (double)rs[o + 3 + k]));
+      }
+      rs[o + k] = nre;
+      rs[o + 3 + k] = nim;
+      d[0] += (double)nre * nre + (double)nim * nim;
+    }
+    ps[idx] = h16_pack(pv);
+  }
+  bsum_lanes<1, NR>(d, partials, pstride);
+}
+
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
@@ -704,11 +771,13 @@ int bcg_mixed_nr(const void* b64, void* x64, const double* offsets, const BCGPar
   int pstride;
   if ((rc = bpartials(&part, &pstride))) return rc;
   const RecGrid rg = rec_grid(NR, pstride);
-  int v32, v64;
-  if ((rc = btune(P_F64, NR, &v64, nullptr)) || (rc = btune(P_F32, NR, &v32, nullptr))) return rc;
+  const int sp = prm.sloppy == P_H16 ? P_H16 : P_F32;  // operator / search-direction precision
+  if (sp == P_H16 && (rc = ensure_links(P_H16))) return rc;
+  int vs, v64;
+  if ((rc = btune(P_F64, NR, &v64, nullptr)) || (rc = btune(sp, NR, &vs, nullptr))) return rc;
   DevBuf tmp64, t64, r64, ps, tmps, ttts, rs, xs, sums, dscal, dglob;
   if ((rc = balloc(&tmp64.p, P_F64, NR)) || (rc = balloc(&t64.p, P_F64, NR)) || (rc = balloc(&r64.p, P_F64, NR)) ||
-      (rc = balloc(&ps.p, P_F32, NR)) || (rc = balloc(&tmps.p, P_F32, NR)) || (rc = balloc(&ttts.p, P_F32, NR)) ||
+      (rc = balloc(&ps.p, sp, NR)) || (rc = balloc(&tmps.p, sp, NR)) || (rc = balloc(&ttts.p, P_F32, NR)) ||
       (rc = balloc(&rs.p, P_F32, NR)) || (rc = balloc(&xs.p, P_F32, NR)))
     return rc;
   if ((rc = dev_alloc(&sums.p, 128 * sizeof(double))) || (rc = dev_alloc(&dscal.p, NR * sizeof(BCGR))) ||
@@ -730,14 +799,14 @@ int bcg_mixed_nr(const void* b64, void* x64, const double* offsets, const BCGPar
     int r2;
     for (int k = 0; k < BKW; k++) {
       BDOp d1;
-      d1.prec = P_F32;
+      d1.prec = sp;
       d1.nr = NR;
       d1.kind = BK_STORE;
       d1.pb = pb ^ 1;
       d1.src = ps.p;
       d1.dst = tmps.p;
       d1.halt_all = &dg->halt_all;
-      d1.variant = v32;
+      d1.variant = vs;
       if ((r2 = bdslash(d1, nullptr))) return r2;
       BDOp d2 = d1;
       d2.kind = BK_CG;
@@ -751,8 +820,12 @@ int bcg_mixed_nr(const void* b64, void* x64, const double* offsets, const BCGPar
       if ((r2 = bdslash(d2, &nb2))) return r2;
       if ((r2 = breduce(3 * NR, nb2, dsum))) return r2;
       hipLaunchKernelGGL(k_bcg_scalars, dim3(1), dim3(64), 0, c.stream, dsum, dsum_rr, NR, ds, dg);
-      hipLaunchKernelGGL(k_bcg_update<NR>, dim3(rg.nb), dim3(256), 0, c.stream, (float*)xs.p, (float*)ps.p,
-                         (float*)rs.p, (const float*)ttts.p, ds, dg, rg.nrec, rg.loop, part, pstride);
+      if (sp == P_H16)
+        hipLaunchKernelGGL(k_bcg_update_h16<NR>, dim3(rg.nb), dim3(256), 0, c.stream, (float*)xs.p, (int4*)ps.p,
+                           (float*)rs.p, (const float*)ttts.p, ds, dg, rg.nrec, rg.loop, part, pstride);
+      else
+        hipLaunchKernelGGL(k_bcg_update<NR>, dim3(rg.nb), dim3(256), 0, c.stream, (float*)xs.p, (float*)ps.p,
+                           (float*)rs.p, (const float*)ttts.p, ds, dg, rg.nrec, rg.loop, part, pstride);
       if ((r2 = breduce(NR, rg.nb, dsum_rr))) return r2;
     }
     MH_CHECK(hipGetLastError());
@@ -797,8 +870,12 @@ int bcg_mixed_nr(const void* b64, void* x64, const double* offsets, const BCGPar
     if ((rc = (int)hipMemcpyAsync(ds, hs.data(), NR * sizeof(BCGR), hipMemcpyHostToDevice, c.stream))) break;
     hg0.iter_cap = (int)std::min<long>(prm.max_iters - iteration, 1L << 30);
     if ((rc = (int)hipMemcpyAsync(dg, &hg0, sizeof(BCGG), hipMemcpyHostToDevice, c.stream))) break;
-    hipLaunchKernelGGL(k_bru<NR>, dim3(rg.nb), dim3(256), 0, c.stream, (const double*)r64.p, (float*)rs.p,
-                       (float*)ps.p, ds, first ? 1 : 0, rg.nrec, rg.loop, part, pstride);
+    if (sp == P_H16)
+      hipLaunchKernelGGL(k_bru_h16<NR>, dim3(rg.nb), dim3(256), 0, c.stream, (const double*)r64.p, (float*)rs.p,
+                         (int4*)ps.p, ds, first ? 1 : 0, rg.nrec, rg.loop, part, pstride);
+    else
+      hipLaunchKernelGGL(k_bru<NR>, dim3(rg.nb), dim3(256), 0, c.stream, (const double*)r64.p, (float*)rs.p,
+                         (float*)ps.p, ds, first ? 1 : 0, rg.nrec, rg.loop, part, pstride);
     if ((rc = (int)hipGetLastError())) break;
     if ((rc = breduce(NR, rg.nb, dsum_rr))) break;
     if (!first) nru++;
@@ -844,6 +921,7 @@ int bmultishift_sets(const void* b64, int nr, int n, const double* offs, const d
     cp.resid = refine_resid[0];
     cp.resid_r = refine_resid;
     cp.delta = prm.delta;
+    cp.sloppy = prm.refine_sloppy;
     BCGStats cs;
     rc = bcg_mixed(b64, x64[j], nr, off, cp, cs);
     if (rc) return rc;
