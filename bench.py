@@ -30,6 +30,8 @@ Usage: python bench.py [--gpus 1] [--steps 10] [--warmup 1] [--no-cpu]
        python bench.py --workload trajectory   (one RHMC trajectory at 48^3 x 96 with the
            rational functions of MILC's HISQ RHMC test; FP64 dH)
        --no-sequential (multishift: leave out the one-source comparison solves)
+       --no-reference (multishift: leave out the FP64 one-source reference solves too,
+           so that only the batched solver runs, e.g. under a profiler)
 """
 import argparse
 import json
@@ -226,6 +228,21 @@ def multishift_workload(args):
         variants.append({"variant": v, "usec": round(us, 2), "desc": G.dev_block_describe(1, 16, v)})
     bytes_site = 16 * 72 + 16 * (24 + 24)
     log(f"batched stencil: {nvar} variants, chosen {chosen} ({desc}) {us_best:.1f} us")
+    # the 16-bit form of the refinement CG: 16-bit fat links (36 B), Naik links
+    # (26 B in the two-row form, else 36 B), 16-byte records in and out
+    nvar16, chosen16, _, desc16 = G.dev_block_tune(3, 16)
+    us16 = G.dev_block_dslash_bench(3, 16, chosen16, 20)
+    bytes_site16 = 8 * 36 + 8 * (26 if G.link_compression(2) == 1 else 36) + 16 * (16 + 16)
+    log(f"16-bit batched stencil: {nvar16} variants, chosen {chosen16} ({desc16}) {us16:.1f} us")
+    if args.kernel_only:
+        # the tuned stencils alone, for counter runs: no solver is launched
+        us_f32_16 = G.dev_block_dslash_bench(1, 16, chosen, 20)
+        G.close()
+        print(json.dumps({"workload": "batched stencils, 16 rhs, 24^3x48",
+                          "f32": {"usec": round(us_f32_16, 2), "alg_bytes_per_site": bytes_site, "desc": desc},
+                          "16bit": {"usec": round(us16, 2), "alg_bytes_per_site": bytes_site16, "desc": desc16}}),
+              flush=True)
+        return
     for i in range(args.warmup):
         G.dev_multishift_block(list(range(16)), offs, MS_STAGE_RESID, MS_RESID)
     t1 = time.perf_counter()
@@ -236,12 +253,15 @@ def multishift_workload(args):
     t_batch = (time.perf_counter() - t1) / args.steps
     log(f"batched: {t_batch:.3f} s per 16-source solve; {st}")
     # FP64 reference of source 0 and the batched solutions' error against it
-    it64, t64, _ = G.dev_multishift(0, offs, 1e-12, 0, out_slot0=40)
-    errs = [G.dev_diff(16 + j, 40 + j) for j in range(n)]
-    it64_target, _, _ = G.dev_multishift(0, offs, MS_RESID, 0)
+    it64 = t64 = it64_target = None
+    errs = [None]
+    if not args.no_reference:
+        it64, t64, _ = G.dev_multishift(0, offs, 1e-12, 0, out_slot0=40)
+        errs = [G.dev_diff(16 + j, 40 + j) for j in range(n)]
+        it64_target, _, _ = G.dev_multishift(0, offs, MS_RESID, 0)
     # one-source solves (same FP32 + refinement scheme), all 16 sources
     t_seq, seq_iters = None, None
-    if not args.no_sequential:
+    if not (args.no_sequential or args.no_reference):
         t1 = time.perf_counter()
         seq_iters = []
         for j in range(16):
@@ -250,13 +270,27 @@ def multishift_workload(args):
         t_seq = time.perf_counter() - t1
     us_f32 = G.dev_dslash_bench(1, mq.EVEN, 20)
     cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and it64_target is not None:
         try:
             with stdout_to_stderr():
                 cpu = cpu_multishift_baseline(G, MS_DIMS, offs, it64_target, 16)
         except Exception as e:  # the CPU leg must not hide the GPU result
             log(f"cpu baseline failed: {e!r}")
     G.close()
+    # the kernel with the larger share of the solve's stencil time (two
+    # applications per multi-shift and per refinement iteration) is the one the
+    # roofline line reports
+    t32 = 2 * st["ms_iters"] * us_best
+    t16 = 2 * sum(st["refine_iters"]) * us16
+    share16 = t16 / (t16 + t32)
+    if share16 >= 0.5:
+        roof = {"bound": "hbm", "achieved": round(bytes_site16 * Vh / us16 / 1e3, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(bytes_site16 * Vh / us16 / 1e3 / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": f"16-bit batched stencil of the refinement, 16 rhs ({desc16})"}
+    else:
+        roof = {"bound": "hbm", "achieved": round(bytes_site * Vh / us_best / 1e3, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(bytes_site * Vh / us_best / 1e3 / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": f"batched stencil of the multi-shift stage, 16 rhs ({desc})"}
     out = {
         "metric": "batched multi-shift RHMC solve: 16 sources x 12 shifts to per-shift FP64 true residual 1e-9",
         "value": round(t_batch, 4),
@@ -278,8 +312,8 @@ def multishift_workload(args):
         "refine_seconds": round(st["refine_seconds"], 4),
         "refine_iters_per_shift": st["refine_iters"],
         "true_rel_resid_per_shift_max": st["true_rel"],
-        "rel_err_vs_fp64_solve_src0": max(errs),
-        "fp64_reference": {"iters": it64, "seconds": round(t64, 4)},
+        "rel_err_vs_fp64_solve_src0": None if errs[0] is None else max(errs),
+        "fp64_reference": None if it64 is None else {"iters": it64, "seconds": round(t64, 4)},
         "sequential_16_one_source_solves_s": None if t_seq is None else round(t_seq, 4),
         "speedup_batched_vs_sequential": None if t_seq is None else round(t_seq / t_batch, 3),
         "sequential_iters": seq_iters,
@@ -288,9 +322,10 @@ def multishift_workload(args):
                            "GBps": round(bytes_site * Vh / us_best / 1e3, 1),
                            "usec_per_rhs": round(us_best / 16, 2),
                            "single_rhs_f32_dslash_usec": round(us_f32, 2)},
-        "roofline": {"bound": "hbm", "achieved": round(bytes_site * Vh / us_best / 1e3, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(bytes_site * Vh / us_best / 1e3 / HBM_PEAK_GBS, 4),
-                     "traffic": None, "kernel": f"batched stencil, 16 rhs ({desc})"},
+        "batched_dslash_16bit": {"usec": round(us16, 2), "alg_bytes_per_site": bytes_site16,
+                                 "GBps": round(bytes_site16 * Vh / us16 / 1e3, 1), "desc": desc16,
+                                 "share_of_stencil_time": round(share16, 3)},
+        "roofline": roof,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
@@ -725,10 +760,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--kernel-only", action="store_true",
-                    help="only the Dslash and copy bursts (for rocprofv3 counter runs)")
+                    help="only the Dslash and copy bursts, or for multishift the tuned batched stencils "
+                         "(for rocprofv3 counter runs)")
     ap.add_argument("--workload", choices=["cg", "c1", "multishift", "hmc", "force", "io", "trajectory"], default="cg")
     ap.add_argument("--no-sequential", action="store_true",
                     help="multishift: leave out the 16 one-source comparison solves")
+    ap.add_argument("--no-reference", action="store_true",
+                    help="multishift: leave out the FP64 one-source reference solves (implies --no-sequential)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); "
