@@ -287,10 +287,12 @@ int milc_hip_dev_gaussian_source(int parity, int slot, double* sum);
 
 /* 16-bit sloppy long links in the two-row form (on = 1, the default): a long
  * link that is a real multiple of a U(3) matrix, L = c U (HISQ and asqtad
- * Naik links), is stored as rows 0 and 1 plus z = det(U)/c, and row 2 =
- * z conj(row0 x row1) is rebuilt in the stencil: 28 instead of 36 bytes per
- * link, 544 instead of 608 bytes per output site of the 16-bit Dslash.  The
- * form is used only when every long link passes |L^dag L - c^2 1| <= 1e-5 c^2;
+ * Naik links), is stored as rows 0 and 1 plus the phase of det(U) in 16
+ * bits, and row 2 = z conj(row0 x row1), z = exp(i phase) / c with c from the
+ * row norms, is rebuilt in the stencil: 26 instead of 36 bytes per link, 528
+ * instead of 608 bytes per output site of the 16-bit Dslash (the 16-bit
+ * batched stencil of the batched CG uses the same links).  The form is used
+ * only when every long link passes |L^dag L - c^2 1| <= 1e-5 c^2;
  * other links stay in the nine-element form.  No counterpart in the
  * reference (QUDA's reconstruct-13 plays this role under MILC's QUDA glue). */
 int milc_hip_set_link_compression(int on);

@@ -55,6 +55,31 @@ def test_block_dslash_every_variant_vs_oracle(prec, nrhs):
                     assert e < tol, (prec, nrhs, var, out_par, s, e)
 
 
+def test_block_dslash_16bit_nine_element_naik_vs_oracle():
+    """The 16-bit batched stencil with the Naik links in the nine-element form
+    (two-row form switched off), 16 rhs, every variant: 2e-4 of the oracle."""
+    dims = (8, 6, 4, 10)
+    L, fat, lng = hisq_setup(dims, 1005)
+    with mq.Lattice(dims) as G:
+        G.set_link_compression(0)  # process-wide setting: restored below
+        try:
+            G.dev_load_links(fat, lng)
+            nvar, chosen, us, desc = G.dev_block_tune(3, 16)
+            assert G.link_compression(2) == 0
+            srcs = [L.gaussian_vector(800 + s, ko.ODD) for s in range(16)]
+            refs = [L.dslash(fat, lng, v, ko.EVEN) for v in srcs]
+            for s, v in enumerate(srcs):
+                G.dev_write(s, ko.ODD, v)
+            sl = L.parity_slice(ko.EVEN)
+            for var in range(nvar):
+                G.dev_block_dslash(3, 16, var, list(range(16)), ko.EVEN, list(range(20, 36)))
+                for s in range(16):
+                    e = rel(G.dev_read(20 + s, ko.EVEN)[sl], refs[s][sl])
+                    assert e < 2e-4, (var, s, e)
+        finally:
+            G.set_link_compression(1)
+
+
 @pytest.fixture(scope="module")
 def l8():
     dims = (8, 8, 8, 8)
