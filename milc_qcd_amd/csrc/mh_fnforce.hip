@@ -715,31 +715,31 @@ __global__ void __launch_bounds__(128) k_sf_adj(Geom g, LinkField U, MF A, MF Gs
   const SfSites n = sf_sites<MU, NU>(g, cb, pb);
   Mat3<double> a, b, t, u, acc;
   if (MODE != 2) {
-  mload(U, pb, NU, n.ymnpm, u);
-  mfl(Gs, po, n.ymn, b);
-  mmul_an(u, b, t);
-  mload(U, po, NU, n.ymn, u);
-  mmul(t, u, acc);
-  mload(U, po, NU, n.ypm, u);
-  mfl(Gs, po, n.ypn, b);
-  mmul(u, b, t);
-  mload(U, pb, NU, cb, u);
-  mmul_na(t, u, a);
-  madd(acc, a);
-  if (fa_init) {
-    mfl_nt(FI, pb, cb, a);
+    mload(U, pb, NU, n.ymnpm, u);
+    mfl(Gs, po, n.ymn, b);
+    mmul_an(u, b, t);
+    mload(U, po, NU, n.ymn, u);
+    mmul(t, u, acc);
+    mload(U, po, NU, n.ypm, u);
+    mfl(Gs, po, n.ypn, b);
+    mmul(u, b, t);
+    mload(U, pb, NU, cb, u);
+    mmul_na(t, u, a);
+    madd(acc, a);
+    if (fa_init) {
+      mfl_nt(FI, pb, cb, a);
 #pragma unroll
-    for (int i = 0; i < 3; i++)
+      for (int i = 0; i < 3; i++)
 #pragma unroll
-      for (int j = 0; j < 3; j++) {
-        double re = ci * a.e[i][j].re, im = ci * a.e[i][j].im;
-        a.e[i][j].re = re + cg * acc.e[i][j].re;
-        a.e[i][j].im = im + cg * acc.e[i][j].im;
-      }
-    mfs_nt(FA, pb, cb, a);
-  } else {
-    mfa_nt(FA, pb, cb, cg, acc);
-  }
+        for (int j = 0; j < 3; j++) {
+          double re = ci * a.e[i][j].re, im = ci * a.e[i][j].im;
+          a.e[i][j].re = re + cg * acc.e[i][j].re;
+          a.e[i][j].im = im + cg * acc.e[i][j].im;
+        }
+      mfs_nt(FA, pb, cb, a);
+    } else {
+      mfa_nt(FA, pb, cb, cg, acc);
+    }
   }
   if (MODE == 1) return;
   // A(y+nu) U_nu(y+mu)^dag G(y)
