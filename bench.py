@@ -872,6 +872,11 @@ def main():
     us = G.dev_copy_bench(copy_bytes, 20)
     stream = 2 * copy_bytes / us / 1e3
     log(f"dslash {dsl}; stream copy {stream:.0f} GB/s")
+    # the mixed CG iteration's kernels inside their own sequence (source 0),
+    # beside the bursts above, and the time per iteration of the timed solves
+    cgi = {k: round(v, 2) for k, v in G.dev_cg_iteration_bench(0, MASS, mq.EVEN, 50).items()}
+    cgi["solve_us_per_iteration"] = round(t_mixed / (sum(mixed_iters) / len(mixed_iters)) * 1e6, 2)
+    log(f"cg iteration in sequence: {cgi}")
 
     # HBM bytes per launch of the 16-bit Dslash from the committed rocprofv3
     # FETCH_SIZE/WRITE_SIZE summary (copy-calibrated), or null if absent
@@ -917,6 +922,7 @@ def main():
         "true_rel_resid_mixed": max(mixed_res),
         "rel_err_mixed_vs_fp64_solution": err,
         "dslash": dsl,
+        "cg_iteration_in_solve_us": cgi,
         "stream_copy_GBps": round(stream, 1),
         "roofline": {"bound": "hbm", "achieved": h16["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": h16["frac_peak"], "traffic": traffic,

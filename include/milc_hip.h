@@ -416,6 +416,12 @@ int milc_hip_dev_read_links(double* fat, double* lng);
 /* Time `reps` Dslash applications on device slots (HIP events on the
  * library stream); returns the average in *usec. */
 int milc_hip_dev_dslash_bench(int prec, int parity, int reps, double* usec);
+/* The kernels of the 16-bit mixed CG iteration timed inside their own
+ * sequence: from the residual of source slot slot_src (x = 0, on `parity`)
+ * `reps` iterations run with HIP events between the kernels, then `reps`
+ * without; us[0..4] = stencil D p, stencil + M^dag M p and its dots, CG
+ * scalars, vector update, one whole iteration (the last without events). */
+int milc_hip_dev_cg_iteration_bench(int slot_src, double mass, int parity, int reps, double* us);
 /* STREAM-style device copy of `bytes` bytes, `reps` times (HIP events):
  * the measured HBM reference next to the vendor peak. */
 int milc_hip_dev_copy_bench(size_t bytes, int reps, double* usec);

@@ -1073,6 +1073,15 @@ extern "C" int milc_hip_dev_dslash(int prec, int slot_src, int parity_out, int s
   return rc;
 }
 
+extern "C" int milc_hip_dev_cg_iteration_bench(int slot_src, double mass, int parity, int reps, double* us) {
+  Context& c = ctx();
+  if (!c.initialised || !us || !c.links[P_F64].valid) return -1;
+  if (!slot_ok(slot_src, true) || (parity != EVEN && parity != ODD)) return -2;
+  int rc;
+  if ((rc = ensure_links(P_H16))) return rc;
+  return cg_iteration_profile(g_slots[slot_src], mass, parity, reps, us);
+}
+
 // precision code of the batched-stencil entry points: 1 float, 2 double,
 // 3 the 16-bit records and links of the batched CG's sloppy operator
 static int block_prec(int prec) { return prec == 2 ? P_F64 : (prec == 3 ? P_H16 : P_F32); }

@@ -1233,6 +1233,82 @@ static int cg_solve_mixed(const VecField& src, VecField& x, double mass, const S
   return rc;
 }
 
+// Per-kernel times of the 16-bit mixed CG iteration inside the solve's own
+// sequence: from the source's residual (x = 0) the iteration kernels run as in
+// cg_solve_mixed, never halting, and HIP events between the kernels time each
+// one over `reps` iterations (the events add their own small gaps); then the
+// same number of iterations runs without events for the time per iteration.
+// us[0..4] = stencil D p, stencil + M^dag M p dots, scalars, update, whole
+// iteration.
+int cg_iteration_profile(const VecField& src, double mass, int parity, int reps, double us[5]) {
+  constexpr int PS = P_H16, PA = kH16Arith;
+  Context& c = ctx();
+  const DevLinks& LS = c.links[PS];
+  if (!LS.valid || !c.links[P_F64].valid) return -3;
+  if (reps < 1 || reps > 1000) return -2;
+  const int pb = pbit(parity);
+  const double msq_x4 = 4.0 * mass * mass;
+  VecField x64, rhi, tmp64, ps, tmps, ttts, rs, xs, dummy{};
+  VecScope owned{&x64, &rhi, &tmp64, &ps, &tmps, &ttts, &rs, &xs};
+  int rc;
+  if ((rc = alloc_vec(x64, P_F64, c.stride)) || (rc = alloc_vec(rhi, P_F64, c.stride)) ||
+      (rc = alloc_vec(tmp64, P_F64, c.stride)) || (rc = alloc_vec(ps, PS, c.stride)) ||
+      (rc = alloc_vec(tmps, PS, c.stride)) || (rc = alloc_vec(ttts, PA, c.stride)) ||
+      (rc = alloc_vec(rs, PA, c.stride)) || (rc = alloc_vec(xs, PA, c.stride)))
+    return rc;
+  if ((rc = vec_zero(x64, P_F64))) return rc;
+  CGScalars h;
+  memset(&h, 0, sizeof h);
+  if ((rc = vec_norm2(src, P_F64, &h.source_norm))) return rc;
+  if ((rc = write_scalars(&h))) return rc;
+  if ((rc = true_residual(src, x64, tmp64, rhi, dummy, 0, pb, msq_x4))) return rc;
+  if ((rc = convert_vec(rhi, P_F64, rs, PA)) || (rc = convert_vec(rhi, P_F64, ps, PS))) return rc;
+  if ((rc = read_scalars(&h))) return rc;
+  h.rsq_stop = 0.0;  // never halts
+  h.rel_mode = 0;
+  h.halt = 0;
+  h.iters = 0;
+  if ((rc = write_scalars(&h))) return rc;
+  for (int k = 0; k < 4; k++) enqueue_cg_iteration<PS, PS, PA>(LS, ps, tmps, rs, ttts, xs, pb, msq_x4);
+  std::vector<hipEvent_t> ev(4 * reps + 1);
+  for (auto& e : ev) MH_CHECK(hipEventCreate(&e));
+  OnExit events([&] {
+    for (auto& e : ev) (void)hipEventDestroy(e);
+  });
+  const int nb = dslash_grid(), nu = nblocks_for(c.Vh, 256);
+  const int* halt = &c.ws.scal->halt;
+  MH_CHECK(hipEventRecord(ev[0], c.stream));
+  for (int i = 0; i < reps; i++) {
+    hipLaunchKernelGGL(k_dslash<PS>, dim3(nb), dim3(256), 0, c.stream, c.geom, LS.fat, LS.lng, ps, tmps, pb ^ 1, halt);
+    MH_CHECK(hipEventRecord(ev[4 * i + 1], c.stream));
+    hipLaunchKernelGGL((k_dslash_mdagm<PS, PS, PA>), dim3(nb), dim3(256), 0, c.stream, c.geom, LS.fat, LS.lng, tmps,
+                       ps, rs, ttts, pb, msq_x4, c.ws.partials, halt);
+    MH_CHECK(hipEventRecord(ev[4 * i + 2], c.stream));
+    hipLaunchKernelGGL(k_cg_scalars, dim3(1), dim3(1024), 0, c.stream, c.ws.partials, partials_rr(), nb, c.ws.scal);
+    MH_CHECK(hipEventRecord(ev[4 * i + 3], c.stream));
+    hipLaunchKernelGGL((k_cg_update<PS, PA, 0>), dim3(nu), dim3(256), 0, c.stream, xs, ps, rs, ttts, c.Vh, c.ws.scal,
+                       partials_rr());
+    MH_CHECK(hipEventRecord(ev[4 * i + 4], c.stream));
+  }
+  MH_CHECK(hipGetLastError());
+  MH_CHECK(hipEventSynchronize(ev[4 * reps]));
+  for (int k = 0; k < 5; k++) us[k] = 0.0;
+  for (int i = 0; i < reps; i++)
+    for (int k = 0; k < 4; k++) {
+      float ms = 0;
+      MH_CHECK(hipEventElapsedTime(&ms, ev[4 * i + k], ev[4 * i + k + 1]));
+      us[k] += 1000.0 * ms / reps;
+    }
+  MH_CHECK(hipEventRecord(ev[0], c.stream));
+  for (int i = 0; i < reps; i++) enqueue_cg_iteration<PS, PS, PA>(LS, ps, tmps, rs, ttts, xs, pb, msq_x4);
+  MH_CHECK(hipEventRecord(ev[1], c.stream));
+  MH_CHECK(hipEventSynchronize(ev[1]));
+  float ms = 0;
+  MH_CHECK(hipEventElapsedTime(&ms, ev[0], ev[1]));
+  us[4] = 1000.0 * ms / reps;
+  return read_scalars(&h);
+}
+
 // The work vectors cg_solve_f64 / cg_solve_mixed allocate, 256-byte aligned.
 size_t cg_scratch_bytes(int ps, int stride) {
   auto a = [&](int prec) { return (vec_bytes(prec, stride) + 255) & ~(size_t)255; };

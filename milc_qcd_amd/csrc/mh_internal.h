@@ -115,6 +115,8 @@ struct ScratchArena {
   ScratchArena& operator=(const ScratchArena&) = delete;
 };
 // Work-vector bytes cg_solve takes from the arena for sloppy precision ps.
+// per-kernel times of the 16-bit mixed CG iteration in its own sequence (us[5])
+int cg_iteration_profile(const VecField& src, double mass, int parity, int reps, double us[5]);
 size_t cg_scratch_bytes(int ps, int stride);
 // hand the pooled free blocks back to the device (end of context)
 void dev_pool_release();

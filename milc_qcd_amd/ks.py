@@ -46,7 +46,7 @@ EXPORTS = [
     "milc_hip_multishift_invert", "milc_hip_dslash", "milc_hip_compute_oprod",
     "milc_hip_load_hisq_links", "milc_hip_load_hisq_links_naik", "milc_hip_dev_gauge_hisq", "milc_hip_dev_source",
     "milc_hip_dev_invert", "milc_hip_dev_read", "milc_hip_dev_write", "milc_hip_dev_read_links",
-    "milc_hip_dev_dslash_bench", "milc_hip_dev_copy_bench", "milc_hip_dev_diff",
+    "milc_hip_dev_dslash_bench", "milc_hip_dev_copy_bench", "milc_hip_dev_diff", "milc_hip_dev_cg_iteration_bench",
     "milc_hip_dev_rhmc_action", "milc_hip_dev_force_level0", "milc_hip_dev_load_links",
     "milc_hip_asqtad_force", "milc_hip_fermion_force_fn_multi",
     "milc_hip_invert_block", "milc_hip_multishift_invert_block", "milc_hip_dev_block_tune",
@@ -194,6 +194,7 @@ def load_library(path=LIB_PATH):
     L.milc_hip_dev_write.argtypes = [I, I, P]
     L.milc_hip_dev_read_links.argtypes = [P, P]
     L.milc_hip_dev_dslash_bench.argtypes = [I, I, I, ctypes.POINTER(D)]
+    L.milc_hip_dev_cg_iteration_bench.argtypes = [I, D, I, I, ctypes.POINTER(D)]
     L.milc_hip_dev_copy_bench.argtypes = [ctypes.c_size_t, I, ctypes.POINTER(D)]
     L.milc_hip_dev_diff.argtypes = [I, I, ctypes.POINTER(D)]
     L.milc_hip_dev_rhmc_action.argtypes = [I, I, ctypes.POINTER(D), ctypes.POINTER(D), D, I, ctypes.POINTER(D)]
@@ -820,6 +821,16 @@ class Lattice:
         us = ctypes.c_double(0)
         _check(self.lib.milc_hip_dev_dslash_bench(prec, parity, reps, ctypes.byref(us)), "milc_hip_dev_dslash_bench")
         return us.value
+
+    def dev_cg_iteration_bench(self, slot, mass, parity=EVEN, reps=50):
+        """16-bit mixed CG iteration kernels timed in their own sequence (us):
+        dslash, dslash + M^dag M dots, scalars, update, whole iteration"""
+        self._activate()
+        us = (ctypes.c_double * 5)()
+        _check(self.lib.milc_hip_dev_cg_iteration_bench(slot, mass, parity, reps, us),
+               "milc_hip_dev_cg_iteration_bench")
+        return {"dslash": us[0], "dslash_mdagm": us[1], "cg_scalars": us[2], "cg_update": us[3],
+                "iteration": us[4]}
 
     def dev_copy_bench(self, nbytes, reps):
         self._activate()

@@ -214,3 +214,21 @@ def test_golden_asqtad_mdaggerm_inverse_on_gpu(oracle):
             e = rel(x[s], ans[s])
             print(f"golden parity {par}: rel {e:.3e}")
             assert e < 1e-6  # the golden answer is stored in FP32
+
+
+def test_cg_iteration_bench_times_every_kernel(hisq8):
+    """milc_hip_dev_cg_iteration_bench: the 16-bit CG iteration's four kernels
+    timed in their own sequence, and a whole iteration without events; all
+    positive, and the four parts (each with its event gap) account for at
+    least the time of an iteration without events, give or take launch jitter."""
+    L, U, fat, lng, W, _ = hisq8
+    with mq.Lattice(DIMS_8) as G:
+        G.dev_load_links(fat, lng)
+        G.dev_write(0, ko.EVEN, L.gaussian_vector(91, ko.EVEN))
+        t = G.dev_cg_iteration_bench(0, 0.05, mq.EVEN, 20)
+        print(t)
+        assert all(v > 0 for v in t.values())
+        parts = t["dslash"] + t["dslash_mdagm"] + t["cg_scalars"] + t["cg_update"]
+        assert parts > 0.5 * t["iteration"]
+        with pytest.raises(mq.MilcHipError):
+            G.dev_cg_iteration_bench(0, 0.05, mq.EVEN, 0)
