@@ -532,15 +532,6 @@ __device__ __forceinline__ void madd_adj(Mat3<double>& a, const Mat3<double>& b)
     }
 }
 
-// neighbour (cb, parity) of site (cb, pb) after hops (d1, s1) then (d2, s2)
-__device__ __forceinline__ int hop2(const Geom& g, int cb, int pb, int d1, int s1, int d2, int s2, int& opb) {
-  Coord c = site_coords(g, cb, pb);
-  int n1 = neighbour(g, c, cb, d1, s1);
-  Coord c1 = site_coords(g, n1, pb ^ 1);
-  opb = pb;
-  return neighbour(g, c1, n1, d2, s2);
-}
-
 // F(x) = eps(x)^with_eps F1_mu(x);  G_mu(x) += c1 F(x)
 __global__ void __launch_bounds__(128) k_sf_init(Geom g, MF F1, int with_eps, MF F, MF G, double c1) {
   int cb = blockIdx.x * blockDim.x + threadIdx.x;
@@ -610,9 +601,9 @@ __host__ inline bool sf_tiled(const Geom& g, int mu, int nu) {
   return (mu == 3 || nu == 3) && g.st % 16 == 0 && g.X[3] % 4 == 0;
 }
 
-// out = S_nu(A) for the mu link (acc != 0: out += S_nu(A))
+// out = S_nu(A) for the mu link
 template <int MU, int NU, int TT>
-__global__ void __launch_bounds__(128) k_sf_fwd(Geom g, LinkField U, MF A, MF out, int acc) {
+__global__ void __launch_bounds__(128) k_sf_fwd(Geom g, LinkField U, MF A, MF out) {
   int cb, pb;
   if (!sf_site<TT>(g, cb, pb)) return;
   const int po = pb ^ 1;
@@ -629,8 +620,7 @@ __global__ void __launch_bounds__(128) k_sf_fwd(Geom g, LinkField U, MF A, MF ou
   mload(U, pb, NU, n.ymnpm, b);
   mmul(t, b, lo);
   madd(up, lo);
-  if (acc) mfa(out, pb, cb, 1.0, up);
-  else mfs(out, pb, cb, up);
+  mfs(out, pb, cb, up);
 }
 
 // out = S_P(AP) + S_Q(AQ) for the mu link: the two 5-staples of one 7-staple
@@ -772,7 +762,7 @@ __global__ void __launch_bounds__(128) k_sf_adj(Geom g, LinkField U, MF A, MF Gs
   mfa_nt(FU, pb, cb, cg, acc);
 }
 
-typedef void (*SfFwdFn)(Geom, LinkField, MF, MF, int);
+typedef void (*SfFwdFn)(Geom, LinkField, MF, MF);
 typedef void (*SfAdjFn)(Geom, LinkField, MF, MF, double, MF, MF, int, MF, double);
 struct SfKernels {
   SfFwdFn fwd;
@@ -803,16 +793,21 @@ static SfKernels sf_kernels(const Geom& g, int mu, int nu) {
 
 // Naik: G_mu(y) += cN [U(y+mu) U(y+2mu) F3(y) + U(y+mu) F3(y-mu) U(y-mu) + F3(y-2mu) U(y-2mu) U(y-mu)],
 // F3 multiplied by eps at its own site when with_eps
-__global__ void __launch_bounds__(128) k_sf_naik(Geom g, LinkField U, int mu, MF F3, int with_eps, double cN, MF G) {
-  int bx, pb;
-  if (!xcd_block((g.Vh + blockDim.x - 1) / blockDim.x, 2, bx, pb)) return;
-  int cb = bx * blockDim.x + threadIdx.x;
-  if (cb >= g.Vh) return;
+// (direction MU at compile time: one coordinate decode, the hops from it)
+template <int MU>
+__global__ void __launch_bounds__(128) k_sf_naik(Geom g, LinkField U, MF F3, int with_eps, double cN, MF G) {
+  constexpr int mu = MU;
+  int cb, pb;
+  if (!sf_site<0>(g, cb, pb)) return;
   int po = pb ^ 1;
-  Coord c = site_coords(g, cb, pb);
-  int p1 = neighbour(g, c, cb, mu, 1), m1 = neighbour(g, c, cb, mu, -1);
-  int q;
-  int p2 = hop2(g, cb, pb, mu, 1, mu, 1, q), m2 = hop2(g, cb, pb, mu, -1, mu, -1, q);
+  const Coord c = site_coords(g, cb, pb);
+  const int x[4] = {c.x, c.y, c.z, c.t};
+  auto at = [&](int d) {
+    int v[4] = {x[0], x[1], x[2], x[3]};
+    v[MU] = sf_wrap(v[MU] + d, g.X[MU]);
+    return (v[0] + g.X[0] * (v[1] + g.X[1] * (v[2] + g.X[2] * v[3]))) >> 1;
+  };
+  const int p1 = at(1), m1 = at(-1), p2 = at(2), m2 = at(-2);
   const double ey = (with_eps && pb) ? -1.0 : 1.0, eo = (with_eps && po) ? -1.0 : 1.0;
   Mat3<double> u1, u2, f, t, acc, b;
   mload(U, po, mu, p1, u1);
@@ -849,7 +844,7 @@ __global__ void __launch_bounds__(128) k_sf_naik(Geom g, LinkField U, int mu, MF
       acc.e[i][j].re += ey * b.e[i][j].re;
       acc.e[i][j].im += ey * b.e[i][j].im;
     }
-  mfa(G, pb, cb, cN, acc);
+  mfa_nt(G, pb, cb, cN, acc);
 }
 
 // mom += TA(scale U G), packed (as k_path_adjoint<1>)
@@ -1549,9 +1544,9 @@ static int smear_adjoint_dev(const LinkField& L, const LinkField& F1, const Link
     const SfKernels k = sf_kernels(c.geom, mu, nu);
     hipLaunchKernelGGL(k.adj[mode], xgrid, blk, 0, c.stream, c.geom, L, A, Gs, cg, FA, FU, init, mF, ci);
   };
-  auto fwd = [&](int mu, int nu, const MF& A, const MF& out, int acc) {
+  auto fwd = [&](int mu, int nu, const MF& A, const MF& out) {
     const SfKernels k = sf_kernels(c.geom, mu, nu);
-    hipLaunchKernelGGL(k.fwd, xgrid, blk, 0, c.stream, c.geom, L, A, out, acc);
+    hipLaunchKernelGGL(k.fwd, xgrid, blk, 0, c.stream, c.geom, L, A, out);
   };
   for (int mu = 0; mu < 4; mu++) {
     const MF Gmu = mf_link(G, mu);
@@ -1566,7 +1561,7 @@ static int smear_adjoint_dev(const LinkField& L, const LinkField& F1, const Link
         mS1[od[i]] = mf_site(S1[i], st);
         mG2[od[i]] = mf_site(G2[i], st);
       }
-      for (int i = 0; i < 3; i++) fwd(mu, od[i], mf_link(L, mu), mS1[od[i]], 0);
+      for (int i = 0; i < 3; i++) fwd(mu, od[i], mf_link(L, mu), mS1[od[i]]);
       if (seven)
         for (int i = 0; i < 3; i++) {
           const int sig = od[i], nu = od[(i + 1) % 3], rho = od[(i + 2) % 3];
@@ -1594,7 +1589,8 @@ static int smear_adjoint_dev(const LinkField& L, const LinkField& F1, const Link
       }
     }
     if (cN != 0.0)
-      hipLaunchKernelGGL(k_sf_naik, xgrid, blk, 0, c.stream, c.geom, L, mu, mf_link(F3, mu), with_eps, cN, Gmu);
+      hipLaunchKernelGGL(mu == 0 ? k_sf_naik<0> : mu == 1 ? k_sf_naik<1> : mu == 2 ? k_sf_naik<2> : k_sf_naik<3>, xgrid,
+                         blk, 0, c.stream, c.geom, L, mf_link(F3, mu), with_eps, cN, Gmu);
   }
   MH_CHECK(hipGetLastError());
   MH_CHECK(hipStreamSynchronize(c.stream));
