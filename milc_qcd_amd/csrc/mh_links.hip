@@ -261,13 +261,15 @@ __global__ void __launch_bounds__(256) k_gen_staple(Geom g, LinkField U, LinkFie
 
 // Up to three generalised staples of stored matrix fields for the mu link in
 // one pass: for job k, S = S_nu(in)(x) = U_nu(x) in(x+nu) U_nu(x+mu)^dag +
-// U_nu(x-nu)^dag in(x-nu) U_nu(x-nu+mu); fat_mu += coef S; out = S when out
-// is given.  The fat link is read and written once for all jobs.
+// U_nu(x-nu)^dag in(x-nu) U_nu(x-nu+mu); fat_mu += coef S; when out is given,
+// out = S (acc == 0) or out += S (acc == 1).  The fat link is read and written
+// once for all jobs.
 struct StapleJob {
   MatField in;
   int nu;
   double coef;
   MatField out;  // data == nullptr: not stored
+  int acc;
 };
 struct StapleJobs {
   StapleJob j[3];
@@ -310,7 +312,18 @@ __global__ void __launch_bounds__(256) k_gen_staples(Geom g, LinkField U, Staple
         F.e[i][j].re += J.coef * up.e[i][j].re;
         F.e[i][j].im += J.coef * up.e[i][j].im;
       }
-    if (J.out.data) st(J.out.data + (size_t)pb * 9 * J.out.stride, J.out.stride, cb, up);
+    if (J.out.data) {
+      double2* o = J.out.data + (size_t)pb * 9 * J.out.stride;
+      if (J.acc) {
+        ld(o, J.out.stride, cb, A);
+        for (int i = 0; i < 3; i++)
+          for (int j = 0; j < 3; j++) {
+            up.e[i][j].re += A.e[i][j].re;
+            up.e[i][j].im += A.e[i][j].im;
+          }
+      }
+      st(o, J.out.stride, cb, up);
+    }
   }
   st(f, fat.stride, cb, F);
 }
@@ -492,50 +505,61 @@ static int alloc_mat(MatField& m, int stride) {
 }
 
 // Optimised fattening: coeff = {one_link, naik, 3-staple, 5-staple, 7-staple, Lepage}
-// Per (mu, nu): the 3-staple of U (stored), then one pass for the Lepage term
-// and the two 5-staples of the 3-staple (stored), then one pass for the two
-// 7-staples of the 5-staples; terms with coefficient 0 are skipped.
+// Per (mu, nu): the 3-staple S_nu(U_mu) (stored), then one pass for the Lepage
+// term and the two 5-staples S_rho(S_nu(U_mu)).  The 7-staple S_sig(S_rho(S_nu))
+// is linear in its 5-staple, so the two 5-staples that share the fourth
+// direction sig (rho and nu swapped) are summed into one field per sig and the
+// six 7-staples of a mu link take three staples in one final pass.  Terms with
+// coefficient 0 are skipped.
 static int fat_links_dev(const LinkField& U, const double coeff[6], LinkField& fat) {
   Context& c = ctx();
   double one_link = coeff[0] - 6.0 * coeff[5];
-  MatField staple, temp, temp2, none{nullptr, 0};
+  MatField staple, five_sum[3], none{nullptr, 0};
   int rc;
-  if ((rc = alloc_mat(staple, c.stride)) || (rc = alloc_mat(temp, c.stride)) || (rc = alloc_mat(temp2, c.stride)))
-    return rc;
+  const bool five = coeff[3] != 0.0 || coeff[4] != 0.0, seven = coeff[4] != 0.0;
+  struct Guard {
+    MatField* m[4];
+    ~Guard() {
+      for (MatField* f : m) dev_free(f->data);
+    }
+  } guard{{&staple, &five_sum[0], &five_sum[1], &five_sum[2]}};
+  for (MatField* f : guard.m) f->data = nullptr;
+  if ((rc = alloc_mat(staple, c.stride))) return rc;
+  for (int s = 0; seven && s < 3; s++)
+    if ((rc = alloc_mat(five_sum[s], c.stride))) return rc;
   // k_fat_init streams: plain grid; the staple kernels read neighbours: XCD block order
   dim3 grid(nb_for(c.Vh), 2), xgrid = xcd_grid(nb_for(c.Vh), 2);
-  const bool five = coeff[3] != 0.0 || coeff[4] != 0.0, seven = coeff[4] != 0.0;
   for (int dir = 0; dir < 4; dir++) {
     hipLaunchKernelGGL(k_fat_init, grid, dim3(256), 0, c.stream, c.geom, U, fat, dir, one_link);
     if (coeff[2] == 0.0 && coeff[5] == 0.0 && !five) continue;
-    for (int nu = 0; nu < 4; nu++) {
-      if (nu == dir) continue;
+    int od[3], n = 0;
+    for (int r = 0; r < 4; r++)
+      if (r != dir) od[n++] = r;
+    bool filled[3] = {false, false, false};
+    for (int i = 0; i < 3; i++) {
+      const int nu = od[i];
       hipLaunchKernelGGL(k_gen_staple, xgrid, dim3(256), 0, c.stream, c.geom, U, U, dir, none, staple, 1, fat, dir,
                          nu, coeff[2]);
-      int rho[2], sig[2], nr = 0;
-      for (int r = 0; r < 4; r++)
-        if (r != dir && r != nu) rho[nr++] = r;
-      for (int k = 0; k < 2; k++) sig[k] = 6 - dir - nu - rho[k];  // the fourth direction
       StapleJobs a{};
-      if (coeff[5] != 0.0) a.j[a.n++] = StapleJob{staple, nu, coeff[5], none};
-      if (five) {
-        a.j[a.n++] = StapleJob{staple, rho[0], coeff[3], seven ? temp : none};
-        a.j[a.n++] = StapleJob{staple, rho[1], coeff[3], seven ? temp2 : none};
-      }
+      if (coeff[5] != 0.0) a.j[a.n++] = StapleJob{staple, nu, coeff[5], none, 0};
+      if (five)
+        for (int k = 1; k <= 2; k++) {
+          // S_rho(S_nu(U_mu)) belongs to the sum of the fourth direction sig
+          const int rho = (k == 1) ? od[(i + 1) % 3] : od[(i + 2) % 3];
+          const int sl = (k == 1) ? (i + 2) % 3 : (i + 1) % 3;
+          a.j[a.n++] = StapleJob{staple, rho, coeff[3], seven ? five_sum[sl] : none, filled[sl] ? 1 : 0};
+          filled[sl] = true;
+        }
       if (a.n) hipLaunchKernelGGL(k_gen_staples, xgrid, dim3(256), 0, c.stream, c.geom, U, a, fat, dir);
-      if (seven) {
-        StapleJobs b{};
-        b.j[b.n++] = StapleJob{temp, sig[0], coeff[4], none};
-        b.j[b.n++] = StapleJob{temp2, sig[1], coeff[4], none};
-        hipLaunchKernelGGL(k_gen_staples, xgrid, dim3(256), 0, c.stream, c.geom, U, b, fat, dir);
-      }
+    }
+    if (seven) {
+      StapleJobs b{};
+      for (int s = 0; s < 3; s++) b.j[b.n++] = StapleJob{five_sum[s], od[s], coeff[4], none, 0};
+      hipLaunchKernelGGL(k_gen_staples, xgrid, dim3(256), 0, c.stream, c.geom, U, b, fat, dir);
     }
   }
   MH_CHECK(hipGetLastError());
   MH_CHECK(hipStreamSynchronize(c.stream));
-  dev_free(staple.data);
-  dev_free(temp.data);
-  dev_free(temp2.data);
   return 0;
 }
 
