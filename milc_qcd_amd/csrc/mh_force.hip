@@ -25,7 +25,8 @@ struct TermSet {
 // with both parities side by side (xcd_block), so the here-vector Z_t(x) is
 // one request for the four lanes and the x/y neighbours come from that XCD's
 // L2.  Each lane keeps its O1_mu and O3_mu (two FP64 3x3 accumulators) over
-// all terms and writes them once.
+// all terms and writes them once, non-temporally (streamed out instead of
+// displacing from the L2 the vectors that the other lanes are still gathering).
 __global__ void __launch_bounds__(256) k_oprod(Geom g, TermSet ts, int nterms, LinkField o1, LinkField o3) {
   const int nbx = (g.Vh + 63) / 64;
   int bx, pb;
@@ -69,8 +70,13 @@ __global__ void __launch_bounds__(256) k_oprod(Geom g, TermSet ts, int nterms, L
   double2* p3 = (double2*)o3.data + (size_t)(pm * 9) * o3.stride + cb;
 #pragma unroll
   for (int e = 0; e < 9; e++) {
-    p1[(size_t)e * o1.stride] = make_double2(a1.e[e / 3][e % 3].re, a1.e[e / 3][e % 3].im);
-    p3[(size_t)e * o3.stride] = make_double2(a3.e[e / 3][e % 3].re, a3.e[e / 3][e % 3].im);
+    nat_d2 v1, v3;
+    v1.x = a1.e[e / 3][e % 3].re;
+    v1.y = a1.e[e / 3][e % 3].im;
+    v3.x = a3.e[e / 3][e % 3].re;
+    v3.y = a3.e[e / 3][e % 3].im;
+    __builtin_nontemporal_store(v1, (nat_d2*)(p1 + (size_t)e * o1.stride));
+    __builtin_nontemporal_store(v3, (nat_d2*)(p3 + (size_t)e * o3.stride));
   }
 }
 

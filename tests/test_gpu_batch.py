@@ -146,6 +146,32 @@ def test_multishift_block_fp32_stage_vs_oracle(l8):
         assert np.array_equal(a, G.dev_read(16, ko.EVEN))
 
 
+def test_multishift_block_seven_shifts_refined_from_1e4_vs_oracle(l8):
+    """2 sources (a batch of 4) and 7 shifts, the FP32 stage stopped at 1e-4 so
+    that every shift's refinement runs sloppy phases and reliable updates down to
+    a 1e-10 FP64 true residual."""
+    L, fat, lng = l8
+    offs = [4 * 0.05 ** 2 + s for s in (3e-4, 1e-3, 3e-3, 1e-2, 3e-2, 0.1, 1.0)]
+    srcs = [L.gaussian_vector(1200 + s, ko.EVEN) for s in range(2)]
+    ref = []
+    for v in srcs:
+        xs, _, _ = L.multicg_offset(fat, lng, v, offs, 1e-14, parity=ko.EVEN, max_iters=5000, nrestart=1)
+        ref.append(xs)
+    sl = L.parity_slice(ko.EVEN)
+    with mq.Lattice(L.dims) as G:
+        G.dev_load_links(fat, lng)
+        for s, v in enumerate(srcs):
+            G.dev_write(s, ko.EVEN, v)
+        st = G.dev_multishift_block([0, 1], offs, 1e-4, 1e-10, parity=ko.EVEN, out_slot0=8)
+        print(st)
+        assert st["converged"] == 1
+        assert max(st["true_rel"]) < 1e-10
+        for s in range(2):
+            for j in range(len(offs)):
+                got = G.dev_read(8 + s * len(offs) + j, ko.EVEN)
+                assert rel(got[sl], ref[s][j][sl]) < 1e-7, (s, j)
+
+
 def test_multishift_invert_block_host_abi(l8):
     """milc_hip_multishift_invert_block with MILC-layout host arrays."""
     L, fat, lng = l8
