@@ -90,9 +90,14 @@ int milc_hip_invert(int external_precision, int internal_precision, double mass,
  * (-D_eo D_oe + offset_j) x_j = b, all offsets > 0.  Solutions are zeroed
  * first.  The base system (smallest offset) stops at target_resid[0];
  * internal_precision 1 runs the vector updates in FP32 with FP64
- * recurrences, 2 runs everything in FP64.  A nonzero target_relresid[j]
- * returns -4: the multi-shift solve has no relative-residual test (the
- * reference glue stops on it, ks_multicg_offset_gpu.c:55-58). */
+ * recurrences, 2 runs everything in FP64.  With args.mixed_precision set,
+ * the FP32 bulk stops at max(target_resid[0], 1e-6) and every shift is then
+ * refined to its own FP64 true residual target_resid[j] (the per-shift CG of
+ * ks_multicg_hybrid_field), up to four consecutive shifts in one batched mixed
+ * CG with an FP32 sloppy operator; final_resid[j] is then that true |r|/|b|.
+ * A nonzero target_relresid[j] returns -4: the multi-shift solve has no
+ * relative-residual test (the reference glue stops on it,
+ * ks_multicg_offset_gpu.c:55-58). */
 int milc_hip_multishift_invert(int external_precision, int internal_precision, int num_offsets,
                                const double* offsets, milc_hip_inv_args args, const double* target_resid,
                                const double* target_relresid, const void* fatlink, const void* longlink,

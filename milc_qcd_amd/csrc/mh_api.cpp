@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "../../include/milc_hip.h"
+#include "mh_batch.h"
 #include "mh_internal.h"
 
 using namespace mh;
@@ -640,23 +641,15 @@ extern "C" int milc_hip_multishift_invert(int external_precision, int internal_p
   std::vector<double> fres(num_offsets, 0.0);
   if (mixed) {
     // per-shift refinement: (offset_j - D D) x_j = b with mass = sqrt(offset_j)/2
-    // (the mass <-> offset relation of ks_multicg.c:142,222)
-    if ((rc = ensure_links(P_F32))) return rc;
-    for (int j = 0; j < num_offsets; j++) {
-      SolveParams sp;
-      sp.parity = parity;
-      sp.max_iters = mp.max_iters;
-      sp.max_restarts = 1;
-      sp.resid = target_resid[j] > 0 ? target_resid[j] : target_resid[0];
-      sp.relresid = 0;
-      sp.sloppy_prec = P_F32;
-      sp.delta = kReliableDelta;
-      sp.batch = 16;
-      SolveStats s2;
-      if ((rc = cg_solve(b, xs[j], 0.5 * std::sqrt(offs[j]), sp, s2))) return rc;
-      total_iters += s2.iters;
-      fres[j] = std::sqrt(s2.final_rsq);
-    }
+    // (the mass <-> offset relation of ks_multicg.c:142,222), FP32 sloppy
+    // operator, up to four shifts per batched CG
+    std::vector<double> res(num_offsets);
+    for (int j = 0; j < num_offsets; j++) res[j] = target_resid[j] > 0 ? target_resid[j] : target_resid[0];
+    long it = 0;
+    int conv = 1;
+    if ((rc = refine_shifts(b, xs, offs.data(), res.data(), parity, mp.max_iters, P_F32, &it, fres.data(), &conv)))
+      return rc;
+    total_iters += (int)it;
   } else {
     for (int j = 0; j < num_offsets; j++) fres[j] = std::sqrt(st.final_rsq);
   }
@@ -1026,7 +1019,6 @@ extern "C" int milc_hip_fermion_force_fn_multi(int precision, double eps, const 
 // ---------------------------------------------------------------------------
 // many right-hand sides: batched stencil, multi-shift and CG
 // ---------------------------------------------------------------------------
-#include "mh_batch.h"
 
 namespace {
 int batch_width(int nsrc) { return nsrc <= 4 ? 4 : (nsrc <= 8 ? 8 : 16); }

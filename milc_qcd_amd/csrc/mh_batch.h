@@ -116,4 +116,13 @@ struct BCGStats {
 // guess; offsets[r] = 4 m^2 (+ sigma) per rhs.
 int bcg_mixed(const void* b64, void* x64, int nr, const double* offsets, const BCGParams& prm, BCGStats& st);
 
+// Refinement of one source's multi-shift solutions x[j] (FP64 fields of the
+// solve parity) to the FP64 true |r|/|b| targets resid[j]: groups of up to four
+// consecutive shifts as one batched mixed CG with the sloppy precision
+// `sloppy` (P_F32 or P_H16).  Adds the iterations (each group once) to *iters,
+// the final true relative residuals to fres[j] (when given) and ANDs the
+// convergence flags into *converged.
+int refine_shifts(const VecField& b, std::vector<VecField>& x, const double* offs, const double* resid, int parity,
+                  int max_iters, int sloppy, long* iters, double* fres, int* converged);
+
 }  // namespace mh

@@ -111,25 +111,14 @@ static int rational_solve(const VecField& phi, int n, const double* offsets, dou
   if (!c.links[bulk].valid) return -3;
   if ((rc = multicg_solve(phi, x, offs, mp, st))) return rc;
   if (!mixed) return 0;
-  // Refinement of every shift to the FP64 true-residual target (the per-shift
-  // CG of ks_multicg_hybrid_field, generic_ks/ks_multicg.c:170-236): mixed CG
-  // from the multi-shift solution, 16-bit sloppy operator, FP64 reliable updates.
-  if ((rc = ensure_links(P_H16))) return rc;
-  for (int t = 0; t < n; t++) {
-    SolveParams sp;
-    sp.parity = parity;
-    sp.max_iters = 20000;
-    sp.max_restarts = 1;
-    sp.resid = resid;
-    sp.relresid = 0;
-    sp.sloppy_prec = P_H16;
-    sp.delta = kReliableDelta;
-    sp.batch = 16;
-    SolveStats s2;
-    if ((rc = cg_solve(phi, x[t], 0.5 * std::sqrt(offs[t]), sp, s2))) return rc;
-    st.iters += s2.iters;
-    st.converged &= s2.converged;
-  }
+  // Refinement of every shift to the FP64 true-residual target from the
+  // multi-shift solution: 16-bit sloppy operator, FP64 reliable updates, up to
+  // four shifts per batched CG.
+  std::vector<double> res(n, resid);
+  long it = 0;
+  if ((rc = refine_shifts(phi, x, offs.data(), res.data(), parity, 20000, P_H16, &it, nullptr, &st.converged)))
+    return rc;
+  st.iters += (int)it;
   return 0;
 }
 
