@@ -91,7 +91,7 @@ int milc_hip_invert(int external_precision, int internal_precision, double mass,
  * first.  The base system (smallest offset) stops at target_resid[0];
  * internal_precision 1 runs the vector updates in FP32 with FP64
  * recurrences, 2 runs everything in FP64.  With args.mixed_precision set,
- * the FP32 bulk stops at max(target_resid[0], 1e-6) and every shift is then
+ * the FP32 bulk stops at max(target_resid[0], 1e-8) and every shift is then
  * refined to its own FP64 true residual target_resid[j] (the per-shift CG of
  * ks_multicg_hybrid_field), up to four consecutive shifts in one batched mixed
  * CG with an FP32 sloppy operator; final_resid[j] is then that true |r|/|b|.

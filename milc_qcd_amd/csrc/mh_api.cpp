@@ -632,7 +632,7 @@ extern "C" int milc_hip_multishift_invert(int external_precision, int internal_p
   mp.max_iters = args.max_iter > 0 ? args.max_iter : 1000;
   // the bulk FP32 solve cannot reach FP64 targets; it stops at its own floor
   // and the per-shift mixed-precision CG below finishes each system
-  mp.resid = mixed ? std::max(target_resid[0], 1e-6) : target_resid[0];
+  mp.resid = mixed ? std::max(target_resid[0], kFp32StageResid) : target_resid[0];
   mp.prec = bulk;
   mp.batch = 16;
   SolveStats st;

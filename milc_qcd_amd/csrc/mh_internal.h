@@ -198,7 +198,9 @@ constexpr double kBatchReliableDelta = 1e-3;
 // solutions, and a deeper stage is cheaper than more refinement (same
 // workload: stage 1e-7 with 176 multi-shift iterations, 84 refinement
 // iterations for the lightest shift, 0.749 s; 1e-8 with 214 and 58, 0.706 s;
-// 3e-9 stalls at the FP32 floor: 219 and 57).
+// 3e-9 stalls at the FP32 floor: 219 and 57).  One source, 48^3x96, 12
+// shifts, refined in groups of four to 1e-10: 1.92 s with the stage at 1e-7,
+// 1.85 s at 1e-8.
 constexpr double kFp32StageResid = 1e-8;
 
 struct SolveStats {

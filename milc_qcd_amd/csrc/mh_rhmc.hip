@@ -104,7 +104,7 @@ static int rational_solve(const VecField& phi, int n, const double* offsets, dou
   MultiParams mp;
   mp.parity = parity;
   mp.max_iters = 20000;
-  mp.resid = mixed ? std::max(resid, 1e-7) : resid;
+  mp.resid = mixed ? std::max(resid, kFp32StageResid) : resid;
   mp.prec = mixed ? P_F32 : P_F64;
   mp.batch = 16;
   int bulk = mp.prec;
