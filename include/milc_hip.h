@@ -94,7 +94,8 @@ int milc_hip_invert(int external_precision, int internal_precision, double mass,
  * the FP32 bulk stops at max(target_resid[0], 1e-8) and every shift is then
  * refined to its own FP64 true residual target_resid[j] (the per-shift CG of
  * ks_multicg_hybrid_field), up to four consecutive shifts in one batched mixed
- * CG with an FP32 sloppy operator; final_resid[j] is then that true |r|/|b|.
+ * CG with an FP32 (mixed_precision 1) or 16-bit (mixed_precision 2) sloppy
+ * operator; final_resid[j] is then that true |r|/|b|.
  * A nonzero target_relresid[j] returns -4: the multi-shift solve has no
  * relative-residual test (the reference glue stops on it,
  * ks_multicg_offset_gpu.c:55-58). */

@@ -642,12 +642,13 @@ extern "C" int milc_hip_multishift_invert(int external_precision, int internal_p
   if (mixed) {
     // per-shift refinement: (offset_j - D D) x_j = b with mass = sqrt(offset_j)/2
     // (the mass <-> offset relation of ks_multicg.c:142,222), FP32 sloppy
-    // operator, up to four shifts per batched CG
+    // operator (16-bit with mixed_precision 2), up to four shifts per batched CG
     std::vector<double> res(num_offsets);
     for (int j = 0; j < num_offsets; j++) res[j] = target_resid[j] > 0 ? target_resid[j] : target_resid[0];
     long it = 0;
     int conv = 1;
-    if ((rc = refine_shifts(b, xs, offs.data(), res.data(), parity, mp.max_iters, P_F32, &it, fres.data(), &conv)))
+    const int sloppy = args.mixed_precision == 2 ? P_H16 : P_F32;
+    if ((rc = refine_shifts(b, xs, offs.data(), res.data(), parity, mp.max_iters, sloppy, &it, fres.data(), &conv)))
       return rc;
     total_iters += (int)it;
   } else {

@@ -164,20 +164,27 @@ def test_multishift_f64_matches_oracle(hisq8, gpu8):
     assert abs(qic[0].final_iters - it_ref) <= 2
 
 
-def test_multishift_mixed_per_shift_residuals(hisq8, gpu8):
+@pytest.mark.parametrize("mixed", [1, 2])
+def test_multishift_mixed_per_shift_residuals(hisq8, gpu8, mixed):
+    """FP32 bulk, then every shift refined to its own FP64 target with an FP32
+    (mixed 1) or 16-bit (mixed 2) sloppy operator; six shifts are refined as a
+    batched group of four and one of two (two unused slots)."""
     L, U, fat, lng, W, _ = hisq8
     src = L.gaussian_vector(3001, ko.EVEN)
     m = 0.02
     offsets = [4 * m * m + s for s in np.geomspace(1e-4, 1.0, 6)]
+    targets = [1e-9, 1e-9, 3e-9, 1e-8, 1e-8, 3e-8]
     ksp = [mq.KsParam(offset=o) for o in offsets]
-    qic = [mq.QuarkInvertControl(max=3000, nrestart=1, parity=mq.EVEN, resid=1e-9) for _ in offsets]
-    xs = gpu8.ks_multicg_offset_field(src, ksp, qic, mixed=1, prec=1)
+    qic = [mq.QuarkInvertControl(max=3000, nrestart=1, parity=mq.EVEN, resid=t) for t in targets]
+    xs = gpu8.ks_multicg_offset_field(src, ksp, qic, mixed=mixed, prec=1)
     bn = np.sqrt(L.norm2(src, ko.EVEN))
     for j, o in enumerate(offsets):
         r = src - L.mdagm(fat, lng, o, xs[j], ko.EVEN)
         tr = np.sqrt(L.norm2(r, ko.EVEN)) / bn
-        print(f"shift {o:.4g}: true resid {tr:.3e}")
-        assert tr < 1.01e-9
+        print(f"shift {o:.4g}: true resid {tr:.3e} (target {targets[j]:g}, reported {qic[j].final_rsq ** 0.5:.3e})")
+        assert tr < 1.01 * targets[j]
+        assert qic[j].converged == 1
+        assert abs(qic[j].final_rsq ** 0.5 - tr) <= 1e-3 * tr
 
 
 def test_oprod_matches_oracle(hisq8, gpu8):
