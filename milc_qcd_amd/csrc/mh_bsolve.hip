@@ -242,68 +242,92 @@ __global__ void k_bms_coef(const BMSR* __restrict__ s, const BMSG* __restrict__ 
   }
 }
 
-// window flush: psim_j and pm_j brought up to date from the ring of residuals
+// Window flush: psim_j and pm_j brought up to date from the ring of residuals,
+//   psim_j += wx p_j + sum_l wb_l r^(l+1),   pm_j = wa p_j + sum_l wc_l r^(l+1)
+// over the md slots of the window (coefficients of the rhs from k_bms_coef;
+// slots past md count as zero), then ring slot 0 takes the last slot.  The
+// field is walked as a flat float array, one float4 per thread and pass, so
+// every access is a whole 16-byte word and a wavefront's words are
+// contiguous (48^3x96, 4 rhs, 9 shifts: 6.0 ms against 7.0 ms for one thread
+// per 6-float record with 8-byte accesses; same results bit for bit).  A
+// record is 6 floats, so a float4 holds parts of one or two records (rhs ra
+// and rb, the next one); each component takes its own record's coefficients
+// and window length.
 template <int NR>
-__global__ void __launch_bounds__(256) k_bms_flush(float* __restrict__ ring, size_t ring_stride, FPtrs pm, FPtrs psim,
-                                                   const float* __restrict__ cf, const int* __restrict__ meta, int n,
-                                                   long nrec, int loop) {
-  const int rhs = (blockIdx.x * blockDim.x + threadIdx.x) % NR;
-  const int md = meta[2 * rhs], nn = meta[2 * rhs + 1];
-  if (md == 0) return;
-  const float* cr = cf + (size_t)rhs * n * BMS_NC;
-  REC_LOOP(idx) {
-    if (idx >= nrec) continue;
-    const size_t o = (size_t)idx * 6;
-    float rv[BKW][6];
+__global__ void __launch_bounds__(256) k_bms_flush4(float* __restrict__ ring, size_t ring_stride, FPtrs pm,
+                                                    FPtrs psim, const float* __restrict__ cf,
+                                                    const int* __restrict__ meta, int n, long nq, int loop) {
+  REC_LOOP(q) {
+    if (q >= nq) continue;
+    const long f0 = q * 4;
+    const long ra_rec = f0 / 6;
+    const int ra = (int)(ra_rec % NR), rb = (int)((ra_rec + 1) % NR);
+    const int na = (int)(ra_rec * 6 + 6 - f0);  // components u < na belong to record ra (na = 2, 4 or 6)
+    const int mda = meta[2 * ra], nna = meta[2 * ra + 1];
+    const int mdb = na < 4 ? meta[2 * rb] : 0, nnb = na < 4 ? meta[2 * rb + 1] : 0;
+    int md[4];
 #pragma unroll
-    for (int l = 0; l < BKW; l++)
+    for (int u = 0; u < 4; u++) md[u] = u < na ? mda : mdb;
+    const int mdmax = max(mda, mdb);
+    if (mdmax == 0) continue;
+    float rv[BKW][4];
 #pragma unroll
-      for (int h = 0; h < 3; h++) {
-        // ring_stride and o are even: 8-byte aligned pairs
-        const float2 t = l < md ? ((const float2*)(ring + (size_t)(l + 1) * ring_stride + o))[h] : make_float2(0.f, 0.f);
-        rv[l][2 * h] = t.x;
-        rv[l][2 * h + 1] = t.y;
-      }
-    // 8-byte accesses: records start at even float offsets (6 floats each)
-    // and coefficient rows hold an even number (BMS_NC) of floats
-    for (int j = 0; j < nn; j++) {
-      const float2* c2 = (const float2*)(cr + (size_t)j * BMS_NC);
-      float c[BMS_NC];
+    for (int l = 0; l < BKW; l++) {
+      const float4 t = l < mdmax ? *(const float4*)(ring + (size_t)(l + 1) * ring_stride + f0)
+                                 : make_float4(0.f, 0.f, 0.f, 0.f);
+      rv[l][0] = l < md[0] ? t.x : 0.f;
+      rv[l][1] = l < md[1] ? t.y : 0.f;
+      rv[l][2] = l < md[2] ? t.z : 0.f;
+      rv[l][3] = l < md[3] ? t.w : 0.f;
+    }
+    const int nnmax = max(mda > 0 ? nna : 0, mdb > 0 ? nnb : 0);
+    for (int j = 0; j < nnmax; j++) {
+      const bool la = mda > 0 && j < nna, lb = mdb > 0 && j < nnb;
+      float ca[BMS_NC], cb[BMS_NC];
+      const float2* c2a = (const float2*)(cf + ((size_t)ra * n + j) * BMS_NC);
+      const float2* c2b = (const float2*)(cf + ((size_t)rb * n + j) * BMS_NC);
 #pragma unroll
       for (int l = 0; l < BMS_NC / 2; l++) {
-        const float2 t = c2[l];
-        c[2 * l] = t.x;
-        c[2 * l + 1] = t.y;
+        const float2 ta = la ? c2a[l] : make_float2(0.f, 0.f);
+        const float2 tb = lb ? c2b[l] : make_float2(0.f, 0.f);
+        ca[2 * l] = ta.x;
+        ca[2 * l + 1] = ta.y;
+        cb[2 * l] = tb.x;
+        cb[2 * l + 1] = tb.y;
       }
-      float2* pmj = (float2*)(pm.v[j] + o);
-      float2* xj = (float2*)(psim.v[j] + o);
+      float4* pmj = (float4*)(pm.v[j] + f0);
+      float4* xj = (float4*)(psim.v[j] + f0);
+      const float4 p4 = *pmj, x4 = *xj;
+      const float pv[4] = {p4.x, p4.y, p4.z, p4.w};
+      float xv[4] = {x4.x, x4.y, x4.z, x4.w}, qv[4] = {p4.x, p4.y, p4.z, p4.w};
 #pragma unroll
-      for (int h = 0; h < 3; h++) {
-        const float2 p2 = pmj[h], x2 = xj[h];
-        const float pv[2] = {p2.x, p2.y};
-        float xv[2] = {x2.x, x2.y}, qv[2];
+      for (int u = 0; u < 4; u++) {
+        const bool a = u < na;
+        if (!(a ? la : lb)) continue;
+        const float* c = a ? ca : cb;
+        float x = xv[u] + c[0] * pv[u];
+        float qq = c[1 + BKW] * pv[u];
 #pragma unroll
-        for (int u = 0; u < 2; u++) {
-          const int k = 2 * h + u;
-          float x = xv[u] + c[0] * pv[u];
-          float q = c[1 + BKW] * pv[u];
-#pragma unroll
-          for (int l = 0; l < BKW; l++) {
-            x += c[1 + l] * rv[l][k];
-            q += c[2 + BKW + l] * rv[l][k];
-          }
-          xv[u] = x;
-          qv[u] = q;
+        for (int l = 0; l < BKW; l++) {
+          x += c[1 + l] * rv[l][u];
+          qq += c[2 + BKW + l] * rv[l][u];
         }
-        xj[h] = make_float2(xv[0], xv[1]);
-        pmj[h] = make_float2(qv[0], qv[1]);
+        xv[u] = x;
+        qv[u] = qq;
       }
+      *xj = make_float4(xv[0], xv[1], xv[2], xv[3]);
+      *pmj = make_float4(qv[0], qv[1], qv[2], qv[3]);
     }
+    // ring slot 0 takes the last slot of the window, per record
+    float4* r0 = (float4*)(ring + f0);
+    float4 o = (mda > 0 && (na >= 4 || mdb > 0)) ? make_float4(0.f, 0.f, 0.f, 0.f) : *r0;
+    float ov[4] = {o.x, o.y, o.z, o.w};
 #pragma unroll
-    for (int l = 0; l < BKW; l++)
-      if (l + 1 == md)
+    for (int u = 0; u < 4; u++)
 #pragma unroll
-        for (int k = 0; k < 6; k++) ring[o + k] = rv[l][k];
+      for (int l = 0; l < BKW; l++)
+        if (l + 1 == md[u]) ov[u] = rv[l][u];
+    *r0 = make_float4(ov[0], ov[1], ov[2], ov[3]);
   }
 }
 
@@ -589,7 +613,8 @@ int bmultishift_nr(const void* b64, const double* offs, int n, const double* res
   int pstride;
   if ((rc = bpartials(&part, &pstride))) return rc;
   const RecGrid rg = rec_grid(NR, pstride);
-  const RecGrid rf = rec_grid(NR, 2048);
+  // the flush walks float4 words: 6 NR / 4 of them per site
+  const RecGrid rf = rec_grid(NR * 6 / 4, 2048);
   int variant;
   if ((rc = btune(P_F32, NR, &variant, nullptr))) return rc;
 
@@ -693,7 +718,7 @@ int bmultishift_nr(const void* b64, const double* offs, int n, const double* res
                          rg.loop);
     }
     hipLaunchKernelGGL(k_bms_coef, dim3(1), dim3(256), 0, c.stream, ds, dg, (float*)dcoef.p, (int*)dmeta.p);
-    hipLaunchKernelGGL(k_bms_flush<NR>, dim3(rf.nb), dim3(256), 0, c.stream, ring0, ring_stride, fpm, fpsim,
+    hipLaunchKernelGGL(k_bms_flush4<NR>, dim3(rf.nb), dim3(256), 0, c.stream, ring0, ring_stride, fpm, fpsim,
                        (const float*)dcoef.p, (const int*)dmeta.p, n, rf.nrec, rf.loop);
     MH_CHECK(hipGetLastError());
     return 0;
