@@ -991,7 +991,11 @@ int bcg_mixed(const void* b64, void* x64, int nr, const double* offsets, const B
 // The batched stencil of four right-hand sides costs about twice the
 // one-source stencil (48^3x96, 16-bit: 995 against 500-550 us; FP32: 1878
 // against about 1000 us), and neighbouring shifts need similar iteration
-// counts, so a group costs about twice its slowest shift.
+// counts, so a group costs about 2.5 times its slowest shift (48^3x96, 12
+// shifts to 1e-10: 119, 113 and 48 iterations for the three groups, 0.91 s).
+// One group of the eight lightest shifts instead of two of four was measured
+// at 0.87 s (an 8-rhs iteration costs 1.75 times a 4-rhs one), not enough for
+// the larger work fields.
 int refine_shifts(const VecField& b, std::vector<VecField>& x, const double* offs, const double* resid, int parity,
                   int max_iters, int sloppy, long* iters, double* fres, int* converged) {
   Context& c = ctx();
