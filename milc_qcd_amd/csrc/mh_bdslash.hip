@@ -523,6 +523,9 @@ int launch_h16(const BDOp& op, int block, int loop_min, int order, int* nb_out) 
 template <int NR>
 int launch_h16_tps(const BDOp& op, const BVariant& v, int* nb) {
   switch (v.tps) {
+    case 1:
+      if constexpr (NR <= 8) return launch_h16<NR, 1>(op, v.block, v.loop, v.order, nb);
+      break;
     case 2:
       if constexpr (NR / 2 <= 8) return launch_h16<NR, 2>(op, v.block, v.loop, v.order, nb);
       break;
@@ -635,7 +638,6 @@ std::vector<BVariant> bvariants(int prec, int nr) {
   for (int order : orders)
     for (int tps : {1, 2, 4, 8}) {
       if (tps > nr || nr / tps > rpt_max) continue;
-      if (prec == P_H16 && tps == 1) continue;
       for (int block : {128, 256})
         for (int loop : {1, 2}) {
           if (order && loop > 1) continue;
