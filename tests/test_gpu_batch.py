@@ -55,6 +55,31 @@ def test_block_dslash_every_variant_vs_oracle(prec, nrhs):
                     assert e < tol, (prec, nrhs, var, out_par, s, e)
 
 
+@pytest.mark.parametrize("prec,nrhs", [(2, 4), (1, 4), (1, 16), (3, 4), (3, 16)])
+def test_block_dslash_site_orders_vs_oracle(prec, nrhs):
+    """Every variant on a lattice where every blocked site order applies
+    (16x16x24x4: blocks of 8x8x8x4, 8x4x4x4 and 4x4x4x4 sites)."""
+    dims = (16, 16, 24, 4)
+    L, fat, lng = hisq_setup(dims, 1007)
+    tol = {2: 1e-13, 1: 2e-6, 3: 2e-4}[prec]
+    with mq.Lattice(dims) as G:
+        G.dev_load_links(fat, lng)
+        nvar, chosen, us, desc = G.dev_block_tune(prec, nrhs)
+        descs = [G.dev_block_describe(prec, nrhs, v) for v in range(nvar)]
+        print(f"prec {prec} nrhs {nrhs}: {nvar} variants, tuned -> {chosen} ({desc})")
+        assert all(any(b in d for d in descs) for b in ("8x8x8x4", "8x4x4x4", "4x4x4x4"))
+        srcs = [L.gaussian_vector(760 + s, ko.ODD) for s in range(nrhs)]
+        refs = [L.dslash(fat, lng, v, ko.EVEN) for v in srcs]
+        for s, v in enumerate(srcs):
+            G.dev_write(s, ko.ODD, v)
+        sl = L.parity_slice(ko.EVEN)
+        for var in range(nvar):
+            G.dev_block_dslash(prec, nrhs, var, list(range(nrhs)), ko.EVEN, list(range(20, 20 + nrhs)))
+            for s in range(nrhs):
+                e = rel(G.dev_read(20 + s, ko.EVEN)[sl], refs[s][sl])
+                assert e < tol, (prec, nrhs, var, descs[var], s, e)
+
+
 def test_block_dslash_16bit_nine_element_naik_vs_oracle():
     """The 16-bit batched stencil with the Naik links in the nine-element form
     (two-row form switched off), 16 rhs, every variant: 2e-4 of the oracle."""
