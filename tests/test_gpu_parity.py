@@ -164,23 +164,23 @@ def test_multishift_f64_matches_oracle(hisq8, gpu8):
     assert abs(qic[0].final_iters - it_ref) <= 2
 
 
-@pytest.mark.parametrize("mixed", [1, 2])
-def test_multishift_mixed_per_shift_residuals(hisq8, gpu8, mixed):
+@pytest.mark.parametrize("mixed,parity", [(1, ko.EVEN), (2, ko.EVEN), (2, ko.ODD)])
+def test_multishift_mixed_per_shift_residuals(hisq8, gpu8, mixed, parity):
     """FP32 bulk, then every shift refined to its own FP64 target with an FP32
     (mixed 1) or 16-bit (mixed 2) sloppy operator; six shifts are refined as a
     batched group of four and one of two (two unused slots)."""
     L, U, fat, lng, W, _ = hisq8
-    src = L.gaussian_vector(3001, ko.EVEN)
+    src = L.gaussian_vector(3001, parity)
     m = 0.02
     offsets = [4 * m * m + s for s in np.geomspace(1e-4, 1.0, 6)]
     targets = [1e-9, 1e-9, 3e-9, 1e-8, 1e-8, 3e-8]
     ksp = [mq.KsParam(offset=o) for o in offsets]
-    qic = [mq.QuarkInvertControl(max=3000, nrestart=1, parity=mq.EVEN, resid=t) for t in targets]
+    qic = [mq.QuarkInvertControl(max=3000, nrestart=1, parity=parity, resid=t) for t in targets]
     xs = gpu8.ks_multicg_offset_field(src, ksp, qic, mixed=mixed, prec=1)
-    bn = np.sqrt(L.norm2(src, ko.EVEN))
+    bn = np.sqrt(L.norm2(src, parity))
     for j, o in enumerate(offsets):
-        r = src - L.mdagm(fat, lng, o, xs[j], ko.EVEN)
-        tr = np.sqrt(L.norm2(r, ko.EVEN)) / bn
+        r = src - L.mdagm(fat, lng, o, xs[j], parity)
+        tr = np.sqrt(L.norm2(r, parity)) / bn
         print(f"shift {o:.4g}: true resid {tr:.3e} (target {targets[j]:g}, reported {qic[j].final_rsq ** 0.5:.3e})")
         assert tr < 1.01 * targets[j]
         assert qic[j].converged == 1
