@@ -187,6 +187,25 @@ def test_multishift_mixed_per_shift_residuals(hisq8, gpu8, mixed, parity):
         assert abs(qic[j].final_rsq ** 0.5 - tr) <= 1e-3 * tr
 
 
+@pytest.mark.parametrize("nshift", [1, 5, 9])
+def test_multishift_mixed_group_edges(hisq8, gpu8, nshift):
+    """Shift counts at the edges of the grouped refinement: one shift (the
+    one-source mixed CG alone), five (a group of four and a lone shift) and
+    nine (two groups and a lone shift); every shift to its FP64 target."""
+    L, U, fat, lng, W, _ = hisq8
+    src = L.gaussian_vector(3100 + nshift, ko.EVEN)
+    m = 0.03
+    offsets = [4 * m * m + s for s in np.geomspace(3e-4, 0.5, nshift)] if nshift > 1 else [4 * m * m + 1e-3]
+    ksp = [mq.KsParam(offset=o) for o in offsets]
+    qic = [mq.QuarkInvertControl(max=3000, nrestart=1, parity=ko.EVEN, resid=1e-9) for _ in offsets]
+    xs = gpu8.ks_multicg_offset_field(src, ksp, qic, mixed=2, prec=1)
+    bn = np.sqrt(L.norm2(src, ko.EVEN))
+    for j, o in enumerate(offsets):
+        tr = np.sqrt(L.norm2(src - L.mdagm(fat, lng, o, xs[j], ko.EVEN), ko.EVEN)) / bn
+        assert tr < 1.01e-9, (nshift, j, tr)
+        assert qic[j].converged == 1
+
+
 def test_oprod_matches_oracle(hisq8, gpu8):
     L, U, fat, lng, W, _ = hisq8
     xs = [L.gaussian_vector(4000 + t, ko.EVENANDODD) for t in range(3)]
